@@ -1,0 +1,311 @@
+// Device layer behind the OpenCL_* facade (include/fpde_opencl_facade.h).
+//
+// Replaces opencl_cuda_base.cpp (OpenCL flavour :20-458, CUDA/NVRTC flavour
+// :459-826) for the reference's GPU history sum.  There is no run-time
+// compilation: kernels are precompiled for gfx950 and bound by name; the
+// program object only records the #define constants the reference prints
+// into its kernel text (sum_alg, OS, BS: fpde_hpc_1d.cpp:1771-1775).
+// Errors are returned as codes (the reference prints and calls exit(0),
+// opencl_cuda_base.cpp:12-17); sizes are 64-bit.
+#include "fpde_internal.h"
+
+#include <cstdlib>
+#include <cstring>
+#include <map>
+#include <mutex>
+#include <string>
+#include <vector>
+
+using namespace fpde;
+
+struct fpde_cl_ctx {
+    int device = 0;
+    hipStream_t stream = nullptr;
+};
+
+struct fpde_cl_buf {
+    void* ptr = nullptr;
+    size_t bytes = 0;
+};
+
+struct fpde_cl_prog {
+    fpde_cl_ctx* ctx = nullptr;
+    int sum_alg = 0;
+    int64_t os = 0;
+    int bs = 16;
+};
+
+struct fpde_cl_kernel {
+    fpde_cl_prog* prog = nullptr;
+    std::string name;
+    struct Arg {
+        bool set = false;
+        bool is_buf = false;
+        fpde_cl_buf* buf = nullptr;
+        std::vector<unsigned char> bytes;
+    };
+    std::vector<Arg> args;
+};
+
+namespace {
+
+// value of "#define NAME <int>" in a kernel source text, or dflt
+int64_t parse_define(const char* src, const char* name, int64_t dflt) {
+    std::string s(src ? src : "");
+    std::string key = std::string("#define ") + name;
+    size_t p = 0;
+    while ((p = s.find(key, p)) != std::string::npos) {
+        size_t q = p + key.size();
+        if (q < s.size() && (s[q] == ' ' || s[q] == '\t')) {
+            return std::strtoll(s.c_str() + q, nullptr, 10);
+        }
+        p = q;
+    }
+    return dflt;
+}
+
+fpde_status arg_double(const fpde_cl_kernel* k, int idx, double* out) {
+    if (idx >= (int)k->args.size() || !k->args[idx].set || k->args[idx].is_buf) {
+        set_error("kernel argument " + std::to_string(idx) + " not set (scalar expected)");
+        return FPDE_ERR_ARG;
+    }
+    const auto& b = k->args[idx].bytes;
+    if (b.size() == sizeof(double)) { std::memcpy(out, b.data(), sizeof(double)); return FPDE_OK; }
+    if (b.size() == sizeof(float)) { float f; std::memcpy(&f, b.data(), sizeof(float)); *out = f; return FPDE_OK; }
+    set_error("kernel argument " + std::to_string(idx) + ": unsupported scalar size");
+    return FPDE_ERR_ARG;
+}
+
+fpde_status arg_int(const fpde_cl_kernel* k, int idx, int64_t* out) {
+    if (idx >= (int)k->args.size() || !k->args[idx].set || k->args[idx].is_buf) {
+        set_error("kernel argument " + std::to_string(idx) + " not set (scalar expected)");
+        return FPDE_ERR_ARG;
+    }
+    const auto& b = k->args[idx].bytes;
+    if (b.size() == 4) { int32_t v; std::memcpy(&v, b.data(), 4); *out = v; return FPDE_OK; }
+    if (b.size() == 8) { int64_t v; std::memcpy(&v, b.data(), 8); *out = v; return FPDE_OK; }
+    set_error("kernel argument " + std::to_string(idx) + ": unsupported integer size");
+    return FPDE_ERR_ARG;
+}
+
+fpde_status arg_buf(const fpde_cl_kernel* k, int idx, fpde_cl_buf** out) {
+    if (idx >= (int)k->args.size() || !k->args[idx].set || !k->args[idx].is_buf) {
+        set_error("kernel argument " + std::to_string(idx) + " not set (buffer expected)");
+        return FPDE_ERR_ARG;
+    }
+    *out = k->args[idx].buf;
+    return FPDE_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+fpde_status fpde_cl_ctx_create(int device, fpde_cl_ctx** out) {
+    if (!out) { set_error("null out"); return FPDE_ERR_ARG; }
+    int n = 0;
+    FPDE_HIP_TRY(hipGetDeviceCount(&n));
+    if (device < 0 || device >= n) { set_error("no such device"); return FPDE_ERR_ARG; }
+    FPDE_HIP_TRY(hipSetDevice(device));
+    auto* c = new fpde_cl_ctx;
+    c->device = device;
+    hipError_t e = hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking);
+    if (e != hipSuccess) { delete c; set_error("hipStreamCreate failed"); return FPDE_ERR_HIP; }
+    *out = c;
+    return FPDE_OK;
+}
+
+fpde_status fpde_cl_ctx_destroy(fpde_cl_ctx* ctx) {
+    if (!ctx) return FPDE_OK;
+    hipStreamSynchronize(ctx->stream);
+    hipStreamDestroy(ctx->stream);
+    delete ctx;
+    return FPDE_OK;
+}
+
+fpde_status fpde_cl_buf_alloc(fpde_cl_ctx* ctx, size_t bytes, const void* host_init, fpde_cl_buf** out) {
+    if (!ctx || !out) { set_error("null argument"); return FPDE_ERR_ARG; }
+    auto* b = new fpde_cl_buf;
+    b->bytes = bytes;
+    hipError_t e = hipMalloc(&b->ptr, bytes ? bytes : 8);
+    if (e != hipSuccess) { delete b; set_error(std::string("hipMalloc: ") + hipGetErrorString(e)); return FPDE_ERR_NOMEM; }
+    if (host_init && bytes) {
+        e = hipMemcpy(b->ptr, host_init, bytes, hipMemcpyHostToDevice);
+        if (e != hipSuccess) { hipFree(b->ptr); delete b; set_error("initial copy failed"); return FPDE_ERR_HIP; }
+    }
+    *out = b;
+    return FPDE_OK;
+}
+
+fpde_status fpde_cl_buf_free(fpde_cl_buf* buf) {
+    if (!buf) return FPDE_OK;
+    hipFree(buf->ptr);
+    delete buf;
+    return FPDE_OK;
+}
+
+size_t fpde_cl_buf_size(const fpde_cl_buf* buf) { return buf ? buf->bytes : 0; }
+void* fpde_cl_buf_device_ptr(const fpde_cl_buf* buf) { return buf ? buf->ptr : nullptr; }
+
+fpde_status fpde_cl_buf_write(fpde_cl_ctx* ctx, fpde_cl_buf* buf, size_t offset, size_t bytes, const void* src) {
+    if (!ctx || !buf || !src) { set_error("null argument"); return FPDE_ERR_ARG; }
+    if (bytes == 0) bytes = buf->bytes - offset;
+    if (offset > buf->bytes || bytes > buf->bytes - offset) { set_error("write out of buffer bounds"); return FPDE_ERR_ARG; }
+    FPDE_HIP_TRY(hipMemcpyAsync((char*)buf->ptr + offset, src, bytes, hipMemcpyHostToDevice, ctx->stream));
+    FPDE_HIP_TRY(hipStreamSynchronize(ctx->stream));
+    return FPDE_OK;
+}
+
+fpde_status fpde_cl_buf_read(fpde_cl_ctx* ctx, fpde_cl_buf* buf, size_t offset, size_t bytes, void* dst) {
+    if (!ctx || !buf || !dst) { set_error("null argument"); return FPDE_ERR_ARG; }
+    if (bytes == 0) bytes = buf->bytes - offset;
+    if (offset > buf->bytes || bytes > buf->bytes - offset) { set_error("read out of buffer bounds"); return FPDE_ERR_ARG; }
+    FPDE_HIP_TRY(hipMemcpyAsync(dst, (char*)buf->ptr + offset, bytes, hipMemcpyDeviceToHost, ctx->stream));
+    FPDE_HIP_TRY(hipStreamSynchronize(ctx->stream));
+    return FPDE_OK;
+}
+
+fpde_status fpde_cl_program_create(fpde_cl_ctx* ctx, const char* source, fpde_cl_prog** out) {
+    if (!ctx || !out) { set_error("null argument"); return FPDE_ERR_ARG; }
+    auto* p = new fpde_cl_prog;
+    p->ctx = ctx;
+    p->sum_alg = (int)parse_define(source, "sum_alg", 0);
+    p->os = parse_define(source, "OS", 0);
+    p->bs = (int)parse_define(source, "BS", 16);
+    *out = p;
+    return FPDE_OK;
+}
+
+fpde_status fpde_cl_program_destroy(fpde_cl_prog* prog) {
+    delete prog;
+    return FPDE_OK;
+}
+
+fpde_status fpde_cl_kernel_create(fpde_cl_prog* prog, const char* name, fpde_cl_kernel** out) {
+    if (!prog || !name || !out) { set_error("null argument"); return FPDE_ERR_ARG; }
+    if (std::strcmp(name, "Sum") != 0) {
+        set_error(std::string("unknown kernel '") + name + "' (available: Sum)");
+        return FPDE_ERR_NOTFOUND;
+    }
+    auto* k = new fpde_cl_kernel;
+    k->prog = prog;
+    k->name = name;
+    k->args.resize(12);
+    *out = k;
+    return FPDE_OK;
+}
+
+fpde_status fpde_cl_kernel_destroy(fpde_cl_kernel* k) {
+    delete k;
+    return FPDE_OK;
+}
+
+fpde_status fpde_cl_kernel_set_arg(fpde_cl_kernel* k, int idx, size_t size, const void* value) {
+    if (!k || idx < 0 || !value) { set_error("bad argument"); return FPDE_ERR_ARG; }
+    if (idx >= (int)k->args.size()) k->args.resize(idx + 1);
+    auto& a = k->args[idx];
+    a.set = true;
+    a.is_buf = false;
+    a.buf = nullptr;
+    a.bytes.assign((const unsigned char*)value, (const unsigned char*)value + size);
+    return FPDE_OK;
+}
+
+fpde_status fpde_cl_kernel_set_buffer_arg(fpde_cl_kernel* k, int idx, fpde_cl_buf* buf) {
+    if (!k || idx < 0 || !buf) { set_error("bad argument"); return FPDE_ERR_ARG; }
+    if (idx >= (int)k->args.size()) k->args.resize(idx + 1);
+    auto& a = k->args[idx];
+    a.set = true;
+    a.is_buf = true;
+    a.buf = buf;
+    a.bytes.clear();
+    return FPDE_OK;
+}
+
+// "Sum" with the argument ABI of fr_ocl_call_sum (fpde_hpc_1d.cpp:1851-1868):
+// 0 S, 1 old, 2 int old_size, 3 tau, 4 Htmp, 5 row2_precalc, 6 kb_3_cache,
+// 7 double sum_param, 8 gtj, 9 gt, 10 alpha, 11 int tstep.
+fpde_status fpde_cl_launch(fpde_cl_ctx* ctx, fpde_cl_kernel* k, int ndims, const size_t* gdims, const size_t* ldims) {
+    (void)ldims;
+    if (!ctx || !k || ndims < 1 || !gdims) { set_error("bad launch arguments"); return FPDE_ERR_ARG; }
+    fpde_cl_buf *S, *old, *tau, *kb;
+    int64_t old_size;
+    double sum_param;
+    fpde_status rc;
+    if ((rc = arg_buf(k, 0, &S)) || (rc = arg_buf(k, 1, &old)) || (rc = arg_int(k, 2, &old_size)) ||
+        (rc = arg_buf(k, 3, &tau)) || (rc = arg_buf(k, 6, &kb)) || (rc = arg_double(k, 7, &sum_param)))
+        return rc;
+    int sum_alg = k->prog->sum_alg;
+    if (sum_alg == 2) {
+        set_error("sum_alg 2 (series-compressed history) is not provided by this library");
+        return FPDE_ERR_ARG;
+    }
+    int64_t os = k->prog->os;
+    int64_t nodes = (int64_t)gdims[0];
+    if (os <= 0) os = nodes;
+    // bounds: the launch may be rounded up past OS (fpde_hpc_1d.cpp:1873-1874)
+    if (nodes > os) nodes = os;
+    if ((int64_t)(S->bytes / sizeof(double)) < nodes) { set_error("S buffer too small"); return FPDE_ERR_ARG; }
+    if (old_size >= 1 && (int64_t)(old->bytes / sizeof(double)) < old_size * os) { set_error("old buffer too small"); return FPDE_ERR_ARG; }
+    if (old_size >= 2 && ((int64_t)(kb->bytes / sizeof(double)) < old_size - 1 || (int64_t)(tau->bytes / sizeof(double)) < old_size - 1)) {
+        set_error("kb/tau buffers too small");
+        return FPDE_ERR_ARG;
+    }
+    rc = launch_ref_sum((double*)S->ptr, (const double*)old->ptr, old_size, os, nodes, (const double*)tau->ptr,
+                        (const double*)kb->ptr, sum_alg, sum_param, k->prog->bs, ctx->stream);
+    if (rc != FPDE_OK) return rc;
+    FPDE_HIP_TRY(hipGetLastError());
+    return FPDE_OK;
+}
+
+fpde_status fpde_cl_finish(fpde_cl_ctx* ctx) {
+    if (!ctx) { set_error("null ctx"); return FPDE_ERR_ARG; }
+    FPDE_HIP_TRY(hipStreamSynchronize(ctx->stream));
+    return FPDE_OK;
+}
+
+// ---------------------------------------------------------------------------
+fpde_status fpde_reference_sum(double* S, const double* old, int64_t old_size, int64_t os, int64_t nodes,
+                               const double* tau, const double* kb, int sum_alg, double sum_param) {
+    if (!S || !old || (old_size >= 2 && (!tau || !kb)) || nodes < 0 || os < nodes) {
+        set_error("bad arguments");
+        return FPDE_ERR_ARG;
+    }
+    if (sum_alg != 0 && sum_alg != 1) { set_error("sum_alg must be 0 or 1"); return FPDE_ERR_ARG; }
+    fpde_status rc = launch_ref_sum(S, old, old_size, os, nodes, tau, kb, sum_alg, sum_param, 16, 0);
+    if (rc != FPDE_OK) return rc;
+    FPDE_HIP_TRY(hipGetLastError());
+    FPDE_HIP_TRY(hipDeviceSynchronize());
+    return FPDE_OK;
+}
+
+fpde_status fpde_history_sum_f64(double* S, const double* D, int64_t n, int64_t nodes, int64_t ld,
+                                 const double* omega) {
+    if (!S || !D || !omega || n < 0 || nodes < 0 || ld < nodes || (ld & 1) || (nodes > (1 << 30))) {
+        set_error("bad arguments (ld must be even and >= nodes)");
+        return FPDE_ERR_ARG;
+    }
+    if (n == 0) {
+        FPDE_HIP_TRY(hipMemset(S, 0, sizeof(double) * nodes));
+        FPDE_HIP_TRY(hipDeviceSynchronize());
+        return FPDE_OK;
+    }
+    // partials then one combine pass
+    double* part = nullptr;
+    FPDE_HIP_TRY(hipMalloc(&part, sizeof(double) * 64 * (size_t)ld));
+    int splits = 0;
+    fpde_status rc = launch_hist_plain(D, ld, (int)nodes, (int)n, (int)n, omega, part, &splits, 0);
+    if (rc == FPDE_OK) {
+        static double zero_corr[4] = {0, 0, 0, 0};
+        (void)zero_corr;
+        // combine: S = sum of partial rows (no near levels, no corrections)
+        rc = launch_hist_combine(S, part, splits, ld, 0, D, ld, nodes, 0, 0, omega, nullptr, 0);
+    }
+    hipError_t e = hipDeviceSynchronize();
+    hipFree(part);
+    if (rc != FPDE_OK) return rc;
+    if (e != hipSuccess) { set_error(hipGetErrorString(e)); return FPDE_ERR_HIP; }
+    return FPDE_OK;
+}
+
+}  // extern "C"

@@ -1,0 +1,134 @@
+// Internal declarations shared by the HIP translation units of libfpde_hip.so.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <cstdint>
+#include <cstddef>
+#include <string>
+
+#include "../../include/fpde_hip.h"
+
+namespace fpde {
+
+// ---------------------------------------------------------------------------
+// error plumbing: every C-ABI entry returns an fpde_status and records text.
+void set_error(const std::string& msg);
+#define FPDE_HIP_TRY(expr)                                                     \
+    do {                                                                       \
+        hipError_t e_ = (expr);                                                \
+        if (e_ != hipSuccess) {                                                \
+            ::fpde::set_error(std::string(#expr " failed: ") +                 \
+                              hipGetErrorString(e_));                          \
+            return FPDE_ERR_HIP;                                               \
+        }                                                                      \
+    } while (0)
+
+// ---------------------------------------------------------------------------
+// van Genuchten-Mualem soil constants (fpde_hpc_1d.cpp:2513-2524).
+constexpr double kVgmN = 1.25;
+constexpr double kVgmS0 = 0.0736;
+constexpr double kVgmS1 = 0.55;
+constexpr double kVgmA = 0.066;
+
+// Conductivity K = ksat*Kr and the inverse water capacity 1/C at suction head
+// P = z - u (metres).  Same quantities as KK()/inv_dw_dh() of the reference
+// (fpde_hpc_1d.cpp:2527-2575, :2625-2657, cv_form 4, k_form 1), evaluated with
+// three transcendental calls: x=(100 a P)^n, y=(1+x)^(1/n), (x/(1+x))^(1-1/n);
+// then theta-s0 = (s1-s0) y/(1+x), Se^(1/m) = 1/(1+x), C = (n-1)(s1-s0) x y/(P(1+x)^2).
+__host__ __device__ inline void vgm_coefs(double u, double z, double ksat, int mode,
+                                          double& K, double& invC) {
+    if (mode == 1) { K = 1.0; invC = 1.0; return; }
+    double P = z - u;
+    if (P <= 0.0) {               // saturated: wetness = s1, Se = 1, Kr = 1, C = 0
+        K = ksat;
+        invC = 1e20;
+        return;
+    }
+    const double n = kVgmN, m = 1.0 - 1.0 / kVgmN;
+    double x = pow(100.0 * kVgmA * P, n);
+    double opx = 1.0 + x;
+    double y = pow(opx, 1.0 / n);
+    double se = y / opx;
+    double t = 1.0 - pow(x / opx, m);
+    K = ksat * sqrt(se) * t * t;
+    double ch = (n - 1.0) * (kVgmS1 - kVgmS0) * x * y / (P * opx * opx);
+    invC = (ch != 0.0) ? 1.0 / ch : 1e20;
+}
+
+// ---------------------------------------------------------------------------
+// Weight sets on the device for a uniform step (see weights.hip):
+//   local   : coefficient of (U^{n+1}-U^n) in the discrete derivative
+//   omega[j-1], j=1..M : Toeplitz weight of D[n-j] (already x Da/tau, folded over orders)
+//   corr[2n+t], t=0,1  : reference-series correction of the weight of D[t] at step n
+struct DeviceWeights {
+    double local = 0.0;
+    double* omega = nullptr;   // [M]
+    double* corr = nullptr;    // [2*M]
+    int M = 0;
+};
+
+// history storage modes
+enum HistMode : int {
+    HIST_PLAIN_F64 = 0,   // FP64 streaming sum over every level, every step
+    HIST_MFMA_F64 = 1,    // time-block GEMM on v_mfma_f64_16x16x4
+    HIST_SPLIT_BF16 = 2,  // time-block GEMM, bf16 hi+lo, fp32 accumulate
+    HIST_SPLIT_F16 = 3,   // time-block GEMM, fp16 hi+lo with per-group scaling
+    HIST_SPLIT_BF16X3 = 4 // time-block GEMM, bf16 hi+mid+lo
+};
+
+constexpr int kGroup = 32;     // history levels per packed group (K of one 16x16x32 MFMA)
+
+
+// Arguments of one 1D step launch (step1d.hip)
+struct Step1DArgs {
+    int N, C;
+    double h, h2inv, ksat, L, utop;
+    int coef_mode;
+    int n;          // step: unknown is U^{n+1}
+    int k;          // Picard iteration (0 first), -1 = finalize only
+    int recover;    // a reduced system from the previous launch is pending
+    int parity;     // reduced-system buffer written by this launch
+    double* U;      // U^n
+    double* D;      // FP64 history [levels][ld]
+    int64_t ld;
+    const double* omega;
+    const double* corr;
+    const double* part;       // far partial sums
+    int nsplit;
+    int64_t part_stride, part_off;
+    int near_lo;              // near terms t in [near_lo, n)
+    double* Sbuf;             // history term of step n, reused by Picard iterations
+    double* xyz;              // [3][ld] chunk factors
+    double* red;              // [2][2*C][4]
+    double* Uit;              // Picard iterate (for the convergence test)
+    double* dmax;             // [2][C] per-chunk max |U^k - U^{k-1}|
+    int* flags;               // [0]: converged flag of the current step
+    double tol;
+};
+
+// Kernel launchers (weights.hip, history.hip, step1d.hip, lod3d.hip)
+fpde_status launch_weights(const fpde_spec& s, DeviceWeights& w, hipStream_t st);
+fpde_status launch_hist_plain(const double* D, int64_t ld, int nodes, int n, int t_hi, const double* omega,
+                              double* part, int* splits_out, hipStream_t st);
+fpde_status launch_ref_sum(double* S, const double* old, int64_t old_size, int64_t os, int64_t nodes,
+                           const double* tau, const double* kb, int sum_alg, double sum_param, int bs,
+                           hipStream_t st);
+fpde_status launch_gemm_f64(const double* D, int64_t ld, int nodes, int n0, int tc, int block,
+                            const double* omega, double* P, int* splits_out, hipStream_t st);
+fpde_status launch_pack_split(int mode, const double* D, int64_t ld, int nodes, int g_lo, int g_hi, void* c0,
+                              void* c1, void* c2, float* scale, hipStream_t st);
+fpde_status launch_gemm_split(int mode, const void* c0, const void* c1, const void* c2, const float* scale,
+                              int64_t ld, int nodes, int n0, int tc, int block, const double* omega, double* P,
+                              int* splits_out, hipStream_t st);
+fpde_status launch_step1d(const Step1DArgs& a, hipStream_t st);
+fpde_status launch_coef3d(const double* U, int n, double h, double ksat, int mode, double* K, double* iC,
+                          hipStream_t st);
+fpde_status launch_lateral3d(double* U, const double* K, const double* iC, const double* H, int n, int axis,
+                             double h2inv, double L, double* Al, double* Bt, hipStream_t st);
+fpde_status launch_zline3d(double* U, double* Uprev, const double* utop, int n, double h, double h2inv,
+                           double ksat, int mode, double L, double* Dlev, hipStream_t st);
+fpde_status launch_hist_combine(double* S, const double* part, int nsplit, int64_t part_stride, int64_t part_off,
+                                const double* D, int64_t ld, int64_t nodes, int n, int near_lo,
+                                const double* omega, const double* corr, hipStream_t st);
+int gemm_max_splits(int64_t nodes);
+
+}  // namespace fpde

@@ -1,0 +1,438 @@
+// History-sum kernels of the Caputo L1 derivative (gfx950).
+//
+// Storage: D[t][i] = U^{t+1}_i - U^t_i, level-major with node index fastest
+// (leading dimension ld, a multiple of 64), so one level is a contiguous row
+// and every wavefront reads 64 consecutive nodes.  The step-n history term
+// is S_n[i] = sum_{t<n} w(n,t) D[t][i] with the Toeplitz weight
+// w(n,t) = omega[n-t-1] (plus the two-level correction added by the step
+// kernel).  Kernels here:
+//   k_hist_plain   : FP64 streaming sum over a t-range, split over t-chunks
+//                    (one partial row per chunk), HBM bound, 8 B per term;
+//   k_ref_sum      : the reference "Sum" kernel semantics on level arrays
+//                    (fpde_hpc_1d.cpp:1247-1284) behind the OpenCL_* facade;
+//   k_gemm_f64     : time-block GEMM  P[ks][r][i] = sum_t omega[n0+r-t-1] D[t][i]
+//                    for the B future steps r of a block, on v_mfma_f64_16x16x4;
+//   k_pack_split / k_gemm_split : the same GEMM on split bf16/fp16 operands
+//                    (hi+lo or hi+mid+lo) with fp32 MFMA accumulation flushed
+//                    to fp64 after every 32-level group.
+#include "fpde_internal.h"
+
+namespace fpde {
+
+// ---------------------------------------------------------------------------
+// Plain FP64 streaming kernel: part[ks*ld + i] = sum_{t in chunk ks, t < t_hi}
+// omega[n-t-1] * D[t*ld + i].  256 threads, 2 nodes per thread (16-B loads).
+__global__ __launch_bounds__(256) void k_hist_plain(const double* __restrict__ D, int64_t ld,
+                                                    int nodes, int n, int t_hi, int chunk,
+                                                    const double* __restrict__ omega,
+                                                    double* __restrict__ part) {
+    int i2 = (blockIdx.x * 256 + threadIdx.x) * 2;
+    if (i2 >= nodes) return;
+    int ks = blockIdx.y;
+    int t0 = ks * chunk;
+    int t1 = min(t_hi, t0 + chunk);
+    const double* __restrict__ w = omega + (n - 1);   // w[-t] = omega[n-t-1]
+    double a0 = 0.0, a1 = 0.0, b0 = 0.0, b1 = 0.0;
+    int t = t0;
+    for (; t + 4 <= t1; t += 4) {
+        double2 d0 = *reinterpret_cast<const double2*>(D + (int64_t)(t + 0) * ld + i2);
+        double2 d1 = *reinterpret_cast<const double2*>(D + (int64_t)(t + 1) * ld + i2);
+        double2 d2 = *reinterpret_cast<const double2*>(D + (int64_t)(t + 2) * ld + i2);
+        double2 d3 = *reinterpret_cast<const double2*>(D + (int64_t)(t + 3) * ld + i2);
+        double w0 = w[-t], w1 = w[-t - 1], w2 = w[-t - 2], w3 = w[-t - 3];
+        a0 = fma(w0, d0.x, a0); a1 = fma(w0, d0.y, a1);
+        b0 = fma(w1, d1.x, b0); b1 = fma(w1, d1.y, b1);
+        a0 = fma(w2, d2.x, a0); a1 = fma(w2, d2.y, a1);
+        b0 = fma(w3, d3.x, b0); b1 = fma(w3, d3.y, b1);
+    }
+    for (; t < t1; t++) {
+        double2 d0 = *reinterpret_cast<const double2*>(D + (int64_t)t * ld + i2);
+        double w0 = w[-t];
+        a0 = fma(w0, d0.x, a0); a1 = fma(w0, d0.y, a1);
+    }
+    double2 r;
+    r.x = a0 + b0;
+    r.y = a1 + b1;
+    *reinterpret_cast<double2*>(part + (int64_t)ks * ld + i2) = r;
+}
+
+// ---------------------------------------------------------------------------
+// The reference Sum kernel, sum_alg 0/1 (fpde_hpc_1d.cpp:1247-1284): newest
+// interval first, chunks of BS levels; sum_alg 1 stops after the first chunk whose
+// weights are all below sum_param in magnitude.  One thread per node.
+__global__ __launch_bounds__(256) void k_ref_sum(double* __restrict__ S, const double* __restrict__ old,
+                                                 int64_t old_size, int64_t os, int64_t nodes,
+                                                 const double* __restrict__ tau,
+                                                 const double* __restrict__ kb, int sum_alg,
+                                                 double sum_param, int bs) {
+    int64_t id = (int64_t)blockIdx.x * 256 + threadIdx.x;
+    if (id >= nodes) return;
+    double time_sum = 0.0;
+    if (old_size >= 2) {
+        for (int64_t t = old_size - 2; t >= 0; t -= bs) {
+            int br = 0, cnt = 0;
+            for (int tt = 0; tt < bs && t - tt >= 0; tt++) {
+                int64_t l = t - tt;
+                double kv = kb[l];
+                double diff = old[(l + 1) * os + id] - old[l * os + id];
+                time_sum += kv * diff / tau[l];
+                if (fabs(kv) < sum_param) br++;
+                cnt++;
+            }
+            if (sum_alg == 1 && br == bs) break;
+            (void)cnt;
+        }
+    }
+    S[id] = time_sum;
+}
+
+// ---------------------------------------------------------------------------
+// MFMA operand types
+typedef double d4_t __attribute__((ext_vector_type(4)));
+typedef float f4_t __attribute__((ext_vector_type(4)));
+typedef __bf16 bf8_t __attribute__((ext_vector_type(8)));
+typedef _Float16 h8_t __attribute__((ext_vector_type(8)));
+
+// FP64 time-block GEMM.  Grid: x = 64-node tiles, y = K splits (t-chunks).
+// Block 256 = 4 waves; wave w owns nodes [16w,16w+16) of the tile and all BR
+// rows (BR/16 MFMA tiles).  v_mfma_f64_16x16x4f64 operand layout:
+// A lane l = A[l%16][l/16], B lane l = B[l/16][l%16], D lane l holds
+// D[4(l/16)+v][l%16], v=0..3.
+template <int BR>
+__global__ __launch_bounds__(256) void k_gemm_f64(const double* __restrict__ D, int64_t ld, int nodes,
+                                                  int n0, int tc, int kchunk,
+                                                  const double* __restrict__ omega,
+                                                  double* __restrict__ P) {
+    constexpr int RT = BR / 16;
+    constexpr int SLAB = 1024;                  // t-values per LDS weight window
+    __shared__ double win[BR + SLAB];
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    const int ks = blockIdx.y;
+    const int tlo = ks * kchunk;
+    const int thi = min(tc, tlo + kchunk);
+    const int node = blockIdx.x * 64 + wave * 16 + (lane & 15);
+    const bool node_ok = node < nodes;
+    const int kk = lane >> 4;          // k index within the 4-deep MFMA
+    d4_t acc[RT];
+#pragma unroll
+    for (int rt = 0; rt < RT; rt++) acc[rt] = (d4_t){0.0, 0.0, 0.0, 0.0};
+    for (int s0 = tlo; s0 < thi; s0 += SLAB) {
+        const int s1 = min(thi, s0 + SLAB);
+        // win[q] = omega[n0 - s1 + q], so omega[n0+r-t-1] = win[r - t - 1 + s1]
+        const int W = BR + (s1 - s0) - 1;
+        __syncthreads();
+        for (int q = threadIdx.x; q < W; q += 256) win[q] = omega[n0 - s1 + q];
+        __syncthreads();
+        for (int t = s0; t < s1; t += 4) {
+            int tt = t + kk;
+            bool ok = tt < s1;
+            double b = (ok && node_ok) ? D[(int64_t)tt * ld + node] : 0.0;
+#pragma unroll
+            for (int rt = 0; rt < RT; rt++) {
+                int r = rt * 16 + (lane & 15);
+                double a = ok ? win[r - tt - 1 + s1] : 0.0;
+                acc[rt] = __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, acc[rt], 0, 0, 0);
+            }
+        }
+    }
+    if (!node_ok) return;
+#pragma unroll
+    for (int rt = 0; rt < RT; rt++) {
+#pragma unroll
+        for (int v = 0; v < 4; v++) {
+            int r = rt * 16 + 4 * (lane >> 4) + v;
+            P[((int64_t)ks * BR + r) * ld + node] = acc[rt][v];
+        }
+    }
+}
+
+// ---------------------------------------------------------------------------
+// Split storage.  Group g holds levels [32g, 32g+32); for node i the 32
+// values of each component are contiguous: comp[(g*ld + i)*32 + e].  That is
+// exactly the B operand of v_mfma_f32_16x16x32_{bf16,f16}: lane l needs
+// B[8(l/16)+e][l%16], e=0..7, i.e. 16 contiguous bytes.
+// fp16 groups carry a power-of-two scale per (group,node): value = scale*(hi+lo).
+template <typename T> struct SplitTraits;
+template <> struct SplitTraits<__bf16> {
+    typedef bf8_t vec;
+    static constexpr bool scaled = false;
+};
+template <> struct SplitTraits<_Float16> {
+    typedef h8_t vec;
+    static constexpr bool scaled = true;
+};
+
+// Pack levels of complete groups [g_lo, g_hi) from the FP64 history.
+// One thread per (group, node).
+template <typename T, int NS>
+__global__ __launch_bounds__(256) void k_pack_split(const double* __restrict__ D, int64_t ld, int nodes,
+                                                    int g_lo, int g_hi, T* __restrict__ c0,
+                                                    T* __restrict__ c1, T* __restrict__ c2,
+                                                    float* __restrict__ scale) {
+    int node = blockIdx.x * 256 + threadIdx.x;
+    int g = g_lo + blockIdx.y;
+    if (node >= ld || g >= g_hi) return;
+    double v[kGroup];
+    double mx = 0.0;
+#pragma unroll
+    for (int e = 0; e < kGroup; e++) {
+        v[e] = (node < nodes) ? D[(int64_t)(g * kGroup + e) * ld + node] : 0.0;
+        mx = fmax(mx, fabs(v[e]));
+    }
+    double sc = 1.0;
+    if (SplitTraits<T>::scaled) {
+        // scaled magnitudes in [2^14, 2^15): hi and lo stay normal fp16 for
+        // entries within 2^-10 of the group maximum
+        if (mx > 0.0) sc = ldexp(1.0, ilogb(mx) - 14);
+        scale[(int64_t)g * ld + node] = (float)sc;
+    }
+    double inv = 1.0 / sc;
+    int64_t base = ((int64_t)g * ld + node) * kGroup;
+#pragma unroll
+    for (int e = 0; e < kGroup; e++) {
+        double x = v[e] * inv;
+        T h = (T)(float)x;
+        double r = x - (double)(float)h;
+        T m = (T)(float)r;
+        c0[base + e] = h;
+        c1[base + e] = m;
+        if (NS == 3) {
+            double r2 = r - (double)(float)m;
+            c2[base + e] = (T)(float)r2;
+        }
+    }
+}
+
+// Split GEMM.  Grid: x = 64-node tiles, y = K splits.  Per 32-level group:
+// wave w (nodes 16w..16w+15) loads its B fragments (one 16-B load per
+// component), the A fragments (Toeplitz weights, split the same way) come
+// from LDS where the block builds them for 4 groups at a time, and the
+// products are accumulated in fp32 by magnitude class, then flushed to fp64.
+template <typename T, int NS, int BR>
+__global__ __launch_bounds__(256) void k_gemm_split(const T* __restrict__ c0, const T* __restrict__ c1,
+                                                    const T* __restrict__ c2,
+                                                    const float* __restrict__ scale, int64_t ld,
+                                                    int nodes, int n0, int tc, int kchunk,
+                                                    const double* __restrict__ omega,
+                                                    double* __restrict__ P) {
+    typedef typename SplitTraits<T>::vec vec_t;
+    constexpr int RT = BR / 16;
+    constexpr int GB = 4;                                  // groups per LDS batch
+    // afrag[comp][gb][rt][lane] : 8 elements each
+    __shared__ vec_t afrag[NS][GB][RT][64];
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    const int ks = blockIdx.y;
+    const int glo = (ks * kchunk) / kGroup;
+    const int ghi = min(tc, ks * kchunk + kchunk) / kGroup;
+    const int node = blockIdx.x * 64 + wave * 16 + (lane & 15);
+    const bool node_ok = node < nodes;
+    d4_t acc[RT];
+#pragma unroll
+    for (int rt = 0; rt < RT; rt++) acc[rt] = (d4_t){0.0, 0.0, 0.0, 0.0};
+
+    for (int gb0 = glo; gb0 < ghi; gb0 += GB) {
+        __syncthreads();
+        // build A fragments: entries (gb, rt, lane) -> 8 weights
+        for (int f = threadIdx.x; f < GB * RT * 64; f += 256) {
+            int fl = f & 63, frt = (f >> 6) % RT, fgb = f / (64 * RT);
+            int g = gb0 + fgb;
+            int r = frt * 16 + (fl & 15);
+            int tbase = g * kGroup + 8 * (fl >> 4);
+            vec_t h, m, l;
+#pragma unroll
+            for (int e = 0; e < 8; e++) {
+                double wv = 0.0;
+                int t = tbase + e;
+                if (g < ghi) wv = omega[n0 + r - t - 1];
+                T hh = (T)(float)wv;
+                double rr = wv - (double)(float)hh;
+                T mm = (T)(float)rr;
+                h[e] = hh;
+                m[e] = mm;
+                if (NS == 3) l[e] = (T)(float)(rr - (double)(float)mm);
+            }
+            afrag[0][fgb][frt][fl] = h;
+            afrag[1][fgb][frt][fl] = m;
+            if (NS == 3) afrag[NS - 1][fgb][frt][fl] = l;
+        }
+        __syncthreads();
+        for (int gi = 0; gi < GB; gi++) {
+            int g = gb0 + gi;
+            if (g >= ghi) break;
+            int64_t off = ((int64_t)g * ld + (node_ok ? node : 0)) * kGroup + 8 * (lane >> 4);
+            vec_t bh = *reinterpret_cast<const vec_t*>(c0 + off);
+            vec_t bm = *reinterpret_cast<const vec_t*>(c1 + off);
+            vec_t bl;
+            if (NS == 3) bl = *reinterpret_cast<const vec_t*>(c2 + off);
+            double sc = 1.0;
+            if (SplitTraits<T>::scaled) sc = node_ok ? (double)scale[(int64_t)g * ld + node] : 0.0;
+#pragma unroll
+            for (int rt = 0; rt < RT; rt++) {
+                vec_t ah = afrag[0][gi][rt][lane];
+                vec_t am = afrag[1][gi][rt][lane];
+                f4_t z = {0.f, 0.f, 0.f, 0.f};
+                f4_t p0, p1, p2;
+                if constexpr (sizeof(T) == 2 && SplitTraits<T>::scaled) {
+                    p0 = __builtin_amdgcn_mfma_f32_16x16x32_f16(ah, bh, z, 0, 0, 0);
+                    p1 = __builtin_amdgcn_mfma_f32_16x16x32_f16(ah, bm, z, 0, 0, 0);
+                    p1 = __builtin_amdgcn_mfma_f32_16x16x32_f16(am, bh, p1, 0, 0, 0);
+                    if (NS == 3) {
+                        vec_t al = afrag[NS - 1][gi][rt][lane];
+                        p2 = __builtin_amdgcn_mfma_f32_16x16x32_f16(ah, bl, z, 0, 0, 0);
+                        p2 = __builtin_amdgcn_mfma_f32_16x16x32_f16(am, bm, p2, 0, 0, 0);
+                        p2 = __builtin_amdgcn_mfma_f32_16x16x32_f16(al, bh, p2, 0, 0, 0);
+                    }
+                } else {
+                    p0 = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ah, bh, z, 0, 0, 0);
+                    p1 = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ah, bm, z, 0, 0, 0);
+                    p1 = __builtin_amdgcn_mfma_f32_16x16x32_bf16(am, bh, p1, 0, 0, 0);
+                    if (NS == 3) {
+                        vec_t al = afrag[NS - 1][gi][rt][lane];
+                        p2 = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ah, bl, z, 0, 0, 0);
+                        p2 = __builtin_amdgcn_mfma_f32_16x16x32_bf16(am, bm, p2, 0, 0, 0);
+                        p2 = __builtin_amdgcn_mfma_f32_16x16x32_bf16(al, bh, p2, 0, 0, 0);
+                    }
+                }
+#pragma unroll
+                for (int v = 0; v < 4; v++) {
+                    double s = (double)p0[v] + (double)p1[v];
+                    if (NS == 3) s += (double)p2[v];
+                    acc[rt][v] = fma(sc, s, acc[rt][v]);
+                }
+            }
+        }
+    }
+    if (!node_ok) return;
+#pragma unroll
+    for (int rt = 0; rt < RT; rt++) {
+#pragma unroll
+        for (int v = 0; v < 4; v++) {
+            int r = rt * 16 + 4 * (lane >> 4) + v;
+            P[((int64_t)ks * BR + r) * ld + node] = acc[rt][v];
+        }
+    }
+}
+
+// ---------------------------------------------------------------------------
+// Host launchers
+fpde_status launch_hist_plain(const double* D, int64_t ld, int nodes, int n, int t_hi,
+                              const double* omega, double* part, int* splits_out,
+                              hipStream_t st) {
+    if (t_hi <= 0) { *splits_out = 0; return FPDE_OK; }
+    int chunk = (t_hi + 63) / 64;
+    if (chunk < 32) chunk = 32;
+    int splits = (t_hi + chunk - 1) / chunk;
+    dim3 grid((nodes + 511) / 512, splits);
+    hipLaunchKernelGGL(k_hist_plain, grid, dim3(256), 0, st, D, ld, nodes, n, t_hi, chunk, omega, part);
+    *splits_out = splits;
+    return FPDE_OK;
+}
+
+fpde_status launch_ref_sum(double* S, const double* old, int64_t old_size, int64_t os,
+                           int64_t nodes, const double* tau, const double* kb, int sum_alg,
+                           double sum_param, int bs, hipStream_t st) {
+    if (nodes <= 0) return FPDE_OK;
+    dim3 grid((unsigned)((nodes + 255) / 256));
+    if (bs < 1) bs = 1;
+    hipLaunchKernelGGL(k_ref_sum, grid, dim3(256), 0, st, S, old, old_size, os, nodes, tau, kb,
+                       sum_alg, sum_param, bs);
+    return FPDE_OK;
+}
+
+// K splits for a GEMM over t < tc: enough workgroups to cover the chip
+// (>= ~2048 with the node tiles), at most 16 partial rows per output.
+int gemm_max_splits(int64_t nodes) {
+    int64_t tiles = (nodes + 63) / 64;
+    int64_t s = (2048 + tiles - 1) / tiles;
+    return (int)(s < 1 ? 1 : (s > 16 ? 16 : s));
+}
+
+static void gemm_split_plan(int tc, int64_t nodes, int* kchunk, int* splits) {
+    int ks = gemm_max_splits(nodes);
+    int ch = (tc + ks - 1) / ks;
+    ch = ((ch + 31) / 32) * 32;
+    if (ch < 32) ch = 32;
+    *kchunk = ch;
+    *splits = (tc + ch - 1) / ch;
+}
+
+fpde_status launch_gemm_f64(const double* D, int64_t ld, int nodes, int n0, int tc, int block,
+                            const double* omega, double* P, int* splits_out, hipStream_t st) {
+    if (tc <= 0) { *splits_out = 0; return FPDE_OK; }
+    int kchunk, splits;
+    gemm_split_plan(tc, nodes, &kchunk, &splits);
+    dim3 grid((nodes + 63) / 64, splits);
+    size_t lds = 0;
+    switch (block) {
+        case 16: hipLaunchKernelGGL(k_gemm_f64<16>, grid, dim3(256), lds, st, D, ld, nodes, n0, tc, kchunk, omega, P); break;
+        case 32: hipLaunchKernelGGL(k_gemm_f64<32>, grid, dim3(256), lds, st, D, ld, nodes, n0, tc, kchunk, omega, P); break;
+        case 64: hipLaunchKernelGGL(k_gemm_f64<64>, grid, dim3(256), lds, st, D, ld, nodes, n0, tc, kchunk, omega, P); break;
+        case 128: hipLaunchKernelGGL(k_gemm_f64<128>, grid, dim3(256), lds, st, D, ld, nodes, n0, tc, kchunk, omega, P); break;
+        default: set_error("block must be 16, 32, 64 or 128"); return FPDE_ERR_ARG;
+    }
+    *splits_out = splits;
+    return FPDE_OK;
+}
+
+template <typename T, int NS>
+static fpde_status pack_t(const double* D, int64_t ld, int nodes, int g_lo, int g_hi, void* c0,
+                          void* c1, void* c2, float* scale, hipStream_t st) {
+    dim3 grid((unsigned)((ld + 255) / 256), g_hi - g_lo);
+    hipLaunchKernelGGL((k_pack_split<T, NS>), grid, dim3(256), 0, st, D, ld, nodes, g_lo, g_hi, (T*)c0,
+                       (T*)c1, (T*)c2, scale);
+    return FPDE_OK;
+}
+
+fpde_status launch_pack_split(int mode, const double* D, int64_t ld, int nodes, int g_lo, int g_hi,
+                              void* c0, void* c1, void* c2, float* scale, hipStream_t st) {
+    if (g_hi <= g_lo) return FPDE_OK;
+    switch (mode) {
+        case HIST_SPLIT_BF16: return pack_t<__bf16, 2>(D, ld, nodes, g_lo, g_hi, c0, c1, c2, scale, st);
+        case HIST_SPLIT_F16: return pack_t<_Float16, 2>(D, ld, nodes, g_lo, g_hi, c0, c1, c2, scale, st);
+        case HIST_SPLIT_BF16X3: return pack_t<__bf16, 3>(D, ld, nodes, g_lo, g_hi, c0, c1, c2, scale, st);
+        default: set_error("bad split mode"); return FPDE_ERR_ARG;
+    }
+}
+
+template <typename T, int NS, int BR>
+static void gemm_split_t(dim3 grid, hipStream_t st, const void* c0, const void* c1, const void* c2,
+                         const float* scale, int64_t ld, int nodes, int n0, int tc, int kchunk,
+                         const double* omega, double* P) {
+    hipLaunchKernelGGL((k_gemm_split<T, NS, BR>), grid, dim3(256), 0, st, (const T*)c0, (const T*)c1,
+                       (const T*)c2, scale, ld, nodes, n0, tc, kchunk, omega, P);
+}
+
+template <typename T, int NS>
+static fpde_status gemm_split_b(int block, dim3 grid, hipStream_t st, const void* c0, const void* c1,
+                                const void* c2, const float* scale, int64_t ld, int nodes, int n0,
+                                int tc, int kchunk, const double* omega, double* P) {
+    switch (block) {
+        case 16: gemm_split_t<T, NS, 16>(grid, st, c0, c1, c2, scale, ld, nodes, n0, tc, kchunk, omega, P); break;
+        case 32: gemm_split_t<T, NS, 32>(grid, st, c0, c1, c2, scale, ld, nodes, n0, tc, kchunk, omega, P); break;
+        case 64: gemm_split_t<T, NS, 64>(grid, st, c0, c1, c2, scale, ld, nodes, n0, tc, kchunk, omega, P); break;
+        case 128: gemm_split_t<T, NS, 128>(grid, st, c0, c1, c2, scale, ld, nodes, n0, tc, kchunk, omega, P); break;
+        default: set_error("block must be 16, 32, 64 or 128"); return FPDE_ERR_ARG;
+    }
+    return FPDE_OK;
+}
+
+fpde_status launch_gemm_split(int mode, const void* c0, const void* c1, const void* c2,
+                              const float* scale, int64_t ld, int nodes, int n0, int tc, int block,
+                              const double* omega, double* P, int* splits_out, hipStream_t st) {
+    if (tc <= 0) { *splits_out = 0; return FPDE_OK; }
+    if (tc % kGroup != 0) { set_error("split GEMM needs a group-aligned cut"); return FPDE_ERR_ARG; }
+    int kchunk, splits;
+    gemm_split_plan(tc, nodes, &kchunk, &splits);
+    dim3 grid((nodes + 63) / 64, splits);
+    fpde_status rc;
+    switch (mode) {
+        case HIST_SPLIT_BF16: rc = gemm_split_b<__bf16, 2>(block, grid, st, c0, c1, c2, scale, ld, nodes, n0, tc, kchunk, omega, P); break;
+        case HIST_SPLIT_F16: rc = gemm_split_b<_Float16, 2>(block, grid, st, c0, c1, c2, scale, ld, nodes, n0, tc, kchunk, omega, P); break;
+        case HIST_SPLIT_BF16X3: rc = gemm_split_b<__bf16, 3>(block, grid, st, c0, c1, c2, scale, ld, nodes, n0, tc, kchunk, omega, P); break;
+        default: set_error("bad split mode"); return FPDE_ERR_ARG;
+    }
+    *splits_out = splits;
+    return rc;
+}
+
+}  // namespace fpde

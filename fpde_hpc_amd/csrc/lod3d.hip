@@ -1,0 +1,183 @@
+// 3D locally-one-dimensional step (gfx950), fpde_hpc_1d.cpp:5144-5626.
+//
+// Field layout idx = z + n*(x + n*y): depth z fastest (the reference's N axis,
+// idx() at :1181).  One step = three sweeps in the reference order:
+//   y (m3d_c=2, carries the history term), x (m3d_c=1), z (m3d_c=0).
+// Every sweep solves, line by line, L*(U - U_in) = div_axis(K grad U) + H
+// with coefficients evaluated at the field the sweep starts from.
+//  - y and x lines: one thread per line (adjacent threads = adjacent z, so
+//    every load is coalesced) running the reference Thomas recurrences
+//    al1/bt1/U1 (:4002-4027, :4585-4590, :4672-4725) with zero flux at both
+//    ends (Al[1]=1, Bt[1]=0; bcN()=0), coefficients precomputed per node;
+//  - z lines: one workgroup per line, one node per thread, PCR in LDS,
+//    Dirichlet top / zero-flux bottom; it also writes D[n] = U^{n+1}-U^n.
+#include "fpde_internal.h"
+
+namespace fpde {
+
+// K and 1/C of every node at the current field.
+__global__ __launch_bounds__(256) void k_coef3d(const double* __restrict__ U, int n, double h,
+                                                double ksat, int mode, double* __restrict__ K,
+                                                double* __restrict__ iC) {
+    int64_t id = (int64_t)blockIdx.x * 256 + threadIdx.x;
+    int64_t nn = (int64_t)n * n * n;
+    if (id >= nn) return;
+    int z = (int)(id % n);
+    double k, ic;
+    vgm_coefs(U[id], z * h, ksat, mode, k, ic);
+    K[id] = k;
+    iC[id] = ic;
+}
+
+// Lateral sweep, thread per line.  axis 1 = x (stride n), 2 = y (stride n*n).
+// Scratch Al/Bt: [n+1][nlines], coalesced.
+__global__ __launch_bounds__(256) void k_lateral3d(double* __restrict__ U, const double* __restrict__ K,
+                                                   const double* __restrict__ iC,
+                                                   const double* __restrict__ H, int n, int axis,
+                                                   double h2inv, double L, double* __restrict__ Al,
+                                                   double* __restrict__ Bt) {
+    int line = blockIdx.x * 256 + threadIdx.x;
+    int nlines = n * n;
+    if (line >= nlines) return;
+    int z = line % n, o = line / n;
+    int64_t base, stride;
+    if (axis == 2) { base = z + (int64_t)n * o; stride = (int64_t)n * n; }   // (z,x=o), along y
+    else { base = z + (int64_t)n * n * o; stride = n; }                        // (z,y=o), along x
+    const int N = n - 1;
+    // al1 with Al[1]=1 (lateral, :4021), bt1 with Bt[1]=bc0()=0 (:4511)
+    double al = 1.0, bt = 0.0;
+    Al[(int64_t)1 * nlines + line] = al;
+    Bt[(int64_t)1 * nlines + line] = bt;
+    for (int i = 1; i < N; i++) {
+        int64_t id = base + stride * i;
+        double Km = K[id - stride], Kp = K[id], ic = iC[id];
+        double A_ = Km * ic * h2inv;
+        double B_ = Kp * ic * h2inv;
+        double R_ = ic * ((Kp * h2inv) + (Km * h2inv)) + L;
+        double Om = -L * U[id] + (H ? H[id] : 0.0);
+        double aln = B_ / (R_ - A_ * al);
+        bt = (aln / B_) * (A_ * bt - Om);
+        al = aln;
+        Al[(int64_t)(i + 1) * nlines + line] = al;
+        Bt[(int64_t)(i + 1) * nlines + line] = bt;
+    }
+    // zero-flux end, U1 :4704-4720 with bcN()=0
+    double uN;
+    if (fabs(bt) > 1e-15) uN = (fabs(1.0 - al) > 1e-15) ? bt / (1.0 - al) : U[base + stride * (N - 1)];
+    else uN = (fabs(1.0 - al) > 1e-15) ? 0.0 : U[base + stride * (N - 1)];
+    U[base + stride * N] = uN;
+    double u = uN;
+    for (int i = N - 1; i >= 0; i--) {
+        u = Al[(int64_t)(i + 1) * nlines + line] * u + Bt[(int64_t)(i + 1) * nlines + line];
+        U[base + stride * i] = u;
+    }
+}
+
+// Depth sweep: block = one z line (n <= 256 threads).  Writes U^{n+1}, the
+// history level D[t] = U^{n+1} - Uprev and refreshes Uprev.
+__global__ __launch_bounds__(256) void k_zline3d(double* __restrict__ U, double* __restrict__ Uprev,
+                                                 const double* __restrict__ utop, int n, double h,
+                                                 double h2inv, double ksat, int mode, double L,
+                                                 double* __restrict__ Dlev) {
+    __shared__ double a[256], b[256], c[256], d[256], sK[256];
+    const int tid = threadIdx.x;
+    const int line = blockIdx.x;                   // x + n*y
+    const int64_t base = (int64_t)n * line;
+    double u = 0.0, k = 0.0, ic = 0.0;
+    if (tid < n) {
+        u = U[base + tid];
+        vgm_coefs(u, tid * h, ksat, mode, k, ic);
+        sK[tid] = k;
+    }
+    __syncthreads();
+    if (tid < n) {
+        double ar, br, cr, dr;
+        if (tid == 0) { ar = 0.0; br = 1.0; cr = 0.0; dr = utop[line]; }
+        else if (tid == n - 1) { ar = -1.0; br = 1.0; cr = 0.0; dr = 0.0; }
+        else {
+            double Km = sK[tid - 1], Kp = k;
+            ar = Km * ic * h2inv;
+            cr = Kp * ic * h2inv;
+            br = -(ic * ((Kp * h2inv) + (Km * h2inv)) + L);
+            dr = -L * u;
+        }
+        a[tid] = ar; b[tid] = br; c[tid] = cr; d[tid] = dr;
+    }
+    __syncthreads();
+    for (int delta = 1; delta < n; delta <<= 1) {
+        double na = 0, nb = 1, nc = 0, nd = 0;
+        if (tid < n) {
+            double k1 = 0, k2 = 0, am = 0, cm = 0, dm = 0, ap = 0, cp = 0, dp = 0;
+            if (tid - delta >= 0) { k1 = a[tid] / b[tid - delta]; am = a[tid - delta]; cm = c[tid - delta]; dm = d[tid - delta]; }
+            if (tid + delta < n) { k2 = c[tid] / b[tid + delta]; ap = a[tid + delta]; cp = c[tid + delta]; dp = d[tid + delta]; }
+            na = -k1 * am;
+            nc = -k2 * cp;
+            nb = b[tid] - k1 * cm - k2 * ap;
+            nd = d[tid] - k1 * dm - k2 * dp;
+        }
+        __syncthreads();
+        if (tid < n) { a[tid] = na; b[tid] = nb; c[tid] = nc; d[tid] = nd; }
+        __syncthreads();
+    }
+    if (tid < n) {
+        double un = d[tid] / b[tid];
+        int64_t id = base + tid;
+        U[id] = un;
+        Dlev[id] = un - Uprev[id];
+        Uprev[id] = un;
+    }
+}
+
+// S[i] = sum of far partials + near levels [near_lo, n) + two-level corrections.
+__global__ __launch_bounds__(256) void k_hist_combine(double* __restrict__ S, const double* __restrict__ part,
+                                                      int nsplit, int64_t part_stride, int64_t part_off,
+                                                      const double* __restrict__ D, int64_t ld, int64_t nodes,
+                                                      int n, int near_lo, const double* __restrict__ omega,
+                                                      const double* __restrict__ corr) {
+    int64_t i = (int64_t)blockIdx.x * 256 + threadIdx.x;
+    if (i >= nodes) return;
+    double far = 0.0;
+    for (int sp = 0; sp < nsplit; sp++) far += part[sp * part_stride + part_off + i];
+    double nr0 = 0.0, nr1 = 0.0;
+    int t = near_lo;
+    for (; t + 2 <= n; t += 2) {
+        nr0 = fma(omega[n - t - 1], D[(int64_t)t * ld + i], nr0);
+        nr1 = fma(omega[n - t - 2], D[(int64_t)(t + 1) * ld + i], nr1);
+    }
+    for (; t < n; t++) nr0 = fma(omega[n - t - 1], D[(int64_t)t * ld + i], nr0);
+    double cr = 0.0;
+    if (n >= 1) cr = corr[2 * n] * D[i];
+    if (n >= 2) cr = fma(corr[2 * n + 1], D[ld + i], cr);
+    S[i] = far + (nr0 + nr1) + cr;
+}
+
+fpde_status launch_coef3d(const double* U, int n, double h, double ksat, int mode, double* K,
+                          double* iC, hipStream_t st) {
+    int64_t nn = (int64_t)n * n * n;
+    hipLaunchKernelGGL(k_coef3d, dim3((unsigned)((nn + 255) / 256)), dim3(256), 0, st, U, n, h, ksat, mode, K, iC);
+    return FPDE_OK;
+}
+
+fpde_status launch_lateral3d(double* U, const double* K, const double* iC, const double* H, int n,
+                             int axis, double h2inv, double L, double* Al, double* Bt, hipStream_t st) {
+    int nl = n * n;
+    hipLaunchKernelGGL(k_lateral3d, dim3((nl + 255) / 256), dim3(256), 0, st, U, K, iC, H, n, axis, h2inv, L, Al, Bt);
+    return FPDE_OK;
+}
+
+fpde_status launch_zline3d(double* U, double* Uprev, const double* utop, int n, double h, double h2inv,
+                           double ksat, int mode, double L, double* Dlev, hipStream_t st) {
+    if (n > 256) { set_error("3D grid: at most 256 nodes per axis"); return FPDE_ERR_ARG; }
+    hipLaunchKernelGGL(k_zline3d, dim3(n * n), dim3(256), 0, st, U, Uprev, utop, n, h, h2inv, ksat, mode, L, Dlev);
+    return FPDE_OK;
+}
+
+fpde_status launch_hist_combine(double* S, const double* part, int nsplit, int64_t part_stride,
+                                int64_t part_off, const double* D, int64_t ld, int64_t nodes, int n,
+                                int near_lo, const double* omega, const double* corr, hipStream_t st) {
+    hipLaunchKernelGGL(k_hist_combine, dim3((unsigned)((nodes + 255) / 256)), dim3(256), 0, st, S, part,
+                       nsplit, part_stride, part_off, D, ld, nodes, n, near_lo, omega, corr);
+    return FPDE_OK;
+}
+
+}  // namespace fpde

@@ -1,0 +1,549 @@
+// Host side of the solver: device state of one solve, the per-step launch
+// plan (history kernels + step kernels), optional hipGraph capture of the
+// step loop, profiling, and the C ABI (include/fpde_hip.h, group 3).
+//
+// The step loop mirrors H_solver::calc_step (fpde_hpc_1d.cpp:5100-5803):
+// init_time_step (weights; here generated once, on the device) -> history sum
+// (time_summing / fr_ocl_call_sum) -> tridiagonal step(s) -> push the new level.
+#include "fpde_internal.h"
+
+#include <algorithm>
+#include <chrono>
+#include <cmath>
+#include <cstring>
+#include <memory>
+#include <random>
+#include <vector>
+
+using namespace fpde;
+
+namespace {
+
+thread_local std::string g_err;
+
+int64_t round_up(int64_t v, int64_t m) { return (v + m - 1) / m * m; }
+
+bool is_split(int mode) { return mode == HIST_SPLIT_BF16 || mode == HIST_SPLIT_F16 || mode == HIST_SPLIT_BF16X3; }
+int n_comp(int mode) { return mode == HIST_SPLIT_BF16X3 ? 3 : 2; }
+
+struct Plan {
+    int64_t nodes = 0;     // unknowns per level
+    int64_t ld = 0;        // padded level stride
+    int64_t levels = 0;    // history levels kept
+    int part_rows = 0;     // rows of the partial-sum buffer
+};
+
+fpde_status validate(const fpde_spec& s) {
+    if (s.dims != 1 && s.dims != 3) { set_error("dims must be 1 or 3"); return FPDE_ERR_ARG; }
+    if (s.steps < 1) { set_error("steps must be >= 1"); return FPDE_ERR_ARG; }
+    if (s.dims == 1 && (s.nodes < 4 || s.nodes > 65536)) { set_error("1D nodes must be in [4, 65536]"); return FPDE_ERR_ARG; }
+    if (s.dims == 3 && (s.nodes < 4 || s.nodes > 256)) { set_error("3D nodes per axis must be in [4, 256]"); return FPDE_ERR_ARG; }
+    if (!(s.tau > 0.0) || !(s.length > 0.0)) { set_error("tau and length must be positive"); return FPDE_ERR_ARG; }
+    if (s.do_q <= 0 && !(s.alpha > 0.0 && s.alpha <= 1.0)) { set_error("alpha must be in (0,1]"); return FPDE_ERR_ARG; }
+    if (s.picard_max < 1 || s.picard_max > 16) { set_error("picard_max must be in [1,16]"); return FPDE_ERR_ARG; }
+    if (s.hist_mode < 0 || s.hist_mode > 4) { set_error("hist_mode must be in [0,4]"); return FPDE_ERR_ARG; }
+    if (s.hist_mode != HIST_PLAIN_F64) {
+        int b = s.block;
+        if (b != 16 && b != 32 && b != 64 && b != 128) { set_error("block must be 16, 32, 64 or 128"); return FPDE_ERR_ARG; }
+    }
+    if (s.dims == 3 && s.picard_max != 1) { set_error("3D LOD uses lagged coefficients (picard_max must be 1)"); return FPDE_ERR_ARG; }
+    return FPDE_OK;
+}
+
+Plan make_plan(const fpde_spec& s) {
+    Plan p;
+    p.nodes = (s.dims == 1) ? s.nodes : (int64_t)s.nodes * s.nodes * s.nodes;
+    p.ld = round_up(p.nodes, 64);
+    p.levels = round_up(s.steps + 1, kGroup);
+    int rows_plain = 64;
+    int rows_gemm = (s.hist_mode == HIST_PLAIN_F64) ? 0 : gemm_max_splits(p.nodes) * s.block;
+    p.part_rows = std::max(rows_plain, rows_gemm);
+    return p;
+}
+
+size_t footprint_bytes(const fpde_spec& s) {
+    Plan p = make_plan(s);
+    size_t b = 0;
+    b += sizeof(double) * (size_t)p.levels * p.ld;               // FP64 history
+    if (is_split(s.hist_mode))
+        b += (size_t)n_comp(s.hist_mode) * 2 * p.levels * p.ld + sizeof(float) * (p.levels / kGroup) * p.ld;
+    b += sizeof(double) * (size_t)p.part_rows * p.ld;            // partial sums
+    b += sizeof(double) * (size_t)p.ld * 8;                      // U, S, xyz, ...
+    if (s.dims == 3) b += sizeof(double) * (size_t)p.ld * 6;     // Uprev, K, iC, Al, Bt
+    b += sizeof(double) * (size_t)(s.steps + 512) * 3;           // weights
+    return b;
+}
+
+}  // namespace
+
+namespace fpde {
+void set_error(const std::string& msg) { g_err = msg; }
+}  // namespace fpde
+
+struct fpde_solver {
+    fpde_spec spec{};
+    Plan plan;
+    hipStream_t stream = nullptr;
+    DeviceWeights w;
+    double* U = nullptr;
+    double* D = nullptr;
+    double* S = nullptr;
+    double* xyz = nullptr;
+    double* red = nullptr;
+    double* part = nullptr;
+    double* Uit = nullptr;
+    double* dmax = nullptr;
+    int* flags = nullptr;
+    void* comp[3] = {nullptr, nullptr, nullptr};
+    float* scale = nullptr;
+    // 3D
+    double* Uprev = nullptr;
+    double* utop = nullptr;
+    double* K = nullptr;
+    double* iC = nullptr;
+    double* Al = nullptr;
+    double* Bt = nullptr;
+    double h = 0.0, utop1d = 0.0;
+    int C = 0;
+    // progress
+    int step = 0;
+    int64_t launches = 0;
+    bool pending = false;
+    bool initialized = false;
+    // GEMM block state
+    int n0 = -1, tc = 0, gsplits = 0, packed_groups = 0;
+    // profiling
+    bool prof = false;
+    struct EvPair { hipEvent_t a, b; int cls; };
+    std::vector<EvPair> evs;
+    size_t ev_used = 0;
+    fpde_stats stats{};
+    std::vector<void*> allocs;
+};
+
+namespace {
+
+fpde_status dalloc(fpde_solver* s, void** p, size_t bytes) {
+    if (bytes == 0) bytes = 8;
+    hipError_t e = hipMalloc(p, bytes);
+    if (e != hipSuccess) {
+        set_error(std::string("hipMalloc failed: ") + hipGetErrorString(e));
+        *p = nullptr;
+        return FPDE_ERR_NOMEM;
+    }
+    s->allocs.push_back(*p);
+    s->stats.device_bytes += bytes;
+    return FPDE_OK;
+}
+#define DALLOC(ptr, bytes)                                              \
+    do {                                                                \
+        fpde_status rc_ = dalloc(s, (void**)&(ptr), (bytes));           \
+        if (rc_ != FPDE_OK) return rc_;                                 \
+    } while (0)
+
+// profiling event pairs: cls 0 = history kernels, 1 = step kernels
+void ev_begin(fpde_solver* s, int cls, size_t* slot) {
+    if (!s->prof) return;
+    if (s->ev_used == s->evs.size()) {
+        fpde_solver::EvPair p;
+        hipEventCreate(&p.a);
+        hipEventCreate(&p.b);
+        s->evs.push_back(p);
+    }
+    *slot = s->ev_used++;
+    s->evs[*slot].cls = cls;
+    hipEventRecord(s->evs[*slot].a, s->stream);
+}
+void ev_end(fpde_solver* s, size_t slot) {
+    if (!s->prof) return;
+    hipEventRecord(s->evs[slot].b, s->stream);
+}
+void ev_collect(fpde_solver* s) {
+    if (!s->prof) return;
+    for (size_t i = 0; i < s->ev_used; i++) {
+        float ms = 0.f;
+        hipEventElapsedTime(&ms, s->evs[i].a, s->evs[i].b);
+        if (s->evs[i].cls == 0) s->stats.ms_history += ms;
+        else s->stats.ms_step += ms;
+    }
+    s->ev_used = 0;
+}
+
+fpde_status solver_alloc(fpde_solver* s) {
+    const fpde_spec& sp = s->spec;
+    const Plan& p = s->plan;
+    FPDE_HIP_TRY(hipStreamCreateWithFlags(&s->stream, hipStreamNonBlocking));
+    int M = sp.steps;
+    s->w.M = M;
+    DALLOC(s->w.omega, sizeof(double) * (size_t)(M + 512));
+    DALLOC(s->w.corr, sizeof(double) * 2 * (size_t)(M + 1));
+    FPDE_HIP_TRY(hipMemsetAsync(s->w.omega, 0, sizeof(double) * (size_t)(M + 512), s->stream));
+    DALLOC(s->U, sizeof(double) * p.ld);
+    DALLOC(s->D, sizeof(double) * (size_t)p.levels * p.ld);
+    FPDE_HIP_TRY(hipMemsetAsync(s->D, 0, sizeof(double) * (size_t)p.levels * p.ld, s->stream));
+    DALLOC(s->S, sizeof(double) * p.ld);
+    DALLOC(s->part, sizeof(double) * (size_t)p.part_rows * p.ld);
+    if (is_split(sp.hist_mode)) {
+        for (int c = 0; c < n_comp(sp.hist_mode); c++) {
+            DALLOC(s->comp[c], 2 * (size_t)p.levels * p.ld);
+        }
+        DALLOC(s->scale, sizeof(float) * (size_t)(p.levels / kGroup) * p.ld);
+    }
+    if (sp.dims == 1) {
+        s->C = (int)((sp.nodes + 255) / 256);
+        DALLOC(s->xyz, sizeof(double) * 3 * p.ld);
+        DALLOC(s->red, sizeof(double) * 2 * 2 * s->C * 4);
+        DALLOC(s->Uit, sizeof(double) * p.ld);
+        DALLOC(s->dmax, sizeof(double) * 2 * s->C);
+        DALLOC(s->flags, sizeof(int) * 16);
+    } else {
+        int n = sp.nodes;
+        DALLOC(s->Uprev, sizeof(double) * p.ld);
+        DALLOC(s->utop, sizeof(double) * (size_t)n * n);
+        DALLOC(s->K, sizeof(double) * p.ld);
+        DALLOC(s->iC, sizeof(double) * p.ld);
+        DALLOC(s->Al, sizeof(double) * (size_t)(n + 1) * n * n);
+        DALLOC(s->Bt, sizeof(double) * (size_t)(n + 1) * n * n);
+    }
+    s->h = sp.length / (sp.nodes - 1);
+    fpde_status rc = launch_weights(sp, s->w, s->stream);
+    if (rc != FPDE_OK) return rc;
+    FPDE_HIP_TRY(hipStreamSynchronize(s->stream));
+    return FPDE_OK;
+}
+
+// far part of the history for step n (GEMM modes: start a new block when due)
+fpde_status far_part(fpde_solver* s, int n, int* nsplit, int64_t* stride, int64_t* off, int* near_lo) {
+    const fpde_spec& sp = s->spec;
+    const Plan& p = s->plan;
+    // levels available in memory: t < avail
+    int avail = s->pending ? n - 1 : n;
+    if (sp.hist_mode == HIST_PLAIN_F64) {
+        int ns = 0;
+        size_t slot = 0;
+        if (avail > 0) {
+            ev_begin(s, 0, &slot);
+            fpde_status rc = launch_hist_plain(s->D, p.ld, (int)p.nodes, n, avail, s->w.omega, s->part, &ns, s->stream);
+            ev_end(s, slot);
+            if (rc != FPDE_OK) return rc;
+            s->stats.history_launches++;
+            s->stats.history_alg_bytes += 8.0 * avail * (double)p.nodes;
+            s->stats.history_alg_flops += 2.0 * avail * (double)p.nodes;
+        }
+        *nsplit = ns;
+        *stride = p.ld;
+        *off = 0;
+        *near_lo = std::max(avail, 0);
+        return FPDE_OK;
+    }
+    int B = sp.block;
+    if (s->n0 < 0 || n - s->n0 >= B) {
+        s->n0 = n;
+        int tcm = std::max(avail, 0);
+        int tc = is_split(sp.hist_mode) ? (tcm / kGroup) * kGroup : tcm;
+        s->tc = tc;
+        size_t slot = 0;
+        ev_begin(s, 0, &slot);
+        fpde_status rc = FPDE_OK;
+        if (is_split(sp.hist_mode)) {
+            int ghi = tc / kGroup;
+            if (ghi > s->packed_groups) {
+                rc = launch_pack_split(sp.hist_mode, s->D, p.ld, (int)p.nodes, s->packed_groups, ghi, s->comp[0],
+                                       s->comp[1], s->comp[2], s->scale, s->stream);
+                s->packed_groups = ghi;
+            }
+            if (rc == FPDE_OK)
+                rc = launch_gemm_split(sp.hist_mode, s->comp[0], s->comp[1], s->comp[2], s->scale, p.ld, (int)p.nodes,
+                                       n, tc, B, s->w.omega, s->part, &s->gsplits, s->stream);
+        } else {
+            rc = launch_gemm_f64(s->D, p.ld, (int)p.nodes, n, tc, B, s->w.omega, s->part, &s->gsplits, s->stream);
+        }
+        ev_end(s, slot);
+        if (rc != FPDE_OK) return rc;
+        if (tc > 0) {
+            s->stats.history_launches++;
+            s->stats.history_alg_bytes += 8.0 * tc * (double)p.nodes * B;
+            s->stats.history_alg_flops += 2.0 * tc * (double)p.nodes * B;
+        }
+    }
+    *nsplit = s->gsplits;
+    *stride = (int64_t)B * p.ld;
+    *off = (int64_t)(n - s->n0) * p.ld;
+    *near_lo = s->tc;
+    return FPDE_OK;
+}
+
+fpde_status launch_1d(fpde_solver* s, int n, int k, int nsplit, int64_t stride, int64_t off, int near_lo) {
+    const fpde_spec& sp = s->spec;
+    Step1DArgs a{};
+    a.N = sp.nodes;
+    a.C = s->C;
+    a.h = s->h;
+    a.h2inv = 1.0 / (s->h * s->h);
+    a.ksat = sp.ksat;
+    a.L = s->w.local;
+    a.utop = s->utop1d;
+    a.coef_mode = sp.coef_mode;
+    a.n = n;
+    a.k = k;
+    a.recover = s->pending ? 1 : 0;
+    a.parity = (int)(s->launches & 1);
+    a.U = s->U;
+    a.D = s->D;
+    a.ld = s->plan.ld;
+    a.omega = s->w.omega;
+    a.corr = s->w.corr;
+    a.part = s->part;
+    a.nsplit = nsplit;
+    a.part_stride = stride;
+    a.part_off = off;
+    a.near_lo = near_lo;
+    a.Sbuf = s->S;
+    a.xyz = s->xyz;
+    a.red = s->red;
+    a.Uit = s->Uit;
+    a.dmax = s->dmax;
+    a.flags = s->flags;
+    a.tol = sp.picard_tol;
+    size_t slot = 0;
+    ev_begin(s, 1, &slot);
+    fpde_status rc = launch_step1d(a, s->stream);
+    ev_end(s, slot);
+    if (rc != FPDE_OK) return rc;
+    s->launches++;
+    s->stats.step_launches++;
+    s->pending = (k >= 0);
+    return FPDE_OK;
+}
+
+fpde_status enqueue_1d_step(fpde_solver* s, int n) {
+    int nsplit = 0, near_lo = 0;
+    int64_t stride = 0, off = 0;
+    fpde_status rc = far_part(s, n, &nsplit, &stride, &off, &near_lo);
+    if (rc != FPDE_OK) return rc;
+    for (int k = 0; k < s->spec.picard_max; k++) {
+        rc = launch_1d(s, n, k, nsplit, stride, off, near_lo);
+        if (rc != FPDE_OK) return rc;
+        s->stats.picard_iters_total++;
+    }
+    return FPDE_OK;
+}
+
+fpde_status finalize_1d(fpde_solver* s) {
+    if (!s->pending) return FPDE_OK;
+    return launch_1d(s, s->step, -1, 0, 0, 0, s->step);
+}
+
+fpde_status enqueue_3d_step(fpde_solver* s, int n) {
+    const fpde_spec& sp = s->spec;
+    const Plan& p = s->plan;
+    int nsplit = 0, near_lo = 0;
+    int64_t stride = 0, off = 0;
+    fpde_status rc = far_part(s, n, &nsplit, &stride, &off, &near_lo);
+    if (rc != FPDE_OK) return rc;
+    int nn = sp.nodes;
+    double h2inv = 1.0 / (s->h * s->h);
+    size_t slot = 0;
+    ev_begin(s, 0, &slot);
+    rc = launch_hist_combine(s->S, s->part, nsplit, stride, off, s->D, p.ld, p.nodes, n, near_lo, s->w.omega,
+                             s->w.corr, s->stream);
+    ev_end(s, slot);
+    if (rc != FPDE_OK) return rc;
+    ev_begin(s, 1, &slot);
+    if ((rc = launch_coef3d(s->U, nn, s->h, sp.ksat, sp.coef_mode, s->K, s->iC, s->stream)) != FPDE_OK) return rc;
+    if ((rc = launch_lateral3d(s->U, s->K, s->iC, s->S, nn, 2, h2inv, s->w.local, s->Al, s->Bt, s->stream)) != FPDE_OK) return rc;
+    if ((rc = launch_coef3d(s->U, nn, s->h, sp.ksat, sp.coef_mode, s->K, s->iC, s->stream)) != FPDE_OK) return rc;
+    if ((rc = launch_lateral3d(s->U, s->K, s->iC, nullptr, nn, 1, h2inv, s->w.local, s->Al, s->Bt, s->stream)) != FPDE_OK) return rc;
+    if ((rc = launch_zline3d(s->U, s->Uprev, s->utop, nn, s->h, h2inv, sp.ksat, sp.coef_mode, s->w.local,
+                             s->D + (size_t)n * p.ld, s->stream)) != FPDE_OK) return rc;
+    ev_end(s, slot);
+    s->stats.step_launches += 5;
+    s->stats.picard_iters_total++;
+    return FPDE_OK;
+}
+
+}  // namespace
+
+// ===========================================================================
+// C ABI
+extern "C" {
+
+const char* fpde_last_error(void) { return g_err.c_str(); }
+int fpde_version(void) { return 100; }
+int fpde_device_count(void) {
+    int n = 0;
+    if (hipGetDeviceCount(&n) != hipSuccess) return 0;
+    return n;
+}
+
+size_t fpde_solver_footprint(const fpde_spec* spec) {
+    if (!spec) return 0;
+    return footprint_bytes(*spec);
+}
+
+fpde_status fpde_solver_create(const fpde_spec* spec, fpde_solver** out) {
+    if (!spec || !out) { set_error("null argument"); return FPDE_ERR_ARG; }
+    *out = nullptr;
+    fpde_status rc = validate(*spec);
+    if (rc != FPDE_OK) return rc;
+    auto s = std::make_unique<fpde_solver>();
+    s->spec = *spec;
+    s->plan = make_plan(*spec);
+    rc = solver_alloc(s.get());
+    if (rc != FPDE_OK) {
+        fpde_solver_destroy(s.release());
+        return rc;
+    }
+    *out = s.release();
+    return FPDE_OK;
+}
+
+fpde_status fpde_solver_destroy(fpde_solver* s) {
+    if (!s) return FPDE_OK;
+    if (s->stream) hipStreamSynchronize(s->stream);
+    for (void* p : s->allocs) hipFree(p);
+    for (auto& e : s->evs) { hipEventDestroy(e.a); hipEventDestroy(e.b); }
+    if (s->stream) hipStreamDestroy(s->stream);
+    delete s;
+    return FPDE_OK;
+}
+
+fpde_status fpde_solver_set_initial(fpde_solver* s, const double* u0) {
+    if (!s || !u0) { set_error("null argument"); return FPDE_ERR_ARG; }
+    if (s->step != 0) { set_error("initial state can only be set before the first step"); return FPDE_ERR_STATE; }
+    const Plan& p = s->plan;
+    FPDE_HIP_TRY(hipMemsetAsync(s->U, 0, sizeof(double) * p.ld, s->stream));
+    FPDE_HIP_TRY(hipMemcpyAsync(s->U, u0, sizeof(double) * p.nodes, hipMemcpyHostToDevice, s->stream));
+    if (s->spec.dims == 1) {
+        s->utop1d = u0[0];
+    } else {
+        int n = s->spec.nodes;
+        std::vector<double> top((size_t)n * n);
+        for (int y = 0; y < n; y++)
+            for (int x = 0; x < n; x++) top[(size_t)y * n + x] = u0[(size_t)n * (x + (size_t)n * y)];
+        FPDE_HIP_TRY(hipMemcpyAsync(s->Uprev, s->U, sizeof(double) * p.ld, hipMemcpyDeviceToDevice, s->stream));
+        FPDE_HIP_TRY(hipMemcpyAsync(s->utop, top.data(), sizeof(double) * top.size(), hipMemcpyHostToDevice, s->stream));
+    }
+    FPDE_HIP_TRY(hipStreamSynchronize(s->stream));
+    s->initialized = true;
+    return FPDE_OK;
+}
+
+fpde_status fpde_solver_advance(fpde_solver* s, int nsteps) {
+    if (!s) { set_error("null solver"); return FPDE_ERR_ARG; }
+    if (!s->initialized) { set_error("set_initial must be called first"); return FPDE_ERR_STATE; }
+    int M = s->spec.steps;
+    int end = (nsteps <= 0) ? M : std::min(M, s->step + nsteps);
+    for (int n = s->step; n < end; n++) {
+        fpde_status rc = (s->spec.dims == 1) ? enqueue_1d_step(s, n) : enqueue_3d_step(s, n);
+        if (rc != FPDE_OK) return rc;
+        s->step = n + 1;
+    }
+    if (s->spec.dims == 1 && s->step == M) {
+        fpde_status rc = finalize_1d(s);
+        if (rc != FPDE_OK) return rc;
+    }
+    FPDE_HIP_TRY(hipGetLastError());
+    return FPDE_OK;
+}
+
+fpde_status fpde_solver_sync(fpde_solver* s) {
+    if (!s) { set_error("null solver"); return FPDE_ERR_ARG; }
+    FPDE_HIP_TRY(hipStreamSynchronize(s->stream));
+    ev_collect(s);
+    return FPDE_OK;
+}
+
+int fpde_solver_step(const fpde_solver* s) { return s ? s->step : -1; }
+
+int fpde_solver_query(fpde_solver* s) {
+    if (!s) return -1;
+    hipError_t e = hipStreamQuery(s->stream);
+    if (e == hipSuccess) return 1;
+    if (e == hipErrorNotReady) return 0;
+    set_error(hipGetErrorString(e));
+    return -1;
+}
+
+fpde_status fpde_solver_get_solution(fpde_solver* s, double* u_out) {
+    if (!s || !u_out) { set_error("null argument"); return FPDE_ERR_ARG; }
+    if (s->spec.dims == 1) {
+        fpde_status rc = finalize_1d(s);
+        if (rc != FPDE_OK) return rc;
+    }
+    FPDE_HIP_TRY(hipMemcpyAsync(u_out, s->U, sizeof(double) * s->plan.nodes, hipMemcpyDeviceToHost, s->stream));
+    FPDE_HIP_TRY(hipStreamSynchronize(s->stream));
+    ev_collect(s);
+    return FPDE_OK;
+}
+
+fpde_status fpde_solver_get_weights(fpde_solver* s, double* local, double* omega, double* corr) {
+    if (!s) { set_error("null solver"); return FPDE_ERR_ARG; }
+    FPDE_HIP_TRY(hipStreamSynchronize(s->stream));
+    if (local) *local = s->w.local;
+    if (omega) FPDE_HIP_TRY(hipMemcpy(omega, s->w.omega, sizeof(double) * s->w.M, hipMemcpyDeviceToHost));
+    if (corr) FPDE_HIP_TRY(hipMemcpy(corr, s->w.corr, sizeof(double) * 2 * s->w.M, hipMemcpyDeviceToHost));
+    return FPDE_OK;
+}
+
+fpde_status fpde_solver_set_profiling(fpde_solver* s, int on) {
+    if (!s) { set_error("null solver"); return FPDE_ERR_ARG; }
+    s->prof = on != 0;
+    return FPDE_OK;
+}
+
+fpde_status fpde_solver_get_stats(fpde_solver* s, fpde_stats* out) {
+    if (!s || !out) { set_error("null argument"); return FPDE_ERR_ARG; }
+    FPDE_HIP_TRY(hipStreamSynchronize(s->stream));
+    ev_collect(s);
+    *out = s->stats;
+    return FPDE_OK;
+}
+
+// --- workload initial conditions (same draws as oracle/fpde_oracle.c) ------
+fpde_status fpde_init_1d(int nodes, double length, uint64_t seed, double* u) {
+    if (!u || nodes < 2) { set_error("bad arguments"); return FPDE_ERR_ARG; }
+    std::mt19937_64 eng(seed);
+    std::uniform_real_distribution<double> amp(0.0, 0.05), ph(0.0, 2.0 * M_PI);
+    double a[4], f[4];
+    for (int k = 0; k < 4; k++) { a[k] = amp(eng); f[k] = ph(eng); }
+    double h = length / (nodes - 1);
+    for (int i = 0; i < nodes; i++) {
+        double z = i * h, v = -1.0;
+        for (int k = 0; k < 4; k++) v += a[k] * std::sin((k + 1) * M_PI * z + f[k]);
+        u[i] = v;
+    }
+    return FPDE_OK;
+}
+
+fpde_status fpde_init_3d(int n, double length, uint64_t seed, double* u) {
+    if (!u || n < 2) { set_error("bad arguments"); return FPDE_ERR_ARG; }
+    std::mt19937_64 eng(seed);
+    std::uniform_real_distribution<double> cd(0.2, 0.8), sd(0.05, 0.15), ad(0.0, 0.05);
+    double c[8][3], sg[8], A[8];
+    for (int b = 0; b < 8; b++) {
+        for (int d = 0; d < 3; d++) c[b][d] = cd(eng) * length;
+        sg[b] = sd(eng) * length;
+        A[b] = ad(eng);
+    }
+    double h = length / (n - 1);
+    for (int y = 0; y < n; y++)
+        for (int x = 0; x < n; x++)
+            for (int z = 0; z < n; z++) {
+                double pz = z * h, px = x * h, py = y * h, v = -1.0;
+                for (int b = 0; b < 8; b++) {
+                    double dx = px - c[b][0], dy = py - c[b][1], dz = pz - c[b][2];
+                    v += A[b] * std::exp(-(dx * dx + dy * dy + dz * dz) / (2.0 * sg[b] * sg[b]));
+                }
+                u[z + (size_t)n * (x + (size_t)n * y)] = v;
+            }
+    return FPDE_OK;
+}
+
+double fpde_ksat_for_fourier(double u_top, double h, double tau, double fo) {
+    double K, iC;
+    vgm_coefs(u_top, 0.0, 1.0, 0, K, iC);
+    return fo * h * h / (tau * K * iC);
+}
+
+}  // extern "C"

@@ -1,0 +1,168 @@
+// FP64 L1 / distributed-order weight generation on the device.
+//
+// For a uniform step tau the weight of the history difference D[t] =
+// U^{t+1}-U^t at step n (unknown U^{n+1}) is, per order a,
+//   Da(a)/tau * b(t,t+1;n+1),   b = int_{t tau}^{(t+1) tau} ((n+1)tau - s)^(-a) ds
+// (precalc_kb_3 + time_summing, fpde_hpc_1d.cpp:4069-4136, Da = 1/Gamma(1-a)
+// at :3598-3601).  With g(t)=t the closed form depends on j = n-t only:
+//   b = tau^(1-a)/(1-a) * ((j+1)^(1-a) - j^(1-a)),
+// evaluated here as j^(1-a)*expm1((1-a)*log1p(1/j)) to avoid the cancellation
+// of the plain difference at large j.  The reference replaces the two oldest
+// intervals (t0 <= a3k = 1) by the Newton-binomial series of _kb_row2
+// (fpde_hpc_1d.cpp:2000-2131, stop at the first |term| <= global_eps); those
+// enter as per-step corrections corr[2n+t] = series - Toeplitz weight.
+// Orders are folded: omega_j = sum_q c_q Da(a_q)/tau b_j(a_q) (distributed
+// order, fr_do.cpp:89-113 with the alpha-integral as a quadrature).
+#include "fpde_internal.h"
+#include <cmath>
+#include <vector>
+
+namespace fpde {
+
+struct OrderSet {
+    int q;
+    double a[64];
+    double c[64];    // c_q * Da(a_q) / tau
+};
+
+__device__ inline double l1_closed_lag(double a, double tau, int j) {
+    // b for lag j >= 1: tau^(1-a)/(1-a) * ((j+1)^(1-a) - j^(1-a));
+    // a == 1: the reference returns 0 for every interval but the newest (:2169-2175)
+    if (a == 1.0) return 0.0;
+    double e = 1.0 - a;
+    double jd = (double)j;
+    return pow(tau, e) / e * pow(jd, e) * expm1(e * log1p(1.0 / jd));
+}
+
+// _kb_row2 mode 0 with g(t)=t (see oracle/fpde_oracle.c orc_kb_series).
+__device__ inline double l1_series(double a, double p0, double p1, double pa, double eps) {
+    if (a == 1.0) return (p1 == pa) ? 1.0 : 0.0;
+    double sum = 0.0, v1 = 1.0, v2 = pow(pa, -a) * p1, v6 = p0 / p1, v3, term;
+    double i = 0.0;
+    int niter = 0;
+    do {
+        if (i != 0.0) {
+            v1 *= -(-a - i + 1.0) / i;
+            v2 *= p1 / pa;
+            v6 *= p0 / p1;
+            v3 = (1.0 / (i + 1.0)) * (1.0 - v6);
+            if (fabs(v3) > fabs(v3) * eps) niter++;
+        } else {
+            v3 = (p1 - p0) / p1;
+        }
+        term = v1 * v2 * v3;
+        sum += term;
+        i += 1.0;
+        niter++;
+        if (niter > 100000) break;
+    } while (fabs(term) > eps);
+    return sum;
+}
+
+__global__ void k_weights_omega(OrderSet os, double tau, int M, double* __restrict__ omega) {
+    int j = blockIdx.x * blockDim.x + threadIdx.x + 1;
+    if (j > M) return;
+    double s = 0.0;
+    for (int q = 0; q < os.q; q++) s += os.c[q] * l1_closed_lag(os.a[q], tau, j);
+    omega[j - 1] = s;
+}
+
+// corr[2n+t] for n in [1,M), t in {0,1}, t < n (after omega is complete)
+__global__ void k_weights_corr(OrderSet os, double tau, int M, double eps,
+                               const double* __restrict__ omega, double* __restrict__ corr) {
+    int idx = blockIdx.x * blockDim.x + threadIdx.x;
+    if (idx >= 2 * M) return;
+    int n = idx >> 1, t = idx & 1;
+    if (t >= n) { corr[idx] = 0.0; return; }
+    double s = 0.0;
+    for (int q = 0; q < os.q; q++)
+        s += os.c[q] * l1_series(os.a[q], t * tau, (t + 1) * tau, (n + 1) * tau, eps);
+    corr[idx] = s - omega[n - t - 1];
+}
+
+static void gauss_legendre(int n, double a, double b, double* x, double* w) {
+    for (int i = 0; i < n; i++) {
+        double z = std::cos(M_PI * (i + 0.75) / (n + 0.5)), pp = 0.0, p1 = 1.0, p2 = 0.0;
+        for (int it = 0; it < 100; it++) {
+            p1 = 1.0; p2 = 0.0;
+            for (int j = 1; j <= n; j++) {
+                double p3 = p2; p2 = p1;
+                p1 = ((2.0 * j - 1.0) * z * p2 - (j - 1.0) * p3) / j;
+            }
+            pp = n * (z * p1 - p2) / (z * z - 1.0);
+            double dz = p1 / pp;
+            z -= dz;
+            if (std::fabs(dz) < 1e-16) break;
+        }
+        p1 = 1.0; p2 = 0.0;
+        for (int j = 1; j <= n; j++) {
+            double p3 = p2; p2 = p1;
+            p1 = ((2.0 * j - 1.0) * z * p2 - (j - 1.0) * p3) / j;
+        }
+        pp = n * (z * p1 - p2) / (z * z - 1.0);
+        x[n - 1 - i] = 0.5 * (a + b) + 0.5 * (b - a) * z;
+        w[n - 1 - i] = (b - a) / ((1.0 - z * z) * pp * pp);
+    }
+}
+
+static double da_of(double a) { return (a != 1.0) ? 1.0 / std::tgamma(1.0 - a) : 1.0; }
+
+// Orders of a run and their combination weights (host side).
+static fpde_status build_orders(const fpde_spec& s, OrderSet& os) {
+    if (s.do_q <= 0) {
+        os.q = 1;
+        os.a[0] = s.alpha;
+        os.c[0] = da_of(s.alpha) / s.tau;
+        return FPDE_OK;
+    }
+    if (s.do_q > 64 || (s.do_rule == 1 && s.do_q < 2)) {
+        set_error("distributed order: do_q must be in [1,64] (>=2 for the trapezoid)");
+        return FPDE_ERR_ARG;
+    }
+    os.q = s.do_q;
+    if (s.do_rule == 0) {
+        double x[64], w[64];
+        gauss_legendre(s.do_q, s.do_a0, s.do_a1, x, w);
+        for (int i = 0; i < s.do_q; i++) {
+            os.a[i] = x[i];
+            os.c[i] = w[i] * std::tgamma(3.0 - x[i]) / 2.0 * da_of(x[i]) / s.tau;
+        }
+    } else {
+        // fr_do.cpp trapezoid: sum_i 0.5*(WA_i CD_i + WA_{i+1} CD_{i+1}) (:107-108)
+        double hstep = (s.do_a1 - s.do_a0) / (s.do_q - 1);
+        for (int i = 0; i < s.do_q; i++) {
+            double a = s.do_a0 + hstep * i;
+            double edge = (i == 0 || i == s.do_q - 1) ? 0.5 : 1.0;
+            os.a[i] = a;
+            os.c[i] = std::tgamma(3.0 - a) / 2.0 * hstep * edge * da_of(a) / s.tau;
+        }
+    }
+    return FPDE_OK;
+}
+
+fpde_status launch_weights(const fpde_spec& s, DeviceWeights& w, hipStream_t st) {
+    OrderSet os;
+    fpde_status rc = build_orders(s, os);
+    if (rc != FPDE_OK) return rc;
+    // local coefficient: Da/tau * kkk0, kkk0 = b(n,n+1;n+1) = tau^(1-a)/(1-a) (alpha=1: 1)
+    double local = 0.0;
+    for (int q = 0; q < os.q; q++) {
+        double a = os.a[q];
+        double kkk0 = (a == 1.0) ? 1.0 : std::pow(s.tau, 1.0 - a) / (1.0 - a);
+        local += os.c[q] * kkk0;
+    }
+    w.local = local;
+    int M = w.M;
+    int tb = 256;
+    hipLaunchKernelGGL(k_weights_omega, dim3((M + tb - 1) / tb), dim3(tb), 0, st, os, s.tau, M, w.omega);
+    if (s.weight_mode == 1) {
+        hipLaunchKernelGGL(k_weights_corr, dim3((2 * M + tb - 1) / tb), dim3(tb), 0, st, os, s.tau, M,
+                           s.series_eps, (const double*)w.omega, w.corr);
+    } else {
+        FPDE_HIP_TRY(hipMemsetAsync(w.corr, 0, sizeof(double) * 2 * (size_t)M, st));
+    }
+    FPDE_HIP_TRY(hipGetLastError());
+    return FPDE_OK;
+}
+
+}  // namespace fpde

@@ -1,0 +1,208 @@
+"""Host-side API of the history-sum solver (Python mirror over the C ABI).
+
+`Solver` owns one solve on the device (fpde_solver); `Server` is the
+continuous-batching request loop (fpde_server).  `workload()` builds the
+configurations of the benchmark suite.  Every call goes to libfpde_hip.so;
+there is no CPU path here.
+"""
+import ctypes
+
+import numpy as np
+
+from ._lib import ResultInfo, Spec, Stats, check, lib
+
+HIST_MODES = {"plain": 0, "mfma_f64": 1, "split_bf16": 2, "split_f16": 3, "split_bf16x3": 4}
+
+_dp = ctypes.POINTER(ctypes.c_double)
+
+
+def _p(a):
+    return a.ctypes.data_as(_dp)
+
+
+def make_spec(nodes, steps, tau, alpha=0.8, dims=1, length=1.0, ksat=None, coef_mode=0,
+              picard_max=1, picard_tol=0.0, weight_mode=1, series_eps=1e-5, do_q=0, do_rule=0,
+              do_a0=0.1, do_a1=1.0, hist_mode="plain", block=64, use_graph=0, fourier=1.0,
+              u_top=-1.0):
+    """fpde_spec; ksat defaults to the value giving mesh Fourier number `fourier`
+    (K*tau/(C*h^2)) at suction -u_top."""
+    if ksat is None:
+        h = length / (nodes - 1)
+        ksat = lib().fpde_ksat_for_fourier(u_top, h, tau, fourier) if coef_mode == 0 else 1.0
+    hm = HIST_MODES[hist_mode] if isinstance(hist_mode, str) else int(hist_mode)
+    return Spec(dims, nodes, steps, tau, alpha, length, ksat, coef_mode, picard_max, picard_tol,
+                weight_mode, series_eps, do_q, do_rule, do_a0, do_a1, hm, block, use_graph,
+                (ctypes.c_int * 4)(0, 0, 0, 0))
+
+
+def init_1d(nodes, seed, length=1.0):
+    u = np.zeros(nodes)
+    check(lib().fpde_init_1d(nodes, length, seed, _p(u)))
+    return u
+
+
+def init_3d(n, seed, length=1.0):
+    u = np.zeros(n * n * n)
+    check(lib().fpde_init_3d(n, length, seed, _p(u)))
+    return u
+
+
+class Solver:
+    """One solve kept on the device (fpde_solver_*)."""
+
+    def __init__(self, spec):
+        self.spec = spec
+        self._h = ctypes.c_void_p()
+        check(lib().fpde_solver_create(ctypes.byref(spec), ctypes.byref(self._h)))
+        self.unknowns = spec.nodes if spec.dims == 1 else spec.nodes ** 3
+
+    def close(self):
+        if self._h:
+            lib().fpde_solver_destroy(self._h)
+            self._h = ctypes.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *a):
+        self.close()
+
+    def set_initial(self, u0):
+        u0 = np.ascontiguousarray(u0, dtype=np.float64)
+        if u0.size != self.unknowns:
+            raise ValueError("u0 has the wrong size")
+        check(lib().fpde_solver_set_initial(self._h, _p(u0)))
+
+    def advance(self, nsteps=0):
+        check(lib().fpde_solver_advance(self._h, int(nsteps)))
+
+    def sync(self):
+        check(lib().fpde_solver_sync(self._h))
+
+    @property
+    def step(self):
+        return lib().fpde_solver_step(self._h)
+
+    def solution(self):
+        u = np.zeros(self.unknowns)
+        check(lib().fpde_solver_get_solution(self._h, _p(u)))
+        return u
+
+    def weights(self):
+        M = self.spec.steps
+        local = ctypes.c_double()
+        omega = np.zeros(M)
+        corr = np.zeros(2 * M)
+        check(lib().fpde_solver_get_weights(self._h, ctypes.byref(local), _p(omega), _p(corr)))
+        return local.value, omega, corr.reshape(M, 2)
+
+    def set_profiling(self, on=True):
+        check(lib().fpde_solver_set_profiling(self._h, 1 if on else 0))
+
+    def stats(self):
+        st = Stats()
+        check(lib().fpde_solver_get_stats(self._h, ctypes.byref(st)))
+        return {f: getattr(st, f) for f, _ in Stats._fields_}
+
+
+def solve(spec, u0):
+    with Solver(spec) as s:
+        s.set_initial(u0)
+        s.advance(0)
+        return s.solution()
+
+
+class Server:
+    """Continuous-batching request server (fpde_server_*)."""
+
+    def __init__(self, mem_budget_bytes, max_active=8):
+        self._h = ctypes.c_void_p()
+        check(lib().fpde_server_create(int(mem_budget_bytes), int(max_active), ctypes.byref(self._h)))
+        self._sizes = {}
+
+    def close(self):
+        if self._h:
+            lib().fpde_server_destroy(self._h)
+            self._h = ctypes.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def submit(self, spec, u0):
+        u0 = np.ascontiguousarray(u0, dtype=np.float64)
+        rid = ctypes.c_int64()
+        check(lib().fpde_server_submit(self._h, ctypes.byref(spec), _p(u0), ctypes.byref(rid)))
+        self._sizes[rid.value] = u0.size
+        return rid.value
+
+    def cancel(self, rid):
+        check(lib().fpde_server_cancel(self._h, rid))
+
+    def poll(self, quantum=64):
+        return lib().fpde_server_poll(self._h, quantum)
+
+    def run(self, stop_after=0):
+        check(lib().fpde_server_run(self._h, stop_after))
+
+    def clock(self):
+        return lib().fpde_server_clock(self._h)
+
+    def info(self, rid):
+        inf = ResultInfo()
+        check(lib().fpde_server_result(self._h, rid, ctypes.byref(inf), None, 0))
+        return {f: getattr(inf, f) for f, _ in ResultInfo._fields_}
+
+    def result(self, rid):
+        u = np.zeros(self._sizes[rid])
+        check(lib().fpde_server_result(self._h, rid, None, _p(u), u.size))
+        return u
+
+
+def footprint(spec):
+    return lib().fpde_solver_footprint(ctypes.byref(spec))
+
+
+# ---------------------------------------------------------------------------
+# Workloads of the benchmark suite (1D / distributed order / 3D / serving).
+def workload(name, hist_mode="plain", steps=None, nodes=None, **kw):
+    """Spec + initial field of a named workload:
+    '1d_small'  : N=1000, M=2000, tau=1e-3, alpha=0.8, seed 1000
+    '1d_large'  : N=10000, M=20000, tau=5e-4, alpha=0.7, seed 1001
+    'do'        : N=4000, M=10000, tau=1e-4... distributed order on [0.1,1],
+                  32 Gauss-Legendre nodes, rho(a)=Gamma(3-a)/2, seed 1002
+    '3d'        : 128^3, M=1000, alpha=0.85, seed 1003
+    """
+    if name == "1d_small":
+        N, M, tau, a, seed = 1000, 2000, 1e-3, 0.8, 1000
+    elif name == "1d_large":
+        N, M, tau, a, seed = 10000, 20000, 5e-4, 0.7, 1001
+    elif name == "do":
+        N, M, tau, a, seed = 4000, 10000, 5e-4, 0.5, 1002
+    elif name == "3d":
+        N, M, tau, a, seed = 128, 1000, 1e-3, 0.85, 1003
+    else:
+        raise ValueError(name)
+    if nodes is not None:
+        N = nodes
+    if steps is not None:
+        M = steps
+    dims = 3 if name == "3d" else 1
+    u0 = init_3d(N, seed) if dims == 3 else init_1d(N, seed)
+    u_top = u0[0]
+    extra = dict(kw)
+    if name == "do":
+        extra.setdefault("do_q", 32)
+        extra.setdefault("do_rule", 0)
+        extra.setdefault("do_a0", 0.1)
+        extra.setdefault("do_a1", 1.0)
+    spec = make_spec(N, M, tau, alpha=a, dims=dims, hist_mode=hist_mode, u_top=u_top, **extra)
+    return spec, u0

@@ -1,0 +1,197 @@
+/*
+ * libfpde_hip.so -- C ABI of the MI355X (gfx950) history-sum solver.
+ *
+ * Plain C types only (pointers, sizes, ints, doubles); no torch or HIP types.
+ * Every function returns an fpde_status; on failure fpde_last_error() holds
+ * the message (thread-local).  Sizes are 64-bit throughout (the reference's
+ * GPU layer uses int sizes, opencl_cuda_class.h:8,44-45).
+ *
+ * Three groups of entry points:
+ *  1. the device-layer ABI behind the OpenCL_* facade classes of
+ *     include/fpde_opencl_facade.h, replacing opencl_cuda_class.h:4-67 /
+ *     opencl_cuda_base.cpp:20-826 for the reference's GPU history sum;
+ *  2. direct kernels of the hot path (history sums, weights);
+ *  3. the solver (1D / 3D / distributed order) and the batched request
+ *     server, which keep all state on the device.
+ */
+#ifndef FPDE_HIP_H
+#define FPDE_HIP_H
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef enum {
+    FPDE_OK = 0,
+    FPDE_ERR_ARG = 1,       /* invalid argument / shape                      */
+    FPDE_ERR_HIP = 2,       /* HIP runtime error                             */
+    FPDE_ERR_NOMEM = 3,     /* device allocation failed / over budget        */
+    FPDE_ERR_NOTFOUND = 4,  /* unknown kernel name                           */
+    FPDE_ERR_STATE = 5      /* call not valid in the object's current state  */
+} fpde_status;
+
+const char *fpde_last_error(void);
+int fpde_version(void);
+/* Number of visible devices (0 when no GPU); never fails. */
+int fpde_device_count(void);
+
+/* ======================================================================
+ * 1. Device layer behind the OpenCL_* facade (opencl_cuda_class.h:4-67).
+ *    Kernels are precompiled for gfx950; a "program" is created from the
+ *    reference's kernel source text only to read its #define constants
+ *    (sum_alg, OS, BS: fpde_hpc_1d.cpp:1771-1775); kernels bind by name.
+ * ====================================================================== */
+typedef struct fpde_cl_ctx fpde_cl_ctx;
+typedef struct fpde_cl_buf fpde_cl_buf;
+typedef struct fpde_cl_prog fpde_cl_prog;
+typedef struct fpde_cl_kernel fpde_cl_kernel;
+
+/* OpenCL_program(int gpu) + create_queue (opencl_cuda_base.cpp:356-406, :448-452). */
+fpde_status fpde_cl_ctx_create(int device, fpde_cl_ctx **out);
+fpde_status fpde_cl_ctx_destroy(fpde_cl_ctx *ctx);
+/* OpenCL_buffer(ct,flags,size,mem) (opencl_cuda_base.cpp:342-351); host_init may be NULL. */
+fpde_status fpde_cl_buf_alloc(fpde_cl_ctx *ctx, size_t bytes, const void *host_init,
+                              fpde_cl_buf **out);
+fpde_status fpde_cl_buf_free(fpde_cl_buf *buf);
+size_t fpde_cl_buf_size(const fpde_cl_buf *buf);
+/* Device address of a buffer (for direct kernel calls below). */
+void *fpde_cl_buf_device_ptr(const fpde_cl_buf *buf);
+/* EnqueueWriteBuffer / EnqueueBuffer (opencl_cuda_base.cpp:210-228): blocking;
+ * bytes == 0 means "whole buffer from offset". */
+fpde_status fpde_cl_buf_write(fpde_cl_ctx *ctx, fpde_cl_buf *buf, size_t offset,
+                              size_t bytes, const void *src);
+fpde_status fpde_cl_buf_read(fpde_cl_ctx *ctx, fpde_cl_buf *buf, size_t offset,
+                             size_t bytes, void *dst);
+/* OpenCL_prg(ct,did,source) (opencl_cuda_base.cpp:55-179): parses #defines. */
+fpde_status fpde_cl_program_create(fpde_cl_ctx *ctx, const char *source, fpde_cl_prog **out);
+fpde_status fpde_cl_program_destroy(fpde_cl_prog *prog);
+/* OpenCL_kernel(ct,prog,name) (opencl_cuda_base.cpp:181-197); name "Sum". */
+fpde_status fpde_cl_kernel_create(fpde_cl_prog *prog, const char *name, fpde_cl_kernel **out);
+fpde_status fpde_cl_kernel_destroy(fpde_cl_kernel *k);
+/* SetArg(idx,size,ptr) / SetBufferArg(buf,idx) (opencl_cuda_base.cpp:198-209). */
+fpde_status fpde_cl_kernel_set_arg(fpde_cl_kernel *k, int idx, size_t size, const void *value);
+fpde_status fpde_cl_kernel_set_buffer_arg(fpde_cl_kernel *k, int idx, fpde_cl_buf *buf);
+/* ExecuteKernel(k,ndims,dims,ldims) (opencl_cuda_base.cpp:229-321). */
+fpde_status fpde_cl_launch(fpde_cl_ctx *ctx, fpde_cl_kernel *k, int ndims,
+                           const size_t *gdims, const size_t *ldims);
+/* Finish() (opencl_cuda_base.cpp:322-341). */
+fpde_status fpde_cl_finish(fpde_cl_ctx *ctx);
+
+/* ======================================================================
+ * 2. Direct hot-path kernels on device pointers (stream = default stream of
+ *    the calling thread; all synchronous on return).
+ * ====================================================================== */
+/* The reference "Sum" kernel, sum_alg 0/1 (fpde_hpc_1d.cpp:1247-1284):
+ * S[id] = sum_{t=old_size-2..0} kb[t]*(old[(t+1)*os+id]-old[t*os+id])/tau[t],
+ * for id < nodes; sum_alg 1 stops at the first 16-level chunk whose weights
+ * are all below sum_param.  old: [old_size][os] doubles, level-major. */
+fpde_status fpde_reference_sum(double *S, const double *old, int64_t old_size,
+                               int64_t os, int64_t nodes, const double *tau,
+                               const double *kb, int sum_alg, double sum_param);
+
+/* Plain FP64 streaming history sum on differences (the solver's plain path):
+ * S[i] = sum_{j=1..n} omega[j-1]*D[(n-j)*ld + i], i < nodes. */
+fpde_status fpde_history_sum_f64(double *S, const double *D, int64_t n, int64_t nodes,
+                                 int64_t ld, const double *omega);
+
+/* ======================================================================
+ * 3. Solver
+ * ====================================================================== */
+typedef struct {
+    int dims;            /* 1 or 3                                               */
+    int nodes;           /* 1D: nodes on [0,length]; 3D: nodes per axis (n^3)    */
+    int steps;           /* M                                                    */
+    double tau;          /* uniform time step                                    */
+    double alpha;        /* Caputo order (single order)                          */
+    double length;       /* domain length                                        */
+    double ksat;         /* conductivity scale K = ksat*Kr                       */
+    int coef_mode;       /* 0 VGM (cv_form 4, k_form 1), 1 linear (K=1, 1/C=1)   */
+    int picard_max;      /* >=1; 1 = lagged coefficients (the reference scheme)  */
+    double picard_tol;   /* stop when max|U^k-U^{k-1}| <= tol (checked on device) */
+    int weight_mode;     /* 0 closed-form L1, 1 reference series for t0<=1       */
+    double series_eps;   /* the reference's global_eps (1e-5)                    */
+    int do_q;            /* 0 single order; >0 distributed order with q nodes    */
+    int do_rule;         /* 0 Gauss-Legendre, 1 trapezoid (fr_do.cpp)            */
+    double do_a0, do_a1; /* order interval for distributed order                 */
+    int hist_mode;       /* 0 plain FP64, 1 FP64 MFMA GEMM, 2 split bf16 hi+lo,
+                            3 split fp16 hi+lo + scaling, 4 split bf16 x3        */
+    int block;           /* future steps per GEMM block (multiple of 16, <=128)  */
+    int use_graph;       /* capture the step loop in hipGraphs                   */
+    int reserved[4];
+} fpde_spec;
+
+typedef struct fpde_solver fpde_solver;
+
+typedef struct {
+    double ms_total;          /* device time of the last run() (hipEvents)    */
+    double ms_history;        /* summed device time of history kernels        */
+    double ms_step;           /* summed device time of step kernels           */
+    int64_t history_launches;
+    int64_t step_launches;
+    double history_alg_bytes; /* algorithmic bytes of the history kernels (8 per term) */
+    double history_alg_flops; /* algorithmic flops (2 per term)               */
+    size_t device_bytes;      /* device memory held by the solver             */
+    int picard_iters_total;   /* Picard iterations actually performed         */
+} fpde_stats;
+
+/* Bytes of device memory a solve of this spec will hold (admission). */
+size_t fpde_solver_footprint(const fpde_spec *spec);
+fpde_status fpde_solver_create(const fpde_spec *spec, fpde_solver **out);
+fpde_status fpde_solver_destroy(fpde_solver *s);
+/* u0: host array of nodes (1D) or nodes^3 (3D, depth index fastest). */
+fpde_status fpde_solver_set_initial(fpde_solver *s, const double *u0);
+/* Enqueue nsteps steps (or all remaining if nsteps<=0); asynchronous. */
+fpde_status fpde_solver_advance(fpde_solver *s, int nsteps);
+fpde_status fpde_solver_sync(fpde_solver *s);
+/* 1 when every enqueued launch of the solver has completed, 0 otherwise
+ * (non-blocking), -1 on error. */
+int fpde_solver_query(fpde_solver *s);
+/* Steps completed so far. */
+int fpde_solver_step(const fpde_solver *s);
+fpde_status fpde_solver_get_solution(fpde_solver *s, double *u_out);
+/* Effective weights (local, omega[M], corr[2M]) the solver uses (host copies). */
+fpde_status fpde_solver_get_weights(fpde_solver *s, double *local, double *omega, double *corr);
+/* Enable per-kernel timing (hipEvents around each launch; slows the run). */
+fpde_status fpde_solver_set_profiling(fpde_solver *s, int on);
+fpde_status fpde_solver_get_stats(fpde_solver *s, fpde_stats *out);
+
+/* Initial conditions of the workloads (host; same draws as the oracle's). */
+fpde_status fpde_init_1d(int nodes, double length, uint64_t seed, double *u);
+fpde_status fpde_init_3d(int nodes, double length, uint64_t seed, double *u);
+/* ksat giving the mesh Fourier number K*tau/(C*h^2) = fo at suction -u_top. */
+double fpde_ksat_for_fourier(double u_top, double h, double tau, double fo);
+
+/* ---------------------------------------------------------------------
+ * Batched request server (continuous batching on one GPU).
+ * --------------------------------------------------------------------- */
+typedef struct fpde_server fpde_server;
+typedef struct {
+    int64_t id;
+    double submit_s, start_s, finish_s;   /* server clock (seconds)            */
+    int status;                           /* 0 done, 1 rejected, 2 cancelled   */
+    int steps_done;
+} fpde_result_info;
+
+fpde_status fpde_server_create(size_t mem_budget_bytes, int max_active, fpde_server **out);
+fpde_status fpde_server_destroy(fpde_server *srv);
+/* Queue a request; u0 is copied.  Rejected (status 1) if its footprint
+ * exceeds the whole budget. */
+fpde_status fpde_server_submit(fpde_server *srv, const fpde_spec *spec, const double *u0,
+                               int64_t *id_out);
+fpde_status fpde_server_cancel(fpde_server *srv, int64_t id);
+/* Run the scheduler until every queued request has finished (or until the
+ * number of finished requests reaches stop_after, if > 0). */
+fpde_status fpde_server_run(fpde_server *srv, int64_t stop_after);
+/* One scheduling round: admit, advance each active solve by up to `quantum`
+ * steps, retire finished ones.  Returns number of active+queued requests. */
+int fpde_server_poll(fpde_server *srv, int quantum);
+fpde_status fpde_server_result(fpde_server *srv, int64_t id, fpde_result_info *info,
+                               double *u_out, size_t u_len);
+double fpde_server_clock(fpde_server *srv);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,106 @@
+/*
+ * FP64 CPU oracle for the Caputo L1 history-sum path of sevaboh/FPDE_HPC.
+ *
+ * TEST INFRASTRUCTURE ONLY.  This library is the checker: tests/, the smoke
+ * entry and bench.py's cpu_baseline leg may load it; the product library
+ * (libfpde_hip.so) never links, loads or calls it.
+ *
+ * It restates, in plain C (C99 + OpenMP), the reference algorithm:
+ *   - L1 weights b(t0,t1;a) = int_{x_t0}^{x_t1} (x_a - s)^(-alpha) ds
+ *     (fpde_hpc_1d.cpp:1933-1996 closed form for g(t)=t, :2000-2131 Newton
+ *     binomial series, :2167-2301 dispatch, :3942-3949 kkk0, :4069-4085
+ *     kb_3_cache);
+ *   - the history sum sum_t kb_3_cache[t]*(old[t+1]-old[t])/tau[t]
+ *     (fpde_hpc_1d.cpp:4086-4164, sum_alg 0, node-outer / t-inner);
+ *   - van Genuchten-Mualem coefficients (fpde_hpc_1d.cpp:2493-2575, :2625-2657);
+ *   - the tridiagonal step A_/B/R + al1/Om1/bt1/back-substitution
+ *     (fpde_hpc_1d.cpp:3095-3142, :4002-4027, :4165-4268, :4491-4590, :4672-4725);
+ *   - the distributed-order derivative (fr_do.cpp:51-113) with either the
+ *     reference trapezoid in alpha or Gauss-Legendre nodes;
+ *   - the 3D locally-one-dimensional step (fpde_hpc_1d.cpp:5144-5626, sweep
+ *     order K -> M -> N, history only in the first sweep).
+ *
+ * Pinning status of each function is listed in the header of fpde_oracle.c.
+ */
+#ifndef FPDE_ORACLE_H
+#define FPDE_ORACLE_H
+#include <stdint.h>
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ---- random numbers: MT19937-64 + libstdc++ uniform_real_distribution ---- */
+typedef struct { uint64_t mt[312]; int mti; } orc_mt64;
+void     orc_mt64_seed(orc_mt64 *s, uint64_t seed);
+uint64_t orc_mt64_next(orc_mt64 *s);
+double   orc_mt64_uniform(orc_mt64 *s, double a, double b);
+
+/* ---- L1 weights (g(t)=t, FK=0) ---- */
+double orc_kb_closed(double alpha, double p0, double p1, double pa);
+double orc_kb_series(double alpha, double p0, double p1, double pa,
+                     double eps, int max_niter, int *niter_out);
+/* kb(alpha,t0,t0+1,a) over a time grid, reference dispatch (a3k == 1):
+ * weight_mode 0 = closed form everywhere, 1 = reference (series for t0<=1). */
+double orc_kb(double alpha, int t0, int a, const double *Time,
+              int weight_mode, double eps);
+/* kb_3_cache for step n (levels 0..n), n entries; returns kkk0. */
+double orc_l1_weights(double alpha, int n, const double *Time,
+                      int weight_mode, double eps, double *kb3);
+
+/* ---- history sum (sum_alg 0): S[i] = sum_t kb3[t]*(old[t+1][i]-old[t][i])/tau[t] */
+void orc_time_summing(int nodes, int levels, const double *old,
+                      const double *kb3, const double *tau, double *S,
+                      int nthreads);
+
+/* ---- VGM coefficients, P = suction head in metres ---- */
+double orc_wetness(double P);
+double orc_inv_dw_dh(double P);
+double orc_kr(double P);
+
+/* ---- Gauss-Legendre nodes/weights on [a,b] ---- */
+void orc_gauss_legendre(int n, double a, double b, double *x, double *w);
+
+/* ---- solver parameters (shared by the 1D and 3D drivers) ---- */
+typedef struct {
+    int    nodes;        /* 1D: nodes on [0,length]; 3D: nodes per axis   */
+    int    steps;        /* M                                             */
+    double tau;          /* uniform time step                             */
+    double alpha;        /* Caputo order (single-order runs)              */
+    double length;       /* domain length (1D depth, 3D cube side)        */
+    double ksat;         /* saturated conductivity scale (nondimensional) */
+    int    coef_mode;    /* 0 = VGM (cv_form 4, k_form 1), 1 = linear K=1, 1/C=1 */
+    int    picard_max;   /* >=1; 1 = lagged coefficients (reference scheme) */
+    double picard_tol;
+    int    weight_mode;  /* 0 closed form, 1 reference series for t0<=1  */
+    double series_eps;   /* global_eps of the reference (1e-5)           */
+    int    do_q;         /* 0 = single order; >0 = distributed order, q nodes */
+    int    do_rule;      /* 0 = Gauss-Legendre, 1 = trapezoid (fr_do.cpp)  */
+    double do_a0, do_a1; /* order interval                                 */
+    int    nthreads;
+} orc_params;
+
+/* 1D solve: u (nodes) holds u0 on entry, u^M on exit.  u_top = u0[0]. */
+int orc_solve_1d(const orc_params *p, double *u);
+/* 3D LOD solve on nodes^3 (depth index fastest); u holds u0 / u^M. */
+int orc_solve_3d(const orc_params *p, double *u);
+
+/* Effective (folded) weights used by the product for uniform tau:
+ * local = coefficient of (U^{n+1}-U^n); omega[j-1] for j=1..M (Toeplitz),
+ * corr[2n], corr[2n+1] = corrections of the t=0,1 weights at step n.       */
+void orc_folded_weights(const orc_params *p, double *local, double *omega,
+                        double *corr);
+
+/* Thomas solve of a_i x_{i-1} + b_i x_i + c_i x_{i+1} = d_i (tests). */
+void orc_thomas(int n, const double *a, const double *b, const double *c,
+                const double *d, double *x);
+
+/* Initial conditions of the workloads. */
+void orc_init_1d(int nodes, double length, uint64_t seed, double *u);
+void orc_init_3d(int nodes, double length, uint64_t seed, double *u);
+/* ksat giving mesh Fourier number K*tau/(C*h^2) = fo at the top node. */
+double orc_ksat_for_fourier(double u_top, double h, double tau, double fo);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,81 @@
+"""CPU checks of the C ABI: the library loads, exports every symbol the
+public header declares, host-side logic agrees with the oracle, and invalid
+specs are rejected before any device work."""
+import ctypes
+import os
+import re
+
+import numpy as np
+import pytest
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+
+
+def declared_symbols():
+    txt = open(os.path.join(ROOT, "include", "fpde_hip.h")).read()
+    txt = re.sub(r"/\*.*?\*/", "", txt, flags=re.S)
+    return sorted(set(re.findall(r"\b(fpde_[a-z0-9_]+)\s*\(", txt)))
+
+
+def test_library_exports_every_declared_symbol(fpde):
+    L = fpde.lib()
+    syms = declared_symbols()
+    assert len(syms) >= 40
+    missing = [s for s in syms if not hasattr(L, s)]
+    assert not missing, missing
+
+
+def test_binding_covers_header(fpde):
+    from fpde_hpc_amd._lib import SIGNATURES
+    bound = {n for n, _, _ in SIGNATURES}
+    assert set(declared_symbols()) == bound
+
+
+def test_init_conditions_match_oracle(fpde, oracle):
+    for n, seed in [(1000, 1000), (10000, 1001), (37, 5)]:
+        assert np.array_equal(fpde.init_1d(n, seed), oracle.init_1d(n, seed))
+    for n, seed in [(16, 1003), (9, 7)]:
+        assert np.array_equal(fpde.init_3d(n, seed), oracle.init_3d(n, seed))
+
+
+def test_ksat_fourier_matches_oracle(fpde, oracle):
+    for ut, h, tau in [(-1.0, 1e-3, 1e-3), (-0.95, 1e-4, 5e-4), (-1.07, 2.5e-4, 1e-3)]:
+        a = fpde.lib().fpde_ksat_for_fourier(ut, h, tau, 1.0)
+        b = oracle.ksat_for_fourier(ut, h, tau, 1.0)
+        assert abs(a - b) <= 1e-14 * abs(b)
+
+
+@pytest.mark.parametrize("bad", [
+    dict(dims=2), dict(nodes=3), dict(steps=0), dict(alpha=0.0), dict(alpha=1.5),
+    dict(picard_max=0), dict(hist_mode=9), dict(block=48), dict(dims=3, nodes=300),
+    dict(tau=-1.0), dict(nodes=70000),
+])
+def test_invalid_specs_rejected(fpde, bad):
+    kw = dict(nodes=100, steps=10, tau=1e-3, alpha=0.5, hist_mode=1, block=64)
+    kw.update(bad)
+    dims = kw.pop("dims", 1)
+    spec = fpde.make_spec(kw.pop("nodes"), kw.pop("steps"), kw.pop("tau"), dims=dims, ksat=1.0, **kw)
+    h = ctypes.c_void_p()
+    rc = fpde.lib().fpde_solver_create(ctypes.byref(spec), ctypes.byref(h))
+    assert rc == 1
+    assert fpde.lib().fpde_last_error()
+
+
+def test_footprint_scales_with_history(fpde):
+    a = fpde.footprint(fpde.make_spec(10000, 20000, 5e-4, hist_mode="plain", ksat=1.0))
+    b = fpde.footprint(fpde.make_spec(10000, 20000, 5e-4, hist_mode="split_bf16", ksat=1.0))
+    assert a >= 8 * 10000 * 20000
+    assert b >= a + 4 * 10000 * 20000 - 1
+
+
+def test_unknown_kernel_name_is_reported(fpde):
+    # program/kernel objects need no device
+    L = fpde.lib()
+    prog = ctypes.c_void_p()
+    assert L.fpde_cl_program_create(ctypes.c_void_p(1), b"#define OS 64\n", ctypes.byref(prog)) == 0
+    k = ctypes.c_void_p()
+    assert L.fpde_cl_kernel_create(prog, b"Nope", ctypes.byref(k)) == 4
+    assert b"Sum" in L.fpde_last_error()
+    assert L.fpde_cl_kernel_create(prog, b"Sum", ctypes.byref(k)) == 0
+    L.fpde_cl_kernel_destroy(k)
+    L.fpde_cl_program_destroy(prog)

@@ -1,0 +1,251 @@
+"""GPU parity: the HIP path (through the C ABI) against the FP64 CPU oracle.
+
+Tolerances (relative = max|u - u_ref| / max|u_ref|):
+  FP64 paths (plain streaming, FP64 MFMA GEMM)     <= 1e-10
+  split bf16/fp16 MFMA paths                       <= 1e-8
+"""
+import ctypes
+
+import numpy as np
+import pytest
+
+pytestmark = pytest.mark.gpu
+
+TOL_F64 = 1e-10
+TOL_MIXED = 1e-8
+
+
+def rel_err(u, ref):
+    return float(np.max(np.abs(u - ref)) / np.max(np.abs(ref)))
+
+
+class Dev:
+    """Small helper over the fpde_cl_* buffer API (device memory for direct kernel calls)."""
+
+    def __init__(self, L):
+        self.L = L
+        self.ctx = ctypes.c_void_p()
+        assert L.fpde_cl_ctx_create(0, ctypes.byref(self.ctx)) == 0, L.fpde_last_error()
+        self.bufs = []
+
+    def put(self, a):
+        a = np.ascontiguousarray(a, dtype=np.float64)
+        b = ctypes.c_void_p()
+        assert self.L.fpde_cl_buf_alloc(self.ctx, a.nbytes, a.ctypes.data, ctypes.byref(b)) == 0
+        self.bufs.append(b)
+        return b
+
+    def zeros(self, n):
+        return self.put(np.zeros(n))
+
+    def ptr(self, b):
+        return self.L.fpde_cl_buf_device_ptr(b)
+
+    def get(self, b, n):
+        out = np.zeros(n)
+        assert self.L.fpde_cl_buf_read(self.ctx, b, 0, n * 8, out.ctypes.data) == 0
+        return out
+
+    def close(self):
+        for b in self.bufs:
+            self.L.fpde_cl_buf_free(b)
+        self.L.fpde_cl_ctx_destroy(self.ctx)
+
+
+def test_history_sum_f64_matches_numpy(fpde):
+    L = fpde.lib()
+    rng = np.random.default_rng(7)
+    for n, nodes in [(1, 10), (5, 130), (97, 1000), (700, 777)]:
+        ld = ((nodes + 63) // 64) * 64
+        D = np.zeros((n, ld))
+        D[:, :nodes] = rng.standard_normal((n, nodes))
+        omega = rng.random(n + 8)
+        ref = np.zeros(nodes)
+        for j in range(1, n + 1):
+            ref += omega[j - 1] * D[n - j, :nodes]
+        dv = Dev(L)
+        try:
+            bD, bw, bS = dv.put(D), dv.put(omega), dv.zeros(ld)
+            rc = L.fpde_history_sum_f64(dv.ptr(bS), dv.ptr(bD), n, nodes, ld, dv.ptr(bw))
+            assert rc == 0, L.fpde_last_error()
+            S = dv.get(bS, nodes)
+        finally:
+            dv.close()
+        assert np.max(np.abs(S - ref)) <= 1e-12 * np.max(np.abs(ref))
+
+
+def test_reference_sum_kernel_through_facade(fpde, oracle):
+    """The reference 'Sum' ABI (fr_ocl_call_sum, fpde_hpc_1d.cpp:1851-1868) vs time_summing."""
+    L = fpde.lib()
+    rng = np.random.default_rng(11)
+    nodes, levels = 333, 41
+    OS = 384
+    old = np.zeros((levels, OS))
+    old[:, :nodes] = np.cumsum(rng.standard_normal((levels, nodes)) * 1e-2, axis=0)
+    Time = oracle.time_grid(levels + 1, 1e-3)
+    kkk0, kb = oracle.l1_weights(0.7, levels - 1, Time, weight_mode=1)
+    tau = np.full(levels + 8, 1e-3)
+    ref = oracle.time_summing(old[:, :nodes], kb, tau)
+    src = b"#define sum_alg 0\n#define OS 384\n#define BS 16\n__kernel void Sum(...)"
+    dv = Dev(L)
+    try:
+        bS, bold, btau, bK = dv.zeros(OS), dv.put(old), dv.put(tau), dv.put(np.concatenate([kb, np.zeros(8)]))
+        bH, bC = dv.zeros(8), dv.zeros(8)
+        prog, k = ctypes.c_void_p(), ctypes.c_void_p()
+        assert L.fpde_cl_program_create(dv.ctx, src, ctypes.byref(prog)) == 0
+        assert L.fpde_cl_kernel_create(prog, b"Sum", ctypes.byref(k)) == 0
+        for idx, b in [(0, bS), (1, bold), (3, btau), (4, bH), (5, bC), (6, bK)]:
+            assert L.fpde_cl_kernel_set_buffer_arg(k, idx, b) == 0
+        iv = ctypes.c_int(levels)
+        assert L.fpde_cl_kernel_set_arg(k, 2, 4, ctypes.byref(iv)) == 0
+        for idx, v in [(7, 1e-5), (8, 0.0), (9, 0.0), (10, 0.7)]:
+            dvv = ctypes.c_double(v)
+            assert L.fpde_cl_kernel_set_arg(k, idx, 8, ctypes.byref(dvv)) == 0
+        iv2 = ctypes.c_int(levels - 1)
+        assert L.fpde_cl_kernel_set_arg(k, 11, 4, ctypes.byref(iv2)) == 0
+        g = (ctypes.c_size_t * 1)(OS)
+        l = (ctypes.c_size_t * 1)(16)
+        assert L.fpde_cl_launch(dv.ctx, k, 1, g, l) == 0, L.fpde_last_error()
+        assert L.fpde_cl_finish(dv.ctx) == 0
+        S = dv.get(bS, nodes)
+        L.fpde_cl_kernel_destroy(k)
+        L.fpde_cl_program_destroy(prog)
+    finally:
+        dv.close()
+    assert np.max(np.abs(S - ref)) <= 1e-13 * np.max(np.abs(ref))
+
+
+@pytest.mark.parametrize("weight_mode", [0, 1])
+@pytest.mark.parametrize("do", [None, (8, 0), (6, 1)])
+def test_weights_match_oracle(fpde, oracle, weight_mode, do):
+    M, tau = 300, 1e-3
+    kw = {}
+    if do:
+        kw = dict(do_q=do[0], do_rule=do[1], do_a0=0.1, do_a1=1.0)
+    spec = fpde.make_spec(50, M, tau, alpha=0.6, weight_mode=weight_mode, **kw)
+    with fpde.Solver(spec) as s:
+        local, omega, corr = s.weights()
+    p = oracle.make_params(50, M, tau, 0.6, weight_mode=weight_mode, ksat=spec.ksat,
+                           do_q=kw.get("do_q", 0), do_rule=kw.get("do_rule", 0))
+    lo, om, co = oracle.folded_weights(p)
+    assert abs(local - lo) <= 1e-13 * abs(lo)
+    assert np.max(np.abs(omega - om) / np.abs(om)) <= 1e-10
+    full = om[None, :] * 0  # noqa
+    # compare full weights of the two oldest intervals (omega + corr)
+    for n in range(1, M):
+        for t in range(min(2, n)):
+            a = omega[n - t - 1] + corr[n, t]
+            b = om[n - t - 1] + co[n, t]
+            assert abs(a - b) <= 1e-10 * abs(b)
+
+
+def _oracle_1d(oracle, spec, u0, picard_tol=0.0):
+    p = oracle.make_params(spec.nodes, spec.steps, spec.tau, spec.alpha, ksat=spec.ksat,
+                           coef_mode=spec.coef_mode, picard_max=spec.picard_max, picard_tol=picard_tol,
+                           weight_mode=spec.weight_mode, series_eps=spec.series_eps, do_q=spec.do_q,
+                           do_rule=spec.do_rule, do_a0=spec.do_a0, do_a1=spec.do_a1)
+    return oracle.solve_1d(p, u0)
+
+
+CASES_1D = [
+    # nodes, steps, alpha, picard, coef_mode
+    (300, 200, 0.8, 1, 0),
+    (517, 150, 0.7, 2, 0),
+    (1000, 300, 0.5, 1, 0),
+    (5, 90, 0.9, 1, 0),
+    (64, 130, 0.3, 3, 1),
+]
+
+
+@pytest.mark.parametrize("case", CASES_1D)
+@pytest.mark.parametrize("mode", ["plain", "mfma_f64", "split_bf16", "split_f16", "split_bf16x3"])
+def test_solve_1d_matches_oracle(fpde, oracle, case, mode):
+    nodes, steps, alpha, picard, coef = case
+    u0 = fpde.init_1d(nodes, 1000 + nodes)
+    spec = fpde.make_spec(nodes, steps, 1e-3, alpha=alpha, picard_max=picard, coef_mode=coef,
+                          hist_mode=mode, block=64, u_top=u0[0])
+    u = fpde.solve(spec, u0)
+    ref = _oracle_1d(oracle, spec, u0)
+    err = rel_err(u, ref)
+    tol = TOL_F64 if mode in ("plain", "mfma_f64") else TOL_MIXED
+    print(f"1D {case} {mode}: rel err {err:.3e}")
+    assert err <= tol
+
+
+@pytest.mark.parametrize("block", [16, 32, 128])
+def test_solve_1d_blocks(fpde, oracle, block):
+    nodes, steps = 700, 260
+    u0 = fpde.init_1d(nodes, 5)
+    for mode in ("mfma_f64", "split_f16"):
+        spec = fpde.make_spec(nodes, steps, 1e-3, alpha=0.75, hist_mode=mode, block=block, u_top=u0[0])
+        u = fpde.solve(spec, u0)
+        ref = _oracle_1d(oracle, spec, u0)
+        assert rel_err(u, ref) <= (TOL_F64 if mode == "mfma_f64" else TOL_MIXED)
+
+
+@pytest.mark.parametrize("mode", ["plain", "mfma_f64", "split_f16"])
+def test_distributed_order_matches_oracle(fpde, oracle, mode):
+    nodes, steps = 400, 160
+    u0 = fpde.init_1d(nodes, 1002)
+    for rule, q in [(0, 8), (1, 6)]:
+        spec = fpde.make_spec(nodes, steps, 1e-3, do_q=q, do_rule=rule, do_a0=0.1, do_a1=1.0,
+                              hist_mode=mode, u_top=u0[0])
+        u = fpde.solve(spec, u0)
+        ref = _oracle_1d(oracle, spec, u0)
+        assert rel_err(u, ref) <= (TOL_MIXED if mode.startswith("split") else TOL_F64)
+
+
+@pytest.mark.parametrize("mode", ["plain", "mfma_f64", "split_f16"])
+def test_solve_3d_matches_oracle(fpde, oracle, mode):
+    n, steps = 12, 70
+    u0 = fpde.init_3d(n, 1003)
+    spec = fpde.make_spec(n, steps, 1e-3, alpha=0.85, dims=3, hist_mode=mode, block=16, u_top=u0[0])
+    u = fpde.solve(spec, u0)
+    p = oracle.make_params(n, steps, 1e-3, 0.85, ksat=spec.ksat)
+    ref = oracle.solve_3d(p, u0)
+    err = rel_err(u, ref)
+    print(f"3D {mode}: rel err {err:.3e}")
+    assert err <= (TOL_MIXED if mode.startswith("split") else TOL_F64)
+
+
+def test_advance_in_pieces_equals_one_shot(fpde):
+    nodes, steps = 400, 150
+    u0 = fpde.init_1d(nodes, 3)
+    spec = fpde.make_spec(nodes, steps, 1e-3, alpha=0.6, hist_mode="mfma_f64", u_top=u0[0])
+    ref = fpde.solve(spec, u0)
+    with fpde.Solver(spec) as s:
+        s.set_initial(u0)
+        for chunk in (1, 7, 30, 50, 100):
+            s.advance(chunk)
+            _ = s.solution()
+        assert s.step == steps
+        u = s.solution()
+    assert rel_err(u, ref) <= 1e-13
+
+
+def test_server_results_same_alone_and_in_mix(fpde):
+    rng = np.random.default_rng(1004)
+    specs = []
+    for i in range(10):
+        if i % 4 == 3:
+            n = int(rng.integers(8, 14))
+            u0 = fpde.init_3d(n, 2000 + i)
+            sp = fpde.make_spec(n, int(rng.integers(20, 60)), 1e-3, alpha=float(rng.uniform(0.3, 0.95)),
+                                dims=3, hist_mode="split_f16", block=16, u_top=u0[0])
+        else:
+            n = int(rng.integers(100, 900))
+            u0 = fpde.init_1d(n, 2000 + i)
+            kw = dict(do_q=4) if i % 5 == 1 else {}
+            sp = fpde.make_spec(n, int(rng.integers(60, 300)), 1e-3, alpha=float(rng.uniform(0.3, 0.95)),
+                                hist_mode="split_f16", u_top=u0[0], **kw)
+        specs.append((sp, u0))
+    alone = [fpde.solve(sp, u0) for sp, u0 in specs]
+    fps = [fpde.footprint(sp) for sp, _ in specs]
+    srv = fpde.Server(mem_budget_bytes=max(sum(fps) // 3, max(fps)), max_active=4)
+    ids = [srv.submit(sp, u0) for sp, u0 in specs]
+    srv.run()
+    for rid, a in zip(ids, alone):
+        info = srv.info(rid)
+        assert info["status"] == 0
+        assert rel_err(srv.result(rid), a) <= 1e-14
+    srv.close()
