@@ -73,6 +73,8 @@ SIGNATURES = [
     ("fpde_reference_sum", ctypes.c_int, [_vp, _vp, ctypes.c_int64, ctypes.c_int64, ctypes.c_int64,
                                           _vp, _vp, ctypes.c_int, ctypes.c_double]),
     ("fpde_history_sum_f64", ctypes.c_int, [_vp, _vp, ctypes.c_int64, ctypes.c_int64, ctypes.c_int64, _vp]),
+    ("fpde_history_block_gemm", ctypes.c_int, [ctypes.c_int, _vp, ctypes.c_int64, ctypes.c_int64,
+                                               ctypes.c_int64, ctypes.c_int, ctypes.c_int, ctypes.c_int, _vp, _vp]),
     ("fpde_solver_footprint", ctypes.c_size_t, [ctypes.POINTER(Spec)]),
     ("fpde_solver_create", ctypes.c_int, [ctypes.POINTER(Spec), ctypes.POINTER(_vp)]),
     ("fpde_solver_destroy", ctypes.c_int, [_vp]),

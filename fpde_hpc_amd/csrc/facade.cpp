@@ -308,4 +308,45 @@ fpde_status fpde_history_sum_f64(double* S, const double* D, int64_t n, int64_t 
     return FPDE_OK;
 }
 
+fpde_status fpde_history_block_gemm(int mode, const double* D, int64_t ld, int64_t nodes, int64_t levels, int n0,
+                                    int tc, int block, const double* omega, double* P) {
+    if (!D || !omega || !P || nodes <= 0 || ld < nodes || ld % 64 || tc < 0 || tc > n0 || tc > levels ||
+        (block != 16 && block != 32 && block != 64 && block != 128) || mode < 1 || mode > 4) {
+        set_error("bad arguments");
+        return FPDE_ERR_ARG;
+    }
+    bool split = mode >= 2;
+    if (split && tc % kGroup) { set_error("split modes need tc % 32 == 0"); return FPDE_ERR_ARG; }
+    int ks = gemm_max_splits(nodes);
+    double* part = nullptr;
+    void* comp[3] = {nullptr, nullptr, nullptr};
+    float* scale = nullptr;
+    fpde_status rc = FPDE_OK;
+    int splits = 0;
+    int64_t lv = ((levels + kGroup - 1) / kGroup) * kGroup;
+    hipError_t e = hipMalloc(&part, sizeof(double) * (size_t)ks * block * ld);
+    if (e == hipSuccess && split) {
+        for (int c = 0; c < 3 && e == hipSuccess; c++) e = hipMalloc(&comp[c], 2 * (size_t)lv * ld);
+        if (e == hipSuccess) e = hipMalloc(&scale, sizeof(float) * (size_t)(lv / kGroup) * ld);
+    }
+    if (e != hipSuccess) { rc = FPDE_ERR_NOMEM; set_error("hipMalloc failed"); }
+    if (rc == FPDE_OK && split) rc = launch_pack_split(mode, D, ld, (int)nodes, 0, tc / kGroup, comp[0], comp[1], comp[2], scale, 0);
+    if (rc == FPDE_OK) {
+        if (split) rc = launch_gemm_split(mode, comp[0], comp[1], comp[2], scale, ld, (int)nodes, n0, tc, block, omega, part, &splits, 0);
+        else rc = launch_gemm_f64(D, ld, (int)nodes, n0, tc, block, omega, part, &splits, 0);
+    }
+    if (rc == FPDE_OK) {
+        // P[r] = sum over K splits (combine kernel, one row at a time)
+        for (int r = 0; r < block && rc == FPDE_OK; r++)
+            rc = launch_hist_combine(P + (size_t)r * ld, part, splits, (int64_t)block * ld, (int64_t)r * ld, D, ld,
+                                     nodes, 0, 0, omega, nullptr, 0);
+    }
+    e = hipDeviceSynchronize();
+    if (rc == FPDE_OK && e != hipSuccess) { rc = FPDE_ERR_HIP; set_error(hipGetErrorString(e)); }
+    hipFree(part);
+    for (int c = 0; c < 3; c++) hipFree(comp[c]);
+    hipFree(scale);
+    return rc;
+}
+
 }  // extern "C"

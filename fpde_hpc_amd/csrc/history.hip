@@ -95,9 +95,10 @@ typedef _Float16 h8_t __attribute__((ext_vector_type(8)));
 
 // FP64 time-block GEMM.  Grid: x = 64-node tiles, y = K splits (t-chunks).
 // Block 256 = 4 waves; wave w owns nodes [16w,16w+16) of the tile and all BR
-// rows (BR/16 MFMA tiles).  v_mfma_f64_16x16x4f64 operand layout:
-// A lane l = A[l%16][l/16], B lane l = B[l/16][l%16], D lane l holds
-// D[4(l/16)+v][l%16], v=0..3.
+// rows (BR/16 MFMA tiles).  v_mfma_f64_16x16x4f64 operand layout (checked
+// on gfx950): A lane l = A[l%16][l/16], B lane l = B[l/16][l%16], and the
+// four results of lane l are D[4v + l/16][l%16], v=0..3 (rows interleaved,
+// unlike the f32-output MFMAs where lane l holds rows 4(l/16)+v).
 template <int BR>
 __global__ __launch_bounds__(256) void k_gemm_f64(const double* __restrict__ D, int64_t ld, int nodes,
                                                   int n0, int tc, int kchunk,
@@ -140,7 +141,7 @@ __global__ __launch_bounds__(256) void k_gemm_f64(const double* __restrict__ D, 
     for (int rt = 0; rt < RT; rt++) {
 #pragma unroll
         for (int v = 0; v < 4; v++) {
-            int r = rt * 16 + 4 * (lane >> 4) + v;
+            int r = rt * 16 + 4 * v + (lane >> 4);
             P[((int64_t)ks * BR + r) * ld + node] = acc[rt][v];
         }
     }
@@ -220,6 +221,7 @@ __global__ __launch_bounds__(256) void k_gemm_split(const T* __restrict__ c0, co
     constexpr int GB = 4;                                  // groups per LDS batch
     // afrag[comp][gb][rt][lane] : 8 elements each
     __shared__ vec_t afrag[NS][GB][RT][64];
+    __shared__ double ainv[GB][RT];                        // 1 / weight scale of a fragment
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
     const int ks = blockIdx.y;
     const int glo = (ks * kchunk) / kGroup;
@@ -238,12 +240,23 @@ __global__ __launch_bounds__(256) void k_gemm_split(const T* __restrict__ c0, co
             int g = gb0 + fgb;
             int r = frt * 16 + (fl & 15);
             int tbase = g * kGroup + 8 * (fl >> 4);
+            // fp16: power-of-two scale per fragment so that its largest weight
+            // sits in [2^14, 2^15) and the lo parts stay normal.  The weights
+            // decrease with the lag n0+r-t, so the extremes are at the corners.
+            double sa = 1.0;
+            if (SplitTraits<T>::scaled && g < ghi) {
+                int lag_lo = n0 + frt * 16 - (g * kGroup + kGroup - 1);
+                int lag_hi = n0 + frt * 16 + 15 - g * kGroup;
+                double wmax = fmax(fabs(omega[lag_lo - 1]), fabs(omega[lag_hi - 1]));
+                if (wmax > 0.0) sa = ldexp(1.0, 14 - ilogb(wmax));
+            }
+            if (fl == 0) ainv[fgb][frt] = 1.0 / sa;
             vec_t h, m, l;
 #pragma unroll
             for (int e = 0; e < 8; e++) {
                 double wv = 0.0;
                 int t = tbase + e;
-                if (g < ghi) wv = omega[n0 + r - t - 1];
+                if (g < ghi) wv = omega[n0 + r - t - 1] * sa;
                 T hh = (T)(float)wv;
                 double rr = wv - (double)(float)hh;
                 T mm = (T)(float)rr;
@@ -293,11 +306,12 @@ __global__ __launch_bounds__(256) void k_gemm_split(const T* __restrict__ c0, co
                         p2 = __builtin_amdgcn_mfma_f32_16x16x32_bf16(al, bh, p2, 0, 0, 0);
                     }
                 }
+                double scr = SplitTraits<T>::scaled ? sc * ainv[gi][rt] : 1.0;
 #pragma unroll
                 for (int v = 0; v < 4; v++) {
                     double s = (double)p0[v] + (double)p1[v];
                     if (NS == 3) s += (double)p2[v];
-                    acc[rt][v] = fma(sc, s, acc[rt][v]);
+                    acc[rt][v] = fma(scr, s, acc[rt][v]);
                 }
             }
         }

@@ -96,6 +96,16 @@ fpde_status fpde_reference_sum(double *S, const double *old, int64_t old_size,
 fpde_status fpde_history_sum_f64(double *S, const double *D, int64_t n, int64_t nodes,
                                  int64_t ld, const double *omega);
 
+/* Time-block history GEMM of the blocked solver paths, for B future steps:
+ * P[r*ld + i] = sum_{t<tc} omega[n0+r-t-1] * D[t*ld + i], r < block, i < nodes,
+ * computed in `mode` (1 FP64 MFMA, 2 split bf16, 3 split fp16+scaling,
+ * 4 split bf16 x3; split modes need tc % 32 == 0).  D: [levels][ld] device
+ * FP64 (levels >= tc rounded up to 32), omega: >= n0+block+512 entries, P:
+ * [block][ld].  Temporary split storage is allocated internally. */
+fpde_status fpde_history_block_gemm(int mode, const double *D, int64_t ld, int64_t nodes,
+                                    int64_t levels, int n0, int tc, int block,
+                                    const double *omega, double *P);
+
 /* ======================================================================
  * 3. Solver
  * ====================================================================== */

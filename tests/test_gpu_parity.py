@@ -115,6 +115,35 @@ def test_reference_sum_kernel_through_facade(fpde, oracle):
     assert np.max(np.abs(S - ref)) <= 1e-13 * np.max(np.abs(ref))
 
 
+@pytest.mark.parametrize("mode,tol", [(1, 1e-13), (2, 2e-5), (3, 5e-7), (4, 2e-7)])
+@pytest.mark.parametrize("block", [16, 64, 128])
+def test_block_gemm_matches_numpy(fpde, mode, tol, block):
+    """P[r,i] = sum_{t<tc} omega[n0+r-t-1] D[t,i]; tol is relative to sum |w||D|."""
+    L = fpde.lib()
+    rng = np.random.default_rng(mode * 100 + block)
+    nodes, ld, tc, n0 = 300, 320, 224, 230
+    levels = 256
+    D = np.zeros((levels, ld))
+    D[:tc, :nodes] = rng.standard_normal((tc, nodes)) * np.exp(rng.uniform(-8, 0, (tc, 1)))
+    omega = np.zeros(n0 + block + 600)
+    j = np.arange(1, n0 + block + 1)
+    omega[: n0 + block] = (j + 1.0) ** 0.3 - j ** 0.3
+    W = np.array([[omega[n0 + r - t - 1] for t in range(tc)] for r in range(block)])
+    ref = W @ D[:tc, :nodes]
+    scale = np.abs(W) @ np.abs(D[:tc, :nodes])
+    dv = Dev(L)
+    try:
+        bD, bw, bP = dv.put(D), dv.put(omega), dv.zeros(block * ld)
+        rc = L.fpde_history_block_gemm(mode, dv.ptr(bD), ld, nodes, levels, n0, tc, block, dv.ptr(bw), dv.ptr(bP))
+        assert rc == 0, L.fpde_last_error()
+        P = dv.get(bP, block * ld).reshape(block, ld)[:, :nodes]
+    finally:
+        dv.close()
+    err = np.max(np.abs(P - ref) / scale)
+    print(f"gemm mode {mode} block {block}: max err/scale {err:.3e}")
+    assert err <= tol
+
+
 @pytest.mark.parametrize("weight_mode", [0, 1])
 @pytest.mark.parametrize("do", [None, (8, 0), (6, 1)])
 def test_weights_match_oracle(fpde, oracle, weight_mode, do):
