@@ -79,6 +79,7 @@ SIGNATURES = [
     ("fpde_solver_create", ctypes.c_int, [ctypes.POINTER(Spec), ctypes.POINTER(_vp)]),
     ("fpde_solver_destroy", ctypes.c_int, [_vp]),
     ("fpde_solver_set_initial", ctypes.c_int, [_vp, _dp]),
+    ("fpde_solver_reset", ctypes.c_int, [_vp, _dp]),
     ("fpde_solver_advance", ctypes.c_int, [_vp, ctypes.c_int]),
     ("fpde_solver_sync", ctypes.c_int, [_vp]),
     ("fpde_solver_query", ctypes.c_int, [_vp]),

@@ -136,15 +136,31 @@ __global__ __launch_bounds__(256) void k_hist_combine(double* __restrict__ S, co
                                                       const double* __restrict__ corr) {
     int64_t i = (int64_t)blockIdx.x * 256 + threadIdx.x;
     if (i >= nodes) return;
+    constexpr int W = 16;      // independent loads per batch
     double far = 0.0;
-    for (int sp = 0; sp < nsplit; sp++) far += part[sp * part_stride + part_off + i];
-    double nr0 = 0.0, nr1 = 0.0;
-    int t = near_lo;
-    for (; t + 2 <= n; t += 2) {
-        nr0 = fma(omega[n - t - 1], D[(int64_t)t * ld + i], nr0);
-        nr1 = fma(omega[n - t - 2], D[(int64_t)(t + 1) * ld + i], nr1);
+    for (int sp0 = 0; sp0 < nsplit; sp0 += W) {
+        double v[W];
+#pragma unroll
+        for (int j = 0; j < W; j++) v[j] = (sp0 + j < nsplit) ? part[(sp0 + j) * part_stride + part_off + i] : 0.0;
+#pragma unroll
+        for (int j = 0; j < W; j++) far += v[j];
     }
-    for (; t < n; t++) nr0 = fma(omega[n - t - 1], D[(int64_t)t * ld + i], nr0);
+    double nr0 = 0.0, nr1 = 0.0;
+    for (int t0 = near_lo; t0 < n; t0 += W) {
+        double v[W], w[W];
+#pragma unroll
+        for (int j = 0; j < W; j++) {
+            int t = t0 + j;
+            bool ok = t < n;
+            v[j] = ok ? D[(int64_t)t * ld + i] : 0.0;
+            w[j] = ok ? omega[n - t - 1] : 0.0;
+        }
+#pragma unroll
+        for (int j = 0; j < W; j += 2) {
+            nr0 = fma(w[j], v[j], nr0);
+            nr1 = fma(w[j + 1], v[j + 1], nr1);
+        }
+    }
     double cr = 0.0;
     if (n >= 1) cr = corr[2 * n] * D[i];
     if (n >= 2) cr = fma(corr[2 * n + 1], D[ld + i], cr);

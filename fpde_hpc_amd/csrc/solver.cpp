@@ -113,7 +113,7 @@ struct fpde_solver {
     // GEMM block state
     int n0 = -1, tc = 0, gsplits = 0, packed_groups = 0;
     // profiling
-    bool prof = false;
+    int prof = 0;
     struct EvPair { hipEvent_t a, b; int cls; };
     std::vector<EvPair> evs;
     size_t ev_used = 0;
@@ -143,7 +143,8 @@ fpde_status dalloc(fpde_solver* s, void** p, size_t bytes) {
 
 // profiling event pairs: cls 0 = history kernels, 1 = step kernels
 void ev_begin(fpde_solver* s, int cls, size_t* slot) {
-    if (!s->prof) return;
+    *slot = (size_t)-1;
+    if (!(s->prof & (1 << cls))) return;
     if (s->ev_used == s->evs.size()) {
         fpde_solver::EvPair p;
         hipEventCreate(&p.a);
@@ -155,11 +156,10 @@ void ev_begin(fpde_solver* s, int cls, size_t* slot) {
     hipEventRecord(s->evs[*slot].a, s->stream);
 }
 void ev_end(fpde_solver* s, size_t slot) {
-    if (!s->prof) return;
+    if (slot == (size_t)-1) return;
     hipEventRecord(s->evs[slot].b, s->stream);
 }
 void ev_collect(fpde_solver* s) {
-    if (!s->prof) return;
     for (size_t i = 0; i < s->ev_used; i++) {
         float ms = 0.f;
         hipEventElapsedTime(&ms, s->evs[i].a, s->evs[i].b);
@@ -429,6 +429,20 @@ fpde_status fpde_solver_set_initial(fpde_solver* s, const double* u0) {
     return FPDE_OK;
 }
 
+fpde_status fpde_solver_reset(fpde_solver* s, const double* u0) {
+    if (!s || !u0) { set_error("null argument"); return FPDE_ERR_ARG; }
+    FPDE_HIP_TRY(hipStreamSynchronize(s->stream));
+    ev_collect(s);
+    s->step = 0;
+    s->launches = 0;
+    s->pending = false;
+    s->n0 = -1;
+    s->tc = 0;
+    s->gsplits = 0;
+    s->packed_groups = 0;
+    return fpde_solver_set_initial(s, u0);
+}
+
 fpde_status fpde_solver_advance(fpde_solver* s, int nsteps) {
     if (!s) { set_error("null solver"); return FPDE_ERR_ARG; }
     if (!s->initialized) { set_error("set_initial must be called first"); return FPDE_ERR_STATE; }
@@ -486,9 +500,14 @@ fpde_status fpde_solver_get_weights(fpde_solver* s, double* local, double* omega
     return FPDE_OK;
 }
 
-fpde_status fpde_solver_set_profiling(fpde_solver* s, int on) {
+fpde_status fpde_solver_set_profiling(fpde_solver* s, int mask) {
     if (!s) { set_error("null solver"); return FPDE_ERR_ARG; }
-    s->prof = on != 0;
+    FPDE_HIP_TRY(hipStreamSynchronize(s->stream));
+    ev_collect(s);
+    size_t keep = s->stats.device_bytes;
+    s->stats = fpde_stats{};          // counters restart with the new mask
+    s->stats.device_bytes = keep;
+    s->prof = mask;
     return FPDE_OK;
 }
 

@@ -24,48 +24,73 @@
 
 namespace fpde {
 
-
 constexpr int kMaxRed = 512;   // reduced rows (2C) per launch: N <= 65536
 
-// In-place parallel cyclic reduction on LDS rows [0,n), 256 threads, one RHS.
-// On return d[r] holds the solution.  Rows outside [0,n) act as identity.
-__device__ inline void pcr_block(double* a, double* b, double* c, double* d, int n) {
+// Parallel cyclic reduction on rows normalised to b = 1 (row r: a_r x_{r-1}
+// + x_r + c_r x_{r+1} = d_r), 256 threads, two rows per thread, ping-pong LDS
+// buffers so each level costs one barrier and one reciprocal per row.
+// Rows outside [0,n) are identity rows.  Returns the buffer holding x in D.
+__device__ inline int pcr_norm(double (*A)[kMaxRed], double (*Cc)[kMaxRed], double (*Dd)[kMaxRed], int n) {
     const int tid = threadIdx.x;
+    int p = 0;
     for (int delta = 1; delta < n; delta <<= 1) {
-        double na[2], nb[2], nc[2], nd[2];
 #pragma unroll
         for (int j = 0; j < 2; j++) {
             int r = tid + 256 * j;
             if (r < n) {
-                double ar = a[r], br = b[r], cr = c[r], dr = d[r];
-                double k1 = 0.0, k2 = 0.0, am = 0.0, cm = 0.0, dm = 0.0, ap = 0.0, cp = 0.0, dp = 0.0;
-                if (r - delta >= 0) { k1 = ar / b[r - delta]; am = a[r - delta]; cm = c[r - delta]; dm = d[r - delta]; }
-                if (r + delta < n) { k2 = cr / b[r + delta]; ap = a[r + delta]; cp = c[r + delta]; dp = d[r + delta]; }
-                na[j] = -k1 * am;
-                nc[j] = -k2 * cp;
-                nb[j] = br - k1 * cm - k2 * ap;
-                nd[j] = dr - k1 * dm - k2 * dp;
+                double ar = A[p][r], cr = Cc[p][r], dr = Dd[p][r];
+                double am = 0.0, cm = 0.0, dm = 0.0, ap = 0.0, cp = 0.0, dp = 0.0;
+                if (r - delta >= 0) { am = A[p][r - delta]; cm = Cc[p][r - delta]; dm = Dd[p][r - delta]; }
+                if (r + delta < n) { ap = A[p][r + delta]; cp = Cc[p][r + delta]; dp = Dd[p][r + delta]; }
+                double inv = 1.0 / (1.0 - ar * cm - cr * ap);
+                A[1 - p][r] = -ar * am * inv;
+                Cc[1 - p][r] = -cr * cp * inv;
+                Dd[1 - p][r] = (dr - ar * dm - cr * dp) * inv;
             }
         }
         __syncthreads();
+        p = 1 - p;
+    }
+    return p;
+}
+
+// Far partial sums plus the near levels t in [near_lo, n-2] already in
+// memory, gathered in 32-wide independent load batches (a sum over m terms
+// costs ~m/32 memory round trips).
+__device__ inline double hist_mem(const Step1DArgs& g, int i, int n) {
+    constexpr int W = 32;
+    double far = 0.0;
+    for (int sp0 = 0; sp0 < g.nsplit; sp0 += W) {
+        double v[W];
 #pragma unroll
-        for (int j = 0; j < 2; j++) {
-            int r = tid + 256 * j;
-            if (r < n) { a[r] = na[j]; b[r] = nb[j]; c[r] = nc[j]; d[r] = nd[j]; }
+        for (int j = 0; j < W; j++)
+            v[j] = (sp0 + j < g.nsplit) ? g.part[(int64_t)(sp0 + j) * g.part_stride + g.part_off + i] : 0.0;
+#pragma unroll
+        for (int j = 0; j < W; j++) far += v[j];
+    }
+    double nr0 = 0.0, nr1 = 0.0;
+    const int t_end = n - 1;   // exclusive: D[n-1] is the newest level, added by the caller
+    for (int t0 = g.near_lo; t0 < t_end; t0 += W) {
+        double v[W], w[W];
+#pragma unroll
+        for (int j = 0; j < W; j++) {
+            int t = t0 + j;
+            bool ok = t < t_end;
+            v[j] = ok ? g.D[(int64_t)t * g.ld + i] : 0.0;
+            w[j] = ok ? g.omega[n - t - 1] : 0.0;
         }
-        __syncthreads();
-    }
 #pragma unroll
-    for (int j = 0; j < 2; j++) {
-        int r = tid + 256 * j;
-        if (r < n) d[r] = d[r] / b[r];
+        for (int j = 0; j < W; j += 2) {
+            nr0 = fma(w[j], v[j], nr0);
+            nr1 = fma(w[j + 1], v[j + 1], nr1);
+        }
     }
-    __syncthreads();
+    return far + (nr0 + nr1);
 }
 
 __global__ __launch_bounds__(256) void k_step1d(Step1DArgs g) {
-    __shared__ double ra[kMaxRed], rb[kMaxRed], rc[kMaxRed], rd[kMaxRed];
-    __shared__ double pa[256], pb[256], pc[256], px[256], py[256], pz[256];
+    __shared__ double rA[2][kMaxRed], rC[2][kMaxRed], rD[2][kMaxRed];
+    __shared__ double pA[2][256], pC[2][256], pX[2][256], pY[2][256], pZ[2][256];
     __shared__ double sK[258];
     const int tid = threadIdx.x;
     const int c = blockIdx.x;
@@ -76,89 +101,93 @@ __global__ __launch_bounds__(256) void k_step1d(Step1DArgs g) {
     const int i = s + tid;
     const bool act = tid < nc;
     const int R2 = 2 * C;
+    const int n = g.n;
+    const bool first = g.k <= 0;          // first Picard iteration of step n (or finalize)
 
-    // ---- 1. finish the previous solve -------------------------------------
+    // ---- 0. loads that do not depend on the pending solve (one round trip) ---
+    double rrow[2][4];
+#pragma unroll
+    for (int j = 0; j < 2; j++) {
+        int r = tid + 256 * j;
+        bool ok = g.recover && r < R2;
+        const double* red = g.red + (size_t)(1 - g.parity) * R2 * 4 + 4 * r;
+#pragma unroll
+        for (int q = 0; q < 4; q++) rrow[j][q] = ok ? red[q] : 0.0;
+    }
+    const bool inner_node = act && i != s && i != e;
+    double fx = 0.0, fy = 0.0, fz = 0.0;
+    if (g.recover && inner_node) { fx = g.xyz[i]; fy = g.xyz[g.ld + i]; fz = g.xyz[2 * g.ld + i]; }
+    double uold = act ? g.U[i] : 0.0;                    // U^{n-1} (recovering, k=0) or U^n
+    double Sk = (!first && act) ? g.Sbuf[i] : 0.0;
+    double dmem = (first && !g.recover && act && n >= 1) ? g.D[(int64_t)(n - 1) * g.ld + i] : 0.0;
+    double d0m = (first && act && n >= 2) ? g.D[i] : 0.0;
+    double d1m = (first && act && n >= 3) ? g.D[g.ld + i] : 0.0;
+    double hm = (first && g.k == 0 && act && n >= 1) ? hist_mem(g, i, n) : 0.0;
+
+    // ---- 1. finish the previous solve: reduced system, then back-substitute --
     double val = 0.0;
+    int pr = 0;
     if (g.recover) {
-        const double* red = g.red + (size_t)(1 - g.parity) * R2 * 4;
-        for (int r = tid; r < R2; r += 256) {
-            ra[r] = red[4 * r + 0];
-            rb[r] = red[4 * r + 1];
-            rc[r] = red[4 * r + 2];
-            rd[r] = red[4 * r + 3];
+#pragma unroll
+        for (int j = 0; j < 2; j++) {
+            int r = tid + 256 * j;
+            if (r < R2) {
+                double ib = 1.0 / rrow[j][1];
+                rA[0][r] = rrow[j][0] * ib;
+                rC[0][r] = rrow[j][2] * ib;
+                rD[0][r] = rrow[j][3] * ib;
+            }
         }
         __syncthreads();
-        pcr_block(ra, rb, rc, rd, R2);
+        pr = pcr_norm(rA, rC, rD, R2);
         if (act) {
-            if (i == s) val = rd[2 * c];
-            else if (i == e) val = rd[2 * c + 1];
-            else {
-                val = g.xyz[i] + g.xyz[g.ld + i] * rd[2 * c] + g.xyz[2 * g.ld + i] * rd[2 * c + 1];
-            }
+            if (i == s) val = rD[pr][2 * c];
+            else if (i == e) val = rD[pr][2 * c + 1];
+            else val = fx + fy * rD[pr][2 * c] + fz * rD[pr][2 * c + 1];
         }
     }
 
     // ---- 2. advance the state -----------------------------------------------
     double un = 0.0, uit = 0.0, S = 0.0;
-    const int n = g.n;
-    if (g.k <= 0) {
+    if (first) {
         if (act) {
-            un = g.U[i];
-            double dnew = 0.0;
-            if (g.recover) {                     // val = U^n, un = U^{n-1}
-                dnew = val - un;
+            un = uold;
+            double dnew = dmem;
+            if (g.recover) {                     // val = U^n, uold = U^{n-1}
+                dnew = val - uold;
                 g.D[(int64_t)(n - 1) * g.ld + i] = dnew;
                 g.U[i] = val;
                 un = val;
-            } else if (n >= 1) {
-                dnew = g.D[(int64_t)(n - 1) * g.ld + i];
             }
-            if (g.k < 0) return;
-            if (n >= 1) {
-                // history term S_n: far partials + near levels + corrections
-                double far = 0.0;
-                for (int sp = 0; sp < g.nsplit; sp++) far += g.part[sp * g.part_stride + g.part_off + i];
-                double nr0 = 0.0, nr1 = 0.0;
-                int t = g.near_lo;
-                for (; t + 2 <= n - 1; t += 2) {
-                    nr0 = fma(g.omega[n - t - 1], g.D[(int64_t)t * g.ld + i], nr0);
-                    nr1 = fma(g.omega[n - t - 2], g.D[(int64_t)(t + 1) * g.ld + i], nr1);
-                }
-                for (; t < n - 1; t++) nr0 = fma(g.omega[n - t - 1], g.D[(int64_t)t * g.ld + i], nr0);
+            if (g.k >= 0 && n >= 1) {
+                // S_n = far + near (in memory) + newest level + two-level corrections
                 double newest = (g.near_lo <= n - 1) ? g.omega[0] * dnew : 0.0;
-                double d0 = (n - 1 == 0) ? dnew : g.D[i];
-                double cr = g.corr[2 * n] * d0;
-                if (n >= 2) {
-                    double d1 = (n - 1 == 1) ? dnew : g.D[g.ld + i];
-                    cr = fma(g.corr[2 * n + 1], d1, cr);
-                }
-                S = far + (nr0 + nr1) + newest + cr;
+                double cr = (n == 1) ? g.corr[2] * dnew : g.corr[2 * n] * d0m;
+                if (n == 2) cr = fma(g.corr[2 * n + 1], dnew, cr);
+                else if (n >= 3) cr = fma(g.corr[2 * n + 1], d1m, cr);
+                S = hm + newest + cr;
             }
-            g.Sbuf[i] = S;
-        } else if (g.k < 0) {
-            return;
+            if (g.k >= 0) g.Sbuf[i] = S;
         }
         uit = un;
-    } else {
-        if (act) {
-            un = g.U[i];
-            S = g.Sbuf[i];
-            uit = val;
-        }
+    } else if (act) {
+        un = uold;
+        S = Sk;
+        uit = val;
     }
-    if (g.k < 0) return;
+    if (g.k < 0) return;                         // finalize only (uniform)
 
     // ---- 3. coefficients at the iterate ------------------------------------
     double K = 0.0, invC = 0.0;
+    if (tid == 0 && c > 0) {                     // K of the node left of the chunk
+        double ulo = g.recover ? rD[pr][2 * c - 1] : g.U[s - 1];
+        double iCl;
+        vgm_coefs(ulo, (s - 1) * g.h, g.ksat, g.coef_mode, K, iCl);
+        sK[0] = K;
+    }
     if (act) {
         vgm_coefs(uit, i * g.h, g.ksat, g.coef_mode, K, invC);
         sK[tid + 1] = K;
-    }
-    if (tid == 0 && c > 0) {
-        double ulo = g.recover ? rd[2 * c - 1] : g.U[s - 1];
-        double Kl, iCl;
-        vgm_coefs(ulo, (s - 1) * g.h, g.ksat, g.coef_mode, Kl, iCl);
-        sK[0] = Kl;
     }
     __syncthreads();
 
@@ -176,59 +205,55 @@ __global__ __launch_bounds__(256) void k_step1d(Step1DArgs g) {
         }
     }
 
-    // ---- 5. chunk elimination: interior rows 1..nc-2 -------------------------
+    // ---- 5. chunk elimination: interior rows 1..nc-2, three right-hand sides
+    // (x: the row's own RHS; y, z: unit values at the chunk ends s and e) ---
     const int ni = nc - 2;
     const int q = tid - 1;
     const bool inner = (tid >= 1) && (tid <= nc - 2);
     if (inner) {
-        pa[q] = (tid == 1) ? 0.0 : ar;
-        pc[q] = (tid == nc - 2) ? 0.0 : cr;
-        pb[q] = br;
-        px[q] = dr;
-        py[q] = (tid == 1) ? -ar : 0.0;
-        pz[q] = (tid == nc - 2) ? -cr : 0.0;
+        double ib = 1.0 / br;
+        pA[0][q] = (tid == 1) ? 0.0 : ar * ib;
+        pC[0][q] = (tid == nc - 2) ? 0.0 : cr * ib;
+        pX[0][q] = dr * ib;
+        pY[0][q] = (tid == 1) ? -ar * ib : 0.0;
+        pZ[0][q] = (tid == nc - 2) ? -cr * ib : 0.0;
     }
     __syncthreads();
+    int pp = 0;
     for (int delta = 1; delta < ni; delta <<= 1) {
-        double na = 0, nb = 1, ncc = 0, nx = 0, ny = 0, nz = 0;
         if (inner) {
-            double k1 = 0, k2 = 0, am = 0, cm = 0, xm = 0, ym = 0, zm = 0, ap = 0, cp = 0, xp = 0, yp = 0, zp = 0;
-            double aq = pa[q], cq = pc[q];
+            double aq = pA[pp][q], cq = pC[pp][q];
+            double am = 0, cm = 0, xm = 0, ym = 0, zm = 0, ap = 0, cp = 0, xp = 0, yp = 0, zp = 0;
             if (q - delta >= 0) {
-                k1 = aq / pb[q - delta];
-                am = pa[q - delta]; cm = pc[q - delta]; xm = px[q - delta]; ym = py[q - delta]; zm = pz[q - delta];
+                am = pA[pp][q - delta]; cm = pC[pp][q - delta];
+                xm = pX[pp][q - delta]; ym = pY[pp][q - delta]; zm = pZ[pp][q - delta];
             }
             if (q + delta < ni) {
-                k2 = cq / pb[q + delta];
-                ap = pa[q + delta]; cp = pc[q + delta]; xp = px[q + delta]; yp = py[q + delta]; zp = pz[q + delta];
+                ap = pA[pp][q + delta]; cp = pC[pp][q + delta];
+                xp = pX[pp][q + delta]; yp = pY[pp][q + delta]; zp = pZ[pp][q + delta];
             }
-            na = -k1 * am;
-            ncc = -k2 * cp;
-            nb = pb[q] - k1 * cm - k2 * ap;
-            nx = px[q] - k1 * xm - k2 * xp;
-            ny = py[q] - k1 * ym - k2 * yp;
-            nz = pz[q] - k1 * zm - k2 * zp;
+            double inv = 1.0 / (1.0 - aq * cm - cq * ap);
+            pA[1 - pp][q] = -aq * am * inv;
+            pC[1 - pp][q] = -cq * cp * inv;
+            pX[1 - pp][q] = (pX[pp][q] - aq * xm - cq * xp) * inv;
+            pY[1 - pp][q] = (pY[pp][q] - aq * ym - cq * yp) * inv;
+            pZ[1 - pp][q] = (pZ[pp][q] - aq * zm - cq * zp) * inv;
         }
         __syncthreads();
-        if (inner) { pa[q] = na; pb[q] = nb; pc[q] = ncc; px[q] = nx; py[q] = ny; pz[q] = nz; }
-        __syncthreads();
+        pp = 1 - pp;
     }
     if (inner) {
-        double ib = 1.0 / pb[q];
-        double x = px[q] * ib, y = py[q] * ib, z = pz[q] * ib;
-        g.xyz[i] = x;
-        g.xyz[g.ld + i] = y;
-        g.xyz[2 * g.ld + i] = z;
-        px[q] = x; py[q] = y; pz[q] = z;
+        g.xyz[i] = pX[pp][q];
+        g.xyz[g.ld + i] = pY[pp][q];
+        g.xyz[2 * g.ld + i] = pZ[pp][q];
     }
-    __syncthreads();
 
     // ---- 6. reduced rows of the chunk ends ----------------------------------
     double* redw = g.red + (size_t)g.parity * R2 * 4;
     if (tid == 0) {
         double x, y, z;
         if (nc == 2) { x = 0.0; y = 0.0; z = 1.0; }
-        else { x = px[0]; y = py[0]; z = pz[0]; }
+        else { x = pX[pp][0]; y = pY[pp][0]; z = pZ[pp][0]; }
         redw[4 * (2 * c) + 0] = ar;
         redw[4 * (2 * c) + 1] = br + cr * y;
         redw[4 * (2 * c) + 2] = cr * z;
@@ -237,7 +262,7 @@ __global__ __launch_bounds__(256) void k_step1d(Step1DArgs g) {
     if (tid == nc - 1) {
         double x, y, z;
         if (nc == 2) { x = 0.0; y = 1.0; z = 0.0; }
-        else { x = px[ni - 1]; y = py[ni - 1]; z = pz[ni - 1]; }
+        else { x = pX[pp][ni - 1]; y = pY[pp][ni - 1]; z = pZ[pp][ni - 1]; }
         redw[4 * (2 * c + 1) + 0] = ar * y;
         redw[4 * (2 * c + 1) + 1] = br + ar * z;
         redw[4 * (2 * c + 1) + 2] = cr;

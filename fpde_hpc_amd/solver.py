@@ -79,6 +79,12 @@ class Solver:
             raise ValueError("u0 has the wrong size")
         check(lib().fpde_solver_set_initial(self._h, _p(u0)))
 
+    def reset(self, u0):
+        u0 = np.ascontiguousarray(u0, dtype=np.float64)
+        if u0.size != self.unknowns:
+            raise ValueError("u0 has the wrong size")
+        check(lib().fpde_solver_reset(self._h, _p(u0)))
+
     def advance(self, nsteps=0):
         check(lib().fpde_solver_advance(self._h, int(nsteps)))
 
@@ -102,8 +108,9 @@ class Solver:
         check(lib().fpde_solver_get_weights(self._h, ctypes.byref(local), _p(omega), _p(corr)))
         return local.value, omega, corr.reshape(M, 2)
 
-    def set_profiling(self, on=True):
-        check(lib().fpde_solver_set_profiling(self._h, 1 if on else 0))
+    def set_profiling(self, mask=3):
+        """bit 0: history kernels, bit 1: step kernels (hipEvents per launch)."""
+        check(lib().fpde_solver_set_profiling(self._h, int(mask)))
 
     def stats(self):
         st = Stats()

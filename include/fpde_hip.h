@@ -152,6 +152,8 @@ fpde_status fpde_solver_create(const fpde_spec *spec, fpde_solver **out);
 fpde_status fpde_solver_destroy(fpde_solver *s);
 /* u0: host array of nodes (1D) or nodes^3 (3D, depth index fastest). */
 fpde_status fpde_solver_set_initial(fpde_solver *s, const double *u0);
+/* Restart the solve from u0 at step 0, keeping every device allocation. */
+fpde_status fpde_solver_reset(fpde_solver *s, const double *u0);
 /* Enqueue nsteps steps (or all remaining if nsteps<=0); asynchronous. */
 fpde_status fpde_solver_advance(fpde_solver *s, int nsteps);
 fpde_status fpde_solver_sync(fpde_solver *s);
@@ -163,8 +165,9 @@ int fpde_solver_step(const fpde_solver *s);
 fpde_status fpde_solver_get_solution(fpde_solver *s, double *u_out);
 /* Effective weights (local, omega[M], corr[2M]) the solver uses (host copies). */
 fpde_status fpde_solver_get_weights(fpde_solver *s, double *local, double *omega, double *corr);
-/* Enable per-kernel timing (hipEvents around each launch; slows the run). */
-fpde_status fpde_solver_set_profiling(fpde_solver *s, int on);
+/* Per-kernel timing with hipEvents on the solver's stream: bit 0 = history
+ * kernels (plain stream / pack+GEMM / combine), bit 1 = step kernels. */
+fpde_status fpde_solver_set_profiling(fpde_solver *s, int mask);
 fpde_status fpde_solver_get_stats(fpde_solver *s, fpde_stats *out);
 
 /* Initial conditions of the workloads (host; same draws as the oracle's). */
