@@ -237,7 +237,11 @@ fpde_status far_part(fpde_solver* s, int n, int* nsplit, int64_t* stride, int64_
         return FPDE_OK;
     }
     int B = sp.block;
-    if (s->n0 < 0 || n - s->n0 >= B) {
+    // A block starts where the levels in memory (t < avail) form a multiple of
+    // B: the GEMM cut tc = avail is then group aligned and the near field of
+    // step n0+r is at most r+1 levels (1D: avail = n-1, since D[n-1] is made by
+    // the step launch itself; 3D: avail = n).
+    if (s->n0 < 0 || n - s->n0 >= B || (avail > 0 && avail % B == 0 && n > s->n0)) {
         s->n0 = n;
         int tcm = std::max(avail, 0);
         int tc = is_split(sp.hist_mode) ? (tcm / kGroup) * kGroup : tcm;

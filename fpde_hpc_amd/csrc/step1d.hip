@@ -55,8 +55,9 @@ __device__ inline int pcr_norm(double (*A)[kMaxRed], double (*Cc)[kMaxRed], doub
 }
 
 // Far partial sums plus the near levels t in [near_lo, n-2] already in
-// memory, gathered in 32-wide independent load batches (a sum over m terms
-// costs ~m/32 memory round trips).
+// memory, gathered in 32-wide batches of independent loads (a sum over m
+// terms costs ~m/32 memory round trips).  Every load is guarded: omega is
+// read only at n-t-1 >= 1.
 __device__ inline double hist_mem(const Step1DArgs& g, int i, int n) {
     constexpr int W = 32;
     double far = 0.0;

@@ -115,6 +115,18 @@ def test_reference_sum_kernel_through_facade(fpde, oracle):
     assert np.max(np.abs(S - ref)) <= 1e-13 * np.max(np.abs(ref))
 
 
+def test_reference_style_cpp_caller_through_facade():
+    """The fr_ocl_call_sum call sequence compiled against fpde_opencl_facade.h."""
+    import os
+    import subprocess
+    exe = os.path.join(os.path.dirname(__file__), "facade", "_build", "reference_style_caller")
+    assert os.path.exists(exe), "build it with __graft_entry__.build()"
+    for args in (["1000", "70"], ["37", "20"]):
+        r = subprocess.run([exe] + args, capture_output=True, text=True, timeout=60)
+        print(r.stdout, r.stderr)
+        assert r.returncode == 0
+
+
 @pytest.mark.parametrize("mode,tol", [(1, 1e-13), (2, 2e-5), (3, 5e-7), (4, 2e-7)])
 @pytest.mark.parametrize("block", [16, 64, 128])
 def test_block_gemm_matches_numpy(fpde, mode, tol, block):

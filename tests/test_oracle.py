@@ -1,0 +1,265 @@
+"""Pins the FP64 CPU oracle (oracle/fpde_oracle.c) against what the
+mathematics and independent libraries fix: known-answer vectors, closed forms,
+invariants, brute force and library routines.  Each check is chosen so that
+a dropped term, a wrong sign or an index shift in the oracle fails it."""
+import math
+
+import numpy as np
+import pytest
+from scipy import integrate
+from scipy.special import gamma as G
+
+
+# ---------------------------------------------------------------- MT19937-64
+def test_mt19937_64_known_answer(oracle):
+    # C++ standard [rand.predef]: the 10000th invocation of a default-
+    # constructed std::mt19937_64 (seed 5489) produces 9981545732273789042.
+    assert oracle.mt64_stream(5489, 10000)[-1] == 9981545732273789042
+
+
+def test_uniform_draws_are_canonical_form(oracle):
+    raw = oracle.mt64_stream(1001, 6)
+    u = oracle.mt64_uniforms(1001, [(0.0, 0.05), (0.0, 2 * math.pi)] * 3)
+    for x, (a, b), v in zip(raw, [(0.0, 0.05), (0.0, 2 * math.pi)] * 3, u):
+        assert v == (float(x) / 2.0 ** 64) * (b - a) + a
+
+
+# ---------------------------------------------------------------- L1 weights
+@pytest.mark.parametrize("alpha", [0.1, 0.5, 0.7, 0.95])
+def test_closed_weight_is_the_integral(oracle, alpha):
+    for t0, t1, a in [(0.0, 0.1, 0.3), (0.2, 0.25, 0.25), (1.0, 1.5, 4.0)]:
+        ref, _ = integrate.quad(lambda s: (a - s) ** (-alpha), t0, t1, limit=200)
+        assert abs(oracle.kb_closed(alpha, t0, t1, a) - ref) <= 1e-9 * abs(ref)
+
+
+def test_closed_weights_telescope(oracle):
+    # sum_{t<=n} b(t,t+1;n+1) = T_{n+1}^(1-a)/(1-a)
+    tau = 1e-3
+    for alpha in (0.3, 0.8):
+        for n in (1, 10, 500):
+            T = oracle.time_grid(n + 1, tau)
+            kkk0, kb3 = oracle.l1_weights(alpha, n, T, weight_mode=0)
+            lhs = kb3.sum() + kkk0
+            rhs = T[n + 1] ** (1 - alpha) / (1 - alpha)
+            assert abs(lhs - rhs) <= 1e-11 * rhs
+
+
+def test_alpha_one_limit(oracle):
+    T = oracle.time_grid(5, 1e-3)
+    kkk0, kb3 = oracle.l1_weights(1.0, 4, T)
+    assert kkk0 == 1.0 and np.all(kb3 == 0.0)
+
+
+@pytest.mark.parametrize("alpha", [0.3, 0.7, 0.9])
+def test_series_weight_converges_to_closed_form(oracle, alpha):
+    tau = 1e-3
+    for n in (1, 2, 7, 300):
+        for t0 in (0, 1):
+            if t0 >= n:
+                continue
+            T = np.arange(n + 2) * tau
+            v, _ = oracle.kb_series(alpha, T[t0], T[t0 + 1], T[n + 1], eps=1e-16)
+            c = oracle.kb_closed(alpha, T[t0], T[t0 + 1], T[n + 1])
+            assert abs(v - c) <= 1e-12 * c
+
+
+def _series_terms(alpha, p0, p1, pa, count):
+    # term_i = (alpha)_i / i! * pa^-alpha * (p1^(i+1) - p0^(i+1)) / ((i+1) pa^i)
+    out = []
+    for i in range(count):
+        poch = math.gamma(alpha + i) / math.gamma(alpha) / math.factorial(i)
+        out.append(poch * pa ** (-alpha) * (p1 ** (i + 1) - p0 ** (i + 1)) / ((i + 1) * pa ** i))
+    return out
+
+
+@pytest.mark.parametrize("alpha", [0.5, 0.8])
+def test_series_stopping_rule_matches_direct_terms(oracle, alpha):
+    """_kb_row2 adds terms until the first one with |term| <= eps (absolute)."""
+    tau, eps = 1e-3, 1e-5
+    for n, t0 in [(1, 0), (2, 0), (2, 1), (9, 1), (50, 0)]:
+        p0, p1, pa = t0 * tau, (t0 + 1) * tau, (n + 1) * tau
+        terms = _series_terms(alpha, p0, p1, pa, 80)
+        k = next(i for i, t in enumerate(terms) if abs(t) <= eps)
+        expect = math.fsum(terms[: k + 1])
+        v, _ = oracle.kb_series(alpha, p0, p1, pa, eps=eps)
+        assert abs(v - expect) <= 1e-12 * abs(expect)
+
+
+def test_reference_dispatch_uses_series_for_two_oldest(oracle):
+    T = oracle.time_grid(12, 1e-3)
+    n = 10
+    for t0 in range(n):
+        w = oracle.lib().orc_kb(0.7, t0, n + 1, T.ctypes.data_as(oracle._dp), 1, 1e-5)
+        if t0 <= 1:
+            s, _ = oracle.kb_series(0.7, T[t0], T[t0 + 1], T[n + 1])
+            assert w == s
+        else:
+            assert w == oracle.kb_closed(0.7, T[t0], T[t0 + 1], T[n + 1])
+
+
+# ---------------------------------------------------------------- history sum
+def test_time_summing_brute_force(oracle):
+    rng = np.random.default_rng(3)
+    levels, nodes = 9, 7
+    old = rng.standard_normal((levels, nodes))
+    kb = rng.random(levels - 1)
+    tau = rng.random(levels) + 0.5
+    S = oracle.time_summing(old, kb, tau)
+    ref = np.zeros(nodes)
+    for i in range(nodes):
+        for t in range(levels - 2, -1, -1):
+            ref[i] += kb[t] * (old[t + 1, i] - old[t, i]) / tau[t]
+    assert np.allclose(S, ref, rtol=1e-14, atol=0)
+
+
+@pytest.mark.parametrize("alpha", [0.2, 0.6, 0.9])
+def test_l1_exact_for_linear_in_time(oracle, alpha):
+    # D^a (c t) = c t^(1-a)/Gamma(2-a): the L1 scheme is exact for linear u
+    tau, n, c = 1e-2, 40, 3.0
+    T = oracle.time_grid(n + 1, tau)
+    kkk0, kb3 = oracle.l1_weights(alpha, n, T, weight_mode=0)
+    old = (c * T[: n + 1])[:, None] * np.ones((1, 2))
+    hist = oracle.time_summing(old, kb3, np.full(n + 1, tau))[0]
+    Da = 1.0 / G(1 - alpha)
+    approx = Da * (hist + kkk0 * c * tau / tau)
+    exact = c * T[n + 1] ** (1 - alpha) / G(2 - alpha)
+    assert abs(approx - exact) <= 1e-11 * exact
+
+
+@pytest.mark.parametrize("alpha", [0.3, 0.7])
+def test_l1_converges_at_order_2_minus_alpha(oracle, alpha):
+    errs = []
+    for n in (50, 100, 200):
+        tau = 1.0 / (n + 1)
+        T = oracle.time_grid(n + 1, tau)
+        kkk0, kb3 = oracle.l1_weights(alpha, n, T, weight_mode=0)
+        u = T ** 2
+        hist = oracle.time_summing(u[: n + 1, None], kb3, np.full(n + 1, tau))[0]
+        Da = 1.0 / G(1 - alpha)
+        approx = Da * (hist + kkk0 * (u[n + 1] - u[n]) / tau)
+        exact = 2 * T[n + 1] ** (2 - alpha) / G(3 - alpha)
+        errs.append(abs(approx - exact))
+    r1, r2 = errs[0] / errs[1], errs[1] / errs[2]
+    assert abs(math.log2(r2) - (2 - alpha)) < 0.1 and abs(math.log2(r1) - (2 - alpha)) < 0.15
+
+
+# ---------------------------------------------------------------- VGM soil
+def test_inverse_capacity_is_derivative_of_wetness(oracle):
+    for P in (0.05, 0.5, 1.0, 2.0, 7.0):
+        d = 1e-6 * P
+        dw = (oracle.wetness(P + d) - oracle.wetness(P - d)) / (2 * d)   # d theta / dP = -C
+        assert abs(1.0 / oracle.inv_dw_dh(P) + dw) <= 1e-6 * abs(dw)
+
+
+def test_vgm_limits(oracle):
+    assert oracle.wetness(-0.1) == 0.55 and oracle.wetness(0.0) == 0.55
+    assert oracle.inv_dw_dh(0.0) == 1e20
+    assert abs(oracle.kr(1e-40) - 1.0) < 1e-6          # Kr -> 1 like x^(1-1/n) at saturation
+    ks = [oracle.kr(P) for P in (0.1, 0.5, 1.0, 2.0, 5.0)]
+    assert all(0 < k < 1 for k in ks) and all(a > b for a, b in zip(ks, ks[1:]))
+
+
+# ---------------------------------------------------------------- linear algebra
+def test_thomas_matches_dense_solve(oracle):
+    rng = np.random.default_rng(5)
+    for n in (2, 5, 64, 301):
+        a = rng.random(n); c = rng.random(n); b = -(a + c + rng.random(n) + 0.1)
+        a[0] = 0.0; c[-1] = 0.0
+        d = rng.standard_normal(n)
+        M = np.diag(b) + np.diag(a[1:], -1) + np.diag(c[:-1], 1)
+        assert np.allclose(oracle.thomas(a, b, c, d), np.linalg.solve(M, d), rtol=1e-12, atol=1e-14)
+
+
+@pytest.mark.parametrize("n", [4, 16, 32])
+def test_gauss_legendre_matches_numpy(oracle, n):
+    x, w = oracle.gauss_legendre(n, 0.1, 1.0)
+    xr, wr = np.polynomial.legendre.leggauss(n)
+    assert np.allclose(x, 0.55 + 0.45 * xr, atol=1e-14)
+    assert np.allclose(w, 0.45 * wr, atol=1e-14)
+
+
+# ---------------------------------------------------------------- solvers
+def _mittag_leffler(alpha, z, terms=120):
+    return sum(z ** k / math.gamma(alpha * k + 1) for k in range(terms))
+
+
+def test_1d_linear_solve_converges_to_mittag_leffler(oracle):
+    """K=1, 1/C=1: D^a u = u_zz on [0,1], u(0)=u_top, u_z(1)=0,
+    u0 = u_top + A sin(pi z/2)  ->  u = u_top + A E_a(-(pi/2)^2 t^a) sin(pi z/2)."""
+    alpha, A, ut, T = 0.6, 0.1, -1.0, 0.2
+    errs = []
+    for nodes, steps in ((81, 100), (161, 200)):
+        z = np.linspace(0, 1, nodes)
+        u0 = ut + A * np.sin(np.pi * z / 2)
+        p = oracle.make_params(nodes, steps, T / steps, alpha, coef_mode=1, weight_mode=0)
+        u = oracle.solve_1d(p, u0)
+        exact = ut + A * _mittag_leffler(alpha, -(np.pi / 2) ** 2 * T ** alpha) * np.sin(np.pi * z / 2)
+        errs.append(np.max(np.abs(u - exact)))
+    assert errs[1] < 0.6 * errs[0] and errs[1] < 2e-3
+
+
+def test_vgm_step_satisfies_discrete_equation(oracle):
+    """One reference step: the result satisfies the tridiagonal rows built
+    from the lagged coefficients (independent residual check)."""
+    nodes, tau, alpha = 50, 1e-3, 0.8
+    u0 = oracle.init_1d(nodes, 3)
+    p = oracle.make_params(nodes, 1, tau, alpha, u_top=u0[0])
+    u1 = oracle.solve_1d(p, u0)
+    h = 1.0 / (nodes - 1)
+    z = np.arange(nodes) * h
+    K = np.array([p.ksat * oracle.kr(zz - uu) for zz, uu in zip(z, u0)])
+    iC = np.array([oracle.inv_dw_dh(zz - uu) for zz, uu in zip(z, u0)])
+    L = (1 / G(1 - alpha)) * tau ** (1 - alpha) / (1 - alpha) / tau
+    for i in range(1, nodes - 1):
+        lhs = (K[i - 1] * iC[i] * u1[i - 1] - iC[i] * (K[i] + K[i - 1]) * u1[i] + K[i] * iC[i] * u1[i + 1]) / h ** 2 \
+            - L * u1[i]
+        assert abs(lhs - (-L * u0[i])) <= 1e-9 * L
+    assert u1[0] == u0[0]
+    assert abs(u1[-1] - u1[-2]) <= 1e-12
+
+
+def test_3d_with_laterally_uniform_field_equals_1d(oracle):
+    """With field-independent coefficients a laterally uniform 3D field
+    follows the 1D scheme along depth (the y sweep adds the history term, the
+    x sweep is an identity).  With VGM coefficients the two differ at O(H/L)
+    because the depth sweep evaluates its coefficients after the y sweep."""
+    n, steps = 10, 12
+    col = oracle.init_1d(n, 7)
+    u3 = np.tile(col, n * n)            # z fastest
+    p3 = oracle.make_params(n, steps, 1e-3, 0.85, coef_mode=1)
+    p1 = oracle.make_params(n, steps, 1e-3, 0.85, coef_mode=1)
+    r3 = oracle.solve_3d(p3, u3).reshape(n * n, n)
+    r1 = oracle.solve_1d(p1, col)
+    assert np.max(np.abs(r3 - r1[None, :])) <= 1e-12
+
+
+def test_distributed_order_derivative_closed_form(oracle):
+    """int_{0.1}^{1} Gamma(3-a)/2 * D^a t^2 da = (t^1.9 - t)/ln t (at t = 1: 0.9)."""
+    M, tau = 400, 1.0 / 400
+    for rule, q, tol in ((0, 32, 2e-3), (1, 65, 4e-3)):
+        p = oracle.make_params(5, M, tau, do_q=q, do_rule=rule, do_a0=0.1, do_a1=1.0, weight_mode=0,
+                               ksat=1.0)
+        local, omega, corr = oracle.folded_weights(p)
+        T = np.arange(M + 1) * tau
+        u = T ** 2
+        n = M - 1                        # derivative at T[M] = 1
+        d = np.diff(u)
+        approx = local * d[n] + sum(omega[n - t - 1] * d[t] for t in range(n))
+        exact = 0.9                      # limit of (t^1.9 - t)/ln t at t = 1
+        assert abs(approx - exact) <= tol * exact
+
+
+def test_folded_weights_equal_per_order_sums(oracle):
+    M, tau = 60, 1e-3
+    p = oracle.make_params(5, M, tau, do_q=6, do_rule=0, weight_mode=1, ksat=1.0)
+    local, omega, corr = oracle.folded_weights(p)
+    x, w = oracle.gauss_legendre(6, 0.1, 1.0)
+    T = np.arange(M + 2) * tau
+    n = M - 1
+    for t in range(n):
+        ref = 0.0
+        for a, wq in zip(x, w):
+            kb = oracle.lib().orc_kb(a, t, n + 1, T.ctypes.data_as(oracle._dp), 1, 1e-5)
+            ref += wq * G(3 - a) / 2 / G(1 - a) * kb / tau
+        got = omega[n - t - 1] + (corr[n, t] if t < 2 else 0.0)
+        assert abs(got - ref) <= 1e-12 * abs(ref)
