@@ -61,7 +61,7 @@ def run_variant(F, workload, mode, block, steps, warmup, prof_mask_timed=None, n
     return dict(spec=spec, u=u_last, u_first=u, times=times, prof=prof, timed=timed_stats)
 
 
-def cpu_baseline(N, M, seconds_hint=20.0):
+def cpu_baseline(N, M, seconds_hint=20.0, alpha=0.7, tau=5e-4, seed=1001):
     """FP64 oracle (oracle/, the reference loop order: node-outer, t-inner,
     OpenMP over nodes) timed on this host: the history sum at the run-average
     history length n = M/2 over all N nodes, plus one oracle step (VGM
@@ -74,21 +74,21 @@ def cpu_baseline(N, M, seconds_hint=20.0):
     n = M // 2
     rng = np.random.default_rng(0)
     old = np.cumsum(rng.standard_normal((n + 1, N)) * 1e-4, axis=0)
-    Time = O.time_grid(n + 1, 5e-4)
-    _, kb = O.l1_weights(0.7, n, Time, weight_mode=1)
-    tau = np.full(n + 2, 5e-4)
+    Time = O.time_grid(n + 1, tau)
+    _, kb = O.l1_weights(alpha, n, Time, weight_mode=1)
+    taus = np.full(n + 2, tau)
     reps = []
     t_start = time.perf_counter()
     while len(reps) < 3 or (time.perf_counter() - t_start < seconds_hint and len(reps) < 10):
         t0 = time.perf_counter()
-        O.time_summing(old, kb, tau, nthreads=cores)
+        O.time_summing(old, kb, taus, nthreads=cores)
         reps.append(time.perf_counter() - t0)
         if time.perf_counter() - t_start > seconds_hint * 2:
             break
     t_sum = statistics.median(reps)
     del old
-    u0 = O.init_1d(N, 1001)
-    p = O.make_params(N, 4, 5e-4, 0.7, u_top=u0[0], nthreads=cores)
+    u0 = O.init_1d(N, seed)
+    p = O.make_params(N, 4, tau, alpha, u_top=u0[0], nthreads=cores)
     t0 = time.perf_counter()
     O.solve_1d(p, u0)
     t_step = (time.perf_counter() - t0) / 4
@@ -100,6 +100,67 @@ def cpu_baseline(N, M, seconds_hint=20.0):
         "sample": (f"oracle time_summing over n={n} levels x N={N} nodes (run-average history "
                    f"length for M={M}), {len(reps)} reps median {t_sum:.3f}s, + oracle step "
                    f"{t_step * 1e3:.2f} ms; extrapolated to the whole run"),
+    }
+
+
+def serve_mix(F, n_req, rate, seed=1004, quantum=64, budget_gb=200.0, max_active=16, hist_mode="split_f16"):
+    """The serving mix: Poisson arrivals at `rate` req/s; 80% 1D (N~U[500,20000],
+    M~U[1000,40000]), 20% 3D (32^3..96^3, M~U[200,1000]); alpha~U(0.3,0.95);
+    25% distributed order (16 Gauss-Legendre nodes on [0.1,1]); seeded heads."""
+    import numpy as np
+    rng = np.random.default_rng(seed)
+    reqs = []
+    t = 0.0
+    for i in range(n_req):
+        t += rng.exponential(1.0 / rate)
+        is3d = rng.random() < 0.2
+        do = rng.random() < 0.25
+        alpha = float(rng.uniform(0.3, 0.95))
+        if is3d:
+            n = int(rng.integers(32, 97))
+            M = int(rng.integers(200, 1001))
+            u0 = F.init_3d(n, 100000 + i)
+            kw = dict(dims=3, block=16)
+        else:
+            n = int(rng.integers(500, 20001))
+            M = int(rng.integers(1000, 40001))
+            u0 = F.init_1d(n, 100000 + i)
+            kw = dict(dims=1, block=64)
+        if do:
+            kw.update(do_q=16, do_rule=0, do_a0=0.1, do_a1=1.0)
+        tau = 1e-3 if is3d else 5e-4
+        spec = F.make_spec(n, M, tau, alpha=alpha, hist_mode=hist_mode, u_top=u0[0], **kw)
+        work = (n ** 3 if is3d else n) * M
+        reqs.append((t, spec, u0, work))
+    srv = F.Server(mem_budget_bytes=int(budget_gb * 2 ** 30), max_active=max_active)
+    ids = []
+    t0 = time.perf_counter()
+    k = 0
+    while k < len(reqs) or srv.poll(quantum) > 0:
+        now = time.perf_counter() - t0
+        while k < len(reqs) and reqs[k][0] <= now:
+            ids.append((srv.submit(reqs[k][1], reqs[k][2]), reqs[k][3]))
+            k += 1
+        if k < len(reqs) and srv.poll(quantum) == 0:
+            time.sleep(max(0.0, min(0.01, reqs[k][0] - (time.perf_counter() - t0))))
+    makespan = time.perf_counter() - t0
+    lat, done_work, status = [], 0, []
+    for rid, work in ids:
+        inf = srv.info(rid)
+        status.append(inf["status"])
+        if inf["status"] == 0:
+            lat.append(inf["finish_s"] - inf["submit_s"])
+            done_work += work
+    srv.close()
+    lat = np.array(lat)
+    return {
+        "requests": n_req, "completed": int((np.array(status) == 0).sum()),
+        "rejected": int((np.array(status) == 1).sum()), "rate_per_s": rate,
+        "p50_latency_s": float(np.percentile(lat, 50)) if lat.size else None,
+        "p99_latency_s": float(np.percentile(lat, 99)) if lat.size else None,
+        "mean_latency_s": float(lat.mean()) if lat.size else None,
+        "makespan_s": makespan, "node_steps_per_s": done_work / makespan,
+        "hist_mode": hist_mode, "max_active": max_active, "quantum": quantum,
     }
 
 
@@ -115,6 +176,9 @@ def main():
     ap.add_argument("--nodes", type=int, default=None)
     ap.add_argument("--msteps", type=int, default=None)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--requests", type=int, default=512, help="serving mix size (--workload serve)")
+    ap.add_argument("--rate", type=float, default=16.0, help="serving arrival rate, requests/s")
+    ap.add_argument("--max-active", type=int, default=16)
     args = ap.parse_args()
     if args.gpus != 1:
         print(json.dumps({"error": "this build runs on one GPU only; --gpus must be 1",
@@ -126,6 +190,17 @@ def main():
     if F.lib().fpde_device_count() < 1:
         print(json.dumps({"error": "no GPU visible"}))
         sys.exit(1)
+    if args.workload == "serve":
+        r = serve_mix(F, args.requests, args.rate, max_active=args.max_active, hist_mode=args.headline)
+        print(json.dumps({
+            "metric": "p50/p99 solve latency and aggregate node*steps/s, serving mix (continuous batching)",
+            "value": r["node_steps_per_s"], "unit": "node*steps/s", "n_gpus": 1, "steps": 1, "warmup": 0,
+            "ms_per_step": r["makespan_s"] * 1e3, "higher_is_better": True, "scaling": "weak",
+            "vs_baseline": None, "dtype": VARIANT_DTYPE.get(args.headline, "f64"), "data": "synthetic",
+            "config": {"workload": "serving mix: Poisson arrivals, 80% 1D / 20% 3D, 25% distributed order",
+                       "parallelism": "single GPU, one stream per active request"},
+            "serving": r}))
+        return
 
     # FP64 plain result first: the error reference for every other variant
     results = {}
@@ -211,7 +286,7 @@ def main():
                 "alg_bytes_per_launch": bpl, "avg_launch_us": ms * 1e3 / max(nl, 1),
                 "note": "latency-bound sequential step chain"}
 
-    cpu = None if args.no_cpu else cpu_baseline(N, M)
+    cpu = None if (args.no_cpu or spec.dims != 1 or spec.do_q) else cpu_baseline(N, M, alpha=spec.alpha, tau=spec.tau)
     out = {
         "metric": "node*steps/s, mixed-MFMA history-sum solver (vs FP64 GPU at <=1e-8 rel err)",
         "value": unknowns * M / t_mean,
@@ -225,8 +300,10 @@ def main():
         "vs_baseline": None,
         "dtype": VARIANT_DTYPE.get(args.headline, "f64"),
         "data": "synthetic (seeded initial head, mt19937_64 seed 1001)",
-        "config": {"workload": f"{args.workload}: 1D time-fractional moisture transport N={N} M={M} "
-                               f"tau={spec.tau} alpha={spec.alpha}; history {args.headline} block {args.block}",
+        "config": {"workload": (f"{args.workload}: {'3D LOD ' + str(N) + '^3' if spec.dims == 3 else '1D N=' + str(N)} "
+                                f"time-fractional moisture transport M={M} tau={spec.tau} "
+                                + (f"distributed order q={spec.do_q} on [{spec.do_a0},{spec.do_a1}]" if spec.do_q
+                                   else f"alpha={spec.alpha}") + f"; history {args.headline} block {args.block}"),
                    "nodes": N, "time_steps": M, "history_mode": args.headline, "block": args.block,
                    "parallelism": "single GPU"},
         "roofline": roof,

@@ -101,8 +101,9 @@ struct Step1DArgs {
     double* red;              // [2][2*C][4]
     double* Uit;              // Picard iterate (for the convergence test)
     double* dmax;             // [2][C] per-chunk max |U^k - U^{k-1}|
-    int* flags;               // [0]: converged flag of the current step
+    int* flags;               // unused
     double tol;
+    int prev_k;               // Picard index of the previous launch (-1: none)
 };
 
 // Kernel launchers (weights.hip, history.hip, step1d.hip, lod3d.hip)

@@ -333,10 +333,16 @@ fpde_status launch_hist_plain(const double* D, int64_t ld, int nodes, int n, int
                               const double* omega, double* part, int* splits_out,
                               hipStream_t st) {
     if (t_hi <= 0) { *splits_out = 0; return FPDE_OK; }
-    int chunk = (t_hi + 63) / 64;
+    // enough workgroups to fill the chip (~4096 with the node tiles), each
+    // streaming >= 32 levels; at most 64 partial rows
+    int tiles = (nodes + 511) / 512;
+    int want = (4096 + tiles - 1) / tiles;
+    if (want > 64) want = 64;
+    if (want < 1) want = 1;
+    int chunk = (t_hi + want - 1) / want;
     if (chunk < 32) chunk = 32;
     int splits = (t_hi + chunk - 1) / chunk;
-    dim3 grid((nodes + 511) / 512, splits);
+    dim3 grid(tiles, splits);
     hipLaunchKernelGGL(k_hist_plain, grid, dim3(256), 0, st, D, ld, nodes, n, t_hi, chunk, omega, part);
     *splits_out = splits;
     return FPDE_OK;

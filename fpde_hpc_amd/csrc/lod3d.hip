@@ -30,12 +30,16 @@ __global__ __launch_bounds__(256) void k_coef3d(const double* __restrict__ U, in
 }
 
 // Lateral sweep, thread per line.  axis 1 = x (stride n), 2 = y (stride n*n).
-// Scratch Al/Bt: [n+1][nlines], coalesced.
+// Scratch Al/Bt: [n+1][nlines], coalesced.  The row data of a line are
+// fetched 16 rows at a time (independent loads) ahead of the recurrence, and
+// the back substitution fetches Al/Bt the same way, so a line costs ~n/16
+// memory round trips instead of n.
 __global__ __launch_bounds__(256) void k_lateral3d(double* __restrict__ U, const double* __restrict__ K,
                                                    const double* __restrict__ iC,
                                                    const double* __restrict__ H, int n, int axis,
                                                    double h2inv, double L, double* __restrict__ Al,
                                                    double* __restrict__ Bt) {
+    constexpr int R = 16;
     int line = blockIdx.x * 256 + threadIdx.x;
     int nlines = n * n;
     if (line >= nlines) return;
@@ -48,18 +52,36 @@ __global__ __launch_bounds__(256) void k_lateral3d(double* __restrict__ U, const
     double al = 1.0, bt = 0.0;
     Al[(int64_t)1 * nlines + line] = al;
     Bt[(int64_t)1 * nlines + line] = bt;
-    for (int i = 1; i < N; i++) {
-        int64_t id = base + stride * i;
-        double Km = K[id - stride], Kp = K[id], ic = iC[id];
-        double A_ = Km * ic * h2inv;
-        double B_ = Kp * ic * h2inv;
-        double R_ = ic * ((Kp * h2inv) + (Km * h2inv)) + L;
-        double Om = -L * U[id] + (H ? H[id] : 0.0);
-        double aln = B_ / (R_ - A_ * al);
-        bt = (aln / B_) * (A_ * bt - Om);
-        al = aln;
-        Al[(int64_t)(i + 1) * nlines + line] = al;
-        Bt[(int64_t)(i + 1) * nlines + line] = bt;
+    double Kprev = K[base];
+    for (int i0 = 1; i0 < N; i0 += R) {
+        double kk[R], ic[R], uu[R], hh[R];
+#pragma unroll
+        for (int j = 0; j < R; j++) {
+            int i = i0 + j;
+            bool ok = i < N;
+            int64_t id = base + stride * (ok ? i : 0);
+            kk[j] = ok ? K[id] : 0.0;
+            ic[j] = ok ? iC[id] : 0.0;
+            uu[j] = ok ? U[id] : 0.0;
+            hh[j] = (ok && H) ? H[id] : 0.0;
+        }
+#pragma unroll
+        for (int j = 0; j < R; j++) {
+            int i = i0 + j;
+            if (i < N) {
+                double Km = Kprev, Kp = kk[j];
+                double A_ = Km * ic[j] * h2inv;
+                double B_ = Kp * ic[j] * h2inv;
+                double R_ = ic[j] * ((Kp * h2inv) + (Km * h2inv)) + L;
+                double Om = -L * uu[j] + hh[j];
+                double aln = B_ / (R_ - A_ * al);
+                bt = (aln / B_) * (A_ * bt - Om);
+                al = aln;
+                Al[(int64_t)(i + 1) * nlines + line] = al;
+                Bt[(int64_t)(i + 1) * nlines + line] = bt;
+                Kprev = Kp;
+            }
+        }
     }
     // zero-flux end, U1 :4704-4720 with bcN()=0
     double uN;
@@ -67,9 +89,23 @@ __global__ __launch_bounds__(256) void k_lateral3d(double* __restrict__ U, const
     else uN = (fabs(1.0 - al) > 1e-15) ? 0.0 : U[base + stride * (N - 1)];
     U[base + stride * N] = uN;
     double u = uN;
-    for (int i = N - 1; i >= 0; i--) {
-        u = Al[(int64_t)(i + 1) * nlines + line] * u + Bt[(int64_t)(i + 1) * nlines + line];
-        U[base + stride * i] = u;
+    for (int i1 = N - 1; i1 >= 0; i1 -= R) {
+        double aa[R], bb[R];
+#pragma unroll
+        for (int j = 0; j < R; j++) {
+            int i = i1 - j;
+            bool ok = i >= 0;
+            aa[j] = ok ? Al[(int64_t)(i + 1) * nlines + line] : 0.0;
+            bb[j] = ok ? Bt[(int64_t)(i + 1) * nlines + line] : 0.0;
+        }
+#pragma unroll
+        for (int j = 0; j < R; j++) {
+            int i = i1 - j;
+            if (i >= 0) {
+                u = aa[j] * u + bb[j];
+                U[base + stride * i] = u;
+            }
+        }
     }
 }
 

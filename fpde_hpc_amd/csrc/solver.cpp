@@ -108,6 +108,7 @@ struct fpde_solver {
     // progress
     int step = 0;
     int64_t launches = 0;
+    int last_k = -1;
     bool pending = false;
     bool initialized = false;
     // GEMM block state
@@ -309,6 +310,7 @@ fpde_status launch_1d(fpde_solver* s, int n, int k, int nsplit, int64_t stride, 
     a.dmax = s->dmax;
     a.flags = s->flags;
     a.tol = sp.picard_tol;
+    a.prev_k = s->last_k;
     size_t slot = 0;
     ev_begin(s, 1, &slot);
     fpde_status rc = launch_step1d(a, s->stream);
@@ -317,6 +319,7 @@ fpde_status launch_1d(fpde_solver* s, int n, int k, int nsplit, int64_t stride, 
     s->launches++;
     s->stats.step_launches++;
     s->pending = (k >= 0);
+    s->last_k = k;
     return FPDE_OK;
 }
 
@@ -439,6 +442,7 @@ fpde_status fpde_solver_reset(fpde_solver* s, const double* u0) {
     ev_collect(s);
     s->step = 0;
     s->launches = 0;
+    s->last_k = -1;
     s->pending = false;
     s->n0 = -1;
     s->tc = 0;

@@ -89,10 +89,21 @@ __device__ inline double hist_mem(const Step1DArgs& g, int i, int n) {
     return far + (nr0 + nr1);
 }
 
+// max over the 256 threads of a block (wave shuffles + LDS across the 4 waves)
+__device__ inline double block_max(double v, double* sh) {
+    for (int off = 32; off > 0; off >>= 1) v = fmax(v, __shfl_xor(v, off));
+    if ((threadIdx.x & 63) == 0) sh[threadIdx.x >> 6] = v;
+    __syncthreads();
+    v = fmax(fmax(sh[0], sh[1]), fmax(sh[2], sh[3]));
+    __syncthreads();
+    return v;
+}
+
 __global__ __launch_bounds__(256) void k_step1d(Step1DArgs g) {
     __shared__ double rA[2][kMaxRed], rC[2][kMaxRed], rD[2][kMaxRed];
     __shared__ double pA[2][256], pC[2][256], pX[2][256], pY[2][256], pZ[2][256];
     __shared__ double sK[258];
+    __shared__ double sred[4];
     const int tid = threadIdx.x;
     const int c = blockIdx.x;
     const int N = g.N, C = g.C;
@@ -123,6 +134,12 @@ __global__ __launch_bounds__(256) void k_step1d(Step1DArgs g) {
     double dmem = (first && !g.recover && act && n >= 1) ? g.D[(int64_t)(n - 1) * g.ld + i] : 0.0;
     double d0m = (first && act && n >= 2) ? g.D[i] : 0.0;
     double d1m = (first && act && n >= 3) ? g.D[g.ld + i] : 0.0;
+    // Picard convergence of the previous launch (decided one launch late: the
+    // test needs the max over every chunk): dmax of the previous launch and the
+    // iterate it stored
+    const bool check_conv = g.recover && g.prev_k >= 1;
+    double dprev = (check_conv && tid < C) ? g.dmax[(size_t)(1 - g.parity) * C + tid] : 0.0;
+    double uitp = ((check_conv || g.k >= 2) && act) ? g.Uit[i] : 0.0;
     double hm = (first && g.k == 0 && act && n >= 1) ? hist_mem(g, i, n) : 0.0;
 
     // ---- 1. finish the previous solve: reduced system, then back-substitute --
@@ -147,6 +164,10 @@ __global__ __launch_bounds__(256) void k_step1d(Step1DArgs g) {
             else val = fx + fy * rD[pr][2 * c] + fz * rD[pr][2 * c + 1];
         }
     }
+    // the previous Picard iterate met the tolerance: it is the step's result
+    bool conv = false;
+    if (check_conv) conv = block_max(dprev, sred) <= g.tol;
+    if (conv) val = uitp;
 
     // ---- 2. advance the state -----------------------------------------------
     double un = 0.0, uit = 0.0, S = 0.0;
@@ -177,6 +198,33 @@ __global__ __launch_bounds__(256) void k_step1d(Step1DArgs g) {
         uit = val;
     }
     if (g.k < 0) return;                         // finalize only (uniform)
+
+    if (!first) {
+        double* dm = g.dmax + (size_t)g.parity * C;
+        if (conv) {
+            // converged: hand the iterate on unchanged through an identity
+            // reduced system and x = U, y = z = 0 (dmax 0 keeps later launches
+            // of this step passing it on)
+            double* redw = g.red + (size_t)g.parity * R2 * 4;
+            if (act && i != s && i != e) { g.xyz[i] = uit; g.xyz[g.ld + i] = 0.0; g.xyz[2 * g.ld + i] = 0.0; }
+            if (tid == 0) {
+                redw[4 * (2 * c) + 0] = 0.0; redw[4 * (2 * c) + 1] = 1.0;
+                redw[4 * (2 * c) + 2] = 0.0; redw[4 * (2 * c) + 3] = uit;
+                dm[c] = 0.0;
+            }
+            if (tid == nc - 1) {
+                redw[4 * (2 * c + 1) + 0] = 0.0; redw[4 * (2 * c + 1) + 1] = 1.0;
+                redw[4 * (2 * c + 1) + 2] = 0.0; redw[4 * (2 * c + 1) + 3] = uit;
+            }
+            return;                              // uniform: conv is block-wide
+        }
+        // |U^{k} - U^{k-1}| over the chunk; U^0 of the step is U^n
+        double prev_it = (g.k == 1) ? un : uitp;
+        double d = act ? fabs(uit - prev_it) : 0.0;
+        d = block_max(d, sred);
+        if (tid == 0) dm[c] = d;
+        if (act) g.Uit[i] = uit;
+    }
 
     // ---- 3. coefficients at the iterate ------------------------------------
     double K = 0.0, invC = 0.0;

@@ -213,6 +213,22 @@ def test_solve_1d_matches_oracle(fpde, oracle, case, mode):
     assert err <= tol
 
 
+@pytest.mark.parametrize("tol", [1e-12, 1e-7, 3e-6])
+@pytest.mark.parametrize("mode", ["plain", "split_f16"])
+def test_picard_tolerance_exit_matches_oracle(fpde, oracle, tol, mode):
+    """Picard on the nonlinear coefficient, at most 3 iterations, stopping once
+    max|U^k - U^{k-1}| <= tol (the convergence test runs on the device)."""
+    nodes, steps = 600, 120
+    u0 = fpde.init_1d(nodes, 77)
+    spec = fpde.make_spec(nodes, steps, 1e-3, alpha=0.8, picard_max=3, picard_tol=tol, hist_mode=mode,
+                          u_top=u0[0], fourier=40.0)
+    u = fpde.solve(spec, u0)
+    ref = _oracle_1d(oracle, spec, u0, picard_tol=tol)
+    err = rel_err(u, ref)
+    print(f"picard tol {tol} {mode}: rel err {err:.3e}")
+    assert err <= (TOL_F64 if mode == "plain" else TOL_MIXED)
+
+
 @pytest.mark.parametrize("block", [16, 32, 128])
 def test_solve_1d_blocks(fpde, oracle, block):
     nodes, steps = 700, 260
