@@ -124,15 +124,25 @@ __global__ __launch_bounds__(256) void k_gemm_f64(const double* __restrict__ D, 
         __syncthreads();
         for (int q = threadIdx.x; q < W; q += 256) win[q] = omega[n0 - s1 + q];
         __syncthreads();
-        for (int t = s0; t < s1; t += 4) {
-            int tt = t + kk;
-            bool ok = tt < s1;
-            double b = (ok && node_ok) ? D[(int64_t)tt * ld + node] : 0.0;
+        // four 4-deep K steps per iteration: the four B loads are issued
+        // before the first MFMA that needs them
+        for (int t = s0; t < s1; t += 16) {
+            double b[4];
 #pragma unroll
-            for (int rt = 0; rt < RT; rt++) {
-                int r = rt * 16 + (lane & 15);
-                double a = ok ? win[r - tt - 1 + s1] : 0.0;
-                acc[rt] = __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, acc[rt], 0, 0, 0);
+            for (int q = 0; q < 4; q++) {
+                int tt = t + 4 * q + kk;
+                b[q] = (tt < s1 && node_ok) ? D[(int64_t)tt * ld + node] : 0.0;
+            }
+#pragma unroll
+            for (int q = 0; q < 4; q++) {
+                int tt = t + 4 * q + kk;
+                bool ok = tt < s1;
+#pragma unroll
+                for (int rt = 0; rt < RT; rt++) {
+                    int r = rt * 16 + (lane & 15);
+                    double a = ok ? win[r - tt - 1 + s1] : 0.0;
+                    acc[rt] = __builtin_amdgcn_mfma_f64_16x16x4f64(a, b[q], acc[rt], 0, 0, 0);
+                }
             }
         }
     }
@@ -233,6 +243,20 @@ __global__ __launch_bounds__(256) void k_gemm_split(const T* __restrict__ c0, co
     for (int rt = 0; rt < RT; rt++) acc[rt] = (d4_t){0.0, 0.0, 0.0, 0.0};
 
     for (int gb0 = glo; gb0 < ghi; gb0 += GB) {
+        // B fragments of the batch first (global loads in flight while the
+        // block builds the A fragments)
+        vec_t bfr[GB][NS];
+        double bsc[GB];
+#pragma unroll
+        for (int gi = 0; gi < GB; gi++) {
+            int g = min(gb0 + gi, ghi - 1);
+            int64_t off = ((int64_t)g * ld + (node_ok ? node : 0)) * kGroup + 8 * (lane >> 4);
+            bfr[gi][0] = *reinterpret_cast<const vec_t*>(c0 + off);
+            bfr[gi][1] = *reinterpret_cast<const vec_t*>(c1 + off);
+            if (NS == 3) bfr[gi][NS - 1] = *reinterpret_cast<const vec_t*>(c2 + off);
+            bsc[gi] = 1.0;
+            if (SplitTraits<T>::scaled) bsc[gi] = node_ok ? (double)scale[(int64_t)g * ld + node] : 0.0;
+        }
         __syncthreads();
         // build A fragments: entries (gb, rt, lane) -> 8 weights
         for (int f = threadIdx.x; f < GB * RT * 64; f += 256) {
@@ -269,50 +293,45 @@ __global__ __launch_bounds__(256) void k_gemm_split(const T* __restrict__ c0, co
             if (NS == 3) afrag[NS - 1][fgb][frt][fl] = l;
         }
         __syncthreads();
+#pragma unroll
         for (int gi = 0; gi < GB; gi++) {
             int g = gb0 + gi;
             if (g >= ghi) break;
-            int64_t off = ((int64_t)g * ld + (node_ok ? node : 0)) * kGroup + 8 * (lane >> 4);
-            vec_t bh = *reinterpret_cast<const vec_t*>(c0 + off);
-            vec_t bm = *reinterpret_cast<const vec_t*>(c1 + off);
-            vec_t bl;
-            if (NS == 3) bl = *reinterpret_cast<const vec_t*>(c2 + off);
-            double sc = 1.0;
-            if (SplitTraits<T>::scaled) sc = node_ok ? (double)scale[(int64_t)g * ld + node] : 0.0;
+            const vec_t bh = bfr[gi][0], bm = bfr[gi][1];
 #pragma unroll
             for (int rt = 0; rt < RT; rt++) {
                 vec_t ah = afrag[0][gi][rt][lane];
                 vec_t am = afrag[1][gi][rt][lane];
+                // one fp32 chain per group: hi*hi first, then the smaller
+                // cross products (their rounding is 2^-24 of the group sum)
                 f4_t z = {0.f, 0.f, 0.f, 0.f};
-                f4_t p0, p1, p2;
-                if constexpr (sizeof(T) == 2 && SplitTraits<T>::scaled) {
-                    p0 = __builtin_amdgcn_mfma_f32_16x16x32_f16(ah, bh, z, 0, 0, 0);
-                    p1 = __builtin_amdgcn_mfma_f32_16x16x32_f16(ah, bm, z, 0, 0, 0);
-                    p1 = __builtin_amdgcn_mfma_f32_16x16x32_f16(am, bh, p1, 0, 0, 0);
+                f4_t p;
+                if constexpr (SplitTraits<T>::scaled) {
+                    p = __builtin_amdgcn_mfma_f32_16x16x32_f16(ah, bh, z, 0, 0, 0);
+                    p = __builtin_amdgcn_mfma_f32_16x16x32_f16(ah, bm, p, 0, 0, 0);
+                    p = __builtin_amdgcn_mfma_f32_16x16x32_f16(am, bh, p, 0, 0, 0);
                     if (NS == 3) {
                         vec_t al = afrag[NS - 1][gi][rt][lane];
-                        p2 = __builtin_amdgcn_mfma_f32_16x16x32_f16(ah, bl, z, 0, 0, 0);
-                        p2 = __builtin_amdgcn_mfma_f32_16x16x32_f16(am, bm, p2, 0, 0, 0);
-                        p2 = __builtin_amdgcn_mfma_f32_16x16x32_f16(al, bh, p2, 0, 0, 0);
+                        vec_t bl = bfr[gi][NS - 1];
+                        p = __builtin_amdgcn_mfma_f32_16x16x32_f16(ah, bl, p, 0, 0, 0);
+                        p = __builtin_amdgcn_mfma_f32_16x16x32_f16(am, bm, p, 0, 0, 0);
+                        p = __builtin_amdgcn_mfma_f32_16x16x32_f16(al, bh, p, 0, 0, 0);
                     }
                 } else {
-                    p0 = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ah, bh, z, 0, 0, 0);
-                    p1 = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ah, bm, z, 0, 0, 0);
-                    p1 = __builtin_amdgcn_mfma_f32_16x16x32_bf16(am, bh, p1, 0, 0, 0);
+                    p = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ah, bh, z, 0, 0, 0);
+                    p = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ah, bm, p, 0, 0, 0);
+                    p = __builtin_amdgcn_mfma_f32_16x16x32_bf16(am, bh, p, 0, 0, 0);
                     if (NS == 3) {
                         vec_t al = afrag[NS - 1][gi][rt][lane];
-                        p2 = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ah, bl, z, 0, 0, 0);
-                        p2 = __builtin_amdgcn_mfma_f32_16x16x32_bf16(am, bm, p2, 0, 0, 0);
-                        p2 = __builtin_amdgcn_mfma_f32_16x16x32_bf16(al, bh, p2, 0, 0, 0);
+                        vec_t bl = bfr[gi][NS - 1];
+                        p = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ah, bl, p, 0, 0, 0);
+                        p = __builtin_amdgcn_mfma_f32_16x16x32_bf16(am, bm, p, 0, 0, 0);
+                        p = __builtin_amdgcn_mfma_f32_16x16x32_bf16(al, bh, p, 0, 0, 0);
                     }
                 }
-                double scr = SplitTraits<T>::scaled ? sc * ainv[gi][rt] : 1.0;
+                double scr = SplitTraits<T>::scaled ? bsc[gi] * ainv[gi][rt] : 1.0;
 #pragma unroll
-                for (int v = 0; v < 4; v++) {
-                    double s = (double)p0[v] + (double)p1[v];
-                    if (NS == 3) s += (double)p2[v];
-                    acc[rt][v] = fma(scr, s, acc[rt][v]);
-                }
+                for (int v = 0; v < 4; v++) acc[rt][v] = fma(scr, (double)p[v], acc[rt][v]);
             }
         }
     }

@@ -228,11 +228,14 @@ __global__ __launch_bounds__(256) void k_step1d(Step1DArgs g) {
 
     // ---- 3. coefficients at the iterate ------------------------------------
     double K = 0.0, invC = 0.0;
-    if (tid == 0 && c > 0) {                     // K of the node left of the chunk
+    // K of the node left of the chunk, on a thread without a node of its own
+    // when there is one (else thread 0 does both evaluations)
+    const int halo_tid = (nc < 256) ? 255 : 0;
+    if (tid == halo_tid && c > 0) {
         double ulo = g.recover ? rD[pr][2 * c - 1] : g.U[s - 1];
-        double iCl;
-        vgm_coefs(ulo, (s - 1) * g.h, g.ksat, g.coef_mode, K, iCl);
-        sK[0] = K;
+        double Kl, iCl;
+        vgm_coefs(ulo, (s - 1) * g.h, g.ksat, g.coef_mode, Kl, iCl);
+        sK[0] = Kl;
     }
     if (act) {
         vgm_coefs(uit, i * g.h, g.ksat, g.coef_mode, K, invC);
