@@ -31,9 +31,13 @@ constexpr double kVgmA = 0.066;
 
 // Conductivity K = ksat*Kr and the inverse water capacity 1/C at suction head
 // P = z - u (metres).  Same quantities as KK()/inv_dw_dh() of the reference
-// (fpde_hpc_1d.cpp:2527-2575, :2625-2657, cv_form 4, k_form 1), evaluated with
-// three transcendental calls: x=(100 a P)^n, y=(1+x)^(1/n), (x/(1+x))^(1-1/n);
-// then theta-s0 = (s1-s0) y/(1+x), Se^(1/m) = 1/(1+x), C = (n-1)(s1-s0) x y/(P(1+x)^2).
+// (fpde_hpc_1d.cpp:2527-2575, :2625-2657, cv_form 4, k_form 1), rewritten with
+// x = (100 a P)^n:  theta - s0 = (s1-s0) (1+x)^(-m),  Se = (1+x)^(-m),
+// Se^(1/m) = 1/(1+x),  Kr = Se^(1/2) (1 - (x/(1+x))^m)^2,
+// C = d theta/dh = (n-1)(s1-s0) x (1+x)^(1/n) / (P (1+x)^2),   m = 1-1/n,
+// evaluated through lx = log x and l1 = log1p(x) (one log, one log1p, three
+// exp): 1-(x/(1+x))^m = -expm1(m (lx - l1)) keeps the digits the direct
+// difference cancels.
 __host__ __device__ inline void vgm_coefs(double u, double z, double ksat, int mode,
                                           double& K, double& invC) {
     if (mode == 1) { K = 1.0; invC = 1.0; return; }
@@ -44,13 +48,13 @@ __host__ __device__ inline void vgm_coefs(double u, double z, double ksat, int m
         return;
     }
     const double n = kVgmN, m = 1.0 - 1.0 / kVgmN;
-    double x = pow(100.0 * kVgmA * P, n);
-    double opx = 1.0 + x;
-    double y = pow(opx, 1.0 / n);
-    double se = y / opx;
-    double t = 1.0 - pow(x / opx, m);
+    double lx = n * log(100.0 * kVgmA * P);
+    double x = exp(lx);
+    double l1 = log1p(x);
+    double se = exp(-m * l1);                      // (1+x)^(-m)
+    double t = -expm1(m * (lx - l1));              // 1 - (x/(1+x))^m
     K = ksat * sqrt(se) * t * t;
-    double ch = (n - 1.0) * (kVgmS1 - kVgmS0) * x * y / (P * opx * opx);
+    double ch = (n - 1.0) * (kVgmS1 - kVgmS0) * x * exp(l1 / n - 2.0 * l1) / P;
     invC = (ch != 0.0) ? 1.0 / ch : 1e20;
 }
 

@@ -109,26 +109,31 @@ __global__ __launch_bounds__(256) void k_lateral3d(double* __restrict__ U, const
     }
 }
 
-// Depth sweep: block = one z line (n <= 256 threads).  Writes U^{n+1}, the
+// Depth sweep: block = one z line, one node per thread (blockDim = n rounded
+// up to 64, n <= 256).  PCR on rows normalised to b = 1 with ping-pong LDS
+// buffers (one barrier and one reciprocal per level).  Writes U^{n+1}, the
 // history level D[t] = U^{n+1} - Uprev and refreshes Uprev.
 __global__ __launch_bounds__(256) void k_zline3d(double* __restrict__ U, double* __restrict__ Uprev,
                                                  const double* __restrict__ utop, int n, double h,
                                                  double h2inv, double ksat, int mode, double L,
                                                  double* __restrict__ Dlev) {
-    __shared__ double a[256], b[256], c[256], d[256], sK[256];
+    __shared__ double a[2][256], c[2][256], d[2][256], sK[256];
     const int tid = threadIdx.x;
     const int line = blockIdx.x;                   // x + n*y
     const int64_t base = (int64_t)n * line;
-    double u = 0.0, k = 0.0, ic = 0.0;
-    if (tid < n) {
+    const bool act = tid < n;
+    double u = 0.0, k = 0.0, ic = 0.0, uprev = 0.0;
+    if (act) {
         u = U[base + tid];
+        uprev = Uprev[base + tid];
         vgm_coefs(u, tid * h, ksat, mode, k, ic);
         sK[tid] = k;
     }
+    double top = utop[line];
     __syncthreads();
-    if (tid < n) {
+    if (act) {
         double ar, br, cr, dr;
-        if (tid == 0) { ar = 0.0; br = 1.0; cr = 0.0; dr = utop[line]; }
+        if (tid == 0) { ar = 0.0; br = 1.0; cr = 0.0; dr = top; }
         else if (tid == n - 1) { ar = -1.0; br = 1.0; cr = 0.0; dr = 0.0; }
         else {
             double Km = sK[tid - 1], Kp = k;
@@ -137,29 +142,30 @@ __global__ __launch_bounds__(256) void k_zline3d(double* __restrict__ U, double*
             br = -(ic * ((Kp * h2inv) + (Km * h2inv)) + L);
             dr = -L * u;
         }
-        a[tid] = ar; b[tid] = br; c[tid] = cr; d[tid] = dr;
+        double ib = 1.0 / br;
+        a[0][tid] = ar * ib; c[0][tid] = cr * ib; d[0][tid] = dr * ib;
     }
     __syncthreads();
+    int p = 0;
     for (int delta = 1; delta < n; delta <<= 1) {
-        double na = 0, nb = 1, nc = 0, nd = 0;
-        if (tid < n) {
-            double k1 = 0, k2 = 0, am = 0, cm = 0, dm = 0, ap = 0, cp = 0, dp = 0;
-            if (tid - delta >= 0) { k1 = a[tid] / b[tid - delta]; am = a[tid - delta]; cm = c[tid - delta]; dm = d[tid - delta]; }
-            if (tid + delta < n) { k2 = c[tid] / b[tid + delta]; ap = a[tid + delta]; cp = c[tid + delta]; dp = d[tid + delta]; }
-            na = -k1 * am;
-            nc = -k2 * cp;
-            nb = b[tid] - k1 * cm - k2 * ap;
-            nd = d[tid] - k1 * dm - k2 * dp;
+        if (act) {
+            double aq = a[p][tid], cq = c[p][tid], dq = d[p][tid];
+            double am = 0, cm = 0, dm = 0, ap = 0, cp = 0, dp = 0;
+            if (tid - delta >= 0) { am = a[p][tid - delta]; cm = c[p][tid - delta]; dm = d[p][tid - delta]; }
+            if (tid + delta < n) { ap = a[p][tid + delta]; cp = c[p][tid + delta]; dp = d[p][tid + delta]; }
+            double inv = 1.0 / (1.0 - aq * cm - cq * ap);
+            a[1 - p][tid] = -aq * am * inv;
+            c[1 - p][tid] = -cq * cp * inv;
+            d[1 - p][tid] = (dq - aq * dm - cq * dp) * inv;
         }
         __syncthreads();
-        if (tid < n) { a[tid] = na; b[tid] = nb; c[tid] = nc; d[tid] = nd; }
-        __syncthreads();
+        p = 1 - p;
     }
-    if (tid < n) {
-        double un = d[tid] / b[tid];
+    if (act) {
+        double un = d[p][tid];
         int64_t id = base + tid;
         U[id] = un;
-        Dlev[id] = un - Uprev[id];
+        Dlev[id] = un - uprev;
         Uprev[id] = un;
     }
 }
@@ -220,7 +226,8 @@ fpde_status launch_lateral3d(double* U, const double* K, const double* iC, const
 fpde_status launch_zline3d(double* U, double* Uprev, const double* utop, int n, double h, double h2inv,
                            double ksat, int mode, double L, double* Dlev, hipStream_t st) {
     if (n > 256) { set_error("3D grid: at most 256 nodes per axis"); return FPDE_ERR_ARG; }
-    hipLaunchKernelGGL(k_zline3d, dim3(n * n), dim3(256), 0, st, U, Uprev, utop, n, h, h2inv, ksat, mode, L, Dlev);
+    int bs = ((n + 63) / 64) * 64;
+    hipLaunchKernelGGL(k_zline3d, dim3(n * n), dim3(bs), 0, st, U, Uprev, utop, n, h, h2inv, ksat, mode, L, Dlev);
     return FPDE_OK;
 }
 
