@@ -179,6 +179,8 @@ def main():
     ap.add_argument("--requests", type=int, default=512, help="serving mix size (--workload serve)")
     ap.add_argument("--rate", type=float, default=16.0, help="serving arrival rate, requests/s")
     ap.add_argument("--max-active", type=int, default=16)
+    ap.add_argument("--calibrate", type=int, default=0,
+                    help="first copy this many MiB device-to-device (known-size kernel for HBM counters)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(json.dumps({"error": "this build runs on one GPU only; --gpus must be 1",
@@ -190,6 +192,51 @@ def main():
     if F.lib().fpde_device_count() < 1:
         print(json.dumps({"error": "no GPU visible"}))
         sys.exit(1)
+    if args.calibrate > 0:
+        import ctypes
+        L = F.lib()
+        ctx, b0, b1 = ctypes.c_void_p(), ctypes.c_void_p(), ctypes.c_void_p()
+        nb = args.calibrate << 20
+        F.check(L.fpde_cl_ctx_create(0, ctypes.byref(ctx)))
+        F.check(L.fpde_cl_buf_alloc(ctx, nb, None, ctypes.byref(b0)))
+        F.check(L.fpde_cl_buf_alloc(ctx, nb, None, ctypes.byref(b1)))
+        for _ in range(3):
+            F.check(L.fpde_cl_buf_copy(ctx, b1, 0, b0, 0, nb))
+        L.fpde_cl_buf_free(b0)
+        L.fpde_cl_buf_free(b1)
+        L.fpde_cl_ctx_destroy(ctx)
+        print(f"[bench] calibration copy: 3 x {args.calibrate} MiB device-to-device", file=sys.stderr, flush=True)
+    if args.workload == "counters":
+        # Short program for rocprofv3 --pmc passes: a known-size device copy
+        # (counter calibration), the plain FP64 history stream (n=2000 levels,
+        # N=10000) and a split-bf16 solve (N=10000, M=2000).
+        import ctypes
+        L = F.lib()
+        ctx = ctypes.c_void_p()
+        F.check(L.fpde_cl_ctx_create(0, ctypes.byref(ctx)))
+        nb = 1 << 30
+        bufs = [ctypes.c_void_p() for _ in range(5)]
+        F.check(L.fpde_cl_buf_alloc(ctx, nb, None, ctypes.byref(bufs[0])))
+        F.check(L.fpde_cl_buf_alloc(ctx, nb, None, ctypes.byref(bufs[1])))
+        F.check(L.fpde_cl_buf_copy(ctx, bufs[1], 0, bufs[0], 0, nb))
+        print("[counters] copy 1 GiB", file=sys.stderr, flush=True)
+        n, nodes, ld = 2000, 10000, 10048
+        D = np.random.default_rng(1).standard_normal((n, ld))
+        w = np.random.default_rng(2).random(n + 8)
+        F.check(L.fpde_cl_buf_alloc(ctx, D.nbytes, D.ctypes.data, ctypes.byref(bufs[2])))
+        F.check(L.fpde_cl_buf_alloc(ctx, w.nbytes, w.ctypes.data, ctypes.byref(bufs[3])))
+        F.check(L.fpde_cl_buf_alloc(ctx, ld * 8, None, ctypes.byref(bufs[4])))
+        p = L.fpde_cl_buf_device_ptr
+        F.check(L.fpde_history_sum_f64(p(bufs[4]), p(bufs[2]), n, nodes, ld, p(bufs[3])))
+        print("[counters] plain history sum n=2000 N=10000", file=sys.stderr, flush=True)
+        spec, u0 = F.workload("1d_large", hist_mode="split_bf16", steps=2000)
+        F.solve(spec, u0)
+        print("[counters] split_bf16 solve N=10000 M=2000", file=sys.stderr, flush=True)
+        for bb in bufs:
+            L.fpde_cl_buf_free(bb)
+        L.fpde_cl_ctx_destroy(ctx)
+        print(json.dumps({"counters_program": "done"}))
+        return
     if args.workload == "serve":
         r = serve_mix(F, args.requests, args.rate, max_active=args.max_active, hist_mode=args.headline)
         print(json.dumps({
@@ -286,6 +333,16 @@ def main():
                 "alg_bytes_per_launch": bpl, "avg_launch_us": ms * 1e3 / max(nl, 1),
                 "note": "latency-bound sequential step chain"}
 
+    # HBM traffic per launch of the dominant kernel, from the committed
+    # rocprofv3 --pmc FETCH_SIZE / WRITE_SIZE summary (calibrated on a copy)
+    try:
+        tr = json.load(open(os.path.join(ROOT, "profiles", "hbm_counters.json")))
+        k = tr["kernels"].get(roof["kernel"])
+        if k and spec.dims == 1 and N == tr["nodes"]:
+            roof["traffic"] = k["fetch_bytes"] + k["write_bytes"]
+            roof["traffic_source"] = "profiles/hbm_counters.json (rocprofv3 --pmc, copy-calibrated)"
+    except (OSError, ValueError, KeyError):
+        pass
     cpu = None if (args.no_cpu or spec.dims != 1 or spec.do_q) else cpu_baseline(N, M, alpha=spec.alpha, tau=spec.tau)
     out = {
         "metric": "node*steps/s, mixed-MFMA history-sum solver (vs FP64 GPU at <=1e-8 rel err)",

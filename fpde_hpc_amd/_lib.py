@@ -61,6 +61,7 @@ SIGNATURES = [
     ("fpde_cl_buf_device_ptr", _vp, [_vp]),
     ("fpde_cl_buf_write", ctypes.c_int, [_vp, _vp, ctypes.c_size_t, ctypes.c_size_t, _vp]),
     ("fpde_cl_buf_read", ctypes.c_int, [_vp, _vp, ctypes.c_size_t, ctypes.c_size_t, _vp]),
+    ("fpde_cl_buf_copy", ctypes.c_int, [_vp, _vp, ctypes.c_size_t, _vp, ctypes.c_size_t, ctypes.c_size_t]),
     ("fpde_cl_program_create", ctypes.c_int, [_vp, ctypes.c_char_p, ctypes.POINTER(_vp)]),
     ("fpde_cl_program_destroy", ctypes.c_int, [_vp]),
     ("fpde_cl_kernel_create", ctypes.c_int, [_vp, ctypes.c_char_p, ctypes.POINTER(_vp)]),

@@ -165,6 +165,20 @@ fpde_status fpde_cl_buf_read(fpde_cl_ctx* ctx, fpde_cl_buf* buf, size_t offset, 
     return FPDE_OK;
 }
 
+fpde_status fpde_cl_buf_copy(fpde_cl_ctx* ctx, fpde_cl_buf* dst, size_t dst_offset, fpde_cl_buf* src,
+                             size_t src_offset, size_t bytes) {
+    if (!ctx || !dst || !src) { set_error("null argument"); return FPDE_ERR_ARG; }
+    if (dst_offset > dst->bytes || bytes > dst->bytes - dst_offset || src_offset > src->bytes ||
+        bytes > src->bytes - src_offset) {
+        set_error("copy out of buffer bounds");
+        return FPDE_ERR_ARG;
+    }
+    FPDE_HIP_TRY(hipMemcpyAsync((char*)dst->ptr + dst_offset, (const char*)src->ptr + src_offset, bytes,
+                                hipMemcpyDeviceToDevice, ctx->stream));
+    FPDE_HIP_TRY(hipStreamSynchronize(ctx->stream));
+    return FPDE_OK;
+}
+
 fpde_status fpde_cl_program_create(fpde_cl_ctx* ctx, const char* source, fpde_cl_prog** out) {
     if (!ctx || !out) { set_error("null argument"); return FPDE_ERR_ARG; }
     auto* p = new fpde_cl_prog;

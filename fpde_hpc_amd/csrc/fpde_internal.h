@@ -135,5 +135,6 @@ fpde_status launch_hist_combine(double* S, const double* part, int nsplit, int64
                                 const double* D, int64_t ld, int64_t nodes, int n, int near_lo,
                                 const double* omega, const double* corr, hipStream_t st);
 int gemm_max_splits(int64_t nodes);
+fpde_status launch_part_reduce(double* P, int splits, int rows, int64_t ld, int nodes, hipStream_t st);
 
 }  // namespace fpde

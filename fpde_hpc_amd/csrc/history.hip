@@ -347,6 +347,27 @@ __global__ __launch_bounds__(256) void k_gemm_split(const T* __restrict__ c0, co
 }
 
 // ---------------------------------------------------------------------------
+// Fold the K-split partial rows of a GEMM block into split 0, in place:
+// P[0][r][i] += sum_{ks>=1} P[ks][r][i] (fixed order, deterministic), so a
+// step reads one far value per node.
+__global__ __launch_bounds__(256) void k_part_reduce(double* __restrict__ P, int splits, int rows,
+                                                     int64_t ld, int nodes) {
+    int i = blockIdx.x * 256 + threadIdx.x;
+    int r = blockIdx.y;
+    if (i >= nodes || r >= rows) return;
+    double s = P[(int64_t)r * ld + i];
+    for (int ks = 1; ks < splits; ks++) s += P[((int64_t)ks * rows + r) * ld + i];
+    P[(int64_t)r * ld + i] = s;
+}
+
+fpde_status launch_part_reduce(double* P, int splits, int rows, int64_t ld, int nodes, hipStream_t st) {
+    if (splits <= 1) return FPDE_OK;
+    dim3 grid((nodes + 255) / 256, rows);
+    hipLaunchKernelGGL(k_part_reduce, grid, dim3(256), 0, st, P, splits, rows, ld, nodes);
+    return FPDE_OK;
+}
+
+// ---------------------------------------------------------------------------
 // Host launchers
 fpde_status launch_hist_plain(const double* D, int64_t ld, int nodes, int n, int t_hi,
                               const double* omega, double* part, int* splits_out,

@@ -263,6 +263,10 @@ fpde_status far_part(fpde_solver* s, int n, int* nsplit, int64_t* stride, int64_
         } else {
             rc = launch_gemm_f64(s->D, p.ld, (int)p.nodes, n, tc, B, s->w.omega, s->part, &s->gsplits, s->stream);
         }
+        if (rc == FPDE_OK && s->gsplits > 1) {
+            rc = launch_part_reduce(s->part, s->gsplits, B, p.ld, (int)p.nodes, s->stream);
+            s->gsplits = 1;
+        }
         ev_end(s, slot);
         if (rc != FPDE_OK) return rc;
         if (tc > 0) {

@@ -64,6 +64,10 @@ fpde_status fpde_cl_buf_write(fpde_cl_ctx *ctx, fpde_cl_buf *buf, size_t offset,
                               size_t bytes, const void *src);
 fpde_status fpde_cl_buf_read(fpde_cl_ctx *ctx, fpde_cl_buf *buf, size_t offset,
                              size_t bytes, void *dst);
+/* Device-to-device copy between buffers (the clEnqueueCopyBuffer analogue;
+ * blocking).  Also the known-size copy that calibrates HBM byte counters. */
+fpde_status fpde_cl_buf_copy(fpde_cl_ctx *ctx, fpde_cl_buf *dst, size_t dst_offset,
+                             fpde_cl_buf *src, size_t src_offset, size_t bytes);
 /* OpenCL_prg(ct,did,source) (opencl_cuda_base.cpp:55-179): parses #defines. */
 fpde_status fpde_cl_program_create(fpde_cl_ctx *ctx, const char *source, fpde_cl_prog **out);
 fpde_status fpde_cl_program_destroy(fpde_cl_prog *prog);
