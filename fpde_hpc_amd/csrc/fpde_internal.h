@@ -68,6 +68,7 @@ struct DeviceWeights {
     double* omega = nullptr;   // [M]
     double* corr = nullptr;    // [2*M]
     int M = 0;
+    int jmax = 0;              // no weight beyond this lag is non-zero (restricted summing)
 };
 
 // history storage modes

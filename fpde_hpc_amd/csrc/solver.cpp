@@ -47,6 +47,14 @@ fpde_status validate(const fpde_spec& s) {
         if (b != 16 && b != 32 && b != 64 && b != 128) { set_error("block must be 16, 32, 64 or 128"); return FPDE_ERR_ARG; }
     }
     if (s.dims == 3 && s.picard_max != 1) { set_error("3D LOD uses lagged coefficients (picard_max must be 1)"); return FPDE_ERR_ARG; }
+    if (s.sum_alg != 0 && s.sum_alg != 1) {
+        set_error("sum_alg must be 0 (full sum) or 1 (restricted sum); the series form 2 is not provided");
+        return FPDE_ERR_ARG;
+    }
+    if (s.sum_alg == 1 && !(s.sum_param >= 0.0 && std::isfinite(s.sum_param))) {
+        set_error("sum_param must be finite and >= 0 for the restricted sum");
+        return FPDE_ERR_ARG;
+    }
     return FPDE_OK;
 }
 
@@ -222,14 +230,17 @@ fpde_status far_part(fpde_solver* s, int n, int* nsplit, int64_t* stride, int64_
     if (sp.hist_mode == HIST_PLAIN_F64) {
         int ns = 0;
         size_t slot = 0;
-        if (avail > 0) {
+        // restricted summing: lags above jmax have zero weight, t >= n - jmax
+        int t_lo = std::max(0, n - s->w.jmax);
+        if (avail > t_lo) {
             ev_begin(s, 0, &slot);
-            fpde_status rc = launch_hist_plain(s->D, p.ld, (int)p.nodes, n, avail, s->w.omega, s->part, &ns, s->stream);
+            fpde_status rc = launch_hist_plain(s->D + (size_t)t_lo * p.ld, p.ld, (int)p.nodes, n - t_lo, avail - t_lo,
+                                               s->w.omega, s->part, &ns, s->stream);
             ev_end(s, slot);
             if (rc != FPDE_OK) return rc;
             s->stats.history_launches++;
-            s->stats.history_alg_bytes += 8.0 * avail * (double)p.nodes;
-            s->stats.history_alg_flops += 2.0 * avail * (double)p.nodes;
+            s->stats.history_alg_bytes += 8.0 * (avail - t_lo) * (double)p.nodes;
+            s->stats.history_alg_flops += 2.0 * (avail - t_lo) * (double)p.nodes;
         }
         *nsplit = ns;
         *stride = p.ld;
@@ -247,6 +258,11 @@ fpde_status far_part(fpde_solver* s, int n, int* nsplit, int64_t* stride, int64_
         int tcm = std::max(avail, 0);
         int tc = is_split(sp.hist_mode) ? (tcm / kGroup) * kGroup : tcm;
         s->tc = tc;
+        // restricted summing: every row of the block needs t >= n - jmax only
+        // (group aligned for the split forms); the GEMM runs on [t_lo, tc)
+        int t_lo = std::max(0, n - s->w.jmax);
+        if (is_split(sp.hist_mode)) t_lo = (t_lo / kGroup) * kGroup;
+        t_lo = std::min(t_lo, tc);
         size_t slot = 0;
         ev_begin(s, 0, &slot);
         fpde_status rc = FPDE_OK;
@@ -257,11 +273,15 @@ fpde_status far_part(fpde_solver* s, int n, int* nsplit, int64_t* stride, int64_
                                        s->comp[1], s->comp[2], s->scale, s->stream);
                 s->packed_groups = ghi;
             }
+            size_t cb = 2 * (size_t)t_lo * p.ld;     // 2-byte components, group-major
+            auto cp = [&](int c) { return s->comp[c] ? (const void*)((const char*)s->comp[c] + cb) : nullptr; };
             if (rc == FPDE_OK)
-                rc = launch_gemm_split(sp.hist_mode, s->comp[0], s->comp[1], s->comp[2], s->scale, p.ld, (int)p.nodes,
-                                       n, tc, B, s->w.omega, s->part, &s->gsplits, s->stream);
+                rc = launch_gemm_split(sp.hist_mode, cp(0), cp(1), cp(2), s->scale + (size_t)(t_lo / kGroup) * p.ld,
+                                       p.ld, (int)p.nodes, n - t_lo, tc - t_lo, B, s->w.omega, s->part, &s->gsplits,
+                                       s->stream);
         } else {
-            rc = launch_gemm_f64(s->D, p.ld, (int)p.nodes, n, tc, B, s->w.omega, s->part, &s->gsplits, s->stream);
+            rc = launch_gemm_f64(s->D + (size_t)t_lo * p.ld, p.ld, (int)p.nodes, n - t_lo, tc - t_lo, B, s->w.omega,
+                                 s->part, &s->gsplits, s->stream);
         }
         if (rc == FPDE_OK && s->gsplits > 1) {
             rc = launch_part_reduce(s->part, s->gsplits, B, p.ld, (int)p.nodes, s->stream);
@@ -269,10 +289,10 @@ fpde_status far_part(fpde_solver* s, int n, int* nsplit, int64_t* stride, int64_
         }
         ev_end(s, slot);
         if (rc != FPDE_OK) return rc;
-        if (tc > 0) {
+        if (tc > t_lo) {
             s->stats.history_launches++;
-            s->stats.history_alg_bytes += 8.0 * tc * (double)p.nodes * B;
-            s->stats.history_alg_flops += 2.0 * tc * (double)p.nodes * B;
+            s->stats.history_alg_bytes += 8.0 * (tc - t_lo) * (double)p.nodes * B;
+            s->stats.history_alg_flops += 2.0 * (tc - t_lo) * (double)p.nodes * B;
         }
     }
     *nsplit = s->gsplits;

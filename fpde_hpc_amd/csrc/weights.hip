@@ -14,6 +14,7 @@
 // Orders are folded: omega_j = sum_q c_q Da(a_q)/tau b_j(a_q) (distributed
 // order, fr_do.cpp:89-113 with the alpha-integral as a quadrature).
 #include "fpde_internal.h"
+#include <algorithm>
 #include <cmath>
 #include <vector>
 
@@ -59,24 +60,46 @@ __device__ inline double l1_series(double a, double p0, double p1, double pa, do
     return sum;
 }
 
-__global__ void k_weights_omega(OrderSet os, double tau, int M, double* __restrict__ omega) {
+// Restricted summing (time_summing sum_alg 1, fpde_hpc_1d.cpp:4117-4131)
+// walks the intervals newest first and stops at the first weight with
+// |b| < cut.  The closed-form b decreases with the lag, so a Toeplitz lag j
+// is kept exactly when b(j) >= cut; cut = 0 keeps every lag.
+__device__ inline bool kept(double b, double cut) { return !(fabs(b) < cut); }
+
+__global__ void k_weights_omega(OrderSet os, double tau, int M, double cut, double* __restrict__ omega) {
     int j = blockIdx.x * blockDim.x + threadIdx.x + 1;
     if (j > M) return;
     double s = 0.0;
-    for (int q = 0; q < os.q; q++) s += os.c[q] * l1_closed_lag(os.a[q], tau, j);
+    for (int q = 0; q < os.q; q++) {
+        double b = l1_closed_lag(os.a[q], tau, j);
+        if (kept(b, cut)) s += os.c[q] * b;
+    }
     omega[j - 1] = s;
 }
 
-// corr[2n+t] for n in [1,M), t in {0,1}, t < n (after omega is complete)
-__global__ void k_weights_corr(OrderSet os, double tau, int M, double eps,
+// corr[2n+t] for n in [1,M), t in {0,1}, t < n (after omega is complete).
+// Interval t in {0,1} uses the series weight (weight_mode 1) or the closed one
+// (weight_mode 0).  Under restricted summing it is reached only when every
+// newer interval was kept: t=1 needs lag n-2 kept (or n < 3), t=0 needs t=1.
+__global__ void k_weights_corr(OrderSet os, double tau, int M, double eps, int series, double cut,
                                const double* __restrict__ omega, double* __restrict__ corr) {
     int idx = blockIdx.x * blockDim.x + threadIdx.x;
     if (idx >= 2 * M) return;
     int n = idx >> 1, t = idx & 1;
     if (t >= n) { corr[idx] = 0.0; return; }
     double s = 0.0;
-    for (int q = 0; q < os.q; q++)
-        s += os.c[q] * l1_series(os.a[q], t * tau, (t + 1) * tau, (n + 1) * tau, eps);
+    for (int q = 0; q < os.q; q++) {
+        double a = os.a[q];
+        bool reach = (n < 3) || kept(l1_closed_lag(a, tau, n - 2), cut);
+        double b1 = series ? l1_series(a, tau, 2.0 * tau, (n + 1) * tau, eps) : l1_closed_lag(a, tau, n - 1);
+        if (t == 0) {
+            reach = reach && (n < 2 || kept(b1, cut));
+            double b0 = series ? l1_series(a, 0.0, tau, (n + 1) * tau, eps) : l1_closed_lag(a, tau, n);
+            if (reach && kept(b0, cut)) s += os.c[q] * b0;
+        } else if (reach && kept(b1, cut)) {
+            s += os.c[q] * b1;
+        }
+    }
     corr[idx] = s - omega[n - t - 1];
 }
 
@@ -153,11 +176,37 @@ fpde_status launch_weights(const fpde_spec& s, DeviceWeights& w, hipStream_t st)
     }
     w.local = local;
     int M = w.M;
+    // Largest kept lag, for skipping history work below the cut.  Bisection
+    // on the decreasing closed form, with a relative margin of 1e-9 and one
+    // extra lag so that rounding never drops a lag the device kernel keeps.
+    w.jmax = M;
+    if (s.sum_alg == 1 && s.sum_param > 0.0) {
+        int jm = 0;
+        for (int q = 0; q < os.q; q++) {
+            double a = os.a[q], e = 1.0 - a;
+            if (a == 1.0) continue;
+            auto b = [&](int j) {
+                double jd = (double)j;
+                return std::pow(s.tau, e) / e * std::pow(jd, e) * std::expm1(e * std::log1p(1.0 / jd));
+            };
+            double cut = s.sum_param * (1.0 - 1e-9);
+            if (b(1) < cut) continue;
+            int lo = 1, hi = M;                      // b(lo) >= cut
+            if (b(hi) >= cut) lo = hi;
+            while (hi - lo > 1) {
+                int mid = lo + (hi - lo) / 2;
+                if (b(mid) >= cut) lo = mid; else hi = mid;
+            }
+            jm = std::max(jm, lo);
+        }
+        w.jmax = std::min(M, jm + 1);
+    }
     int tb = 256;
-    hipLaunchKernelGGL(k_weights_omega, dim3((M + tb - 1) / tb), dim3(tb), 0, st, os, s.tau, M, w.omega);
-    if (s.weight_mode == 1) {
+    double cut = (s.sum_alg == 1) ? s.sum_param : 0.0;
+    hipLaunchKernelGGL(k_weights_omega, dim3((M + tb - 1) / tb), dim3(tb), 0, st, os, s.tau, M, cut, w.omega);
+    if (s.weight_mode == 1 || s.sum_alg == 1) {
         hipLaunchKernelGGL(k_weights_corr, dim3((2 * M + tb - 1) / tb), dim3(tb), 0, st, os, s.tau, M,
-                           s.series_eps, (const double*)w.omega, w.corr);
+                           s.series_eps, s.weight_mode == 1 ? 1 : 0, cut, (const double*)w.omega, w.corr);
     } else {
         FPDE_HIP_TRY(hipMemsetAsync(w.corr, 0, sizeof(double) * 2 * (size_t)M, st));
     }

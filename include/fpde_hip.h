@@ -132,8 +132,13 @@ typedef struct {
     int hist_mode;       /* 0 plain FP64, 1 FP64 MFMA GEMM, 2 split bf16 hi+lo,
                             3 split fp16 hi+lo + scaling, 4 split bf16 x3        */
     int block;           /* future steps per GEMM block (multiple of 16, <=128)  */
-    int use_graph;       /* capture the step loop in hipGraphs                   */
-    int reserved[4];
+    int use_graph;       /* reserved (the step loop is issued on one stream)     */
+    int sum_alg;         /* 0 full history sum; 1 truncated: newest intervals
+                            only, down to the first weight with |kb| < sum_param
+                            (time_summing sum_alg 1, fpde_hpc_1d.cpp:4117-4131);
+                            single order only                                   */
+    double sum_param;    /* truncation threshold on the L1 weight kb             */
+    int reserved[2];
 } fpde_spec;
 
 typedef struct fpde_solver fpde_solver;

@@ -168,11 +168,12 @@ double orc_l1_weights(double alpha, int n, const double *Time,
 }
 
 /* ------------------------------------------------------------------------ */
-/* time_summing, sum_alg 0 (fpde_hpc_1d.cpp:4110-4136): node-outer, t-inner,
- * newest interval first; old is level-major [levels][nodes]. */
+/* time_summing (fpde_hpc_1d.cpp:4110-4136): node-outer, t-inner, newest
+ * interval first; old is level-major [levels][nodes].  sum_alg 1 (restricted
+ * summing, :4117-4131) breaks before the first weight with |kb| < sum_param. */
 void orc_time_summing(int nodes, int levels, const double *old,
                       const double *kb3, const double *tau, double *S,
-                      int nthreads)
+                      int nthreads, int sum_alg, double sum_param)
 {
 #ifdef _OPENMP
     if (nthreads <= 0) nthreads = omp_get_max_threads();
@@ -182,6 +183,7 @@ void orc_time_summing(int nodes, int levels, const double *old,
         double time_sum = 0.0;
         for (int t = levels - 2; t >= 0; t--) {
             double kv = kb3[t];
+            if (sum_alg == 1 && fabs(kv) < sum_param) break;
             double diff = old[(size_t)(t + 1) * nodes + id] - old[(size_t)t * nodes + id];
             time_sum += kv * diff / tau[t];
         }
@@ -390,7 +392,7 @@ static void history_term(weights_ctx *w, const orc_params *p, int n, int nodes,
         double Da = da_of(a);
         L += w->cw[q] * Da * kkk0 / w->tau[n];
         if (n >= 1) {
-            orc_time_summing(nodes, n + 1, old, w->kb3, w->tau, tmp, p->nthreads);
+            orc_time_summing(nodes, n + 1, old, w->kb3, w->tau, tmp, p->nthreads, p->sum_alg, p->sum_param);
             for (int i = 0; i < nodes; i++) H[i] += w->cw[q] * Da * tmp[i];
         }
     }

@@ -47,10 +47,11 @@ double orc_kb(double alpha, int t0, int a, const double *Time,
 double orc_l1_weights(double alpha, int n, const double *Time,
                       int weight_mode, double eps, double *kb3);
 
-/* ---- history sum (sum_alg 0): S[i] = sum_t kb3[t]*(old[t+1][i]-old[t][i])/tau[t] */
+/* ---- history sum: S[i] = sum_t kb3[t]*(old[t+1][i]-old[t][i])/tau[t];
+ * sum_alg 1 stops (newest first) at the first |kb3[t]| < sum_param. */
 void orc_time_summing(int nodes, int levels, const double *old,
                       const double *kb3, const double *tau, double *S,
-                      int nthreads);
+                      int nthreads, int sum_alg, double sum_param);
 
 /* ---- VGM coefficients, P = suction head in metres ---- */
 double orc_wetness(double P);
@@ -77,6 +78,8 @@ typedef struct {
     int    do_rule;      /* 0 = Gauss-Legendre, 1 = trapezoid (fr_do.cpp)  */
     double do_a0, do_a1; /* order interval                                 */
     int    nthreads;
+    int    sum_alg;      /* 0 full sum, 1 truncated (time_summing sum_alg 1)   */
+    double sum_param;    /* truncation threshold on |kb|                      */
 } orc_params;
 
 /* 1D solve: u (nodes) holds u0 on entry, u^M on exit.  u_top = u0[0]. */

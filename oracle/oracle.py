@@ -24,7 +24,7 @@ class Params(ctypes.Structure):
         ("coef_mode", ctypes.c_int), ("picard_max", ctypes.c_int), ("picard_tol", ctypes.c_double),
         ("weight_mode", ctypes.c_int), ("series_eps", ctypes.c_double), ("do_q", ctypes.c_int),
         ("do_rule", ctypes.c_int), ("do_a0", ctypes.c_double), ("do_a1", ctypes.c_double),
-        ("nthreads", ctypes.c_int),
+        ("nthreads", ctypes.c_int), ("sum_alg", ctypes.c_int), ("sum_param", ctypes.c_double),
     ]
 
 
@@ -51,7 +51,8 @@ def lib():
         L.orc_kb.restype = ctypes.c_double
         L.orc_l1_weights.argtypes = [ctypes.c_double, ctypes.c_int, _dp, ctypes.c_int, ctypes.c_double, _dp]
         L.orc_l1_weights.restype = ctypes.c_double
-        L.orc_time_summing.argtypes = [ctypes.c_int, ctypes.c_int, _dp, _dp, _dp, _dp, ctypes.c_int]
+        L.orc_time_summing.argtypes = [ctypes.c_int, ctypes.c_int, _dp, _dp, _dp, _dp, ctypes.c_int,
+                                       ctypes.c_int, ctypes.c_double]
         for f in ("orc_wetness", "orc_inv_dw_dh", "orc_kr"):
             getattr(L, f).argtypes = [ctypes.c_double]
             getattr(L, f).restype = ctypes.c_double
@@ -109,13 +110,13 @@ def l1_weights(alpha, n, Time, weight_mode=1, eps=1e-5):
     return kkk0, kb3[:n]
 
 
-def time_summing(old, kb3, tau, nthreads=0):
+def time_summing(old, kb3, tau, nthreads=0, sum_alg=0, sum_param=1e-5):
     old = np.ascontiguousarray(old, dtype=np.float64)
     levels, nodes = old.shape
     kb3 = np.ascontiguousarray(kb3, dtype=np.float64)
     tau = np.ascontiguousarray(tau, dtype=np.float64)
     S = np.zeros(nodes)
-    lib().orc_time_summing(nodes, levels, _p(old), _p(kb3), _p(tau), _p(S), nthreads)
+    lib().orc_time_summing(nodes, levels, _p(old), _p(kb3), _p(tau), _p(S), nthreads, sum_alg, sum_param)
     return S
 
 
@@ -149,7 +150,7 @@ def thomas(a, b, c, d):
 def make_params(nodes, steps, tau, alpha=0.8, length=1.0, ksat=None, coef_mode=0,
                 picard_max=1, picard_tol=1e-12, weight_mode=1, series_eps=1e-5,
                 do_q=0, do_rule=0, do_a0=0.1, do_a1=1.0, nthreads=0, fourier=1.0,
-                u_top=None):
+                u_top=None, sum_alg=0, sum_param=1e-5):
     if ksat is None:
         if coef_mode == 0:
             h = length / (nodes - 1)
@@ -157,7 +158,7 @@ def make_params(nodes, steps, tau, alpha=0.8, length=1.0, ksat=None, coef_mode=0
         else:
             ksat = 1.0
     return Params(nodes, steps, tau, alpha, length, ksat, coef_mode, picard_max, picard_tol,
-                  weight_mode, series_eps, do_q, do_rule, do_a0, do_a1, nthreads)
+                  weight_mode, series_eps, do_q, do_rule, do_a0, do_a1, nthreads, sum_alg, sum_param)
 
 
 def init_1d(nodes, seed, length=1.0):

@@ -4,6 +4,8 @@ specs are rejected before any device work."""
 import ctypes
 import os
 import re
+import subprocess
+import tempfile
 
 import numpy as np
 import pytest
@@ -48,7 +50,7 @@ def test_ksat_fourier_matches_oracle(fpde, oracle):
 @pytest.mark.parametrize("bad", [
     dict(dims=2), dict(nodes=3), dict(steps=0), dict(alpha=0.0), dict(alpha=1.5),
     dict(picard_max=0), dict(hist_mode=9), dict(block=48), dict(dims=3, nodes=300),
-    dict(tau=-1.0), dict(nodes=70000),
+    dict(tau=-1.0), dict(nodes=70000), dict(sum_alg=2), dict(sum_alg=1, sum_param=-1.0),
 ])
 def test_invalid_specs_rejected(fpde, bad):
     kw = dict(nodes=100, steps=10, tau=1e-3, alpha=0.5, hist_mode=1, block=64)
@@ -79,3 +81,21 @@ def test_unknown_kernel_name_is_reported(fpde):
     assert L.fpde_cl_kernel_create(prog, b"Sum", ctypes.byref(k)) == 0
     L.fpde_cl_kernel_destroy(k)
     L.fpde_cl_program_destroy(prog)
+
+
+@pytest.mark.parametrize("name", ["fpde_spec", "fpde_stats", "fpde_result_info"])
+def test_struct_layout_matches_binding(fpde, name):
+    """The ctypes structures have the C header's size and field offsets (gcc)."""
+    from fpde_hpc_amd import _lib
+    cls = {"fpde_spec": _lib.Spec, "fpde_stats": _lib.Stats, "fpde_result_info": _lib.ResultInfo}[name]
+    fields = [f for f, _ in cls._fields_]
+    body = "".join(f'printf("%zu\\n", offsetof({name}, {f}));' for f in fields)
+    src = ('#include <stdio.h>\n#include "fpde_hip.h"\nint main(void){printf("%zu\\n", sizeof('
+           + name + '));' + body + 'return 0;}\n')
+    with tempfile.TemporaryDirectory() as d:
+        c, exe = os.path.join(d, "l.c"), os.path.join(d, "l")
+        open(c, "w").write(src)
+        subprocess.check_call(["gcc", "-std=c99", "-I", os.path.join(ROOT, "include"), c, "-o", exe])
+        out = [int(v) for v in subprocess.check_output([exe]).split()]
+    assert out[0] == ctypes.sizeof(cls)
+    assert out[1:] == [getattr(cls, f).offset for f in fields]

@@ -180,12 +180,16 @@ def test_weights_match_oracle(fpde, oracle, weight_mode, do):
             assert abs(a - b) <= 1e-10 * abs(b)
 
 
+def _oracle_params(oracle, spec, picard_tol=0.0):
+    return oracle.make_params(spec.nodes, spec.steps, spec.tau, spec.alpha, ksat=spec.ksat,
+                              coef_mode=spec.coef_mode, picard_max=spec.picard_max, picard_tol=picard_tol,
+                              weight_mode=spec.weight_mode, series_eps=spec.series_eps, do_q=spec.do_q,
+                              do_rule=spec.do_rule, do_a0=spec.do_a0, do_a1=spec.do_a1,
+                              sum_alg=spec.sum_alg, sum_param=spec.sum_param)
+
+
 def _oracle_1d(oracle, spec, u0, picard_tol=0.0):
-    p = oracle.make_params(spec.nodes, spec.steps, spec.tau, spec.alpha, ksat=spec.ksat,
-                           coef_mode=spec.coef_mode, picard_max=spec.picard_max, picard_tol=picard_tol,
-                           weight_mode=spec.weight_mode, series_eps=spec.series_eps, do_q=spec.do_q,
-                           do_rule=spec.do_rule, do_a0=spec.do_a0, do_a1=spec.do_a1)
-    return oracle.solve_1d(p, u0)
+    return oracle.solve_1d(_oracle_params(oracle, spec, picard_tol), u0)
 
 
 CASES_1D = [
@@ -306,3 +310,139 @@ def test_server_results_same_alone_and_in_mix(fpde):
         assert info["status"] == 0
         assert rel_err(srv.result(rid), a) <= 1e-14
     srv.close()
+
+
+# ------------------------------------------------------------ restricted summing
+def _lag_weights(alpha, tau, M):
+    j = np.arange(1, M + 1, dtype=np.float64)
+    return tau ** (1 - alpha) / (1 - alpha) * ((j + 1) ** (1 - alpha) - j ** (1 - alpha))
+
+
+def _cut_between(alphas, tau, M, lag):
+    """A cut between the weights of lags `lag` and `lag+1` of the first order,
+    moved until no order has a weight within 1e-6 (relative) of it, so that
+    rounding cannot decide which lags are kept."""
+    b = _lag_weights(alphas[0], tau, M)
+    cut = float(np.sqrt(b[lag - 1] * b[lag]))
+    for _ in range(200):
+        near = [np.min(np.abs(_lag_weights(a, tau, M) / cut - 1.0)) for a in alphas]
+        if min(near) > 1e-6:
+            return cut
+        cut *= 1.0 + 3e-4
+    raise AssertionError("no cut with a margin")
+
+
+def test_reference_sum_kernel_restricted_through_facade(fpde, oracle):
+    """'Sum' with sum_alg 1 (fpde_hpc_1d.cpp:1259-1283): newest first, in chunks of
+    BS levels; the first chunk whose weights are all below sum_param is still
+    added, and nothing older.  With the cut between kb[23] and kb[24] and BS 16
+    the chunks are t=39..24 (kept), 23..8 (all below: added, then stop)."""
+    L = fpde.lib()
+    rng = np.random.default_rng(12)
+    nodes, levels, OS = 200, 41, 256
+    old = np.zeros((levels, OS))
+    old[:, :nodes] = np.cumsum(rng.standard_normal((levels, nodes)) * 1e-2, axis=0)
+    Time = oracle.time_grid(levels + 1, 1e-3)
+    _, kb = oracle.l1_weights(0.7, levels - 1, Time, weight_mode=1)
+    cut = 0.5 * (kb[23] + kb[24])
+    assert np.all(np.abs(kb[:24]) < cut) and np.all(np.abs(kb[24:]) >= cut)
+    tau = np.full(levels + 8, 1e-3)
+    ref = oracle.time_summing(old[8:, :nodes], kb[8:], tau[8:])
+    src = b"#define sum_alg 1\n#define OS 256\n#define BS 16\n__kernel void Sum(...)"
+    dv = Dev(L)
+    try:
+        bS, bold, btau, bK = dv.zeros(OS), dv.put(old), dv.put(tau), dv.put(np.concatenate([kb, np.zeros(8)]))
+        bH, bC = dv.zeros(8), dv.zeros(8)
+        prog, k = ctypes.c_void_p(), ctypes.c_void_p()
+        assert L.fpde_cl_program_create(dv.ctx, src, ctypes.byref(prog)) == 0
+        assert L.fpde_cl_kernel_create(prog, b"Sum", ctypes.byref(k)) == 0
+        for idx, b in [(0, bS), (1, bold), (3, btau), (4, bH), (5, bC), (6, bK)]:
+            assert L.fpde_cl_kernel_set_buffer_arg(k, idx, b) == 0
+        iv = ctypes.c_int(levels)
+        assert L.fpde_cl_kernel_set_arg(k, 2, 4, ctypes.byref(iv)) == 0
+        for idx, v in [(7, cut), (8, 0.0), (9, 0.0), (10, 0.7)]:
+            dvv = ctypes.c_double(v)
+            assert L.fpde_cl_kernel_set_arg(k, idx, 8, ctypes.byref(dvv)) == 0
+        iv2 = ctypes.c_int(levels - 1)
+        assert L.fpde_cl_kernel_set_arg(k, 11, 4, ctypes.byref(iv2)) == 0
+        g = (ctypes.c_size_t * 1)(OS)
+        l = (ctypes.c_size_t * 1)(16)
+        assert L.fpde_cl_launch(dv.ctx, k, 1, g, l) == 0, L.fpde_last_error()
+        assert L.fpde_cl_finish(dv.ctx) == 0
+        S = dv.get(bS, nodes)
+        L.fpde_cl_kernel_destroy(k)
+        L.fpde_cl_program_destroy(prog)
+    finally:
+        dv.close()
+    assert np.max(np.abs(S - ref)) <= 1e-13 * np.max(np.abs(ref))
+
+
+@pytest.mark.parametrize("weight_mode", [0, 1])
+def test_restricted_weights_match_oracle(fpde, oracle, weight_mode):
+    """Toeplitz weights are zero beyond the cut lag; the two oldest intervals
+    keep their weight only while every newer interval is kept."""
+    M, tau, alpha = 300, 1e-3, 0.6
+    cut = _cut_between([alpha], tau, M, 70)
+    spec = fpde.make_spec(50, M, tau, alpha=alpha, weight_mode=weight_mode, sum_alg=1, sum_param=cut)
+    with fpde.Solver(spec) as s:
+        local, omega, corr = s.weights()
+    p = oracle.make_params(50, M, tau, alpha, weight_mode=weight_mode, ksat=spec.ksat)
+    lo, om, co = oracle.folded_weights(p)
+    keep = _lag_weights(alpha, tau, M) >= cut
+    assert keep[69] and not keep[70]
+    assert np.array_equal(omega[70:], np.zeros(M - 70))
+    assert np.max(np.abs(omega[:70] - om[:70]) / np.abs(om[:70])) <= 1e-10
+    T = oracle.time_grid(M + 1, tau)
+    Da = omega[0] / _lag_weights(alpha, tau, 1)[0]
+    for n in range(1, M):
+        # newest first: interval t survives when all lags n-t' (t' >= t) pass
+        ok = True
+        for t in range(n - 1, -1, -1):
+            b = oracle.kb_series(alpha, T[t], T[t + 1], T[n + 1])[0] if (weight_mode == 1 and t <= 1) \
+                else oracle.kb_closed(alpha, T[t], T[t + 1], T[n + 1])
+            ok = ok and abs(b) >= cut
+            if t <= 1:
+                got = omega[n - t - 1] + corr[n, t]
+                want = Da * b if ok else 0.0
+                assert abs(got - want) <= 1e-10 * abs(Da * b), (n, t, got, want)
+
+
+@pytest.mark.parametrize("mode", ["plain", "mfma_f64", "split_bf16", "split_f16"])
+def test_restricted_sum_1d_matches_oracle(fpde, oracle, mode):
+    nodes, steps, alpha, tau = 500, 400, 0.5, 1e-3
+    u0 = fpde.init_1d(nodes, 44)
+    for lag in (37, 150):
+        cut = _cut_between([alpha], tau, steps, lag)
+        spec = fpde.make_spec(nodes, steps, tau, alpha=alpha, hist_mode=mode, block=32, u_top=u0[0],
+                              sum_alg=1, sum_param=cut)
+        u = fpde.solve(spec, u0)
+        ref = _oracle_1d(oracle, spec, u0)
+        full = _oracle_1d(oracle, fpde.make_spec(nodes, steps, tau, alpha=alpha, u_top=u0[0]), u0)
+        err = rel_err(u, ref)
+        print(f"restricted 1D lag {lag} {mode}: rel err {err:.3e}, cut effect {rel_err(ref, full):.3e}")
+        assert rel_err(ref, full) > 1e3 * TOL_MIXED     # the cut changes the answer
+        assert err <= (TOL_MIXED if mode.startswith("split") else TOL_F64)
+
+
+def test_restricted_sum_distributed_order_and_3d(fpde, oracle):
+    nodes, steps, tau = 300, 200, 1e-3
+    u0 = fpde.init_1d(nodes, 45)
+    q = 6
+    alphas = [0.1 + 0.9 * i / (q - 1) for i in range(q)]
+    cut = _cut_between([0.5] + alphas[:-1], tau, steps, 40)
+    for mode in ("plain", "mfma_f64"):
+        spec = fpde.make_spec(nodes, steps, tau, do_q=q, do_rule=1, do_a0=0.1, do_a1=1.0, hist_mode=mode,
+                              block=16, u_top=u0[0], sum_alg=1, sum_param=cut)
+        u = fpde.solve(spec, u0)
+        assert rel_err(u, _oracle_1d(oracle, spec, u0)) <= TOL_F64
+    n = 10
+    u3 = fpde.init_3d(n, 46)
+    cut3 = _cut_between([0.85], tau, 90, 20)
+    for mode in ("plain", "split_f16"):
+        spec = fpde.make_spec(n, 90, tau, alpha=0.85, dims=3, hist_mode=mode, block=16, u_top=u3[0],
+                              sum_alg=1, sum_param=cut3)
+        u = fpde.solve(spec, u3)
+        p = oracle.make_params(n, 90, tau, 0.85, ksat=spec.ksat, sum_alg=1, sum_param=cut3)
+        err = rel_err(u, oracle.solve_3d(p, u3))
+        print(f"restricted 3D {mode}: rel err {err:.3e}")
+        assert err <= (TOL_MIXED if mode.startswith("split") else TOL_F64)

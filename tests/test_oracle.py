@@ -112,6 +112,33 @@ def test_time_summing_brute_force(oracle):
     assert np.allclose(S, ref, rtol=1e-14, atol=0)
 
 
+def test_time_summing_restricted_stops_at_first_small_weight(oracle):
+    """sum_alg 1 (fpde_hpc_1d.cpp:4117-4131) walks newest first and stops at the
+    first |kb| < sum_param, even when older weights are large again: the result
+    is the full sum over the newer levels only."""
+    rng = np.random.default_rng(5)
+    levels, nodes = 9, 6
+    old = rng.standard_normal((levels, nodes))
+    tau = rng.random(levels) + 0.5
+    kb = np.array([5.0, 4.0, 3.0, 1e-9, 2.0, 1.5, 1.2, 1.1])
+    S = oracle.time_summing(old, kb, tau, sum_alg=1, sum_param=1e-5)
+    newer = oracle.time_summing(old[4:], kb[4:], tau[4:])
+    assert np.array_equal(S, newer)
+    # a cut of 0 keeps everything; a cut above every weight keeps nothing
+    assert np.array_equal(oracle.time_summing(old, kb, tau, sum_alg=1, sum_param=0.0),
+                          oracle.time_summing(old, kb, tau))
+    assert not np.any(oracle.time_summing(old, kb, tau, sum_alg=1, sum_param=10.0))
+
+
+def test_restricted_solve_with_zero_cut_is_the_full_solve(oracle):
+    u0 = oracle.init_1d(80, 9)
+    full = oracle.solve_1d(oracle.make_params(80, 60, 1e-3, 0.6, u_top=u0[0]), u0)
+    cut0 = oracle.solve_1d(oracle.make_params(80, 60, 1e-3, 0.6, u_top=u0[0], sum_alg=1, sum_param=0.0), u0)
+    assert np.array_equal(full, cut0)
+    cut = oracle.solve_1d(oracle.make_params(80, 60, 1e-3, 0.6, u_top=u0[0], sum_alg=1, sum_param=1e-2), u0)
+    assert 0 < np.max(np.abs(cut - full)) < 1e-2 * np.max(np.abs(full))
+
+
 @pytest.mark.parametrize("alpha", [0.2, 0.6, 0.9])
 def test_l1_exact_for_linear_in_time(oracle, alpha):
     # D^a (c t) = c t^(1-a)/Gamma(2-a): the L1 scheme is exact for linear u
