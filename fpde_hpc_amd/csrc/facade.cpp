@@ -250,16 +250,38 @@ fpde_status fpde_cl_launch(fpde_cl_ctx* ctx, fpde_cl_kernel* k, int ndims, const
         (rc = arg_buf(k, 3, &tau)) || (rc = arg_buf(k, 6, &kb)) || (rc = arg_double(k, 7, &sum_param)))
         return rc;
     int sum_alg = k->prog->sum_alg;
-    if (sum_alg == 2) {
-        set_error("sum_alg 2 (series-compressed history) is not provided by this library");
-        return FPDE_ERR_ARG;
-    }
     int64_t os = k->prog->os;
     int64_t nodes = (int64_t)gdims[0];
     if (os <= 0) os = nodes;
     // bounds: the launch may be rounded up past OS (fpde_hpc_1d.cpp:1873-1874)
     if (nodes > os) nodes = os;
     if ((int64_t)(S->bytes / sizeof(double)) < nodes) { set_error("S buffer too small"); return FPDE_ERR_ARG; }
+    if (sum_alg == 2) {
+        // series moments: 4 Htmp [nodes][P], 5 row2_precalc [P], 8 gtj, 9 gt,
+        // 10 alpha, 11 tstep (fr_ocl_call_sum, fpde_hpc_1d.cpp:1837-1868)
+        fpde_cl_buf *H, *coef;
+        double gtj, gt, alpha;
+        int64_t tstep;
+        if ((rc = arg_buf(k, 4, &H)) || (rc = arg_buf(k, 5, &coef)) || (rc = arg_double(k, 8, &gtj)) ||
+            (rc = arg_double(k, 9, &gt)) || (rc = arg_double(k, 10, &alpha)) || (rc = arg_int(k, 11, &tstep)))
+            return rc;
+        if (!(sum_param >= 1.0 && sum_param <= (double)(1 << 20))) { set_error("sum_alg 2: sum_param (terms) must be >= 1"); return FPDE_ERR_ARG; }
+        int P = (int)sum_param;
+        if ((int64_t)(H->bytes / sizeof(double)) < nodes * P || (int64_t)(coef->bytes / sizeof(double)) < P) {
+            set_error("sum_alg 2: Htmp needs nodes*sum_param and row2_precalc sum_param doubles");
+            return FPDE_ERR_ARG;
+        }
+        if (old_size >= 2 && ((int64_t)(old->bytes / sizeof(double)) < 2 * os || tstep < 0 ||
+                              (int64_t)(tau->bytes / sizeof(double)) <= tstep)) {
+            set_error("sum_alg 2: old needs two levels and tau must hold tstep");
+            return FPDE_ERR_ARG;
+        }
+        rc = launch_ref_series((double*)S->ptr, (const double*)old->ptr, old_size, os, nodes, (const double*)tau->ptr,
+                               (int)tstep, (double*)H->ptr, (const double*)coef->ptr, P, gtj, gt, alpha, ctx->stream);
+        if (rc != FPDE_OK) return rc;
+        FPDE_HIP_TRY(hipGetLastError());
+        return FPDE_OK;
+    }
     if (old_size >= 1 && (int64_t)(old->bytes / sizeof(double)) < old_size * os) { set_error("old buffer too small"); return FPDE_ERR_ARG; }
     if (old_size >= 2 && ((int64_t)(kb->bytes / sizeof(double)) < old_size - 1 || (int64_t)(tau->bytes / sizeof(double)) < old_size - 1)) {
         set_error("kb/tau buffers too small");

@@ -69,7 +69,37 @@ struct DeviceWeights {
     double* corr = nullptr;    // [2*M]
     int M = 0;
     int jmax = 0;              // no weight beyond this lag is non-zero (restricted summing)
+    // series-compressed history (sum_alg 2): per step n, the moment update
+    // coefficients ser_c[n][j] and the series coefficients ser_v[n][j] (Da/tau
+    // folded in), j < ser_P
+    int ser_P = 0;
+    double* ser_c = nullptr;   // [M][P]
+    double* ser_v = nullptr;   // [M][P]
 };
+
+constexpr int kMaxSeriesTerms = 64;
+
+// sum_alg 2 moment update and series of one node (time_summing,
+// fpde_hpc_1d.cpp:4143-4150): H[j][i] += c_j d, S = sum_j v_j H[j][i] in the
+// reference's order (j ascending).  Loads are issued 8 at a time.
+__device__ inline double series_update(double* __restrict__ H, int64_t ld, int64_t i, int P, double d,
+                                       const double* __restrict__ c, const double* __restrict__ v) {
+    double s = 0.0;
+    for (int j0 = 0; j0 < P; j0 += 8) {
+        double h[8];
+#pragma unroll
+        for (int j = 0; j < 8; j++) h[j] = (j0 + j < P) ? H[(int64_t)(j0 + j) * ld + i] : 0.0;
+#pragma unroll
+        for (int j = 0; j < 8; j++) {
+            if (j0 + j < P) {
+                double hn = h[j] + c[j0 + j] * d;
+                H[(int64_t)(j0 + j) * ld + i] = hn;
+                s += v[j0 + j] * hn;
+            }
+        }
+    }
+    return s;
+}
 
 // history storage modes
 enum HistMode : int {
@@ -109,6 +139,12 @@ struct Step1DArgs {
     int* flags;               // unused
     double tol;
     int prev_k;               // Picard index of the previous launch (-1: none)
+    // series-compressed history (ser_P > 0): moments H [P][ld], coefficient
+    // rows of step n; D is then a ring of two levels (row t & 1)
+    int ser_P;
+    double* H;
+    const double* ser_c;
+    const double* ser_v;
 };
 
 // Kernel launchers (weights.hip, history.hip, step1d.hip, lod3d.hip)
@@ -118,6 +154,9 @@ fpde_status launch_hist_plain(const double* D, int64_t ld, int nodes, int n, int
 fpde_status launch_ref_sum(double* S, const double* old, int64_t old_size, int64_t os, int64_t nodes,
                            const double* tau, const double* kb, int sum_alg, double sum_param, int bs,
                            hipStream_t st);
+fpde_status launch_ref_series(double* S, const double* old, int64_t old_size, int64_t os, int64_t nodes,
+                              const double* tau, int tstep, double* Htmp, const double* coef, int P, double gtj,
+                              double gt, double alpha, hipStream_t st);
 fpde_status launch_gemm_f64(const double* D, int64_t ld, int nodes, int n0, int tc, int block,
                             const double* omega, double* P, int* splits_out, hipStream_t st);
 fpde_status launch_pack_split(int mode, const double* D, int64_t ld, int nodes, int g_lo, int g_hi, void* c0,
@@ -132,6 +171,8 @@ fpde_status launch_lateral3d(double* U, const double* K, const double* iC, const
                              double h2inv, double L, double* Al, double* Bt, hipStream_t st);
 fpde_status launch_zline3d(double* U, double* Uprev, const double* utop, int n, double h, double h2inv,
                            double ksat, int mode, double L, double* Dlev, hipStream_t st);
+fpde_status launch_series3d(double* S, double* H, int P, const double* dnew, const double* ser_c,
+                            const double* ser_v, int64_t nodes, int64_t ld, hipStream_t st);
 fpde_status launch_hist_combine(double* S, const double* part, int nsplit, int64_t part_stride, int64_t part_off,
                                 const double* D, int64_t ld, int64_t nodes, int n, int near_lo,
                                 const double* omega, const double* corr, hipStream_t st);

@@ -87,6 +87,52 @@ __global__ __launch_bounds__(256) void k_ref_sum(double* __restrict__ S, const d
 }
 
 // ---------------------------------------------------------------------------
+// The reference Sum kernel, sum_alg 2 (fpde_hpc_1d.cpp:1285-1302): per node,
+// the moments Htmp[id*P + j] += gt^{j+1} coef[j] (old[1][id] - old[0][id])
+// (non-finite factors read as 0, as the reference kernel does), then
+// S = sum_j v1_j Htmp[id*P + j] / tau[tstep] with the coefficients of
+// _kb_row2_fixed_coefs (:1159-1180): v1_0 = gtj^-alpha,
+// v1_j = v1_{j-1} (alpha+j-1)/(j gtj).
+__global__ __launch_bounds__(256) void k_ref_series(double* __restrict__ S, const double* __restrict__ old,
+                                                    int64_t old_size, int64_t os, int64_t nodes,
+                                                    const double* __restrict__ tau, int tstep,
+                                                    double* __restrict__ Htmp, const double* __restrict__ coef,
+                                                    int P, double gtj, double gt, double alpha) {
+    int64_t id = (int64_t)blockIdx.x * 256 + threadIdx.x;
+    if (id >= nodes) return;
+    double ts = 0.0;
+    if (old_size >= 2) {
+        double cur = old[os + id] - old[id];
+        double* h = Htmp + id * P;
+        double m = gt;
+        for (int j = 0; j < P; j++) {
+            double cc = coef[j];
+            if (!isfinite(cc)) cc = 0.0;
+            double mm = isfinite(m) ? m : 0.0;
+            h[j] += mm * cc * cur;
+            m *= gt;
+        }
+        double v1 = pow(gtj, -alpha);
+        for (int j = 0; j < P; j++) {
+            ts += (isfinite(v1) ? v1 : 0.0) * h[j];
+            double i = (double)(j + 1);
+            v1 *= -(-alpha - i + 1.0) / (i * gtj);
+        }
+        ts /= tau[tstep];
+    }
+    S[id] = ts;
+}
+
+fpde_status launch_ref_series(double* S, const double* old, int64_t old_size, int64_t os, int64_t nodes,
+                              const double* tau, int tstep, double* Htmp, const double* coef, int P, double gtj,
+                              double gt, double alpha, hipStream_t st) {
+    if (nodes <= 0) return FPDE_OK;
+    hipLaunchKernelGGL(k_ref_series, dim3((unsigned)((nodes + 255) / 256)), dim3(256), 0, st, S, old, old_size, os,
+                       nodes, tau, tstep, Htmp, coef, P, gtj, gt, alpha);
+    return FPDE_OK;
+}
+
+// ---------------------------------------------------------------------------
 // MFMA operand types
 typedef double d4_t __attribute__((ext_vector_type(4)));
 typedef float f4_t __attribute__((ext_vector_type(4)));

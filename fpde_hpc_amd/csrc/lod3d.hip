@@ -209,6 +209,26 @@ __global__ __launch_bounds__(256) void k_hist_combine(double* __restrict__ S, co
     S[i] = far + (nr0 + nr1) + cr;
 }
 
+// Series-compressed history (sum_alg 2): add the newest interval dnew to the
+// moments and sum the series, S[i] = sum_j v_j (H[j][i] += c_j dnew[i]);
+// dnew == nullptr (step 0) gives S = 0.
+__global__ __launch_bounds__(256) void k_series3d(double* __restrict__ S, double* __restrict__ H, int P,
+                                                  const double* __restrict__ dnew,
+                                                  const double* __restrict__ ser_c,
+                                                  const double* __restrict__ ser_v, int64_t nodes, int64_t ld) {
+    int64_t i = (int64_t)blockIdx.x * 256 + threadIdx.x;
+    if (i >= nodes) return;
+    if (!dnew) { S[i] = 0.0; return; }
+    S[i] = series_update(H, ld, i, P, dnew[i], ser_c, ser_v);
+}
+
+fpde_status launch_series3d(double* S, double* H, int P, const double* dnew, const double* ser_c,
+                            const double* ser_v, int64_t nodes, int64_t ld, hipStream_t st) {
+    hipLaunchKernelGGL(k_series3d, dim3((unsigned)((nodes + 255) / 256)), dim3(256), 0, st, S, H, P, dnew, ser_c,
+                       ser_v, nodes, ld);
+    return FPDE_OK;
+}
+
 fpde_status launch_coef3d(const double* U, int n, double h, double ksat, int mode, double* K,
                           double* iC, hipStream_t st) {
     int64_t nn = (int64_t)n * n * n;

@@ -47,13 +47,24 @@ fpde_status validate(const fpde_spec& s) {
         if (b != 16 && b != 32 && b != 64 && b != 128) { set_error("block must be 16, 32, 64 or 128"); return FPDE_ERR_ARG; }
     }
     if (s.dims == 3 && s.picard_max != 1) { set_error("3D LOD uses lagged coefficients (picard_max must be 1)"); return FPDE_ERR_ARG; }
-    if (s.sum_alg != 0 && s.sum_alg != 1) {
-        set_error("sum_alg must be 0 (full sum) or 1 (restricted sum); the series form 2 is not provided");
+    if (s.sum_alg < 0 || s.sum_alg > 2) {
+        set_error("sum_alg must be 0 (full sum), 1 (restricted sum) or 2 (series moments)");
         return FPDE_ERR_ARG;
     }
     if (s.sum_alg == 1 && !(s.sum_param >= 0.0 && std::isfinite(s.sum_param))) {
         set_error("sum_param must be finite and >= 0 for the restricted sum");
         return FPDE_ERR_ARG;
+    }
+    if (s.sum_alg == 2) {
+        if (!(s.sum_param >= 1.0 && s.sum_param <= kMaxSeriesTerms && s.sum_param == std::floor(s.sum_param))) {
+            set_error("sum_alg 2: sum_param is the number of series terms, an integer in [1, 64]");
+            return FPDE_ERR_ARG;
+        }
+        if (s.do_q > 0) { set_error("sum_alg 2 is for a single order (do_q must be 0)"); return FPDE_ERR_ARG; }
+        if (s.hist_mode != HIST_PLAIN_F64) {
+            set_error("sum_alg 2 keeps no history levels: hist_mode must be 0 (plain)");
+            return FPDE_ERR_ARG;
+        }
     }
     return FPDE_OK;
 }
@@ -62,10 +73,11 @@ Plan make_plan(const fpde_spec& s) {
     Plan p;
     p.nodes = (s.dims == 1) ? s.nodes : (int64_t)s.nodes * s.nodes * s.nodes;
     p.ld = round_up(p.nodes, 64);
-    p.levels = round_up(s.steps + 1, kGroup);
+    // sum_alg 2 keeps moments instead of levels: D is a ring of the last two
+    p.levels = (s.sum_alg == 2) ? 2 : round_up(s.steps + 1, kGroup);
     int rows_plain = 64;
     int rows_gemm = (s.hist_mode == HIST_PLAIN_F64) ? 0 : gemm_max_splits(p.nodes) * s.block;
-    p.part_rows = std::max(rows_plain, rows_gemm);
+    p.part_rows = (s.sum_alg == 2) ? 1 : std::max(rows_plain, rows_gemm);
     return p;
 }
 
@@ -79,6 +91,11 @@ size_t footprint_bytes(const fpde_spec& s) {
     b += sizeof(double) * (size_t)p.ld * 8;                      // U, S, xyz, ...
     if (s.dims == 3) b += sizeof(double) * (size_t)p.ld * 6;     // Uprev, K, iC, Al, Bt
     b += sizeof(double) * (size_t)(s.steps + 512) * 3;           // weights
+    if (s.sum_alg == 2) {
+        size_t P = (size_t)s.sum_param;
+        b += sizeof(double) * P * p.ld;                              // moments
+        b += sizeof(double) * 2 * P * (size_t)s.steps;               // coefficient tables
+    }
     return b;
 }
 
@@ -101,6 +118,7 @@ struct fpde_solver {
     double* part = nullptr;
     double* Uit = nullptr;
     double* dmax = nullptr;
+    double* H = nullptr;       // sum_alg 2 moments [P][ld]
     int* flags = nullptr;
     void* comp[3] = {nullptr, nullptr, nullptr};
     float* scale = nullptr;
@@ -187,6 +205,13 @@ fpde_status solver_alloc(fpde_solver* s) {
     DALLOC(s->w.omega, sizeof(double) * (size_t)(M + 512));
     DALLOC(s->w.corr, sizeof(double) * 2 * (size_t)(M + 1));
     FPDE_HIP_TRY(hipMemsetAsync(s->w.omega, 0, sizeof(double) * (size_t)(M + 512), s->stream));
+    if (sp.sum_alg == 2) {
+        int P = (int)sp.sum_param;
+        s->w.ser_P = P;
+        DALLOC(s->w.ser_c, sizeof(double) * (size_t)M * P);
+        DALLOC(s->w.ser_v, sizeof(double) * (size_t)M * P);
+        DALLOC(s->H, sizeof(double) * (size_t)P * p.ld);
+    }
     DALLOC(s->U, sizeof(double) * p.ld);
     DALLOC(s->D, sizeof(double) * (size_t)p.levels * p.ld);
     FPDE_HIP_TRY(hipMemsetAsync(s->D, 0, sizeof(double) * (size_t)p.levels * p.ld, s->stream));
@@ -225,6 +250,13 @@ fpde_status solver_alloc(fpde_solver* s) {
 fpde_status far_part(fpde_solver* s, int n, int* nsplit, int64_t* stride, int64_t* off, int* near_lo) {
     const fpde_spec& sp = s->spec;
     const Plan& p = s->plan;
+    if (sp.sum_alg == 2) {           // series moments: no history levels to sum
+        *nsplit = 0;
+        *stride = p.ld;
+        *off = 0;
+        *near_lo = n;
+        return FPDE_OK;
+    }
     // levels available in memory: t < avail
     int avail = s->pending ? n - 1 : n;
     if (sp.hist_mode == HIST_PLAIN_F64) {
@@ -335,6 +367,13 @@ fpde_status launch_1d(fpde_solver* s, int n, int k, int nsplit, int64_t stride, 
     a.flags = s->flags;
     a.tol = sp.picard_tol;
     a.prev_k = s->last_k;
+    a.ser_P = s->w.ser_P;
+    if (a.ser_P > 0) {
+        size_t row = (size_t)std::min(n, sp.steps - 1) * a.ser_P;   // the finalize launch reads no row
+        a.H = s->H;
+        a.ser_c = s->w.ser_c + row;
+        a.ser_v = s->w.ser_v + row;
+    }
     size_t slot = 0;
     ev_begin(s, 1, &slot);
     fpde_status rc = launch_step1d(a, s->stream);
@@ -375,9 +414,17 @@ fpde_status enqueue_3d_step(fpde_solver* s, int n) {
     int nn = sp.nodes;
     double h2inv = 1.0 / (s->h * s->h);
     size_t slot = 0;
+    const bool series = s->w.ser_P > 0;
     ev_begin(s, 0, &slot);
-    rc = launch_hist_combine(s->S, s->part, nsplit, stride, off, s->D, p.ld, p.nodes, n, near_lo, s->w.omega,
-                             s->w.corr, s->stream);
+    if (series) {
+        size_t row = (size_t)n * s->w.ser_P;
+        const double* dnew = (n >= 1) ? s->D + (size_t)((n - 1) & 1) * p.ld : nullptr;
+        rc = launch_series3d(s->S, s->H, s->w.ser_P, dnew, s->w.ser_c + row, s->w.ser_v + row, p.nodes, p.ld,
+                             s->stream);
+    } else {
+        rc = launch_hist_combine(s->S, s->part, nsplit, stride, off, s->D, p.ld, p.nodes, n, near_lo, s->w.omega,
+                                 s->w.corr, s->stream);
+    }
     ev_end(s, slot);
     if (rc != FPDE_OK) return rc;
     ev_begin(s, 1, &slot);
@@ -386,7 +433,7 @@ fpde_status enqueue_3d_step(fpde_solver* s, int n) {
     if ((rc = launch_coef3d(s->U, nn, s->h, sp.ksat, sp.coef_mode, s->K, s->iC, s->stream)) != FPDE_OK) return rc;
     if ((rc = launch_lateral3d(s->U, s->K, s->iC, nullptr, nn, 1, h2inv, s->w.local, s->Al, s->Bt, s->stream)) != FPDE_OK) return rc;
     if ((rc = launch_zline3d(s->U, s->Uprev, s->utop, nn, s->h, h2inv, sp.ksat, sp.coef_mode, s->w.local,
-                             s->D + (size_t)n * p.ld, s->stream)) != FPDE_OK) return rc;
+                             s->D + (size_t)(series ? (n & 1) : n) * p.ld, s->stream)) != FPDE_OK) return rc;
     ev_end(s, slot);
     s->stats.step_launches += 5;
     s->stats.picard_iters_total++;
@@ -445,6 +492,7 @@ fpde_status fpde_solver_set_initial(fpde_solver* s, const double* u0) {
     const Plan& p = s->plan;
     FPDE_HIP_TRY(hipMemsetAsync(s->U, 0, sizeof(double) * p.ld, s->stream));
     FPDE_HIP_TRY(hipMemcpyAsync(s->U, u0, sizeof(double) * p.nodes, hipMemcpyHostToDevice, s->stream));
+    if (s->H) FPDE_HIP_TRY(hipMemsetAsync(s->H, 0, sizeof(double) * (size_t)s->w.ser_P * p.ld, s->stream));
     if (s->spec.dims == 1) {
         s->utop1d = u0[0];
     } else {

@@ -131,16 +131,18 @@ __global__ __launch_bounds__(256) void k_step1d(Step1DArgs g) {
     if (g.recover && inner_node) { fx = g.xyz[i]; fy = g.xyz[g.ld + i]; fz = g.xyz[2 * g.ld + i]; }
     double uold = act ? g.U[i] : 0.0;                    // U^{n-1} (recovering, k=0) or U^n
     double Sk = (!first && act) ? g.Sbuf[i] : 0.0;
-    double dmem = (first && !g.recover && act && n >= 1) ? g.D[(int64_t)(n - 1) * g.ld + i] : 0.0;
-    double d0m = (first && act && n >= 2) ? g.D[i] : 0.0;
-    double d1m = (first && act && n >= 3) ? g.D[g.ld + i] : 0.0;
+    const bool series = g.ser_P > 0;
+    const int64_t drow = series ? ((n - 1) & 1) : (n - 1);    // row of D[n-1]
+    double dmem = (first && !g.recover && act && n >= 1) ? g.D[drow * g.ld + i] : 0.0;
+    double d0m = (first && !series && act && n >= 2) ? g.D[i] : 0.0;
+    double d1m = (first && !series && act && n >= 3) ? g.D[g.ld + i] : 0.0;
     // Picard convergence of the previous launch (decided one launch late: the
     // test needs the max over every chunk): dmax of the previous launch and the
     // iterate it stored
     const bool check_conv = g.recover && g.prev_k >= 1;
     double dprev = (check_conv && tid < C) ? g.dmax[(size_t)(1 - g.parity) * C + tid] : 0.0;
     double uitp = ((check_conv || g.k >= 2) && act) ? g.Uit[i] : 0.0;
-    double hm = (first && g.k == 0 && act && n >= 1) ? hist_mem(g, i, n) : 0.0;
+    double hm = (first && !series && g.k == 0 && act && n >= 1) ? hist_mem(g, i, n) : 0.0;
 
     // ---- 1. finish the previous solve: reduced system, then back-substitute --
     double val = 0.0;
@@ -177,11 +179,14 @@ __global__ __launch_bounds__(256) void k_step1d(Step1DArgs g) {
             double dnew = dmem;
             if (g.recover) {                     // val = U^n, uold = U^{n-1}
                 dnew = val - uold;
-                g.D[(int64_t)(n - 1) * g.ld + i] = dnew;
+                g.D[drow * g.ld + i] = dnew;
                 g.U[i] = val;
                 un = val;
             }
-            if (g.k >= 0 && n >= 1) {
+            if (g.k >= 0 && n >= 1 && series) {
+                // moments += newest interval, then the series (sum_alg 2)
+                S = series_update(g.H, g.ld, i, g.ser_P, dnew, g.ser_c, g.ser_v);
+            } else if (g.k >= 0 && n >= 1) {
                 // S_n = far + near (in memory) + newest level + two-level corrections
                 double newest = (g.near_lo <= n - 1) ? g.omega[0] * dnew : 0.0;
                 double cr = (n == 1) ? g.corr[2] * dnew : g.corr[2 * n] * d0m;

@@ -103,6 +103,62 @@ __global__ void k_weights_corr(OrderSet os, double tau, int M, double eps, int s
     corr[idx] = s - omega[n - t - 1];
 }
 
+// Series-compressed history (time_summing sum_alg 2, fpde_hpc_1d.cpp:4137-4151).
+// With g(t)=t the kernel (a-s)^-a = sum_i (a)_i/i! a^{-a-i} s^i turns the
+// history into moments H_j = sum_t int_{t_t}^{t_{t+1}} s^j ds D_t, updated
+// once per step with the newest interval and summed with the coefficients
+// of _kb_row2_fixed_coefs (:1159-1180).  Row n (unknown U^{n+1}):
+//   ser_c[n][j] = t_n^{j+1} v3_j  for j < min(P, K_n), else 0, where v3_j and
+//     K_n are the cached terms of the _kb_row2 series of [t_{n-1}, t_n] at
+//     a = t_{n+1} (eps = series_eps, :2015, :2114-2119; the series stops
+//     after the first |term| <= eps);
+//   ser_v[n][j] = Da/tau * t_{n+1}^-a prod_{i=1..j} (a+i-1)/(i t_{n+1}).
+// One thread per step n; the recurrences are the reference's.
+__global__ void k_series_tables(double a, double tau, int M, int P, double eps, double scale,
+                                double* __restrict__ ser_c, double* __restrict__ ser_v) {
+    int n = blockIdx.x * blockDim.x + threadIdx.x;
+    if (n >= M) return;
+    double* c = ser_c + (size_t)n * P;
+    double* v = ser_v + (size_t)n * P;
+    if (n == 0 || a == 1.0) {                     // a == 1: no cached terms (:2011-2014)
+        for (int j = 0; j < P; j++) { c[j] = 0.0; v[j] = 0.0; }
+        if (n == 0) return;
+    }
+    const double p0 = (n - 1) * tau, p1 = n * tau, pa = (n + 1) * tau;
+    if (a != 1.0) {
+        double v1 = 1.0, v2 = pow(pa, -a) * p1, v6 = p0 / p1, mult = p1;
+        bool live = true;                         // the series has not stopped yet
+        int niter = 0;
+        for (int j = 0; j < P; j++) {
+            double v3 = 0.0;
+            if (live) {
+                if (j != 0) {
+                    double i = (double)j;
+                    v1 *= -(-a - i + 1.0) / i;
+                    v2 *= p1 / pa;
+                    v6 *= p0 / p1;
+                    v3 = (1.0 / (i + 1.0)) * (1.0 - v6);
+                    if (fabs(v3) > fabs(v3) * eps) niter++;
+                } else {
+                    v3 = (p1 - p0) / p1;
+                }
+                c[j] = mult * v3;
+                mult *= p1;
+                niter++;
+                if (!(fabs(v1 * v2 * v3) > eps) || niter > 100000) live = false;
+            } else {
+                c[j] = 0.0;
+            }
+        }
+    }
+    double gtj = pa, w = scale * pow(gtj, -a);
+    for (int j = 0; j < P; j++) {
+        v[j] = w;
+        double i = (double)(j + 1);
+        w *= -(-a - i + 1.0) / (i * gtj);
+    }
+}
+
 static void gauss_legendre(int n, double a, double b, double* x, double* w) {
     for (int i = 0; i < n; i++) {
         double z = std::cos(M_PI * (i + 0.75) / (n + 0.5)), pp = 0.0, p1 = 1.0, p2 = 0.0;
@@ -209,6 +265,11 @@ fpde_status launch_weights(const fpde_spec& s, DeviceWeights& w, hipStream_t st)
                            s.series_eps, s.weight_mode == 1 ? 1 : 0, cut, (const double*)w.omega, w.corr);
     } else {
         FPDE_HIP_TRY(hipMemsetAsync(w.corr, 0, sizeof(double) * 2 * (size_t)M, st));
+    }
+    if (s.sum_alg == 2) {
+        if (!w.ser_c || !w.ser_v || w.ser_P < 1) { set_error("series tables not allocated"); return FPDE_ERR_ARG; }
+        hipLaunchKernelGGL(k_series_tables, dim3((M + 63) / 64), dim3(64), 0, st, os.a[0], s.tau, M, w.ser_P,
+                           s.series_eps, os.c[0], w.ser_c, w.ser_v);
     }
     FPDE_HIP_TRY(hipGetLastError());
     return FPDE_OK;

@@ -77,7 +77,12 @@ fpde_status fpde_cl_kernel_destroy(fpde_cl_kernel *k);
 /* SetArg(idx,size,ptr) / SetBufferArg(buf,idx) (opencl_cuda_base.cpp:198-209). */
 fpde_status fpde_cl_kernel_set_arg(fpde_cl_kernel *k, int idx, size_t size, const void *value);
 fpde_status fpde_cl_kernel_set_buffer_arg(fpde_cl_kernel *k, int idx, fpde_cl_buf *buf);
-/* ExecuteKernel(k,ndims,dims,ldims) (opencl_cuda_base.cpp:229-321). */
+/* ExecuteKernel(k,ndims,dims,ldims) (opencl_cuda_base.cpp:229-321).  "Sum"
+ * runs with the argument ABI of fr_ocl_call_sum (fpde_hpc_1d.cpp:1851-1868);
+ * sum_alg 0/1 read 0 S, 1 old, 2 old_size, 3 tau, 6 kb_3_cache, 7 sum_param;
+ * sum_alg 2 also 4 Htmp [nodes][sum_param], 5 row2_precalc, 8 gtj, 9 gt,
+ * 10 alpha, 11 tstep, with the series coefficients of time_summing
+ * (:4137-4151, _kb_row2_fixed_coefs :1159-1180). */
 fpde_status fpde_cl_launch(fpde_cl_ctx *ctx, fpde_cl_kernel *k, int ndims,
                            const size_t *gdims, const size_t *ldims);
 /* Finish() (opencl_cuda_base.cpp:322-341). */
@@ -133,11 +138,16 @@ typedef struct {
                             3 split fp16 hi+lo + scaling, 4 split bf16 x3        */
     int block;           /* future steps per GEMM block (multiple of 16, <=128)  */
     int use_graph;       /* reserved (the step loop is issued on one stream)     */
-    int sum_alg;         /* 0 full history sum; 1 truncated: newest intervals
-                            only, down to the first weight with |kb| < sum_param
-                            (time_summing sum_alg 1, fpde_hpc_1d.cpp:4117-4131);
-                            single order only                                   */
-    double sum_param;    /* truncation threshold on the L1 weight kb             */
+    int sum_alg;         /* time_summing form (fpde_hpc_1d.cpp:4086-4164):
+                            0 full history sum;
+                            1 restricted: newest intervals only, down to the
+                              first weight with |kb| < sum_param (:4117-4131;
+                              per order for distributed order);
+                            2 series moments: P = sum_param terms of the
+                              binomial series kept per node instead of the
+                              levels (:4137-4151); single order, hist_mode 0,
+                              P in [1, 64]                                      */
+    double sum_param;    /* sum_alg 1: threshold on kb; sum_alg 2: terms P       */
     int reserved[2];
 } fpde_spec;
 

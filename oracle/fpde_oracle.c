@@ -193,6 +193,74 @@ void orc_time_summing(int nodes, int levels, const double *old,
 }
 
 /* ------------------------------------------------------------------------ */
+/* sum_alg 2.  Under sum_alg 2, precalc_kb_3 (fpde_hpc_1d.cpp:4076-4081) calls
+ * kb(alpha, tstep-1, tstep, tstep+1, .., time_i1, 0), which takes the series
+ * branch for every t1 != a (:2280-2285) and caches v3 of each evaluated term
+ * in row2_precalc[time_i1][0] (:2114-2119; eps = global_eps for idx ==
+ * time_i1, :2015).  alpha == 1 returns before the cache (:2011-2014). */
+int orc_series_row(double alpha, double p0, double p1, double pa, double eps,
+                   int cap, double *v3out)
+{
+    double v1 = 1.0, v2, v3, v6, term, i = 0.0;
+    int niter = 0, k = 0;
+    if (alpha == 1.0) return 0;
+    v2 = pow(pa, -alpha) * p1;                                /* :2031 */
+    v6 = p0 / p1;                                             /* :2032 */
+    do {
+        if (i != 0.0) {
+            v1 *= -(-alpha - i + 1.0) / i;                    /* :2045 */
+            v2 *= p1 / pa;                                    /* :2049 */
+            v6 *= p0 / p1;                                    /* :2050 */
+        }
+        if (i == 0.0)
+            v3 = (p1 - p0) / p1;                              /* :2070 */
+        else {
+            v3 = (1.0 / (i + 1.0)) * (1.0 - v6);              /* :2078 */
+            if (fabs(v3) > fabs(v3) * eps) niter++;           /* one inner pass */
+        }
+        if (k < cap) v3out[k] = v3;
+        k++;
+        term = v1 * v2 * v3;
+        i += 1.0;
+        niter++;
+        if (niter > 100000) break;                            /* integr_max_niter */
+    } while (fabs(term) > eps);
+    return k;
+}
+
+/* _kb_row2_fixed_coefs, fpde_hpc_1d.cpp:1159-1180 (right == 0). */
+static double series_fixed_coefs(double gtj, double gtjma, const double *c, int n, double alpha)
+{
+    double sum = 0.0, v1 = gtjma, i = 0.0;
+    int ii = 0;
+    do {
+        sum += v1 * c[ii];
+        i += 1.0;
+        ii++;
+        v1 *= -(-alpha - i + 1.0) / (i * gtj);
+    } while (ii != n);
+    return sum;
+}
+
+void orc_series_step(int nodes, int P, double *H, const double *v3, int K,
+                     double p1, double gtj, double alpha, double tau,
+                     const double *u0, const double *u1, double *S)
+{
+    double gtjma = pow(gtj, -alpha);
+    for (int id = 0; id < nodes; id++) {
+        double *h = H + (size_t)id * P;
+        double diff = u1[id] - u0[id];
+        double mult = p1;                                     /* :4143 */
+        for (int j = 0; j < P; j++)
+            if (K > j) {                                      /* :4145 */
+                h[j] += mult * v3[j] * diff;                  /* :4147 */
+                mult *= p1;
+            }
+        S[id] = series_fixed_coefs(gtj, gtjma, h, P, alpha) / tau;   /* :4150 */
+    }
+}
+
+/* ------------------------------------------------------------------------ */
 /* van Genuchten-Mualem soil, fpde_hpc_1d.cpp:2513-2524 (cv_form 4),
  * :2558-2571 (1/C), :2634-2635 (k_form 1).  P = suction head [m]. */
 #define VGM_N 1.25
@@ -307,6 +375,8 @@ typedef struct {
     double *Time;   /* cumulative time grid, as init_time_step builds it */
     double *tau;
     double *kb3;    /* per-order scratch */
+    double *Hs;     /* sum_alg 2: moments [q][nodes][P] (Htmp, :3342-3348) */
+    double *v3;     /* sum_alg 2: coefficient row scratch */
 } weights_ctx;
 
 static void wctx_init(weights_ctx *w, const orc_params *p)
@@ -315,11 +385,13 @@ static void wctx_init(weights_ctx *w, const orc_params *p)
     w->Time = (double *)malloc(sizeof(double) * (p->steps + 2));
     w->tau = (double *)malloc(sizeof(double) * (p->steps + 1));
     w->kb3 = (double *)malloc(sizeof(double) * (p->steps + 1));
+    w->Hs = NULL;
+    w->v3 = NULL;
     w->Time[0] = 0.0;
     for (int k = 1; k <= p->steps + 1; k++) w->Time[k] = w->Time[k - 1] + p->tau;  /* :3944 */
     for (int k = 0; k <= p->steps; k++) w->tau[k] = p->tau;
 }
-static void wctx_free(weights_ctx *w) { free(w->Time); free(w->tau); free(w->kb3); }
+static void wctx_free(weights_ctx *w) { free(w->Time); free(w->tau); free(w->kb3); free(w->Hs); free(w->v3); }
 
 /* ------------------------------------------------------------------------ */
 /* Tridiagonal line solve of the reference step for one line of nodes.
@@ -385,13 +457,25 @@ static void history_term(weights_ctx *w, const orc_params *p, int n, int nodes,
                          const double *old, double *H, double *tmp, double *Lout)
 {
     double L = 0.0;
+    int P = (int)p->sum_param;
     for (int i = 0; i < nodes; i++) H[i] = 0.0;
+    if (p->sum_alg == 2 && w->Hs == NULL) {
+        w->Hs = (double *)calloc((size_t)w->q * nodes * P, sizeof(double));
+        w->v3 = (double *)malloc(sizeof(double) * P);
+    }
     for (int q = 0; q < w->q; q++) {
         double a = w->al[q];
         double kkk0 = orc_l1_weights(a, n, w->Time, p->weight_mode, p->series_eps, w->kb3);
         double Da = da_of(a);
         L += w->cw[q] * Da * kkk0 / w->tau[n];
-        if (n >= 1) {
+        if (n >= 1 && p->sum_alg == 2) {
+            /* moments of the newest interval [n-1, n], then the series at n+1 */
+            const double *T = w->Time;
+            int K = orc_series_row(a, T[n - 1], T[n], T[n + 1], p->series_eps, P, w->v3);
+            orc_series_step(nodes, P, w->Hs + (size_t)q * nodes * P, w->v3, K, T[n], T[n + 1], a,
+                            w->tau[n], old + (size_t)(n - 1) * nodes, old + (size_t)n * nodes, tmp);
+            for (int i = 0; i < nodes; i++) H[i] += w->cw[q] * Da * tmp[i];
+        } else if (n >= 1) {
             orc_time_summing(nodes, n + 1, old, w->kb3, w->tau, tmp, p->nthreads, p->sum_alg, p->sum_param);
             for (int i = 0; i < nodes; i++) H[i] += w->cw[q] * Da * tmp[i];
         }

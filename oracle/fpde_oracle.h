@@ -11,7 +11,9 @@
  *     binomial series, :2167-2301 dispatch, :3942-3949 kkk0, :4069-4085
  *     kb_3_cache);
  *   - the history sum sum_t kb_3_cache[t]*(old[t+1]-old[t])/tau[t]
- *     (fpde_hpc_1d.cpp:4086-4164, sum_alg 0, node-outer / t-inner);
+ *     (fpde_hpc_1d.cpp:4086-4164: sum_alg 0 full, node-outer / t-inner;
+ *     sum_alg 1 restricted; sum_alg 2 series moments, :4137-4151 with
+ *     _kb_row2_fixed_coefs :1159-1180);
  *   - van Genuchten-Mualem coefficients (fpde_hpc_1d.cpp:2493-2575, :2625-2657);
  *   - the tridiagonal step A_/B/R + al1/Om1/bt1/back-substitution
  *     (fpde_hpc_1d.cpp:3095-3142, :4002-4027, :4165-4268, :4491-4590, :4672-4725);
@@ -53,6 +55,20 @@ void orc_time_summing(int nodes, int levels, const double *old,
                       const double *kb3, const double *tau, double *S,
                       int nthreads, int sum_alg, double sum_param);
 
+/* ---- series-compressed history (time_summing sum_alg 2) ----
+ * Coefficients the reference caches for the newest interval [p0,p1] at
+ * a = pa (row2_precalc[time_i1][0], filled by _kb_row2): v3[i] for every
+ * term the series evaluates (it stops after the first |term| <= eps).
+ * Returns the number of terms K; the first min(K, cap) go to v3. */
+int orc_series_row(double alpha, double p0, double p1, double pa, double eps,
+                   int cap, double *v3);
+/* One step of sum_alg 2 for every node (fpde_hpc_1d.cpp:4137-4151):
+ * H[id*P+j] += p1^(j+1) * v3[j] * (u1[id]-u0[id]) for j < min(P, K), then
+ * S[id] = _kb_row2_fixed_coefs(gtj, gtj^-alpha, H[id*P..], P) / tau. */
+void orc_series_step(int nodes, int P, double *H, const double *v3, int K,
+                     double p1, double gtj, double alpha, double tau,
+                     const double *u0, const double *u1, double *S);
+
 /* ---- VGM coefficients, P = suction head in metres ---- */
 double orc_wetness(double P);
 double orc_inv_dw_dh(double P);
@@ -78,8 +94,8 @@ typedef struct {
     int    do_rule;      /* 0 = Gauss-Legendre, 1 = trapezoid (fr_do.cpp)  */
     double do_a0, do_a1; /* order interval                                 */
     int    nthreads;
-    int    sum_alg;      /* 0 full sum, 1 truncated (time_summing sum_alg 1)   */
-    double sum_param;    /* truncation threshold on |kb|                      */
+    int    sum_alg;      /* 0 full sum, 1 truncated (sum_alg 1), 2 series (sum_alg 2) */
+    double sum_param;    /* sum_alg 1: threshold on |kb|; sum_alg 2: series terms P  */
 } orc_params;
 
 /* 1D solve: u (nodes) holds u0 on entry, u^M on exit.  u_top = u0[0]. */

@@ -53,6 +53,10 @@ def lib():
         L.orc_l1_weights.restype = ctypes.c_double
         L.orc_time_summing.argtypes = [ctypes.c_int, ctypes.c_int, _dp, _dp, _dp, _dp, ctypes.c_int,
                                        ctypes.c_int, ctypes.c_double]
+        L.orc_series_row.argtypes = [ctypes.c_double] * 5 + [ctypes.c_int, _dp]
+        L.orc_series_row.restype = ctypes.c_int
+        L.orc_series_step.argtypes = [ctypes.c_int, ctypes.c_int, _dp, _dp, ctypes.c_int] + \
+            [ctypes.c_double] * 4 + [_dp, _dp, _dp]
         for f in ("orc_wetness", "orc_inv_dw_dh", "orc_kr"):
             getattr(L, f).argtypes = [ctypes.c_double]
             getattr(L, f).restype = ctypes.c_double
@@ -117,6 +121,25 @@ def time_summing(old, kb3, tau, nthreads=0, sum_alg=0, sum_param=1e-5):
     tau = np.ascontiguousarray(tau, dtype=np.float64)
     S = np.zeros(nodes)
     lib().orc_time_summing(nodes, levels, _p(old), _p(kb3), _p(tau), _p(S), nthreads, sum_alg, sum_param)
+    return S
+
+
+def series_row(alpha, p0, p1, pa, eps=1e-5, cap=64):
+    """(K, v3[:min(K, cap)]) of the sum_alg 2 coefficient row."""
+    v3 = np.zeros(max(cap, 1))
+    K = lib().orc_series_row(alpha, p0, p1, pa, eps, cap, _p(v3))
+    return K, v3[:min(K, cap)]
+
+
+def series_step(H, v3, K, p1, gtj, alpha, tau, u0, u1):
+    """One sum_alg 2 step: updates H [nodes, P] in place, returns S."""
+    nodes, P = H.shape
+    assert H.flags.c_contiguous and H.dtype == np.float64
+    v3 = np.ascontiguousarray(np.concatenate([v3, np.zeros(P)])[:max(P, len(v3))], dtype=np.float64)
+    u0 = np.ascontiguousarray(u0, dtype=np.float64)
+    u1 = np.ascontiguousarray(u1, dtype=np.float64)
+    S = np.zeros(nodes)
+    lib().orc_series_step(nodes, P, _p(H), _p(v3), K, p1, gtj, alpha, tau, _p(u0), _p(u1), _p(S))
     return S
 
 

@@ -50,7 +50,10 @@ def test_ksat_fourier_matches_oracle(fpde, oracle):
 @pytest.mark.parametrize("bad", [
     dict(dims=2), dict(nodes=3), dict(steps=0), dict(alpha=0.0), dict(alpha=1.5),
     dict(picard_max=0), dict(hist_mode=9), dict(block=48), dict(dims=3, nodes=300),
-    dict(tau=-1.0), dict(nodes=70000), dict(sum_alg=2), dict(sum_alg=1, sum_param=-1.0),
+    dict(tau=-1.0), dict(nodes=70000), dict(sum_alg=3), dict(sum_alg=1, sum_param=-1.0),
+    dict(sum_alg=2, sum_param=0.0, hist_mode=0), dict(sum_alg=2, sum_param=65.0, hist_mode=0),
+    dict(sum_alg=2, sum_param=2.5, hist_mode=0), dict(sum_alg=2, sum_param=8.0, hist_mode=1),
+    dict(sum_alg=2, sum_param=8.0, hist_mode=0, do_q=4),
 ])
 def test_invalid_specs_rejected(fpde, bad):
     kw = dict(nodes=100, steps=10, tau=1e-3, alpha=0.5, hist_mode=1, block=64)
@@ -68,6 +71,9 @@ def test_footprint_scales_with_history(fpde):
     b = fpde.footprint(fpde.make_spec(10000, 20000, 5e-4, hist_mode="split_bf16", ksat=1.0))
     assert a >= 8 * 10000 * 20000
     assert b >= a + 4 * 10000 * 20000 - 1
+    # series moments (sum_alg 2): P values per node instead of M levels
+    c = fpde.footprint(fpde.make_spec(10000, 20000, 5e-4, hist_mode="plain", ksat=1.0, sum_alg=2, sum_param=32))
+    assert c < a / 50
 
 
 def test_unknown_kernel_name_is_reported(fpde):

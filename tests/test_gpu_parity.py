@@ -446,3 +446,85 @@ def test_restricted_sum_distributed_order_and_3d(fpde, oracle):
         err = rel_err(u, oracle.solve_3d(p, u3))
         print(f"restricted 3D {mode}: rel err {err:.3e}")
         assert err <= (TOL_MIXED if mode.startswith("split") else TOL_F64)
+
+
+# ---------------------------------------------------------- series moments (sum_alg 2)
+@pytest.mark.parametrize("P", [1, 10, 64])
+@pytest.mark.parametrize("picard", [1, 3])
+def test_series_history_1d_matches_oracle(fpde, oracle, P, picard):
+    nodes, steps, alpha, tau = 400, 300, 0.7, 1e-3
+    u0 = fpde.init_1d(nodes, 50 + P)
+    spec = fpde.make_spec(nodes, steps, tau, alpha=alpha, u_top=u0[0], picard_max=picard,
+                          picard_tol=1e-9 if picard > 1 else 0.0, sum_alg=2, sum_param=float(P))
+    u = fpde.solve(spec, u0)
+    ref = _oracle_1d(oracle, spec, u0, picard_tol=spec.picard_tol)
+    err = rel_err(u, ref)
+    print(f"series P={P} picard={picard}: rel err {err:.3e}")
+    assert err <= TOL_F64
+
+
+def test_series_history_pieces_and_3d(fpde, oracle):
+    nodes, steps = 300, 120
+    u0 = fpde.init_1d(nodes, 51)
+    spec = fpde.make_spec(nodes, steps, 1e-3, alpha=0.6, u_top=u0[0], sum_alg=2, sum_param=12.0)
+    with fpde.Solver(spec) as s:
+        s.set_initial(u0)
+        for chunk in (1, 7, 40, 1000):
+            s.advance(chunk)
+            s.solution()            # finalize between pieces
+        u = s.solution()
+    assert rel_err(u, _oracle_1d(oracle, spec, u0)) <= TOL_F64
+    n = 10
+    u3 = fpde.init_3d(n, 52)
+    spec3 = fpde.make_spec(n, 60, 1e-3, alpha=0.85, dims=3, u_top=u3[0], sum_alg=2, sum_param=10.0)
+    u = fpde.solve(spec3, u3)
+    p = oracle.make_params(n, 60, 1e-3, 0.85, ksat=spec3.ksat, sum_alg=2, sum_param=10.0)
+    err = rel_err(u, oracle.solve_3d(p, u3))
+    print(f"series 3D: rel err {err:.3e}")
+    assert err <= TOL_F64
+
+
+def test_reference_sum_kernel_series_through_facade(fpde, oracle):
+    """'Sum' with sum_alg 2 (fr_ocl_call_sum, fpde_hpc_1d.cpp:1837-1868): two
+    consecutive steps update the moments Htmp and return the series sum, as
+    time_summing's sum_alg 2 branch (:4137-4151) does."""
+    L = fpde.lib()
+    rng = np.random.default_rng(13)
+    nodes, OS, P, alpha, tau = 150, 192, 16, 0.7, 1e-3
+    T = oracle.time_grid(60, tau)
+    H0 = rng.standard_normal((OS, P)) * 1e-3
+    Href = H0[:nodes].copy()
+    src = b"#define sum_alg 2\n#define OS 192\n#define BS 16\n__kernel void Sum(...)"
+    dv = Dev(L)
+    try:
+        bS, bH = dv.zeros(OS), dv.put(H0)
+        btau, bK = dv.put(np.full(64, tau)), dv.zeros(8)
+        prog, k = ctypes.c_void_p(), ctypes.c_void_p()
+        assert L.fpde_cl_program_create(dv.ctx, src, ctypes.byref(prog)) == 0
+        assert L.fpde_cl_kernel_create(prog, b"Sum", ctypes.byref(k)) == 0
+        for n in (40, 41):
+            old = rng.standard_normal((2, OS))
+            K, v3 = oracle.series_row(alpha, T[n - 1], T[n], T[n + 1], 1e-5, cap=P)
+            assert K >= P
+            ref = oracle.series_step(Href, v3, K, T[n], T[n + 1], alpha, tau, old[0, :nodes], old[1, :nodes])
+            bold, bC = dv.put(old), dv.put(v3)
+            for idx, b in [(0, bS), (1, bold), (3, btau), (4, bH), (5, bC), (6, bK)]:
+                assert L.fpde_cl_kernel_set_buffer_arg(k, idx, b) == 0
+            for idx, v in [(2, 2), (11, n)]:
+                iv = ctypes.c_int(v)
+                assert L.fpde_cl_kernel_set_arg(k, idx, 4, ctypes.byref(iv)) == 0
+            for idx, v in [(7, float(P)), (8, T[n + 1]), (9, T[n]), (10, alpha)]:
+                dvv = ctypes.c_double(v)
+                assert L.fpde_cl_kernel_set_arg(k, idx, 8, ctypes.byref(dvv)) == 0
+            g = (ctypes.c_size_t * 1)(nodes)
+            assert L.fpde_cl_launch(dv.ctx, k, 1, g, None) == 0, L.fpde_last_error()
+            assert L.fpde_cl_finish(dv.ctx) == 0
+            S = dv.get(bS, nodes)
+            H = dv.get(bH, OS * P).reshape(OS, P)
+            assert np.max(np.abs(S - ref)) <= 1e-12 * np.max(np.abs(ref))
+            assert np.allclose(H[:nodes], Href, rtol=1e-13, atol=1e-18)
+            assert np.array_equal(H[nodes:], H0[nodes:])
+        L.fpde_cl_kernel_destroy(k)
+        L.fpde_cl_program_destroy(prog)
+    finally:
+        dv.close()

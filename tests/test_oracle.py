@@ -139,6 +139,60 @@ def test_restricted_solve_with_zero_cut_is_the_full_solve(oracle):
     assert 0 < np.max(np.abs(cut - full)) < 1e-2 * np.max(np.abs(full))
 
 
+# ------------------------------------------------- series history (sum_alg 2)
+@pytest.mark.parametrize("alpha,n", [(0.3, 1), (0.7, 2), (0.8, 40), (0.5, 300)])
+def test_series_row_terms_and_stopping_rule(oracle, alpha, n):
+    """v3_i and the term count K against the binomial series written directly:
+    term_i = (alpha)_i/i! a^-alpha p1 (p1/a)^i v3_i, v3_i = (p1^{i+1}-p0^{i+1})/((i+1) p1^{i+1});
+    the series stops after the first |term| <= eps."""
+    from scipy.special import gammaln
+    tau, eps = 1e-3, 1e-5
+    T = oracle.time_grid(n + 1, tau)
+    p0, p1, pa = T[n - 1], T[n], T[n + 1]
+    K, v3 = oracle.series_row(alpha, p0, p1, pa, eps, cap=100000)
+    i = np.arange(K, dtype=np.float64)
+    v3d = (1.0 - (p0 / p1) ** (i + 1)) / (i + 1)
+    assert np.allclose(v3, v3d, rtol=1e-9, atol=1e-300)
+    coef = np.exp(gammaln(alpha + i) - gammaln(alpha) - gammaln(i + 1))     # (alpha)_i / i!
+    term = coef * pa ** -alpha * p1 * (p1 / pa) ** i * v3d
+    assert abs(term[-1]) <= eps * (1 + 1e-9)
+    assert np.all(np.abs(term[:-1]) > eps * (1 - 1e-9))
+
+
+def _series_run(oracle, levels, alpha, tau, P, eps):
+    T = oracle.time_grid(len(levels), tau)
+    H = np.zeros((levels.shape[1], P))
+    out = []
+    for n in range(1, len(levels) - 1):
+        K, v3 = oracle.series_row(alpha, T[n - 1], T[n], T[n + 1], eps, cap=P)
+        out.append(oracle.series_step(H, v3, K, T[n], T[n + 1], alpha, tau, levels[n - 1], levels[n]))
+    return T, out
+
+
+def test_series_one_moment_telescopes(oracle):
+    """P = 1 keeps the zeroth moment sum_t (T_{t+1}-T_t) D_t = tau (U^n - U^0):
+    S_n = T_{n+1}^-alpha (U^n - U^0)."""
+    rng = np.random.default_rng(8)
+    lv = np.cumsum(rng.standard_normal((12, 5)), axis=0)
+    T, out = _series_run(oracle, lv, 0.6, 1e-2, 1, 1e-5)
+    for n, S in enumerate(out, start=1):
+        assert np.allclose(S, T[n + 1] ** -0.6 * (lv[n] - lv[0]), rtol=1e-11, atol=1e-12)
+
+
+@pytest.mark.parametrize("alpha", [0.35, 0.8])
+def test_series_converges_to_full_history_sum(oracle, alpha):
+    """With many terms and a tiny eps the moment series is the binomial
+    expansion of every closed-form weight: it reproduces the sum_alg 0 sum."""
+    rng = np.random.default_rng(9)
+    tau = 0.25
+    lv = np.cumsum(rng.standard_normal((8, 4)), axis=0)
+    T, out = _series_run(oracle, lv, alpha, tau, 400, 1e-17)
+    for n, S in enumerate(out, start=1):
+        _, kb3 = oracle.l1_weights(alpha, n, T, weight_mode=0)
+        ref = oracle.time_summing(lv[: n + 1], kb3, np.full(n + 1, tau))
+        assert np.allclose(S, ref, rtol=1e-10, atol=1e-12 * np.max(np.abs(ref))), n
+
+
 @pytest.mark.parametrize("alpha", [0.2, 0.6, 0.9])
 def test_l1_exact_for_linear_in_time(oracle, alpha):
     # D^a (c t) = c t^(1-a)/Gamma(2-a): the L1 scheme is exact for linear u
