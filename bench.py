@@ -3,7 +3,8 @@
 
 Workload (default): 1D time-fractional moisture transport, N=10000 nodes,
 M=20000 steps, tau=5e-4, alpha=0.7, seeded initial head (seed 1001); FP64
-weights, FP64 partitioned-Thomas step, lagged VGM coefficients.  A bench
+weights, FP64 partitioned-Thomas step, Picard iteration on the VGM
+coefficients (at most 3 iterations, tol 1e-12; --picard 1 = lagged).  A bench
 "step" is one complete solve of that workload (all M time steps) with the
 initial field already in HBM; value = N*M / (wall time of one solve).
 
@@ -30,8 +31,10 @@ VARIANT_DTYPE = {"plain": "f64", "mfma_f64": "f64", "split_bf16": "bf16x2->f32->
                  "split_f16": "f16x2->f32->f64", "split_bf16x3": "bf16x3->f32->f64"}
 
 
-def run_variant(F, workload, mode, block, steps, warmup, prof_mask_timed=None, nodes=None, msteps=None):
-    spec, u0 = F.workload(workload, hist_mode=mode, block=block, nodes=nodes, steps=msteps)
+def run_variant(F, workload, mode, block, steps, warmup, prof_mask_timed=None, nodes=None, msteps=None,
+                picard=(1, 0.0)):
+    kw = {} if workload == "3d" else dict(picard_max=picard[0], picard_tol=picard[1])
+    spec, u0 = F.workload(workload, hist_mode=mode, block=block, nodes=nodes, steps=msteps, **kw)
     s = F.Solver(spec)
     s.set_initial(u0)
     # profiled warmup: per-class device times with hipEvents on the solver stream
@@ -61,7 +64,7 @@ def run_variant(F, workload, mode, block, steps, warmup, prof_mask_timed=None, n
     return dict(spec=spec, u=u_last, u_first=u, times=times, prof=prof, timed=timed_stats)
 
 
-def cpu_baseline(N, M, seconds_hint=20.0, alpha=0.7, tau=5e-4, seed=1001):
+def cpu_baseline(N, M, seconds_hint=20.0, alpha=0.7, tau=5e-4, seed=1001, picard=(1, 0.0)):
     """FP64 oracle (oracle/, the reference loop order: node-outer, t-inner,
     OpenMP over nodes) timed on this host: the history sum at the run-average
     history length n = M/2 over all N nodes, plus one oracle step (VGM
@@ -88,7 +91,7 @@ def cpu_baseline(N, M, seconds_hint=20.0, alpha=0.7, tau=5e-4, seed=1001):
     t_sum = statistics.median(reps)
     del old
     u0 = O.init_1d(N, seed)
-    p = O.make_params(N, 4, tau, alpha, u_top=u0[0], nthreads=cores)
+    p = O.make_params(N, 4, tau, alpha, u_top=u0[0], nthreads=cores, picard_max=picard[0], picard_tol=picard[1])
     t0 = time.perf_counter()
     O.solve_1d(p, u0)
     t_step = (time.perf_counter() - t0) / 4
@@ -179,6 +182,10 @@ def main():
     ap.add_argument("--requests", type=int, default=512, help="serving mix size (--workload serve)")
     ap.add_argument("--rate", type=float, default=16.0, help="serving arrival rate, requests/s")
     ap.add_argument("--max-active", type=int, default=16)
+    ap.add_argument("--picard", type=int, default=3,
+                    help="Picard iterations per step on the nonlinear coefficient (1D; 3D LOD uses 1)")
+    ap.add_argument("--picard-tol", type=float, default=1e-12,
+                    help="Picard stops once max|U^k - U^(k-1)| <= tol")
     ap.add_argument("--calibrate", type=int, default=0,
                     help="first copy this many MiB device-to-device (known-size kernel for HBM counters)")
     args = ap.parse_args()
@@ -249,6 +256,7 @@ def main():
             "serving": r}))
         return
 
+    picard = (args.picard, args.picard_tol)
     # FP64 plain result first: the error reference for every other variant
     results = {}
     compare = [c for c in args.compare.split(",") if c]
@@ -258,13 +266,13 @@ def main():
     for mode in order:
         K = args.steps if mode == args.headline else 1
         r = run_variant(F, args.workload, mode, args.block, K, args.warmup, prof_mask_timed=None,
-                        nodes=args.nodes, msteps=args.msteps)
+                        nodes=args.nodes, msteps=args.msteps, picard=picard)
         # decide the dominant kernel class from the profiled warmup, then time
         # it live over the timed region with events on the solver stream
         prof = r["prof"]
         dom = 0 if prof["ms_history"] >= prof["ms_step"] else 1
         r2 = run_variant(F, args.workload, mode, args.block, K, 0, prof_mask_timed=1 << dom,
-                         nodes=args.nodes, msteps=args.msteps) if mode == args.headline else None
+                         nodes=args.nodes, msteps=args.msteps, picard=picard) if mode == args.headline else None
         r["dominant"] = dom
         r["timed_live"] = r2
         results[mode] = r
@@ -325,13 +333,18 @@ def main():
         nl = statistics.mean(x["step_launches"] for x in ts)
         # step kernel algorithmic bytes per launch: U^n in, U^{n+1} and D out (24 B/node)
         # plus the near-field history terms it reads (8 B per term)
-        bpl = 24.0 * unknowns
+        # a launch covers M/launches steps (cooperative block launches: a
+        # whole history block; otherwise one Picard iteration)
+        steps_per_launch = M / max(nl, 1)
+        bpl = 24.0 * unknowns * steps_per_launch
         achieved = bpl / (ms * 1e-3 / max(nl, 1)) / 1e9
-        roof = {"kernel": "k_step1d" if spec.dims == 1 else "lod3d sweeps", "bound": "hbm",
+        kname = ("k_step1d_block" if steps_per_launch > 1.5 else "k_step1d") if spec.dims == 1 else "lod3d sweeps"
+        roof = {"kernel": kname, "bound": "hbm",
                 "achieved": achieved, "peak": MI355X_HBM_GBS, "unit": "GB/s",
                 "frac": achieved / MI355X_HBM_GBS, "traffic": None,
                 "alg_bytes_per_launch": bpl, "avg_launch_us": ms * 1e3 / max(nl, 1),
-                "note": "latency-bound sequential step chain"}
+                "steps_per_launch": steps_per_launch,
+                "note": "latency-bound sequential step chain (Picard iterations, grid barriers)"}
 
     # HBM traffic per launch of the dominant kernel, from the committed
     # rocprofv3 --pmc FETCH_SIZE / WRITE_SIZE summary (calibrated on a copy)
@@ -343,7 +356,8 @@ def main():
             roof["traffic_source"] = "profiles/hbm_counters.json (rocprofv3 --pmc, copy-calibrated)"
     except (OSError, ValueError, KeyError):
         pass
-    cpu = None if (args.no_cpu or spec.dims != 1 or spec.do_q) else cpu_baseline(N, M, alpha=spec.alpha, tau=spec.tau)
+    cpu = None if (args.no_cpu or spec.dims != 1 or spec.do_q) else cpu_baseline(
+        N, M, alpha=spec.alpha, tau=spec.tau, picard=(spec.picard_max, spec.picard_tol))
     out = {
         "metric": "node*steps/s, mixed-MFMA history-sum solver (vs FP64 GPU at <=1e-8 rel err)",
         "value": unknowns * M / t_mean,
@@ -360,8 +374,11 @@ def main():
         "config": {"workload": (f"{args.workload}: {'3D LOD ' + str(N) + '^3' if spec.dims == 3 else '1D N=' + str(N)} "
                                 f"time-fractional moisture transport M={M} tau={spec.tau} "
                                 + (f"distributed order q={spec.do_q} on [{spec.do_a0},{spec.do_a1}]" if spec.do_q
-                                   else f"alpha={spec.alpha}") + f"; history {args.headline} block {args.block}"),
+                                   else f"alpha={spec.alpha}") + f"; history {args.headline} block {args.block}"
+                                + (f"; Picard <= {spec.picard_max} iterations, tol {spec.picard_tol:g}"
+                                   if spec.dims == 1 else "; LOD with lagged coefficients")),
                    "nodes": N, "time_steps": M, "history_mode": args.headline, "block": args.block,
+                   "picard_max": spec.picard_max, "picard_tol": spec.picard_tol,
                    "parallelism": "single GPU"},
         "roofline": roof,
         "cpu_baseline": cpu,

@@ -26,7 +26,7 @@ class Spec(ctypes.Structure):
         ("picard_tol", ctypes.c_double), ("weight_mode", ctypes.c_int), ("series_eps", ctypes.c_double),
         ("do_q", ctypes.c_int), ("do_rule", ctypes.c_int), ("do_a0", ctypes.c_double),
         ("do_a1", ctypes.c_double), ("hist_mode", ctypes.c_int), ("block", ctypes.c_int),
-        ("use_graph", ctypes.c_int), ("sum_alg", ctypes.c_int), ("sum_param", ctypes.c_double),
+        ("step_mode", ctypes.c_int), ("sum_alg", ctypes.c_int), ("sum_param", ctypes.c_double),
         ("reserved", ctypes.c_int * 2),
     ]
 

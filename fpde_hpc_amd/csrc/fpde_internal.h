@@ -35,9 +35,9 @@ constexpr double kVgmA = 0.066;
 // x = (100 a P)^n:  theta - s0 = (s1-s0) (1+x)^(-m),  Se = (1+x)^(-m),
 // Se^(1/m) = 1/(1+x),  Kr = Se^(1/2) (1 - (x/(1+x))^m)^2,
 // C = d theta/dh = (n-1)(s1-s0) x (1+x)^(1/n) / (P (1+x)^2),   m = 1-1/n,
-// evaluated through lx = log x and l1 = log1p(x) (one log, one log1p, three
-// exp): 1-(x/(1+x))^m = -expm1(m (lx - l1)) keeps the digits the direct
-// difference cancels.
+// evaluated through lx = log x and l1 = log1p(x) (one log, one log1p, two
+// exp, one expm1): 1-(x/(1+x))^m = -expm1(m (lx - l1)) keeps the digits the
+// direct difference cancels, and (1+x)^(1/n) / (1+x)^2 = Se / (1+x).
 __host__ __device__ inline void vgm_coefs(double u, double z, double ksat, int mode,
                                           double& K, double& invC) {
     if (mode == 1) { K = 1.0; invC = 1.0; return; }
@@ -54,8 +54,19 @@ __host__ __device__ inline void vgm_coefs(double u, double z, double ksat, int m
     double se = exp(-m * l1);                      // (1+x)^(-m)
     double t = -expm1(m * (lx - l1));              // 1 - (x/(1+x))^m
     K = ksat * sqrt(se) * t * t;
-    double ch = (n - 1.0) * (kVgmS1 - kVgmS0) * x * exp(l1 / n - 2.0 * l1) / P;
-    invC = (ch != 0.0) ? 1.0 / ch : 1e20;
+    double den = (n - 1.0) * (kVgmS1 - kVgmS0) * x * se;     // C (1+x) P
+    invC = (den != 0.0) ? (1.0 + x) * P / den : 1e20;
+}
+
+// 1/x for the normalised pivots of the tridiagonal reductions: the hardware
+// reciprocal refined by two Newton steps (within an ulp or two of 1/x; the
+// pivots are O(1) or well inside the normal range).
+__device__ inline double rcp_nr(double x) {
+    double r = __builtin_amdgcn_rcp(x);
+    double e = fma(-x, r, 1.0);
+    r = fma(r, e, r);
+    e = fma(-x, r, 1.0);
+    return fma(r, e, r);
 }
 
 // ---------------------------------------------------------------------------
@@ -139,6 +150,16 @@ struct Step1DArgs {
     int* flags;               // unused
     double tol;
     int prev_k;               // Picard index of the previous launch (-1: none)
+    int extrap;               // Picard starts from 2 U^n - U^{n-1} (picard_max > 1)
+    double* Uit0;             // that initial iterate (convergence test of iteration 1)
+    double* uend;             // [2][2C] U^n at the chunk ends, by step parity
+    // cooperative block form (k_step1d_block): steps [n, n_end) in one launch
+    int n_end;
+    int picard_max;
+    int n0;                   // first step of the history block (far partial row = n - n0)
+    unsigned* bar;            // grid barrier counters, one per launch parity (bar[0], bar[16])
+    int epar;                 // parity of this cooperative launch
+    int* err;                 // set when a grid barrier times out
     // series-compressed history (ser_P > 0): moments H [P][ld], coefficient
     // rows of step n; D is then a ring of two levels (row t & 1)
     int ser_P;
@@ -165,6 +186,8 @@ fpde_status launch_gemm_split(int mode, const void* c0, const void* c1, const vo
                               int64_t ld, int nodes, int n0, int tc, int block, const double* omega, double* P,
                               int* splits_out, hipStream_t st);
 fpde_status launch_step1d(const Step1DArgs& a, hipStream_t st);
+fpde_status launch_step1d_block(const Step1DArgs& a, hipStream_t st);
+bool step1d_block_supported(int C);
 fpde_status launch_coef3d(const double* U, int n, double h, double ksat, int mode, double* K, double* iC,
                           hipStream_t st);
 fpde_status launch_lateral3d(double* U, const double* K, const double* iC, const double* H, int n, int axis,

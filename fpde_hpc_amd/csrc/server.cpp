@@ -117,6 +117,9 @@ fpde_status fpde_server_submit(fpde_server* srv, const fpde_spec* spec, const do
     auto r = std::make_unique<Request>();
     r->id = srv->next_id++;
     r->spec = *spec;
+    // requests share the device: the cooperative block form needs its
+    // workgroups co-resident, so served solves launch per Picard iteration
+    r->spec.step_mode = 1;
     r->submit_s = srv->now();
     r->bytes = fpde_solver_footprint(spec);
     if (id_out) *id_out = r->id;

@@ -46,6 +46,7 @@ fpde_status validate(const fpde_spec& s) {
         int b = s.block;
         if (b != 16 && b != 32 && b != 64 && b != 128) { set_error("block must be 16, 32, 64 or 128"); return FPDE_ERR_ARG; }
     }
+    if (s.step_mode != 0 && s.step_mode != 1) { set_error("step_mode must be 0 or 1"); return FPDE_ERR_ARG; }
     if (s.dims == 3 && s.picard_max != 1) { set_error("3D LOD uses lagged coefficients (picard_max must be 1)"); return FPDE_ERR_ARG; }
     if (s.sum_alg < 0 || s.sum_alg > 2) {
         set_error("sum_alg must be 0 (full sum), 1 (restricted sum) or 2 (series moments)");
@@ -119,6 +120,12 @@ struct fpde_solver {
     double* Uit = nullptr;
     double* dmax = nullptr;
     double* H = nullptr;       // sum_alg 2 moments [P][ld]
+    double* Uit0 = nullptr;    // initial Picard iterate
+    double* uend = nullptr;    // chunk-end values by step parity
+    unsigned* bar = nullptr;   // grid barrier of the cooperative 1D block launches
+    int* err = nullptr;        // barrier time-out flag
+    bool block_mode = false;   // 1D steps run as cooperative block launches
+    int64_t coop_launches = 0; // selects the barrier counter of the next one
     int* flags = nullptr;
     void* comp[3] = {nullptr, nullptr, nullptr};
     float* scale = nullptr;
@@ -228,6 +235,17 @@ fpde_status solver_alloc(fpde_solver* s) {
         DALLOC(s->xyz, sizeof(double) * 3 * p.ld);
         DALLOC(s->red, sizeof(double) * 2 * 2 * s->C * 4);
         DALLOC(s->Uit, sizeof(double) * p.ld);
+        DALLOC(s->Uit0, sizeof(double) * p.ld);
+        DALLOC(s->uend, sizeof(double) * 2 * 2 * s->C);
+        DALLOC(s->bar, 128);
+        DALLOC(s->err, 64);
+        FPDE_HIP_TRY(hipMemsetAsync(s->bar, 0, 128, s->stream));
+        FPDE_HIP_TRY(hipMemsetAsync(s->err, 0, 64, s->stream));
+        // cooperative block launches pay off where a launch covers many steps:
+        // the GEMM history forms (one launch per block) and the series form;
+        // the plain form needs its history kernel between every two steps
+        bool blocked_history = sp.hist_mode != HIST_PLAIN_F64 || sp.sum_alg == 2;
+        s->block_mode = (sp.step_mode == 0) && blocked_history && step1d_block_supported(s->C);
         DALLOC(s->dmax, sizeof(double) * 2 * s->C);
         DALLOC(s->flags, sizeof(int) * 16);
     } else {
@@ -367,6 +385,9 @@ fpde_status launch_1d(fpde_solver* s, int n, int k, int nsplit, int64_t stride, 
     a.flags = s->flags;
     a.tol = sp.picard_tol;
     a.prev_k = s->last_k;
+    a.extrap = sp.picard_max > 1 ? 1 : 0;
+    a.Uit0 = s->Uit0;
+    a.uend = s->uend;
     a.ser_P = s->w.ser_P;
     if (a.ser_P > 0) {
         size_t row = (size_t)std::min(n, sp.steps - 1) * a.ser_P;   // the finalize launch reads no row
@@ -395,6 +416,91 @@ fpde_status enqueue_1d_step(fpde_solver* s, int n) {
         rc = launch_1d(s, n, k, nsplit, stride, off, near_lo);
         if (rc != FPDE_OK) return rc;
         s->stats.picard_iters_total++;
+    }
+    return FPDE_OK;
+}
+
+// Cooperative block launch for the steps [n, n_end) (no pending solve
+// between launches: each launch leaves U^{n_end} and D[n_end-1] in memory).
+fpde_status launch_1d_block(fpde_solver* s, int n, int n_end, int n0, int nsplit, int64_t stride, int near_lo) {
+    const fpde_spec& sp = s->spec;
+    Step1DArgs a{};
+    a.N = sp.nodes;
+    a.C = s->C;
+    a.h = s->h;
+    a.h2inv = 1.0 / (s->h * s->h);
+    a.ksat = sp.ksat;
+    a.L = s->w.local;
+    a.utop = s->utop1d;
+    a.coef_mode = sp.coef_mode;
+    a.n = n;
+    a.n_end = n_end;
+    a.n0 = n0;
+    a.picard_max = sp.picard_max;
+    a.tol = sp.picard_tol;
+    a.extrap = sp.picard_max > 1 ? 1 : 0;
+    a.U = s->U;
+    a.D = s->D;
+    a.ld = s->plan.ld;
+    a.omega = s->w.omega;
+    a.corr = s->w.corr;
+    a.part = s->part;
+    a.nsplit = nsplit;
+    a.part_stride = stride;
+    a.near_lo = near_lo;
+    a.red = s->red;
+    a.dmax = s->dmax;
+    a.bar = s->bar;
+    a.epar = (int)(s->coop_launches & 1);
+    a.err = s->err;
+    a.ser_P = s->w.ser_P;
+    if (a.ser_P > 0) {
+        a.H = s->H;
+        a.ser_c = s->w.ser_c;
+        a.ser_v = s->w.ser_v;
+    }
+    size_t slot = 0;
+    ev_begin(s, 1, &slot);
+    fpde_status rc = launch_step1d_block(a, s->stream);
+    ev_end(s, slot);
+    if (rc != FPDE_OK) return rc;
+    s->launches++;
+    s->coop_launches++;
+    s->stats.step_launches++;
+    s->stats.picard_iters_total += (int)((int64_t)(n_end - n) * sp.picard_max);
+    return FPDE_OK;
+}
+
+// 1D steps [n, end) in cooperative block launches: one launch per history
+// block (GEMM forms), per step (plain: its history kernel runs between
+// steps), or per 256 steps (series moments, no history kernels).
+fpde_status enqueue_1d_blocks(fpde_solver* s, int n, int end) {
+    const fpde_spec& sp = s->spec;
+    while (n < end) {
+        int nsplit = 0, near_lo = 0;
+        int64_t stride = 0, off = 0;
+        fpde_status rc = far_part(s, n, &nsplit, &stride, &off, &near_lo);
+        if (rc != FPDE_OK) return rc;
+        int n_end, n0;
+        if (sp.sum_alg == 2) { n_end = std::min(end, n + 256); n0 = n; }
+        else if (sp.hist_mode == HIST_PLAIN_F64) { n_end = n + 1; n0 = n; }   // step_mode 0 never selects this
+        else { n_end = std::min(end, s->n0 + sp.block); n0 = s->n0; }
+        rc = launch_1d_block(s, n, n_end, n0, nsplit, stride, near_lo);
+        if (rc != FPDE_OK) return rc;
+        n = n_end;
+        s->step = n;
+    }
+    return FPDE_OK;
+}
+
+fpde_status check_barrier(fpde_solver* s) {
+    if (!s->block_mode) return FPDE_OK;
+    int e = 0;
+    FPDE_HIP_TRY(hipMemcpy(&e, s->err, sizeof(int), hipMemcpyDeviceToHost));
+    if (e) {
+        set_error("1D cooperative step: a grid barrier timed out (workgroups not co-resident); "
+                  "recreate the solver with step_mode 1");
+        return FPDE_ERR_HIP;
     }
     return FPDE_OK;
 }
@@ -520,6 +626,9 @@ fpde_status fpde_solver_reset(fpde_solver* s, const double* u0) {
     s->tc = 0;
     s->gsplits = 0;
     s->packed_groups = 0;
+    if (s->bar) FPDE_HIP_TRY(hipMemsetAsync(s->bar, 0, 128, s->stream));
+    if (s->err) FPDE_HIP_TRY(hipMemsetAsync(s->err, 0, 64, s->stream));
+    s->coop_launches = 0;
     return fpde_solver_set_initial(s, u0);
 }
 
@@ -528,6 +637,12 @@ fpde_status fpde_solver_advance(fpde_solver* s, int nsteps) {
     if (!s->initialized) { set_error("set_initial must be called first"); return FPDE_ERR_STATE; }
     int M = s->spec.steps;
     int end = (nsteps <= 0) ? M : std::min(M, s->step + nsteps);
+    if (s->spec.dims == 1 && s->block_mode) {
+        fpde_status rc = enqueue_1d_blocks(s, s->step, end);
+        if (rc != FPDE_OK) return rc;
+        FPDE_HIP_TRY(hipGetLastError());
+        return FPDE_OK;
+    }
     for (int n = s->step; n < end; n++) {
         fpde_status rc = (s->spec.dims == 1) ? enqueue_1d_step(s, n) : enqueue_3d_step(s, n);
         if (rc != FPDE_OK) return rc;
@@ -545,7 +660,7 @@ fpde_status fpde_solver_sync(fpde_solver* s) {
     if (!s) { set_error("null solver"); return FPDE_ERR_ARG; }
     FPDE_HIP_TRY(hipStreamSynchronize(s->stream));
     ev_collect(s);
-    return FPDE_OK;
+    return check_barrier(s);
 }
 
 int fpde_solver_step(const fpde_solver* s) { return s ? s->step : -1; }
@@ -568,7 +683,7 @@ fpde_status fpde_solver_get_solution(fpde_solver* s, double* u_out) {
     FPDE_HIP_TRY(hipMemcpyAsync(u_out, s->U, sizeof(double) * s->plan.nodes, hipMemcpyDeviceToHost, s->stream));
     FPDE_HIP_TRY(hipStreamSynchronize(s->stream));
     ev_collect(s);
-    return FPDE_OK;
+    return check_barrier(s);
 }
 
 fpde_status fpde_solver_get_weights(fpde_solver* s, double* local, double* omega, double* corr) {

@@ -42,7 +42,7 @@ __device__ inline int pcr_norm(double (*A)[kMaxRed], double (*Cc)[kMaxRed], doub
                 double am = 0.0, cm = 0.0, dm = 0.0, ap = 0.0, cp = 0.0, dp = 0.0;
                 if (r - delta >= 0) { am = A[p][r - delta]; cm = Cc[p][r - delta]; dm = Dd[p][r - delta]; }
                 if (r + delta < n) { ap = A[p][r + delta]; cp = Cc[p][r + delta]; dp = Dd[p][r + delta]; }
-                double inv = 1.0 / (1.0 - ar * cm - cr * ap);
+                double inv = rcp_nr(1.0 - ar * cm - cr * ap);
                 A[1 - p][r] = -ar * am * inv;
                 Cc[1 - p][r] = -cr * cp * inv;
                 Dd[1 - p][r] = (dr - ar * dm - cr * dp) * inv;
@@ -54,18 +54,63 @@ __device__ inline int pcr_norm(double (*A)[kMaxRed], double (*Cc)[kMaxRed], doub
     return p;
 }
 
+// The same reduction by one wave on its own LDS slice (A/Cc/Dd: [2][128]),
+// for n <= 128 rows (rows r = lane, lane + 64).  Every wave of the block
+// solves the reduced system redundantly, so no workgroup barrier is needed:
+// LDS operations of one wave complete in order, and the wavefront fences
+// keep the compiler from moving them across a level.
+__device__ inline int pcr_wave(double* A, double* Cc, double* Dd, int n) {
+    const int lane = threadIdx.x & 63;
+    int p = 0;
+    for (int delta = 1; delta < n; delta <<= 1) {
+        double na[2], nc[2], nd[2];
+#pragma unroll
+        for (int j = 0; j < 2; j++) {
+            int r = lane + 64 * j;
+            na[j] = 0.0; nc[j] = 0.0; nd[j] = 0.0;
+            if (r < n) {
+                const double* a = A + p * 128;
+                const double* cc = Cc + p * 128;
+                const double* d = Dd + p * 128;
+                double ar = a[r], cr = cc[r], dr = d[r];
+                double am = 0.0, cm = 0.0, dm = 0.0, ap = 0.0, cp = 0.0, dp = 0.0;
+                if (r - delta >= 0) { am = a[r - delta]; cm = cc[r - delta]; dm = d[r - delta]; }
+                if (r + delta < n) { ap = a[r + delta]; cp = cc[r + delta]; dp = d[r + delta]; }
+                double inv = rcp_nr(1.0 - ar * cm - cr * ap);
+                na[j] = -ar * am * inv;
+                nc[j] = -cr * cp * inv;
+                nd[j] = (dr - ar * dm - cr * dp) * inv;
+            }
+        }
+#pragma unroll
+        for (int j = 0; j < 2; j++) {
+            int r = lane + 64 * j;
+            if (r < n) {
+                A[(1 - p) * 128 + r] = na[j];
+                Cc[(1 - p) * 128 + r] = nc[j];
+                Dd[(1 - p) * 128 + r] = nd[j];
+            }
+        }
+        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+        __builtin_amdgcn_wave_barrier();
+        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+        p = 1 - p;
+    }
+    return p;
+}
+
 // Far partial sums plus the near levels t in [near_lo, n-2] already in
 // memory, gathered in 32-wide batches of independent loads (a sum over m
 // terms costs ~m/32 memory round trips).  Every load is guarded: omega is
 // read only at n-t-1 >= 1.
-__device__ inline double hist_mem(const Step1DArgs& g, int i, int n) {
+__device__ inline double hist_mem(const Step1DArgs& g, int i, int n, int64_t part_off) {
     constexpr int W = 32;
     double far = 0.0;
     for (int sp0 = 0; sp0 < g.nsplit; sp0 += W) {
         double v[W];
 #pragma unroll
         for (int j = 0; j < W; j++)
-            v[j] = (sp0 + j < g.nsplit) ? g.part[(int64_t)(sp0 + j) * g.part_stride + g.part_off + i] : 0.0;
+            v[j] = (sp0 + j < g.nsplit) ? g.part[(int64_t)(sp0 + j) * g.part_stride + part_off + i] : 0.0;
 #pragma unroll
         for (int j = 0; j < W; j++) far += v[j];
     }
@@ -116,11 +161,16 @@ __global__ __launch_bounds__(256) void k_step1d(Step1DArgs g) {
     const int n = g.n;
     const bool first = g.k <= 0;          // first Picard iteration of step n (or finalize)
 
+    // reduced systems of up to 128 rows are solved by every wave on its own
+    // LDS slice (no workgroup barrier), larger ones by the whole block
+    const bool wave_red = R2 <= 128;
+    const int wave = tid >> 6, lane = tid & 63;
+
     // ---- 0. loads that do not depend on the pending solve (one round trip) ---
     double rrow[2][4];
 #pragma unroll
     for (int j = 0; j < 2; j++) {
-        int r = tid + 256 * j;
+        int r = wave_red ? lane + 64 * j : tid + 256 * j;
         bool ok = g.recover && r < R2;
         const double* red = g.red + (size_t)(1 - g.parity) * R2 * 4 + 4 * r;
 #pragma unroll
@@ -141,29 +191,61 @@ __global__ __launch_bounds__(256) void k_step1d(Step1DArgs g) {
     // iterate it stored
     const bool check_conv = g.recover && g.prev_k >= 1;
     double dprev = (check_conv && tid < C) ? g.dmax[(size_t)(1 - g.parity) * C + tid] : 0.0;
-    double uitp = ((check_conv || g.k >= 2) && act) ? g.Uit[i] : 0.0;
-    double hm = (first && !series && g.k == 0 && act && n >= 1) ? hist_mem(g, i, n) : 0.0;
+    // previous iterate: the initial one for iteration 1, else the stored one
+    double uitp = 0.0;
+    if ((check_conv || g.k >= 1) && act) uitp = (g.k == 1) ? g.Uit0[i] : g.Uit[i];
+    // halo node s-1 (owned by chunk c-1, whose workgroup rewrites U and D in
+    // this launch): the converged iterate and U^{n-1} come from buffers that
+    // no workgroup writes in this launch
+    const int halo_tid = (nc < 256) ? 255 : 0;
+    const bool halo = tid == halo_tid && c > 0;
+    const bool extrap0 = first && g.k == 0 && g.extrap && n >= 1;
+    double uit_halo = (halo && check_conv) ? g.Uit[s - 1] : 0.0;
+    double uprev_halo = (halo && extrap0) ? g.uend[(size_t)((n - 1) & 1) * R2 + 2 * c - 1] : 0.0;
+    double hm = (first && !series && g.k == 0 && act && n >= 1) ? hist_mem(g, i, n, g.part_off) : 0.0;
 
     // ---- 1. finish the previous solve: reduced system, then back-substitute --
     double val = 0.0;
-    int pr = 0;
+    const double* xs = nullptr;          // solution of the reduced system
     if (g.recover) {
+        if (wave_red) {
+            double* A = &rA[0][0] + wave * 256;
+            double* Cc = &rC[0][0] + wave * 256;
+            double* Dd = &rD[0][0] + wave * 256;
 #pragma unroll
-        for (int j = 0; j < 2; j++) {
-            int r = tid + 256 * j;
-            if (r < R2) {
-                double ib = 1.0 / rrow[j][1];
-                rA[0][r] = rrow[j][0] * ib;
-                rC[0][r] = rrow[j][2] * ib;
-                rD[0][r] = rrow[j][3] * ib;
+            for (int j = 0; j < 2; j++) {
+                int r = lane + 64 * j;
+                if (r < R2) {
+                    double ib = rcp_nr(rrow[j][1]);
+                    A[r] = rrow[j][0] * ib;
+                    Cc[r] = rrow[j][2] * ib;
+                    Dd[r] = rrow[j][3] * ib;
+                }
             }
+            __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+            __builtin_amdgcn_wave_barrier();
+            __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+            int pr = pcr_wave(A, Cc, Dd, R2);
+            xs = Dd + pr * 128;
+        } else {
+#pragma unroll
+            for (int j = 0; j < 2; j++) {
+                int r = tid + 256 * j;
+                if (r < R2) {
+                    double ib = rcp_nr(rrow[j][1]);
+                    rA[0][r] = rrow[j][0] * ib;
+                    rC[0][r] = rrow[j][2] * ib;
+                    rD[0][r] = rrow[j][3] * ib;
+                }
+            }
+            __syncthreads();
+            int pr = pcr_norm(rA, rC, rD, R2);
+            xs = &rD[pr][0];
         }
-        __syncthreads();
-        pr = pcr_norm(rA, rC, rD, R2);
         if (act) {
-            if (i == s) val = rD[pr][2 * c];
-            else if (i == e) val = rD[pr][2 * c + 1];
-            else val = fx + fy * rD[pr][2 * c] + fz * rD[pr][2 * c + 1];
+            if (i == s) val = xs[2 * c];
+            else if (i == e) val = xs[2 * c + 1];
+            else val = fx + fy * xs[2 * c] + fz * xs[2 * c + 1];
         }
     }
     // the previous Picard iterate met the tolerance: it is the step's result
@@ -195,8 +277,18 @@ __global__ __launch_bounds__(256) void k_step1d(Step1DArgs g) {
                 S = hm + newest + cr;
             }
             if (g.k >= 0) g.Sbuf[i] = S;
+            // Picard starts from the extrapolation 2 U^n - U^{n-1}; the
+            // iterate is kept for the convergence test of the next launch and
+            // the chunk ends of U^n for the next step's halo
+            uit = un;
+            if (g.extrap && g.k == 0) {
+                if (n >= 1) uit = un + dnew;
+                g.Uit0[i] = uit;
+                double* ue = g.uend + (size_t)(n & 1) * R2;
+                if (i == s) ue[2 * c] = un;
+                if (i == e) ue[2 * c + 1] = un;
+            }
         }
-        uit = un;
     } else if (act) {
         un = uold;
         S = Sk;
@@ -223,9 +315,9 @@ __global__ __launch_bounds__(256) void k_step1d(Step1DArgs g) {
             }
             return;                              // uniform: conv is block-wide
         }
-        // |U^{k} - U^{k-1}| over the chunk; U^0 of the step is U^n
-        double prev_it = (g.k == 1) ? un : uitp;
-        double d = act ? fabs(uit - prev_it) : 0.0;
+        // |U^{k} - U^{k-1}| over the chunk; U^0 of the step is the initial
+        // iterate stored by the k = 0 launch
+        double d = act ? fabs(uit - uitp) : 0.0;
         d = block_max(d, sred);
         if (tid == 0) dm[c] = d;
         if (act) g.Uit[i] = uit;
@@ -234,10 +326,11 @@ __global__ __launch_bounds__(256) void k_step1d(Step1DArgs g) {
     // ---- 3. coefficients at the iterate ------------------------------------
     double K = 0.0, invC = 0.0;
     // K of the node left of the chunk, on a thread without a node of its own
-    // when there is one (else thread 0 does both evaluations)
-    const int halo_tid = (nc < 256) ? 255 : 0;
-    if (tid == halo_tid && c > 0) {
-        double ulo = g.recover ? rD[pr][2 * c - 1] : g.U[s - 1];
+    // when there is one (else thread 0 does both evaluations), at the same
+    // iterate as the chunk's own nodes
+    if (halo) {
+        double ulo = g.recover ? (conv ? uit_halo : xs[2 * c - 1]) : g.U[s - 1];
+        if (extrap0) ulo += ulo - uprev_halo;
         double Kl, iCl;
         vgm_coefs(ulo, (s - 1) * g.h, g.ksat, g.coef_mode, Kl, iCl);
         sK[0] = Kl;
@@ -268,7 +361,7 @@ __global__ __launch_bounds__(256) void k_step1d(Step1DArgs g) {
     const int q = tid - 1;
     const bool inner = (tid >= 1) && (tid <= nc - 2);
     if (inner) {
-        double ib = 1.0 / br;
+        double ib = rcp_nr(br);
         pA[0][q] = (tid == 1) ? 0.0 : ar * ib;
         pC[0][q] = (tid == nc - 2) ? 0.0 : cr * ib;
         pX[0][q] = dr * ib;
@@ -289,7 +382,7 @@ __global__ __launch_bounds__(256) void k_step1d(Step1DArgs g) {
                 ap = pA[pp][q + delta]; cp = pC[pp][q + delta];
                 xp = pX[pp][q + delta]; yp = pY[pp][q + delta]; zp = pZ[pp][q + delta];
             }
-            double inv = 1.0 / (1.0 - aq * cm - cq * ap);
+            double inv = rcp_nr(1.0 - aq * cm - cq * ap);
             pA[1 - pp][q] = -aq * am * inv;
             pC[1 - pp][q] = -cq * cp * inv;
             pX[1 - pp][q] = (pX[pp][q] - aq * xm - cq * xp) * inv;
@@ -325,6 +418,278 @@ __global__ __launch_bounds__(256) void k_step1d(Step1DArgs g) {
         redw[4 * (2 * c + 1) + 2] = cr;
         redw[4 * (2 * c + 1) + 3] = dr - ar * x;
     }
+}
+
+// ---------------------------------------------------------------------------
+// Cooperative form: one launch runs the steps [n, n_end) of a history block
+// with every Picard iteration inside, all C workgroups co-resident.  Per
+// iteration each workgroup builds its rows at the iterate (VGM, chunk PCR)
+// and publishes its two reduced rows with the convergence measure of the
+// previous iterate; one grid barrier later every workgroup has the whole
+// reduced system, decides convergence the reference way (stop once
+// max|U^k - U^{k-1}| <= tol, the iterate U^k is the result) and otherwise
+// solves the reduced system and back-substitutes its chunk.  A step's
+// history needs only the workgroup's own nodes (its own earlier writes of D)
+// and the precomputed far partials, so steps follow each other without a
+// barrier; U and D of every step are written as in the per-launch form.
+
+constexpr unsigned kSpinLimit = 1u << 18;     // bounded wait of a grid barrier
+
+// Grid barrier over the co-resident workgroups.  Each launch counts its
+// arrivals on its own counter (cnt, zeroed by the previous launch), so the
+// j-th barrier of a launch waits for the count (j+1)*nwg: one atomic add per
+// workgroup and relaxed polling, with a single agent-scope release before the
+// arrival and a single acquire after it (on thread 0; the workgroup barriers
+// order the other threads' accesses around them).  Returns false (and sets
+// *err) after a bounded wait.
+__device__ inline bool grid_sync(unsigned* cnt, unsigned target, int* err, int* sh_ok) {
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        int ok = 1;
+        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent");
+        __hip_atomic_fetch_add(cnt, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        unsigned spins = 0;
+        while (__hip_atomic_load(cnt, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) < target) {
+            __builtin_amdgcn_s_sleep(1);
+            if (++spins > kSpinLimit) {
+                ok = 0;
+                __hip_atomic_store(err, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+                break;
+            }
+        }
+        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
+        *sh_ok = ok;
+    }
+    __syncthreads();
+    return *sh_ok != 0;
+}
+
+__global__ __launch_bounds__(256) void k_step1d_block(Step1DArgs g) {
+    __shared__ double rA[2][kMaxRed], rC[2][kMaxRed], rD[2][kMaxRed];
+    __shared__ double pA[2][256], pC[2][256], pX[2][256], pY[2][256], pZ[2][256];
+    __shared__ double sK[258];
+    __shared__ double sred[4];
+    __shared__ int sh_ok;
+    const int tid = threadIdx.x;
+    const int c = blockIdx.x;
+    const int N = g.N, C = g.C;
+    const int s = (int)(((int64_t)c * N) / C);
+    const int e = (int)(((int64_t)(c + 1) * N) / C) - 1;
+    const int nc = e - s + 1;
+    const int i = s + tid;
+    const bool act = tid < nc;
+    const int R2 = 2 * C;
+    const bool series = g.ser_P > 0;
+    const int halo_tid = (nc < 256) ? 255 : 0;
+    const bool halo = tid == halo_tid && c > 0;
+    const int ni = nc - 2;
+    const int q = tid - 1;
+    const bool inner = (tid >= 1) && (tid <= nc - 2);
+
+    // state carried from step to step: U^n and D[n-1] of the own node and of
+    // the halo node s-1 (from memory at the first step of the launch)
+    const int n_first = g.n;
+    double un = act ? g.U[i] : 0.0;
+    double dlast = 0.0;
+    if (act && n_first >= 1) dlast = g.D[(series ? ((n_first - 1) & 1) : (n_first - 1)) * g.ld + i];
+    double uh = halo ? g.U[s - 1] : 0.0;
+    double dh = 0.0;
+    if (halo && n_first >= 1) dh = g.D[(series ? ((n_first - 1) & 1) : (n_first - 1)) * g.ld + s - 1];
+    // D[0], D[1] (corrections of the two oldest intervals): from memory when
+    // written by an earlier launch, else taken when this launch makes them
+    double d0m = (!series && act && n_first >= 1) ? g.D[i] : 0.0;
+    double d1m = (!series && act && n_first >= 2) ? g.D[g.ld + i] : 0.0;
+    int bk = 0;                                                    // barriers passed in this launch
+    // this launch counts on bar[16*epar]; it zeroes the other counter for the
+    // next launch (the previous launch, its last user, has finished)
+    unsigned* cnt = g.bar + 16 * g.epar;
+    if (c == 0 && tid == 0) __hip_atomic_store(g.bar + 16 * (1 - g.epar), 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+
+    for (int n = n_first; n < g.n_end; n++) {
+        // ---- history term of step n -------------------------------------
+        double S = 0.0;
+        if (n >= 1 && act) {
+            if (series) {
+                S = series_update(g.H, g.ld, i, g.ser_P, dlast, g.ser_c + (size_t)n * g.ser_P,
+                                  g.ser_v + (size_t)n * g.ser_P);
+            } else {
+                double hm = hist_mem(g, i, n, (int64_t)(n - g.n0) * g.ld);
+                double newest = (g.near_lo <= n - 1) ? g.omega[0] * dlast : 0.0;
+                double cr = (n == 1) ? g.corr[2] * dlast : g.corr[2 * n] * (n >= 2 ? d0m : 0.0);
+                if (n == 2) cr = fma(g.corr[2 * n + 1], dlast, cr);
+                else if (n >= 3) cr = fma(g.corr[2 * n + 1], d1m, cr);
+                S = hm + newest + cr;
+            }
+        }
+        // Picard starts from 2 U^n - U^{n-1} (a single pass: lagged at U^n)
+        double uit = un, uith = uh;
+        if (g.extrap && n >= 1) { uit = un + dlast; uith = uh + dh; }
+        double dpub = 0.0;                   // max |U^k - U^{k-1}| of the chunk, k >= 1
+        for (int k = 0; k < g.picard_max; k++) {
+            // the last build is skipped when U^k already met the tolerance:
+            // decide that first (one barrier on the convergence measure);
+            // earlier iterations publish it with their reduced rows
+            const bool check_first = k >= 1 && k == g.picard_max - 1;
+            if (check_first) {
+                if (tid == 0) g.dmax[(size_t)(bk & 1) * C + c] = dpub;
+                if (!grid_sync(cnt, (unsigned)(bk + 1) * (unsigned)C, g.err, &sh_ok)) return;
+                bk++;
+                double dv = 0.0;
+                for (int j = tid; j < C; j += 256) dv = fmax(dv, g.dmax[(size_t)((bk - 1) & 1) * C + j]);
+                if (block_max(dv, sred) <= g.tol) break;      // U^k is the result (uniform decision)
+            }
+            // ---- coefficients at the iterate, rows, chunk elimination ----
+            double K = 0.0, invC = 0.0;
+            if (halo) {
+                double Kl, iCl;
+                vgm_coefs(uith, (s - 1) * g.h, g.ksat, g.coef_mode, Kl, iCl);
+                sK[0] = Kl;
+            }
+            if (act) {
+                vgm_coefs(uit, i * g.h, g.ksat, g.coef_mode, K, invC);
+                sK[tid + 1] = K;
+            }
+            __syncthreads();
+            double ar = 0.0, br = 1.0, cr = 0.0, dr = 0.0;
+            if (act) {
+                if (i == 0) { ar = 0.0; br = 1.0; cr = 0.0; dr = g.utop; }
+                else if (i == N - 1) { ar = -1.0; br = 1.0; cr = 0.0; dr = 0.0; }
+                else {
+                    double Km = sK[tid], Kp = sK[tid + 1];
+                    double A = Km * invC * g.h2inv;
+                    double B = Kp * invC * g.h2inv;
+                    double R = invC * (Kp * g.h2inv + Km * g.h2inv) + g.L;
+                    ar = A; br = -R; cr = B; dr = -g.L * un + S;
+                }
+            }
+            if (inner) {
+                double ib = rcp_nr(br);
+                pA[0][q] = (tid == 1) ? 0.0 : ar * ib;
+                pC[0][q] = (tid == nc - 2) ? 0.0 : cr * ib;
+                pX[0][q] = dr * ib;
+                pY[0][q] = (tid == 1) ? -ar * ib : 0.0;
+                pZ[0][q] = (tid == nc - 2) ? -cr * ib : 0.0;
+            }
+            __syncthreads();
+            int pp = 0;
+            for (int delta = 1; delta < ni; delta <<= 1) {
+                if (inner) {
+                    double aq = pA[pp][q], cq = pC[pp][q];
+                    double am = 0, cm = 0, xm = 0, ym = 0, zm = 0, ap = 0, cp = 0, xp = 0, yp = 0, zp = 0;
+                    if (q - delta >= 0) {
+                        am = pA[pp][q - delta]; cm = pC[pp][q - delta];
+                        xm = pX[pp][q - delta]; ym = pY[pp][q - delta]; zm = pZ[pp][q - delta];
+                    }
+                    if (q + delta < ni) {
+                        ap = pA[pp][q + delta]; cp = pC[pp][q + delta];
+                        xp = pX[pp][q + delta]; yp = pY[pp][q + delta]; zp = pZ[pp][q + delta];
+                    }
+                    double inv = rcp_nr(1.0 - aq * cm - cq * ap);
+                    pA[1 - pp][q] = -aq * am * inv;
+                    pC[1 - pp][q] = -cq * cp * inv;
+                    pX[1 - pp][q] = (pX[pp][q] - aq * xm - cq * xp) * inv;
+                    pY[1 - pp][q] = (pY[pp][q] - aq * ym - cq * yp) * inv;
+                    pZ[1 - pp][q] = (pZ[pp][q] - aq * zm - cq * zp) * inv;
+                }
+                __syncthreads();
+                pp = 1 - pp;
+            }
+            double fx = 0.0, fy = 0.0, fz = 0.0;
+            if (inner) { fx = pX[pp][q]; fy = pY[pp][q]; fz = pZ[pp][q]; }
+            // reduced rows of the chunk ends and the convergence measure
+            double* redw = g.red + (size_t)(bk & 1) * R2 * 4;
+            if (tid == 0) {
+                double x, y, z;
+                if (nc == 2) { x = 0.0; y = 0.0; z = 1.0; }
+                else { x = pX[pp][0]; y = pY[pp][0]; z = pZ[pp][0]; }
+                redw[4 * (2 * c) + 0] = ar;
+                redw[4 * (2 * c) + 1] = br + cr * y;
+                redw[4 * (2 * c) + 2] = cr * z;
+                redw[4 * (2 * c) + 3] = dr - cr * x;
+                g.dmax[(size_t)(bk & 1) * C + c] = dpub;
+            }
+            if (tid == nc - 1) {
+                double x, y, z;
+                if (nc == 2) { x = 0.0; y = 1.0; z = 0.0; }
+                else { x = pX[pp][ni - 1]; y = pY[pp][ni - 1]; z = pZ[pp][ni - 1]; }
+                redw[4 * (2 * c + 1) + 0] = ar * y;
+                redw[4 * (2 * c + 1) + 1] = br + ar * z;
+                redw[4 * (2 * c + 1) + 2] = cr;
+                redw[4 * (2 * c + 1) + 3] = dr - ar * x;
+            }
+            if (!grid_sync(cnt, (unsigned)(bk + 1) * (unsigned)C, g.err, &sh_ok)) return;
+            // ---- every workgroup: convergence of U^k, then the reduced solve
+            const double* redr = g.red + (size_t)(bk & 1) * R2 * 4;
+            bk++;
+            double rrow[2][4];
+#pragma unroll
+            for (int j = 0; j < 2; j++) {
+                int r = tid + 256 * j;
+                bool ok = r < R2;
+#pragma unroll
+                for (int qq = 0; qq < 4; qq++) rrow[j][qq] = ok ? redr[4 * r + qq] : 0.0;
+            }
+            if (k >= 1 && !check_first) {
+                double dv = 0.0;
+                for (int j = tid; j < C; j += 256) dv = fmax(dv, g.dmax[(size_t)((bk - 1) & 1) * C + j]);
+                if (block_max(dv, sred) <= g.tol) break;      // U^k is the result (uniform decision)
+            }
+#pragma unroll
+            for (int j = 0; j < 2; j++) {
+                int r = tid + 256 * j;
+                if (r < R2) {
+                    double ib = rcp_nr(rrow[j][1]);
+                    rA[0][r] = rrow[j][0] * ib;
+                    rC[0][r] = rrow[j][2] * ib;
+                    rD[0][r] = rrow[j][3] * ib;
+                }
+            }
+            __syncthreads();
+            int pr = pcr_norm(rA, rC, rD, R2);
+            double unew = 0.0;
+            if (act) {
+                if (i == s) unew = rD[pr][2 * c];
+                else if (i == e) unew = rD[pr][2 * c + 1];
+                else unew = fx + fy * rD[pr][2 * c] + fz * rD[pr][2 * c + 1];
+            }
+            double unewh = halo ? rD[pr][2 * c - 1] : 0.0;
+            double d = act ? fabs(unew - uit) : 0.0;
+            dpub = block_max(d, sred);               // also orders the LDS reads before reuse
+            uit = unew;
+            uith = unewh;
+        }
+        // ---- step n is done: the last iterate is U^{n+1} -------------------
+        const double result = uit, resulth = uith;
+        double dn = result - un;
+        if (act) {
+            g.D[(series ? (n & 1) : n) * g.ld + i] = dn;
+            g.U[i] = result;
+        }
+        dh = resulth - uh;
+        uh = resulth;
+        dlast = dn;
+        un = result;
+        if (n == 0) d0m = dn;
+        if (n == 1) d1m = dn;
+    }
+}
+
+bool step1d_block_supported(int C) {
+    int dev = 0, coop = 0, cus = 0, per_cu = 0;
+    if (hipGetDevice(&dev) != hipSuccess) return false;
+    if (hipDeviceGetAttribute(&coop, hipDeviceAttributeCooperativeLaunch, dev) != hipSuccess || !coop) return false;
+    if (hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess) return false;
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_step1d_block, 256, 0) != hipSuccess) return false;
+    return (int64_t)per_cu * cus >= C;
+}
+
+fpde_status launch_step1d_block(const Step1DArgs& a, hipStream_t st) {
+    if (2 * a.C > kMaxRed) { set_error("1D line too long for the reduced system (N <= 65536)"); return FPDE_ERR_ARG; }
+    if (a.n_end <= a.n) return FPDE_OK;
+    Step1DArgs arg = a;
+    void* params[] = {&arg};
+    FPDE_HIP_TRY(hipLaunchCooperativeKernel((const void*)k_step1d_block, dim3(a.C), dim3(256), params, 0, st));
+    return FPDE_OK;
 }
 
 fpde_status launch_step1d(const Step1DArgs& a, hipStream_t st) {

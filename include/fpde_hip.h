@@ -137,7 +137,12 @@ typedef struct {
     int hist_mode;       /* 0 plain FP64, 1 FP64 MFMA GEMM, 2 split bf16 hi+lo,
                             3 split fp16 hi+lo + scaling, 4 split bf16 x3        */
     int block;           /* future steps per GEMM block (multiple of 16, <=128)  */
-    int use_graph;       /* reserved (the step loop is issued on one stream)     */
+    int step_mode;       /* 1D step launches: 0 = cooperative blocks (one launch
+                            per history block, every Picard iteration inside,
+                            grid barriers between iterations) for the GEMM and
+                            series history forms when the device supports
+                            cooperative launch, else per iteration;
+                            1 = one launch per Picard iteration                 */
     int sum_alg;         /* time_summing form (fpde_hpc_1d.cpp:4086-4164):
                             0 full history sum;
                             1 restricted: newest intervals only, down to the

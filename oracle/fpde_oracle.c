@@ -504,6 +504,11 @@ int orc_solve_1d(const orc_params *p, double *u)
         history_term(&w, p, s, n, old, H, tmp, &L);
         const double *un = old + (size_t)s * n;
         memcpy(uit, un, sizeof(double) * n);
+        /* Picard (picard_max > 1) starts from the linear extrapolation
+         * 2 U^n - U^{n-1}; a single pass keeps the lagged coefficients at U^n
+         * (the reference scheme, KK() inside A_/B/R, :3095-3142). */
+        if (p->picard_max > 1 && s >= 1)
+            for (int i = 0; i < n; i++) uit[i] = 2.0 * un[i] - un[i - n];
         for (int k = 0; k < p->picard_max; k++) {
             line_solve(n, un, uit, H, zc, h, L, 0, utop, &cc, Al, Bt, K, iC, unew);
             double dmax = 0.0;

@@ -233,6 +233,46 @@ def test_picard_tolerance_exit_matches_oracle(fpde, oracle, tol, mode):
     assert err <= (TOL_F64 if mode == "plain" else TOL_MIXED)
 
 
+@pytest.mark.parametrize("tol", [0.0, 1e-5, 1e-7])
+@pytest.mark.parametrize("picard", [2, 3])
+def test_picard_chunk_halo_tight(fpde, oracle, tol, picard):
+    """Stiff coefficients (mesh Fourier number 40) make every Picard iterate
+    sensitive to the coefficient at the node left of each chunk: that node
+    must see the same iterate (the extrapolated start, the converged iterate
+    when the tolerance was met one launch late) as the chunk's own nodes.
+    Checked to 1e-12 over three chunks."""
+    nodes, steps = 600, 60
+    u0 = fpde.init_1d(nodes, 78)
+    spec = fpde.make_spec(nodes, steps, 1e-3, alpha=0.8, picard_max=picard, picard_tol=tol, hist_mode="plain",
+                          u_top=u0[0], fourier=40.0)
+    u = fpde.solve(spec, u0)
+    err = rel_err(u, _oracle_1d(oracle, spec, u0, picard_tol=tol))
+    print(f"picard {picard} tol {tol}: rel err {err:.3e}")
+    assert err <= 1e-12
+
+
+@pytest.mark.parametrize("mode", ["plain", "mfma_f64", "split_bf16"])
+@pytest.mark.parametrize("picard,tol", [(1, 0.0), (3, 1e-12), (3, 1e-5)])
+def test_cooperative_block_launch_equals_per_iteration_launch(fpde, mode, picard, tol):
+    """The two 1D launch forms (one cooperative launch per history block with
+    grid barriers between Picard iterations; one launch per iteration) run
+    the same arithmetic: results agree to rounding, also when advanced in
+    pieces that cut blocks.  (The split forms place their GEMM cut one level
+    apart in the two forms, so they agree to the split precision.)"""
+    nodes, steps = 900, 150
+    u0 = fpde.init_1d(nodes, 91)
+    out = []
+    for step_mode in (0, 1):
+        spec = fpde.make_spec(nodes, steps, 1e-3, alpha=0.75, hist_mode=mode, block=32, picard_max=picard,
+                              picard_tol=tol, u_top=u0[0], fourier=10.0, step_mode=step_mode)
+        with fpde.Solver(spec) as s:
+            s.set_initial(u0)
+            for chunk in (7, 50, 1000):
+                s.advance(chunk)
+            out.append(s.solution())
+    assert rel_err(out[0], out[1]) <= (TOL_MIXED if mode.startswith("split") else 1e-13)
+
+
 @pytest.mark.parametrize("block", [16, 32, 128])
 def test_solve_1d_blocks(fpde, oracle, block):
     nodes, steps = 700, 260
@@ -297,8 +337,9 @@ def test_server_results_same_alone_and_in_mix(fpde):
             n = int(rng.integers(100, 900))
             u0 = fpde.init_1d(n, 2000 + i)
             kw = dict(do_q=4) if i % 5 == 1 else {}
+            # the server launches 1D steps per Picard iteration: the same here
             sp = fpde.make_spec(n, int(rng.integers(60, 300)), 1e-3, alpha=float(rng.uniform(0.3, 0.95)),
-                                hist_mode="split_f16", u_top=u0[0], **kw)
+                                hist_mode="split_f16", u_top=u0[0], step_mode=1, **kw)
         specs.append((sp, u0))
     alone = [fpde.solve(sp, u0) for sp, u0 in specs]
     fps = [fpde.footprint(sp) for sp, _ in specs]
