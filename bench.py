@@ -236,7 +236,8 @@ def main():
         p = L.fpde_cl_buf_device_ptr
         F.check(L.fpde_history_sum_f64(p(bufs[4]), p(bufs[2]), n, nodes, ld, p(bufs[3])))
         print("[counters] plain history sum n=2000 N=10000", file=sys.stderr, flush=True)
-        spec, u0 = F.workload("1d_large", hist_mode="split_bf16", steps=2000)
+        spec, u0 = F.workload("1d_large", hist_mode="split_bf16", steps=2000, picard_max=args.picard,
+                              picard_tol=args.picard_tol)
         F.solve(spec, u0)
         print("[counters] split_bf16 solve N=10000 M=2000", file=sys.stderr, flush=True)
         for bb in bufs:

@@ -91,7 +91,10 @@ __global__ void k_weights_corr(OrderSet os, double tau, int M, double eps, int s
     for (int q = 0; q < os.q; q++) {
         double a = os.a[q];
         bool reach = (n < 3) || kept(l1_closed_lag(a, tau, n - 2), cut);
-        double b1 = series ? l1_series(a, tau, 2.0 * tau, (n + 1) * tau, eps) : l1_closed_lag(a, tau, n - 1);
+        // interval 1 exists for n >= 2 only (at n = 1 it is the newest one,
+        // whose series would not converge: t1 = a)
+        double b1 = 0.0;
+        if (n >= 2) b1 = series ? l1_series(a, tau, 2.0 * tau, (n + 1) * tau, eps) : l1_closed_lag(a, tau, n - 1);
         if (t == 0) {
             reach = reach && (n < 2 || kept(b1, cut));
             double b0 = series ? l1_series(a, 0.0, tau, (n + 1) * tau, eps) : l1_closed_lag(a, tau, n);
