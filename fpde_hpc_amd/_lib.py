@@ -87,6 +87,7 @@ SIGNATURES = [
     ("fpde_solver_query", ctypes.c_int, [_vp]),
     ("fpde_solver_step", ctypes.c_int, [_vp]),
     ("fpde_solver_get_solution", ctypes.c_int, [_vp, _dp]),
+    ("fpde_solver_write_results", ctypes.c_int, [_vp, ctypes.c_char_p, ctypes.c_char_p]),
     ("fpde_solver_get_weights", ctypes.c_int, [_vp, _dp, _dp, _dp]),
     ("fpde_solver_set_profiling", ctypes.c_int, [_vp, ctypes.c_int]),
     ("fpde_solver_get_stats", ctypes.c_int, [_vp, ctypes.POINTER(Stats)]),

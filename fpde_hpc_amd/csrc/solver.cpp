@@ -8,6 +8,7 @@
 #include "fpde_internal.h"
 
 #include <algorithm>
+#include <cstdio>
 #include <chrono>
 #include <cmath>
 #include <cstring>
@@ -684,6 +685,44 @@ fpde_status fpde_solver_get_solution(fpde_solver* s, double* u_out) {
     FPDE_HIP_TRY(hipStreamSynchronize(s->stream));
     ev_collect(s);
     return check_barrier(s);
+}
+
+fpde_status fpde_solver_write_results(fpde_solver* s, const char* path_txt, const char* path_3d) {
+    if (!s || !path_txt) { set_error("null argument"); return FPDE_ERR_ARG; }
+    const fpde_spec& sp = s->spec;
+    std::vector<double> u((size_t)s->plan.nodes);
+    fpde_status rc = fpde_solver_get_solution(s, u.data());
+    if (rc != FPDE_OK) return rc;
+    FILE* f = std::fopen(path_txt, "a");
+    if (!f) { set_error(std::string("cannot open ") + path_txt); return FPDE_ERR_ARG; }
+    const double t = s->step * sp.tau, h = s->h;
+    // wetness() with cv_form 4 (fpde_hpc_1d.cpp:2493-2526) at P = z - u
+    auto theta = [](double P) {
+        const double n = kVgmN, s0 = kVgmS0, s1 = kVgmS1, a = kVgmA;
+        P *= 100.0;
+        if (P <= 0.0) return s1;
+        return s0 + (s1 - s0) / std::pow(1.0 + std::pow(a * P, n), 1.0 - 1.0 / n);
+    };
+    const int n = sp.nodes;
+    for (int i = 0; i < n; i++) {
+        double z = i * h;
+        if (sp.dims == 1) {
+            std::fprintf(f, "%g %g %g %g %g %g \n", t, z, u[i], 0.0, 0.0, theta(z - u[i]));
+        } else {
+            size_t id = (size_t)i + (size_t)n * ((size_t)(n / 2) + (size_t)n * (size_t)(n / 2));
+            std::fprintf(f, "%g %g %g %g %g %g %g %g \n", t, z, (n / 2) * h, (n / 2) * h, u[id], 0.0, 0.0,
+                         theta(z - u[id]));
+        }
+    }
+    std::fclose(f);
+    if (sp.dims == 3 && path_3d) {
+        FILE* g = std::fopen(path_3d, "ab");
+        if (!g) { set_error(std::string("cannot open ") + path_3d); return FPDE_ERR_ARG; }
+        size_t w = std::fwrite(u.data(), sizeof(double), u.size(), g);
+        std::fclose(g);
+        if (w != u.size()) { set_error("short write to the 3D results file"); return FPDE_ERR_ARG; }
+    }
+    return FPDE_OK;
 }
 
 fpde_status fpde_solver_get_weights(fpde_solver* s, double* local, double* omega, double* corr) {

@@ -6,6 +6,7 @@ configurations of the benchmark suite.  Every call goes to libfpde_hip.so;
 there is no CPU path here.
 """
 import ctypes
+import os
 
 import numpy as np
 
@@ -99,6 +100,12 @@ class Solver:
         u = np.zeros(self.unknowns)
         check(lib().fpde_solver_get_solution(self._h, _p(u)))
         return u
+
+    def write_results(self, path_txt="results.txt", path_3d=None):
+        """Append the current solution in the reference driver's results.txt
+        (and, in 3D, results3d.txt) formats."""
+        check(lib().fpde_solver_write_results(self._h, os.fsencode(path_txt),
+                                              None if path_3d is None else os.fsencode(path_3d)))
 
     def weights(self):
         M = self.spec.steps

@@ -187,6 +187,15 @@ int fpde_solver_query(fpde_solver *s);
 /* Steps completed so far. */
 int fpde_solver_step(const fpde_solver *s);
 fpde_status fpde_solver_get_solution(fpde_solver *s, double *u_out);
+/* Output files of the reference driver (fpde_hpc_1d.cpp:9498-9526, opened
+ * at :8945-8947), for the current solution at time step*tau: appends to
+ * path_txt one row per depth node, "t z u 0 0 theta" (1D) or
+ * "t z x y u 0 0 theta" along the depth line through the centre x = y =
+ * nodes/2 (3D), printed with %g; the two zeros stand for the C and T fields
+ * the moisture-only model does not have, theta is the VGM wetness at
+ * P = z - u.  3D also appends the whole field, nodes^3 FP64 values in
+ * solver order (depth fastest, no halo layer), to path_3d (NULL: skipped). */
+fpde_status fpde_solver_write_results(fpde_solver *s, const char *path_txt, const char *path_3d);
 /* Effective weights (local, omega[M], corr[2M]) the solver uses (host copies). */
 fpde_status fpde_solver_get_weights(fpde_solver *s, double *local, double *omega, double *corr);
 /* Per-kernel timing with hipEvents on the solver's stream: bit 0 = history

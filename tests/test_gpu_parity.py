@@ -569,3 +569,39 @@ def test_reference_sum_kernel_series_through_facade(fpde, oracle):
         L.fpde_cl_program_destroy(prog)
     finally:
         dv.close()
+
+
+def test_results_files_in_reference_format(fpde, oracle, tmp_path):
+    """results.txt rows (fpde_hpc_1d.cpp:9498-9526): t z u C T theta (1D) and
+    t z x y u C T theta along the centre depth line plus the raw FP64 field
+    (3D), theta = VGM wetness at P = z - u."""
+    u0 = fpde.init_1d(50, 12)
+    spec = fpde.make_spec(50, 20, 1e-3, alpha=0.7, u_top=u0[0])
+    txt = tmp_path / "results.txt"
+    with fpde.Solver(spec) as s:
+        s.set_initial(u0)
+        s.advance(0)
+        u = s.solution()
+        s.write_results(str(txt))
+        s.write_results(str(txt))            # appends
+    rows = np.loadtxt(txt)
+    assert rows.shape == (100, 6)
+    z = np.arange(50) / 49.0
+    ref = np.column_stack([np.full(50, 20 * 1e-3), z, u, np.zeros(50), np.zeros(50),
+                           [oracle.wetness(zz - uu) for zz, uu in zip(z, u)]])
+    assert np.allclose(rows[:50], ref, rtol=6e-6, atol=1e-12) and np.array_equal(rows[:50], rows[50:])
+    n = 8
+    u3 = fpde.init_3d(n, 13)
+    spec3 = fpde.make_spec(n, 5, 1e-3, alpha=0.85, dims=3, u_top=u3[0])
+    t3, raw = tmp_path / "r3.txt", tmp_path / "results3d.txt"
+    with fpde.Solver(spec3) as s:
+        s.set_initial(u3)
+        s.advance(0)
+        u = s.solution()
+        s.write_results(str(t3), str(raw))
+    rows = np.loadtxt(t3)
+    assert rows.shape == (n, 8)
+    line = u.reshape(n, n, n)[n // 2, n // 2, :]          # y, x, depth
+    assert np.allclose(rows[:, 4], line, rtol=6e-6)
+    assert np.allclose(rows[:, 2], (n // 2) / (n - 1)) and np.allclose(rows[:, 3], (n // 2) / (n - 1))
+    assert np.array_equal(np.fromfile(raw, dtype=np.float64), u)
