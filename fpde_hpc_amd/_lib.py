@@ -27,7 +27,7 @@ class Spec(ctypes.Structure):
         ("do_q", ctypes.c_int), ("do_rule", ctypes.c_int), ("do_a0", ctypes.c_double),
         ("do_a1", ctypes.c_double), ("hist_mode", ctypes.c_int), ("block", ctypes.c_int),
         ("step_mode", ctypes.c_int), ("sum_alg", ctypes.c_int), ("sum_param", ctypes.c_double),
-        ("reserved", ctypes.c_int * 2),
+        ("taus", ctypes.c_void_p), ("reserved", ctypes.c_int * 2),
     ]
 
 

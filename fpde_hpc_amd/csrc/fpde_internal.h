@@ -4,6 +4,7 @@
 #include <cstdint>
 #include <cstddef>
 #include <string>
+#include <vector>
 
 #include "../../include/fpde_hip.h"
 
@@ -86,7 +87,21 @@ struct DeviceWeights {
     int ser_P = 0;
     double* ser_c = nullptr;   // [M][P]
     double* ser_v = nullptr;   // [M][P]
+    // prescribed non-uniform grid (spec.taus): cumulative times, step lengths,
+    // the local coefficient of every step (host and device copies)
+    bool grid = false;
+    double* Time = nullptr;    // [M+2]
+    double* taus = nullptr;    // [M+1]
+    double* Lv = nullptr;      // [M]
+    std::vector<double> hLv;
 };
+
+// Dense weight rows of a prescribed grid (weights.hip): rows r < rows are
+// the steps n = n_first + r; entry j of row r is w_n(t) for t = n-1-j
+// (reversed: omega-style, w_n(t) at index n-t-1) or t = j (direct), and 0
+// outside t in [t_lo, n).
+fpde_status launch_weights_grid(const fpde_spec& s, const DeviceWeights& w, int n_first, int rows, int len,
+                                int t_lo, int reversed, double* W, int64_t ldw, hipStream_t st);
 
 constexpr int kMaxSeriesTerms = 64;
 
@@ -149,6 +164,8 @@ struct Step1DArgs {
     double* dmax;             // [2][C] per-chunk max |U^k - U^{k-1}|
     int* flags;               // unused
     double tol;
+    int64_t omega_stride;     // per-step weight rows (prescribed grid): omega + (n - n0) * stride
+    const double* Lv;         // per-step local coefficient (prescribed grid), else L
     int prev_k;               // Picard index of the previous launch (-1: none)
     int extrap;               // Picard starts from 2 U^n - U^{n-1} (picard_max > 1)
     double* Uit0;             // that initial iterate (convergence test of iteration 1)
@@ -182,6 +199,8 @@ fpde_status launch_gemm_f64(const double* D, int64_t ld, int nodes, int n0, int 
                             const double* omega, double* P, int* splits_out, hipStream_t st);
 fpde_status launch_pack_split(int mode, const double* D, int64_t ld, int nodes, int g_lo, int g_hi, void* c0,
                               void* c1, void* c2, float* scale, hipStream_t st);
+fpde_status launch_gemm_f64_dense(const double* D, int64_t ld, int nodes, int tc, int block, const double* W,
+                                  int64_t ldw, double* P, int* splits_out, hipStream_t st);
 fpde_status launch_gemm_split(int mode, const void* c0, const void* c1, const void* c2, const float* scale,
                               int64_t ld, int nodes, int n0, int tc, int block, const double* omega, double* P,
                               int* splits_out, hipStream_t st);

@@ -203,6 +203,66 @@ __global__ __launch_bounds__(256) void k_gemm_f64(const double* __restrict__ D, 
     }
 }
 
+// FP64 time-block GEMM with dense weights (a prescribed non-uniform grid,
+// where the weights are not Toeplitz): P[ks][r][i] = sum_t W[r][t] D[t][i],
+// W row-major with leading dimension ldw.  Same tiling and MFMA layout as
+// k_gemm_f64; the A operand comes from an LDS tile of BR x 64 weights.
+template <int BR>
+__global__ __launch_bounds__(256) void k_gemm_f64_dense(const double* __restrict__ D, int64_t ld, int nodes,
+                                                        int tc, int kchunk, const double* __restrict__ W,
+                                                        int64_t ldw, double* __restrict__ P) {
+    constexpr int RT = BR / 16;
+    constexpr int SL = 64;
+    __shared__ double win[BR][SL + 1];
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    const int ks = blockIdx.y;
+    const int tlo = ks * kchunk;
+    const int thi = min(tc, tlo + kchunk);
+    const int node = blockIdx.x * 64 + wave * 16 + (lane & 15);
+    const bool node_ok = node < nodes;
+    const int kk = lane >> 4;
+    d4_t acc[RT];
+#pragma unroll
+    for (int rt = 0; rt < RT; rt++) acc[rt] = (d4_t){0.0, 0.0, 0.0, 0.0};
+    for (int s0 = tlo; s0 < thi; s0 += SL) {
+        const int s1 = min(thi, s0 + SL);
+        __syncthreads();
+        for (int q = threadIdx.x; q < BR * SL; q += 256) {
+            int r = q / SL, c = q % SL;
+            win[r][c] = (s0 + c < s1) ? W[(int64_t)r * ldw + s0 + c] : 0.0;
+        }
+        __syncthreads();
+        for (int t = s0; t < s1; t += 16) {
+            double b[4];
+#pragma unroll
+            for (int q = 0; q < 4; q++) {
+                int tt = t + 4 * q + kk;
+                b[q] = (tt < s1 && node_ok) ? D[(int64_t)tt * ld + node] : 0.0;
+            }
+#pragma unroll
+            for (int q = 0; q < 4; q++) {
+                int tt = t + 4 * q + kk;
+                bool ok = tt < s1;
+#pragma unroll
+                for (int rt = 0; rt < RT; rt++) {
+                    int r = rt * 16 + (lane & 15);
+                    double a = ok ? win[r][tt - s0] : 0.0;
+                    acc[rt] = __builtin_amdgcn_mfma_f64_16x16x4f64(a, b[q], acc[rt], 0, 0, 0);
+                }
+            }
+        }
+    }
+    if (!node_ok) return;
+#pragma unroll
+    for (int rt = 0; rt < RT; rt++) {
+#pragma unroll
+        for (int v = 0; v < 4; v++) {
+            int r = rt * 16 + 4 * v + (lane >> 4);
+            P[((int64_t)ks * BR + r) * ld + node] = acc[rt][v];
+        }
+    }
+}
+
 // ---------------------------------------------------------------------------
 // Split storage.  Group g holds levels [32g, 32g+32); for node i the 32
 // values of each component are contiguous: comp[(g*ld + i)*32 + e].  That is
@@ -460,6 +520,24 @@ static void gemm_split_plan(int tc, int64_t nodes, int* kchunk, int* splits) {
     if (ch < 32) ch = 32;
     *kchunk = ch;
     *splits = (tc + ch - 1) / ch;
+}
+
+fpde_status launch_gemm_f64_dense(const double* D, int64_t ld, int nodes, int tc, int block, const double* W,
+                                  int64_t ldw, double* P, int* splits_out, hipStream_t st) {
+    if (tc <= 0) { *splits_out = 0; return FPDE_OK; }
+    int kchunk, splits;
+    gemm_split_plan(tc, nodes, &kchunk, &splits);
+    dim3 grid((unsigned)((nodes + 63) / 64), (unsigned)splits);
+    switch (block) {
+        case 16: hipLaunchKernelGGL(k_gemm_f64_dense<16>, grid, dim3(256), 0, st, D, ld, nodes, tc, kchunk, W, ldw, P); break;
+        case 32: hipLaunchKernelGGL(k_gemm_f64_dense<32>, grid, dim3(256), 0, st, D, ld, nodes, tc, kchunk, W, ldw, P); break;
+        case 64: hipLaunchKernelGGL(k_gemm_f64_dense<64>, grid, dim3(256), 0, st, D, ld, nodes, tc, kchunk, W, ldw, P); break;
+        case 128: hipLaunchKernelGGL(k_gemm_f64_dense<128>, grid, dim3(256), 0, st, D, ld, nodes, tc, kchunk, W, ldw, P); break;
+        default: set_error("block must be 16, 32, 64 or 128"); return FPDE_ERR_ARG;
+    }
+    *splits_out = splits;
+    FPDE_HIP_TRY(hipGetLastError());
+    return FPDE_OK;
 }
 
 fpde_status launch_gemm_f64(const double* D, int64_t ld, int nodes, int n0, int tc, int block,

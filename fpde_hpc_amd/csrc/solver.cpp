@@ -57,6 +57,16 @@ fpde_status validate(const fpde_spec& s) {
         set_error("sum_param must be finite and >= 0 for the restricted sum");
         return FPDE_ERR_ARG;
     }
+    if (s.taus) {
+        if (s.dims != 1) { set_error("a prescribed time grid (taus) is for 1D solves"); return FPDE_ERR_ARG; }
+        if (s.hist_mode != HIST_PLAIN_F64 && s.hist_mode != HIST_MFMA_F64) {
+            set_error("a prescribed time grid needs hist_mode 0 (plain) or 1 (FP64 MFMA with dense weights)");
+            return FPDE_ERR_ARG;
+        }
+        if (s.sum_alg != 0) { set_error("a prescribed time grid needs sum_alg 0"); return FPDE_ERR_ARG; }
+        for (int k = 0; k < s.steps; k++)
+            if (!(s.taus[k] > 0.0 && std::isfinite(s.taus[k]))) { set_error("taus must be positive and finite"); return FPDE_ERR_ARG; }
+    }
     if (s.sum_alg == 2) {
         if (!(s.sum_param >= 1.0 && s.sum_param <= kMaxSeriesTerms && s.sum_param == std::floor(s.sum_param))) {
             set_error("sum_alg 2: sum_param is the number of series terms, an integer in [1, 64]");
@@ -98,6 +108,10 @@ size_t footprint_bytes(const fpde_spec& s) {
         b += sizeof(double) * P * p.ld;                              // moments
         b += sizeof(double) * 2 * P * (size_t)s.steps;               // coefficient tables
     }
+    if (s.taus) {                                                    // grid, weight rows
+        size_t M = (size_t)s.steps, B = (size_t)s.block;
+        b += sizeof(double) * (3 * M + 64 + B * (round_up(M + 64, 64) + B + 64));
+    }
     return b;
 }
 
@@ -123,6 +137,14 @@ struct fpde_solver {
     double* H = nullptr;       // sum_alg 2 moments [P][ld]
     double* Uit0 = nullptr;    // initial Picard iterate
     double* uend = nullptr;    // chunk-end values by step parity
+    // prescribed grid: host copy of the step lengths, weight rows of the
+    // current step (plain) or block (far: direct, near: reversed per step)
+    std::vector<double> taus_h;
+    double* Wrow = nullptr;
+    double* Wfar = nullptr;
+    double* Wnear = nullptr;
+    int64_t ldwf = 0, ldwn = 0;
+    const double* omega_cur = nullptr;   // weights of the step being enqueued
     unsigned* bar = nullptr;   // grid barrier of the cooperative 1D block launches
     int* err = nullptr;        // barrier time-out flag
     bool block_mode = false;   // 1D steps run as cooperative block launches
@@ -210,6 +232,25 @@ fpde_status solver_alloc(fpde_solver* s) {
     FPDE_HIP_TRY(hipStreamCreateWithFlags(&s->stream, hipStreamNonBlocking));
     int M = sp.steps;
     s->w.M = M;
+    if (sp.taus) {
+        s->w.grid = true;
+        std::vector<double> T((size_t)M + 2, 0.0), tt((size_t)M + 1);
+        for (int k = 0; k <= M; k++) tt[k] = sp.taus[k < M ? k : M - 1];
+        for (int k = 1; k <= M + 1; k++) T[k] = T[k - 1] + tt[k - 1];       // init_time_step (:3944)
+        DALLOC(s->w.Time, sizeof(double) * (M + 2));
+        DALLOC(s->w.taus, sizeof(double) * (M + 1));
+        DALLOC(s->w.Lv, sizeof(double) * M);
+        FPDE_HIP_TRY(hipMemcpyAsync(s->w.Time, T.data(), sizeof(double) * (M + 2), hipMemcpyHostToDevice, s->stream));
+        FPDE_HIP_TRY(hipMemcpyAsync(s->w.taus, tt.data(), sizeof(double) * (M + 1), hipMemcpyHostToDevice, s->stream));
+        DALLOC(s->Wrow, sizeof(double) * (size_t)(M + 64));
+        if (sp.hist_mode == HIST_MFMA_F64) {
+            s->ldwf = round_up(M + 64, 64);
+            s->ldwn = sp.block + 64;
+            DALLOC(s->Wfar, sizeof(double) * (size_t)sp.block * s->ldwf);
+            DALLOC(s->Wnear, sizeof(double) * (size_t)sp.block * s->ldwn);
+        }
+        FPDE_HIP_TRY(hipStreamSynchronize(s->stream));    // the host vectors above go out of scope
+    }
     DALLOC(s->w.omega, sizeof(double) * (size_t)(M + 512));
     DALLOC(s->w.corr, sizeof(double) * 2 * (size_t)(M + 1));
     FPDE_HIP_TRY(hipMemsetAsync(s->w.omega, 0, sizeof(double) * (size_t)(M + 512), s->stream));
@@ -283,10 +324,17 @@ fpde_status far_part(fpde_solver* s, int n, int* nsplit, int64_t* stride, int64_
         size_t slot = 0;
         // restricted summing: lags above jmax have zero weight, t >= n - jmax
         int t_lo = std::max(0, n - s->w.jmax);
+        s->omega_cur = s->w.omega;
+        if (s->w.grid) {
+            // prescribed grid: this step's weight row, omega-style (reversed)
+            fpde_status rc = launch_weights_grid(sp, s->w, n, 1, std::max(n, 1), 0, 1, s->Wrow, 0, s->stream);
+            if (rc != FPDE_OK) return rc;
+            s->omega_cur = s->Wrow;
+        }
         if (avail > t_lo) {
             ev_begin(s, 0, &slot);
             fpde_status rc = launch_hist_plain(s->D + (size_t)t_lo * p.ld, p.ld, (int)p.nodes, n - t_lo, avail - t_lo,
-                                               s->w.omega, s->part, &ns, s->stream);
+                                               s->omega_cur, s->part, &ns, s->stream);
             ev_end(s, slot);
             if (rc != FPDE_OK) return rc;
             s->stats.history_launches++;
@@ -330,6 +378,15 @@ fpde_status far_part(fpde_solver* s, int n, int* nsplit, int64_t* stride, int64_
                 rc = launch_gemm_split(sp.hist_mode, cp(0), cp(1), cp(2), s->scale + (size_t)(t_lo / kGroup) * p.ld,
                                        p.ld, (int)p.nodes, n - t_lo, tc - t_lo, B, s->w.omega, s->part, &s->gsplits,
                                        s->stream);
+        } else if (s->w.grid) {
+            // prescribed grid: dense weights of the block, far (t < tc) and
+            // near (t in [tc, n), one reversed row per step)
+            rc = launch_weights_grid(sp, s->w, n, B, std::max(tc, 1), 0, 0, s->Wfar, s->ldwf, s->stream);
+            if (rc == FPDE_OK)
+                rc = launch_weights_grid(sp, s->w, n, B, B + 32, tc, 1, s->Wnear, s->ldwn, s->stream);
+            if (rc == FPDE_OK)
+                rc = launch_gemm_f64_dense(s->D, p.ld, (int)p.nodes, tc, B, s->Wfar, s->ldwf, s->part, &s->gsplits,
+                                           s->stream);
         } else {
             rc = launch_gemm_f64(s->D + (size_t)t_lo * p.ld, p.ld, (int)p.nodes, n - t_lo, tc - t_lo, B, s->w.omega,
                                  s->part, &s->gsplits, s->stream);
@@ -350,6 +407,7 @@ fpde_status far_part(fpde_solver* s, int n, int* nsplit, int64_t* stride, int64_
     *stride = (int64_t)B * p.ld;
     *off = (int64_t)(n - s->n0) * p.ld;
     *near_lo = s->tc;
+    s->omega_cur = s->w.grid ? s->Wnear + (size_t)(n - s->n0) * s->ldwn : s->w.omega;
     return FPDE_OK;
 }
 
@@ -361,7 +419,7 @@ fpde_status launch_1d(fpde_solver* s, int n, int k, int nsplit, int64_t stride, 
     a.h = s->h;
     a.h2inv = 1.0 / (s->h * s->h);
     a.ksat = sp.ksat;
-    a.L = s->w.local;
+    a.L = (s->w.grid && n < sp.steps) ? s->w.hLv[n] : s->w.local;
     a.utop = s->utop1d;
     a.coef_mode = sp.coef_mode;
     a.n = n;
@@ -371,7 +429,7 @@ fpde_status launch_1d(fpde_solver* s, int n, int k, int nsplit, int64_t stride, 
     a.U = s->U;
     a.D = s->D;
     a.ld = s->plan.ld;
-    a.omega = s->w.omega;
+    a.omega = s->omega_cur ? s->omega_cur : s->w.omega;
     a.corr = s->w.corr;
     a.part = s->part;
     a.nsplit = nsplit;
@@ -443,7 +501,9 @@ fpde_status launch_1d_block(fpde_solver* s, int n, int n_end, int n0, int nsplit
     a.U = s->U;
     a.D = s->D;
     a.ld = s->plan.ld;
-    a.omega = s->w.omega;
+    a.omega = s->w.grid ? (sp.hist_mode == HIST_PLAIN_F64 ? s->Wrow : s->Wnear) : s->w.omega;
+    a.omega_stride = (s->w.grid && sp.hist_mode != HIST_PLAIN_F64) ? s->ldwn : 0;
+    a.Lv = s->w.grid ? s->w.Lv : nullptr;
     a.corr = s->w.corr;
     a.part = s->part;
     a.nsplit = nsplit;
@@ -573,6 +633,10 @@ fpde_status fpde_solver_create(const fpde_spec* spec, fpde_solver** out) {
     if (rc != FPDE_OK) return rc;
     auto s = std::make_unique<fpde_solver>();
     s->spec = *spec;
+    if (spec->taus) {
+        s->taus_h.assign(spec->taus, spec->taus + spec->steps);
+        s->spec.taus = s->taus_h.data();
+    }
     s->plan = make_plan(*spec);
     rc = solver_alloc(s.get());
     if (rc != FPDE_OK) {

@@ -103,7 +103,8 @@ __device__ inline int pcr_wave(double* A, double* Cc, double* Dd, int n) {
 // memory, gathered in 32-wide batches of independent loads (a sum over m
 // terms costs ~m/32 memory round trips).  Every load is guarded: omega is
 // read only at n-t-1 >= 1.
-__device__ inline double hist_mem(const Step1DArgs& g, int i, int n, int64_t part_off) {
+__device__ inline double hist_mem(const Step1DArgs& g, int i, int n, int64_t part_off,
+                                  const double* __restrict__ omega) {
     constexpr int W = 32;
     double far = 0.0;
     for (int sp0 = 0; sp0 < g.nsplit; sp0 += W) {
@@ -123,7 +124,7 @@ __device__ inline double hist_mem(const Step1DArgs& g, int i, int n, int64_t par
             int t = t0 + j;
             bool ok = t < t_end;
             v[j] = ok ? g.D[(int64_t)t * g.ld + i] : 0.0;
-            w[j] = ok ? g.omega[n - t - 1] : 0.0;
+            w[j] = ok ? omega[n - t - 1] : 0.0;
         }
 #pragma unroll
         for (int j = 0; j < W; j += 2) {
@@ -202,7 +203,7 @@ __global__ __launch_bounds__(256) void k_step1d(Step1DArgs g) {
     const bool extrap0 = first && g.k == 0 && g.extrap && n >= 1;
     double uit_halo = (halo && check_conv) ? g.Uit[s - 1] : 0.0;
     double uprev_halo = (halo && extrap0) ? g.uend[(size_t)((n - 1) & 1) * R2 + 2 * c - 1] : 0.0;
-    double hm = (first && !series && g.k == 0 && act && n >= 1) ? hist_mem(g, i, n, g.part_off) : 0.0;
+    double hm = (first && !series && g.k == 0 && act && n >= 1) ? hist_mem(g, i, n, g.part_off, g.omega) : 0.0;
 
     // ---- 1. finish the previous solve: reduced system, then back-substitute --
     double val = 0.0;
@@ -507,14 +508,18 @@ __global__ __launch_bounds__(256) void k_step1d_block(Step1DArgs g) {
 
     for (int n = n_first; n < g.n_end; n++) {
         // ---- history term of step n -------------------------------------
+        // weights of step n: one Toeplitz vector, or the step's own row
+        // (prescribed grid); local coefficient likewise
+        const double* om = g.omega + (int64_t)(n - g.n0) * g.omega_stride;
+        const double Ln = g.Lv ? g.Lv[n] : g.L;
         double S = 0.0;
         if (n >= 1 && act) {
             if (series) {
                 S = series_update(g.H, g.ld, i, g.ser_P, dlast, g.ser_c + (size_t)n * g.ser_P,
                                   g.ser_v + (size_t)n * g.ser_P);
             } else {
-                double hm = hist_mem(g, i, n, (int64_t)(n - g.n0) * g.ld);
-                double newest = (g.near_lo <= n - 1) ? g.omega[0] * dlast : 0.0;
+                double hm = hist_mem(g, i, n, (int64_t)(n - g.n0) * g.ld, om);
+                double newest = (g.near_lo <= n - 1) ? om[0] * dlast : 0.0;
                 double cr = (n == 1) ? g.corr[2] * dlast : g.corr[2 * n] * (n >= 2 ? d0m : 0.0);
                 if (n == 2) cr = fma(g.corr[2 * n + 1], dlast, cr);
                 else if (n >= 3) cr = fma(g.corr[2 * n + 1], d1m, cr);
@@ -558,8 +563,8 @@ __global__ __launch_bounds__(256) void k_step1d_block(Step1DArgs g) {
                     double Km = sK[tid], Kp = sK[tid + 1];
                     double A = Km * invC * g.h2inv;
                     double B = Kp * invC * g.h2inv;
-                    double R = invC * (Kp * g.h2inv + Km * g.h2inv) + g.L;
-                    ar = A; br = -R; cr = B; dr = -g.L * un + S;
+                    double R = invC * (Kp * g.h2inv + Km * g.h2inv) + Ln;
+                    ar = A; br = -R; cr = B; dr = -Ln * un + S;
                 }
             }
             if (inner) {

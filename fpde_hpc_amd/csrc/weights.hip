@@ -162,6 +162,35 @@ __global__ void k_series_tables(double a, double tau, int M, int P, double eps, 
     }
 }
 
+// Prescribed non-uniform grid: the weight of interval t at step n (unknown
+// U^{n+1}) is w_n(t) = sum_q cw_q Da_q kb_q(T_t, T_{t+1}; T_{n+1}) / tau_t, the
+// kb_3_cache of precalc_kb_3 over Time[] (fpde_hpc_1d.cpp:4069-4085) divided by
+// tau[t] in time_summing (:4131).  The closed form ((a-t0)^e - (a-t1)^e)/e,
+// e = 1-alpha, is evaluated as (a-t1)^e expm1(e log1p((t1-t0)/(a-t1)))/e; the
+// two oldest intervals take the _kb_row2 series in weight_mode 1 (:2265).
+__device__ inline double kb_grid(double a, double p0, double p1, double pa, int t, int series, double eps) {
+    if (a == 1.0) return 0.0;                    // history intervals have t1 < a
+    if (series && t <= 1) return l1_series(a, p0, p1, pa, eps);
+    double e = 1.0 - a, x = pa - p1;
+    return pow(x, e) * expm1(e * log1p((p1 - p0) / x)) / e;
+}
+
+__global__ void k_weights_grid(OrderSet os, const double* __restrict__ Time, const double* __restrict__ taus,
+                               int n_first, int rows, int len, int t_lo, int reversed, int series, double eps,
+                               double* __restrict__ W, int64_t ldw) {
+    int64_t idx = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (idx >= (int64_t)rows * len) return;
+    int r = (int)(idx / len), j = (int)(idx % len);
+    int n = n_first + r;
+    int t = reversed ? n - 1 - j : j;
+    double w = 0.0;
+    if (t >= t_lo && t >= 0 && t < n) {
+        for (int q = 0; q < os.q; q++) w += os.c[q] * kb_grid(os.a[q], Time[t], Time[t + 1], Time[n + 1], t, series, eps);
+        w /= taus[t];
+    }
+    W[(int64_t)r * ldw + j] = w;
+}
+
 static void gauss_legendre(int n, double a, double b, double* x, double* w) {
     for (int i = 0; i < n; i++) {
         double z = std::cos(M_PI * (i + 0.75) / (n + 0.5)), pp = 0.0, p1 = 1.0, p2 = 0.0;
@@ -222,10 +251,55 @@ static fpde_status build_orders(const fpde_spec& s, OrderSet& os) {
     return FPDE_OK;
 }
 
+// orders with c_q = weight_q * Da(a_q) (no 1/tau: a grid divides per interval)
+static fpde_status build_orders_grid(const fpde_spec& s, OrderSet& os) {
+    fpde_status rc = build_orders(s, os);
+    if (rc != FPDE_OK) return rc;
+    for (int q = 0; q < os.q; q++) os.c[q] *= s.tau;
+    return FPDE_OK;
+}
+
+fpde_status launch_weights_grid(const fpde_spec& s, const DeviceWeights& w, int n_first, int rows, int len,
+                                int t_lo, int reversed, double* W, int64_t ldw, hipStream_t st) {
+    if (rows <= 0 || len <= 0) return FPDE_OK;
+    OrderSet os;
+    fpde_status rc = build_orders_grid(s, os);
+    if (rc != FPDE_OK) return rc;
+    int64_t total = (int64_t)rows * len;
+    hipLaunchKernelGGL(k_weights_grid, dim3((unsigned)((total + 255) / 256)), dim3(256), 0, st, os,
+                       (const double*)w.Time, (const double*)w.taus, n_first, rows, len, t_lo, reversed,
+                       s.weight_mode == 1 ? 1 : 0, s.series_eps, W, ldw);
+    FPDE_HIP_TRY(hipGetLastError());
+    return FPDE_OK;
+}
+
 fpde_status launch_weights(const fpde_spec& s, DeviceWeights& w, hipStream_t st) {
     OrderSet os;
     fpde_status rc = build_orders(s, os);
     if (rc != FPDE_OK) return rc;
+    w.jmax = w.M;                                // every lag kept (restricted summing narrows it below)
+    if (w.grid) {
+        // prescribed grid: the local coefficient of every step (kkk0 =
+        // b(n,n+1;n+1) = tau_n^(1-a)/(1-a), divided by tau_n, :3942-3949);
+        // the weights themselves are generated per step or per block
+        int M = w.M;
+        w.hLv.assign(M, 0.0);
+        OrderSet og;
+        build_orders_grid(s, og);
+        for (int n = 0; n < M; n++) {
+            double tn = s.taus[n], L = 0.0;
+            for (int q = 0; q < og.q; q++) {
+                double a = og.a[q];
+                double kkk0 = (a == 1.0) ? 1.0 : std::pow(tn, 1.0 - a) / (1.0 - a);
+                L += og.c[q] * kkk0 / tn;
+            }
+            w.hLv[n] = L;
+        }
+        w.local = w.hLv[0];
+        FPDE_HIP_TRY(hipMemcpyAsync(w.Lv, w.hLv.data(), sizeof(double) * M, hipMemcpyHostToDevice, st));
+        FPDE_HIP_TRY(hipMemsetAsync(w.corr, 0, sizeof(double) * 2 * (size_t)M, st));
+        return FPDE_OK;
+    }
     // local coefficient: Da/tau * kkk0, kkk0 = b(n,n+1;n+1) = tau^(1-a)/(1-a) (alpha=1: 1)
     double local = 0.0;
     for (int q = 0; q < os.q; q++) {

@@ -153,6 +153,12 @@ typedef struct {
                               levels (:4137-4151); single order, hist_mode 0,
                               P in [1, 64]                                      */
     double sum_param;    /* sum_alg 1: threshold on kb; sum_alg 2: terms P       */
+    const double *taus;  /* NULL: uniform tau.  Else a prescribed non-uniform
+                            grid: the M step lengths tau_0..tau_{M-1}
+                            (Time[k+1] = Time[k] + tau_k, init_time_step
+                            fpde_hpc_1d.cpp:3942-3949), copied at create; the
+                            weights are then per step (not Toeplitz): 1D,
+                            hist_mode 0 or 1, sum_alg 0                         */
     int reserved[2];
 } fpde_spec;
 

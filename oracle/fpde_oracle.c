@@ -388,8 +388,9 @@ static void wctx_init(weights_ctx *w, const orc_params *p)
     w->Hs = NULL;
     w->v3 = NULL;
     w->Time[0] = 0.0;
-    for (int k = 1; k <= p->steps + 1; k++) w->Time[k] = w->Time[k - 1] + p->tau;  /* :3944 */
-    for (int k = 0; k <= p->steps; k++) w->tau[k] = p->tau;
+    for (int k = 0; k <= p->steps; k++)
+        w->tau[k] = p->taus ? p->taus[k < p->steps ? k : p->steps - 1] : p->tau;
+    for (int k = 1; k <= p->steps + 1; k++) w->Time[k] = w->Time[k - 1] + w->tau[k - 1];  /* :3944 */
 }
 static void wctx_free(weights_ctx *w) { free(w->Time); free(w->tau); free(w->kb3); free(w->Hs); free(w->v3); }
 

@@ -96,6 +96,8 @@ typedef struct {
     int    nthreads;
     int    sum_alg;      /* 0 full sum, 1 truncated (sum_alg 1), 2 series (sum_alg 2) */
     double sum_param;    /* sum_alg 1: threshold on |kb|; sum_alg 2: series terms P  */
+    const double *taus;  /* NULL: uniform tau; else the M step lengths tau_0..tau_{M-1}
+                            (Time[k+1] = Time[k] + tau_k, init_time_step :3942-3949) */
 } orc_params;
 
 /* 1D solve: u (nodes) holds u0 on entry, u^M on exit.  u_top = u0[0]. */

@@ -25,6 +25,7 @@ class Params(ctypes.Structure):
         ("weight_mode", ctypes.c_int), ("series_eps", ctypes.c_double), ("do_q", ctypes.c_int),
         ("do_rule", ctypes.c_int), ("do_a0", ctypes.c_double), ("do_a1", ctypes.c_double),
         ("nthreads", ctypes.c_int), ("sum_alg", ctypes.c_int), ("sum_param", ctypes.c_double),
+        ("taus", ctypes.c_void_p),
     ]
 
 
@@ -173,15 +174,20 @@ def thomas(a, b, c, d):
 def make_params(nodes, steps, tau, alpha=0.8, length=1.0, ksat=None, coef_mode=0,
                 picard_max=1, picard_tol=1e-12, weight_mode=1, series_eps=1e-5,
                 do_q=0, do_rule=0, do_a0=0.1, do_a1=1.0, nthreads=0, fourier=1.0,
-                u_top=None, sum_alg=0, sum_param=1e-5):
+                u_top=None, sum_alg=0, sum_param=1e-5, taus=None):
     if ksat is None:
         if coef_mode == 0:
             h = length / (nodes - 1)
             ksat = lib().orc_ksat_for_fourier(u_top if u_top is not None else -1.0, h, tau, fourier)
         else:
             ksat = 1.0
-    return Params(nodes, steps, tau, alpha, length, ksat, coef_mode, picard_max, picard_tol,
-                  weight_mode, series_eps, do_q, do_rule, do_a0, do_a1, nthreads, sum_alg, sum_param)
+    p = Params(nodes, steps, tau, alpha, length, ksat, coef_mode, picard_max, picard_tol,
+               weight_mode, series_eps, do_q, do_rule, do_a0, do_a1, nthreads, sum_alg, sum_param, None)
+    if taus is not None:
+        p._taus = np.ascontiguousarray(taus, dtype=np.float64)    # kept alive with the struct
+        assert p._taus.size == steps
+        p.taus = p._taus.ctypes.data
+    return p
 
 
 def init_1d(nodes, seed, length=1.0):

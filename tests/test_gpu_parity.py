@@ -605,3 +605,54 @@ def test_results_files_in_reference_format(fpde, oracle, tmp_path):
     assert np.allclose(rows[:, 4], line, rtol=6e-6)
     assert np.allclose(rows[:, 2], (n // 2) / (n - 1)) and np.allclose(rows[:, 3], (n // 2) / (n - 1))
     assert np.array_equal(np.fromfile(raw, dtype=np.float64), u)
+
+
+# ------------------------------------------------- prescribed non-uniform time grid
+def _oracle_grid(oracle, spec, u0, taus, picard_tol=0.0):
+    p = oracle.make_params(spec.nodes, spec.steps, spec.tau, spec.alpha, ksat=spec.ksat,
+                           coef_mode=spec.coef_mode, picard_max=spec.picard_max, picard_tol=picard_tol,
+                           weight_mode=spec.weight_mode, series_eps=spec.series_eps, do_q=spec.do_q,
+                           do_rule=spec.do_rule, do_a0=spec.do_a0, do_a1=spec.do_a1, taus=taus)
+    return oracle.solve_1d(p, u0)
+
+
+@pytest.mark.parametrize("mode", ["plain", "mfma_f64"])
+@pytest.mark.parametrize("step_mode", [0, 1])
+@pytest.mark.parametrize("picard", [1, 3])
+def test_nonuniform_grid_matches_oracle(fpde, oracle, mode, step_mode, picard):
+    """Non-Toeplitz weights: every step has its own weight row over Time[]
+    (plain: a per-step row; FP64 MFMA: dense far block + near rows)."""
+    nodes, steps = 300, 150
+    rng = np.random.default_rng(60 + picard)
+    taus = 1e-3 * (0.4 + 1.2 * rng.random(steps))
+    u0 = fpde.init_1d(nodes, 61)
+    spec = fpde.make_spec(nodes, steps, 1e-3, alpha=0.7, hist_mode=mode, block=32, picard_max=picard,
+                          picard_tol=1e-12 if picard > 1 else 0.0, u_top=u0[0], step_mode=step_mode, taus=taus)
+    u = fpde.solve(spec, u0)
+    err = rel_err(u, _oracle_grid(oracle, spec, u0, taus, picard_tol=spec.picard_tol))
+    print(f"grid {mode} step_mode {step_mode} picard {picard}: rel err {err:.3e}")
+    assert err <= TOL_F64
+
+
+def test_graded_grid_and_distributed_order(fpde, oracle):
+    nodes, steps = 400, 200
+    k = np.arange(steps + 1) / steps
+    taus = 0.2 * np.diff(k ** 2)                      # graded towards t = 0
+    u0 = fpde.init_1d(nodes, 62)
+    for mode, kw in [("mfma_f64", {}), ("plain", dict(do_q=6, do_rule=1, do_a0=0.1, do_a1=1.0)),
+                     ("mfma_f64", dict(do_q=8, do_rule=0, do_a0=0.1, do_a1=1.0))]:
+        spec = fpde.make_spec(nodes, steps, 1e-3, alpha=0.6, hist_mode=mode, block=64, u_top=u0[0], taus=taus, **kw)
+        u = fpde.solve(spec, u0)
+        err = rel_err(u, _oracle_grid(oracle, spec, u0, taus))
+        print(f"graded {mode} {kw}: rel err {err:.3e}")
+        assert err <= TOL_F64
+
+
+def test_uniform_tau_array_equals_uniform_solve(fpde):
+    nodes, steps = 500, 130
+    u0 = fpde.init_1d(nodes, 63)
+    for mode in ("plain", "mfma_f64"):
+        a = fpde.solve(fpde.make_spec(nodes, steps, 1e-3, alpha=0.8, hist_mode=mode, u_top=u0[0]), u0)
+        b = fpde.solve(fpde.make_spec(nodes, steps, 1e-3, alpha=0.8, hist_mode=mode, u_top=u0[0],
+                                      taus=np.full(steps, 1e-3)), u0)
+        assert rel_err(a, b) <= 1e-12

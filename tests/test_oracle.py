@@ -179,6 +179,29 @@ def test_series_one_moment_telescopes(oracle):
         assert np.allclose(S, T[n + 1] ** -0.6 * (lv[n] - lv[0]), rtol=1e-11, atol=1e-12)
 
 
+@pytest.mark.parametrize("alpha", [0.3, 0.75])
+def test_l1_exact_for_linear_in_time_on_a_nonuniform_grid(oracle, alpha):
+    """L1 integrates the kernel exactly against piecewise-linear u on any grid:
+    for u = c t the history plus the local term is c T^(1-a)/Gamma(2-a)."""
+    rng = np.random.default_rng(4)
+    n, c = 30, 2.5
+    taus = 1e-2 * (0.3 + rng.random(n + 1))
+    T = np.concatenate([[0.0], np.cumsum(taus)])
+    kkk0, kb3 = oracle.l1_weights(alpha, n, T, weight_mode=0)
+    old = (c * T[: n + 1])[:, None] * np.ones((1, 3))
+    hist = oracle.time_summing(old, kb3, taus)[0]
+    approx = (hist + kkk0 * c * taus[n] / taus[n]) / G(1 - alpha)
+    exact = c * T[n + 1] ** (1 - alpha) / G(2 - alpha)
+    assert abs(approx - exact) <= 1e-11 * exact
+
+
+def test_uniform_tau_array_is_the_uniform_solve(oracle):
+    u0 = oracle.init_1d(60, 3)
+    a = oracle.solve_1d(oracle.make_params(60, 40, 1e-3, 0.7, u_top=u0[0]), u0)
+    b = oracle.solve_1d(oracle.make_params(60, 40, 1e-3, 0.7, u_top=u0[0], taus=np.full(40, 1e-3)), u0)
+    assert np.array_equal(a, b)
+
+
 @pytest.mark.parametrize("alpha", [0.35, 0.8])
 def test_series_converges_to_full_history_sum(oracle, alpha):
     """With many terms and a tiny eps the moment series is the binomial
