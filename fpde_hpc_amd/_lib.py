@@ -27,7 +27,7 @@ class Spec(ctypes.Structure):
         ("do_q", ctypes.c_int), ("do_rule", ctypes.c_int), ("do_a0", ctypes.c_double),
         ("do_a1", ctypes.c_double), ("hist_mode", ctypes.c_int), ("block", ctypes.c_int),
         ("step_mode", ctypes.c_int), ("sum_alg", ctypes.c_int), ("sum_param", ctypes.c_double),
-        ("taus", ctypes.c_void_p), ("reserved", ctypes.c_int * 2),
+        ("taus", ctypes.c_void_p), ("var_tau", ctypes.c_int), ("reserved", ctypes.c_int),
     ]
 
 
@@ -88,6 +88,9 @@ SIGNATURES = [
     ("fpde_solver_step", ctypes.c_int, [_vp]),
     ("fpde_solver_get_solution", ctypes.c_int, [_vp, _dp]),
     ("fpde_solver_write_results", ctypes.c_int, [_vp, ctypes.c_char_p, ctypes.c_char_p]),
+    ("fpde_solver_advance_tau", ctypes.c_int, [_vp, ctypes.c_double]),
+    ("fpde_solver_copy", ctypes.c_int, [_vp, _vp]),
+    ("fpde_varT_step", ctypes.c_int, [_vp, _vp, _vp, _dp, ctypes.c_double, ctypes.c_double, _dp]),
     ("fpde_solver_get_weights", ctypes.c_int, [_vp, _dp, _dp, _dp]),
     ("fpde_solver_set_profiling", ctypes.c_int, [_vp, ctypes.c_int]),
     ("fpde_solver_get_stats", ctypes.c_int, [_vp, ctypes.POINTER(Stats)]),

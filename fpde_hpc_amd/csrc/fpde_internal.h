@@ -102,6 +102,10 @@ struct DeviceWeights {
 // outside t in [t_lo, n).
 fpde_status launch_weights_grid(const fpde_spec& s, const DeviceWeights& w, int n_first, int rows, int len,
                                 int t_lo, int reversed, double* W, int64_t ldw, hipStream_t st);
+// local coefficient of a step of length tn: sum_q cw_q Da_q kkk0(tn) / tn,
+// kkk0 = b(n,n+1;n+1) = tn^(1-a)/(1-a)  (init_time_step, fpde_hpc_1d.cpp:3942-3949)
+double grid_local(const fpde_spec& s, double tn);
+fpde_status launch_set_grid_point(const DeviceWeights& w, int n, double t_next, double tau, double L, hipStream_t st);
 
 constexpr int kMaxSeriesTerms = 64;
 

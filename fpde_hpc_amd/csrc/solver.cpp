@@ -57,6 +57,12 @@ fpde_status validate(const fpde_spec& s) {
         set_error("sum_param must be finite and >= 0 for the restricted sum");
         return FPDE_ERR_ARG;
     }
+    if (s.var_tau) {
+        if (s.var_tau != 1 || s.dims != 1 || s.hist_mode != HIST_PLAIN_F64 || s.sum_alg != 0 || s.taus) {
+            set_error("var_tau solvers are 1D, hist_mode 0, sum_alg 0, without a prescribed grid");
+            return FPDE_ERR_ARG;
+        }
+    }
     if (s.taus) {
         if (s.dims != 1) { set_error("a prescribed time grid (taus) is for 1D solves"); return FPDE_ERR_ARG; }
         if (s.hist_mode != HIST_PLAIN_F64 && s.hist_mode != HIST_MFMA_F64) {
@@ -108,7 +114,7 @@ size_t footprint_bytes(const fpde_spec& s) {
         b += sizeof(double) * P * p.ld;                              // moments
         b += sizeof(double) * 2 * P * (size_t)s.steps;               // coefficient tables
     }
-    if (s.taus) {                                                    // grid, weight rows
+    if (s.taus || s.var_tau) {                                       // grid, weight rows
         size_t M = (size_t)s.steps, B = (size_t)s.block;
         b += sizeof(double) * (3 * M + 64 + B * (round_up(M + 64, 64) + B + 64));
     }
@@ -140,6 +146,7 @@ struct fpde_solver {
     // prescribed grid: host copy of the step lengths, weight rows of the
     // current step (plain) or block (far: direct, near: reversed per step)
     std::vector<double> taus_h;
+    std::vector<double> Time_h;          // var_tau: the grid so far
     double* Wrow = nullptr;
     double* Wfar = nullptr;
     double* Wnear = nullptr;
@@ -232,6 +239,19 @@ fpde_status solver_alloc(fpde_solver* s) {
     FPDE_HIP_TRY(hipStreamCreateWithFlags(&s->stream, hipStreamNonBlocking));
     int M = sp.steps;
     s->w.M = M;
+    if (sp.var_tau) {
+        // the grid is filled step by step (fpde_solver_advance_tau)
+        s->w.grid = true;
+        s->Time_h.assign((size_t)M + 2, 0.0);
+        s->taus_h.assign((size_t)M + 1, 0.0);
+        DALLOC(s->w.Time, sizeof(double) * (M + 2));
+        DALLOC(s->w.taus, sizeof(double) * (M + 1));
+        DALLOC(s->w.Lv, sizeof(double) * M);
+        FPDE_HIP_TRY(hipMemsetAsync(s->w.Time, 0, sizeof(double) * (M + 2), s->stream));
+        FPDE_HIP_TRY(hipMemsetAsync(s->w.taus, 0, sizeof(double) * (M + 1), s->stream));
+        FPDE_HIP_TRY(hipMemsetAsync(s->w.Lv, 0, sizeof(double) * M, s->stream));
+        DALLOC(s->Wrow, sizeof(double) * (size_t)(M + 64));
+    }
     if (sp.taus) {
         s->w.grid = true;
         std::vector<double> T((size_t)M + 2, 0.0), tt((size_t)M + 1);
@@ -718,6 +738,120 @@ fpde_status fpde_solver_advance(fpde_solver* s, int nsteps) {
         if (rc != FPDE_OK) return rc;
     }
     FPDE_HIP_TRY(hipGetLastError());
+    return FPDE_OK;
+}
+
+fpde_status fpde_solver_advance_tau(fpde_solver* s, double tau) {
+    if (!s) { set_error("null solver"); return FPDE_ERR_ARG; }
+    if (!s->spec.var_tau) { set_error("advance_tau needs a var_tau solver"); return FPDE_ERR_STATE; }
+    if (!s->initialized) { set_error("set_initial must be called first"); return FPDE_ERR_STATE; }
+    if (!(tau > 0.0 && std::isfinite(tau))) { set_error("tau must be positive and finite"); return FPDE_ERR_ARG; }
+    int n = s->step;
+    if (n >= s->spec.steps) { set_error("var_tau solver: step capacity reached"); return FPDE_ERR_STATE; }
+    s->taus_h[n] = tau;
+    s->Time_h[n + 1] = s->Time_h[n] + tau;                         // init_time_step (:3944)
+    s->w.hLv[n] = grid_local(s->spec, tau);
+    fpde_status rc = launch_set_grid_point(s->w, n, s->Time_h[n + 1], tau, s->w.hLv[n], s->stream);
+    if (rc != FPDE_OK) return rc;
+    rc = enqueue_1d_step(s, n);
+    if (rc != FPDE_OK) return rc;
+    s->step = n + 1;
+    rc = finalize_1d(s);
+    if (rc != FPDE_OK) return rc;
+    FPDE_HIP_TRY(hipGetLastError());
+    return FPDE_OK;
+}
+
+fpde_status fpde_solver_copy(fpde_solver* dst, const fpde_solver* src) {
+    if (!dst || !src) { set_error("null solver"); return FPDE_ERR_ARG; }
+    if (dst == src) return FPDE_OK;
+    const fpde_spec &a = dst->spec, &b = src->spec;
+    if (!a.var_tau || !b.var_tau || a.nodes != b.nodes || a.dims != 1) {
+        set_error("copy needs two var_tau 1D solvers of the same nodes");
+        return FPDE_ERR_ARG;
+    }
+    if (src->step > a.steps) { set_error("copy: destination capacity is below the source's steps"); return FPDE_ERR_ARG; }
+    if (src->pending) { set_error("copy: the source has a pending solve"); return FPDE_ERR_STATE; }
+    FPDE_HIP_TRY(hipStreamSynchronize(src->stream));
+    FPDE_HIP_TRY(hipStreamSynchronize(dst->stream));
+    const Plan& p = dst->plan;
+    int n = src->step;
+    FPDE_HIP_TRY(hipMemcpyAsync(dst->U, src->U, sizeof(double) * p.ld, hipMemcpyDeviceToDevice, dst->stream));
+    if (n > 0)
+        FPDE_HIP_TRY(hipMemcpyAsync(dst->D, src->D, sizeof(double) * (size_t)n * p.ld, hipMemcpyDeviceToDevice,
+                                    dst->stream));
+    FPDE_HIP_TRY(hipMemcpyAsync(dst->uend, src->uend, sizeof(double) * 2 * 2 * dst->C, hipMemcpyDeviceToDevice,
+                                dst->stream));
+    size_t np = (size_t)std::min(a.steps, b.steps);
+    FPDE_HIP_TRY(hipMemcpyAsync(dst->w.Time, src->w.Time, sizeof(double) * (np + 2), hipMemcpyDeviceToDevice, dst->stream));
+    FPDE_HIP_TRY(hipMemcpyAsync(dst->w.taus, src->w.taus, sizeof(double) * (np + 1), hipMemcpyDeviceToDevice, dst->stream));
+    FPDE_HIP_TRY(hipMemcpyAsync(dst->w.Lv, src->w.Lv, sizeof(double) * np, hipMemcpyDeviceToDevice, dst->stream));
+    std::copy(src->Time_h.begin(), src->Time_h.begin() + np + 2, dst->Time_h.begin());
+    std::copy(src->taus_h.begin(), src->taus_h.begin() + np + 1, dst->taus_h.begin());
+    std::copy(src->w.hLv.begin(), src->w.hLv.begin() + np, dst->w.hLv.begin());
+    FPDE_HIP_TRY(hipStreamSynchronize(dst->stream));
+    dst->step = n;
+    dst->pending = false;
+    dst->last_k = -1;
+    dst->utop1d = src->utop1d;
+    dst->initialized = src->initialized;
+    return FPDE_OK;
+}
+
+fpde_status fpde_varT_step(fpde_solver* s0, fpde_solver* s2, fpde_solver* s3, double* tau, double eps,
+                           double tau_max, double* err_out) {
+    if (!s0 || !s2 || !s3 || !tau) { set_error("null argument"); return FPDE_ERR_ARG; }
+    const int64_t nn = s0->plan.nodes;
+    std::vector<double> u0((size_t)nn), u2((size_t)nn);
+    // solve_T_half_step_and_check (:8778-8793) and solution_difference (:8723-8749)
+    auto check = [&](double t, double* err) -> fpde_status {
+        fpde_status rc = fpde_solver_copy(s3, s0);
+        if (rc == FPDE_OK) rc = fpde_solver_advance_tau(s0, t);
+        if (rc == FPDE_OK) rc = fpde_solver_advance_tau(s2, t / 2.0);
+        if (rc == FPDE_OK) rc = fpde_solver_advance_tau(s2, t / 2.0);
+        if (rc == FPDE_OK) rc = fpde_solver_get_solution(s0, u0.data());
+        if (rc == FPDE_OK) rc = fpde_solver_get_solution(s2, u2.data());
+        if (rc != FPDE_OK) return rc;
+        double e = 0.0;
+        for (int64_t i = 0; i < nn; i++) e += (u0[i] - u2[i]) * (u0[i] - u2[i]);
+        *err = e;
+        return FPDE_OK;
+    };
+    auto restore = [&]() -> fpde_status {
+        fpde_status rc = fpde_solver_copy(s0, s3);
+        if (rc == FPDE_OK) rc = fpde_solver_copy(s2, s3);
+        return rc;
+    };
+    double t = *tau, err = 0.0;
+    fpde_status rc = check(t, &err);
+    if (rc != FPDE_OK) return rc;
+    if (err > eps) {
+        // decrease the step until the check passes (:8985-9004)
+        const double t_first = t;
+        while (err > eps) {
+            if ((rc = restore()) != FPDE_OK) return rc;
+            t /= 2.0;
+            if ((rc = check(t, &err)) != FPDE_OK) return rc;
+            if (t < 1e-10) {
+                // the reference then takes one more step of the first length
+                // (:8996-9000, with init_tau an int there); here of t_first
+                t = t_first;
+                if ((rc = fpde_solver_advance_tau(s0, t)) != FPDE_OK) return rc;
+                break;
+            }
+        }
+    } else if (t < tau_max) {
+        // increase the step while the check passes (:9006-9021); the step of
+        // the last check is kept, also when it failed or passed tau_max
+        while (err <= eps) {
+            if ((rc = restore()) != FPDE_OK) return rc;
+            t *= 2.0;
+            if ((rc = check(t, &err)) != FPDE_OK) return rc;
+            if (t > tau_max) break;
+        }
+    }
+    *tau = t;
+    if (err_out) *err_out = err;
     return FPDE_OK;
 }
 

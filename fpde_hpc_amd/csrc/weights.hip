@@ -191,6 +191,19 @@ __global__ void k_weights_grid(OrderSet os, const double* __restrict__ Time, con
     W[(int64_t)r * ldw + j] = w;
 }
 
+// one point of a grid built step by step (var_tau)
+__global__ void k_set_grid_point(double* Time, double* taus, double* Lv, int n, double t_next, double tau, double L) {
+    Time[n + 1] = t_next;
+    taus[n] = tau;
+    Lv[n] = L;
+}
+
+fpde_status launch_set_grid_point(const DeviceWeights& w, int n, double t_next, double tau, double L, hipStream_t st) {
+    hipLaunchKernelGGL(k_set_grid_point, dim3(1), dim3(1), 0, st, w.Time, w.taus, w.Lv, n, t_next, tau, L);
+    FPDE_HIP_TRY(hipGetLastError());
+    return FPDE_OK;
+}
+
 static void gauss_legendre(int n, double a, double b, double* x, double* w) {
     for (int i = 0; i < n; i++) {
         double z = std::cos(M_PI * (i + 0.75) / (n + 0.5)), pp = 0.0, p1 = 1.0, p2 = 0.0;
@@ -259,6 +272,18 @@ static fpde_status build_orders_grid(const fpde_spec& s, OrderSet& os) {
     return FPDE_OK;
 }
 
+double grid_local(const fpde_spec& s, double tn) {
+    OrderSet og;
+    if (build_orders_grid(s, og) != FPDE_OK) return 0.0;
+    double L = 0.0;
+    for (int q = 0; q < og.q; q++) {
+        double a = og.a[q];
+        double kkk0 = (a == 1.0) ? 1.0 : std::pow(tn, 1.0 - a) / (1.0 - a);
+        L += og.c[q] * kkk0 / tn;
+    }
+    return L;
+}
+
 fpde_status launch_weights_grid(const fpde_spec& s, const DeviceWeights& w, int n_first, int rows, int len,
                                 int t_lo, int reversed, double* W, int64_t ldw, hipStream_t st) {
     if (rows <= 0 || len <= 0) return FPDE_OK;
@@ -284,19 +309,12 @@ fpde_status launch_weights(const fpde_spec& s, DeviceWeights& w, hipStream_t st)
         // the weights themselves are generated per step or per block
         int M = w.M;
         w.hLv.assign(M, 0.0);
-        OrderSet og;
-        build_orders_grid(s, og);
-        for (int n = 0; n < M; n++) {
-            double tn = s.taus[n], L = 0.0;
-            for (int q = 0; q < og.q; q++) {
-                double a = og.a[q];
-                double kkk0 = (a == 1.0) ? 1.0 : std::pow(tn, 1.0 - a) / (1.0 - a);
-                L += og.c[q] * kkk0 / tn;
-            }
-            w.hLv[n] = L;
+        if (s.taus) {
+            for (int n = 0; n < M; n++) w.hLv[n] = grid_local(s, s.taus[n]);
+            FPDE_HIP_TRY(hipMemcpyAsync(w.Lv, w.hLv.data(), sizeof(double) * M, hipMemcpyHostToDevice, st));
+            FPDE_HIP_TRY(hipStreamSynchronize(st));
         }
         w.local = w.hLv[0];
-        FPDE_HIP_TRY(hipMemcpyAsync(w.Lv, w.hLv.data(), sizeof(double) * M, hipMemcpyHostToDevice, st));
         FPDE_HIP_TRY(hipMemsetAsync(w.corr, 0, sizeof(double) * 2 * (size_t)M, st));
         return FPDE_OK;
     }

@@ -24,7 +24,7 @@ def _p(a):
 def make_spec(nodes, steps, tau, alpha=0.8, dims=1, length=1.0, ksat=None, coef_mode=0,
               picard_max=1, picard_tol=0.0, weight_mode=1, series_eps=1e-5, do_q=0, do_rule=0,
               do_a0=0.1, do_a1=1.0, hist_mode="plain", block=64, step_mode=0, fourier=1.0,
-              u_top=-1.0, sum_alg=0, sum_param=1e-5, taus=None):
+              u_top=-1.0, sum_alg=0, sum_param=1e-5, taus=None, var_tau=0):
     """fpde_spec; ksat defaults to the value giving mesh Fourier number `fourier`
     (K*tau/(C*h^2)) at suction -u_top.  `taus` (one length per step) makes the
     time grid non-uniform."""
@@ -34,7 +34,7 @@ def make_spec(nodes, steps, tau, alpha=0.8, dims=1, length=1.0, ksat=None, coef_
     hm = HIST_MODES[hist_mode] if isinstance(hist_mode, str) else int(hist_mode)
     spec = Spec(dims, nodes, steps, tau, alpha, length, ksat, coef_mode, picard_max, picard_tol,
                 weight_mode, series_eps, do_q, do_rule, do_a0, do_a1, hm, block, step_mode,
-                sum_alg, sum_param, None, (ctypes.c_int * 2)(0, 0))
+                sum_alg, sum_param, None, var_tau, 0)
     if taus is not None:
         spec._taus = np.ascontiguousarray(taus, dtype=np.float64)   # alive until the solver copies it
         if spec._taus.size != steps:
@@ -107,6 +107,14 @@ class Solver:
         u = np.zeros(self.unknowns)
         check(lib().fpde_solver_get_solution(self._h, _p(u)))
         return u
+
+    def advance_tau(self, tau):
+        """One step of length tau (var_tau solvers)."""
+        check(lib().fpde_solver_advance_tau(self._h, tau))
+
+    def copy_from(self, other):
+        """Take other's state (var_tau solvers)."""
+        check(lib().fpde_solver_copy(self._h, other._h))
 
     def write_results(self, path_txt="results.txt", path_3d=None):
         """Append the current solution in the reference driver's results.txt
@@ -186,6 +194,14 @@ class Server:
         u = np.zeros(self._sizes[rid])
         check(lib().fpde_server_result(self._h, rid, None, _p(u), u.size))
         return u
+
+
+def varT_step(s0, s2, s3, tau, eps, tau_max):
+    """One outer step of the reference's step doubling; returns (tau, err)."""
+    t = ctypes.c_double(tau)
+    e = ctypes.c_double(0.0)
+    check(lib().fpde_varT_step(s0._h, s2._h, s3._h, ctypes.byref(t), eps, tau_max, ctypes.byref(e)))
+    return t.value, e.value
 
 
 def footprint(spec):

@@ -159,7 +159,10 @@ typedef struct {
                             fpde_hpc_1d.cpp:3942-3949), copied at create; the
                             weights are then per step (not Toeplitz): 1D,
                             hist_mode 0 or 1, sum_alg 0                         */
-    int reserved[2];
+    int var_tau;         /* 1: the step length is given step by step at run time
+                            (fpde_solver_advance_tau, fpde_varT_step); 1D,
+                            hist_mode 0, sum_alg 0, taus NULL; steps = capacity */
+    int reserved;
 } fpde_spec;
 
 typedef struct fpde_solver fpde_solver;
@@ -186,6 +189,23 @@ fpde_status fpde_solver_set_initial(fpde_solver *s, const double *u0);
 fpde_status fpde_solver_reset(fpde_solver *s, const double *u0);
 /* Enqueue nsteps steps (or all remaining if nsteps<=0); asynchronous. */
 fpde_status fpde_solver_advance(fpde_solver *s, int nsteps);
+/* var_tau solvers: one step of length tau (Time[n+1] = Time[n] + tau,
+ * init_time_step fpde_hpc_1d.cpp:3942-3949), completed before returning to the
+ * stream (U^{n+1} and D[n] in memory). */
+fpde_status fpde_solver_advance_tau(fpde_solver *s, double tau);
+/* H_solver::copy (fpde_hpc_1d.cpp:5805-5986): dst takes src's state (solution,
+ * history levels, time grid, step count).  Both var_tau solvers of the same
+ * nodes; dst's capacity must hold src's steps. */
+fpde_status fpde_solver_copy(fpde_solver *dst, const fpde_solver *src);
+/* One outer step of the reference's variable time step (fpde_hpc_1d.cpp:
+ * 8978-9022 with solve_T_half_step_and_check :8778-8793): s3 keeps s0's state,
+ * s0 takes one step of *tau, s2 two steps of *tau/2, err = sum_i (U0_i - U2_i)^2
+ * (solution_difference :8723-8749); while err > eps both restart from s3 with
+ * *tau halved (below 1e-10 the first tau is taken as it is); else, while
+ * err <= eps and *tau < tau_max, both restart with *tau doubled.  On return s0
+ * holds the last step taken, *tau its length and *err its error. */
+fpde_status fpde_varT_step(fpde_solver *s0, fpde_solver *s2, fpde_solver *s3, double *tau,
+                           double eps, double tau_max, double *err);
 fpde_status fpde_solver_sync(fpde_solver *s);
 /* 1 when every enqueued launch of the solver has completed, 0 otherwise
  * (non-blocking), -1 on error. */

@@ -656,3 +656,79 @@ def test_uniform_tau_array_equals_uniform_solve(fpde):
         b = fpde.solve(fpde.make_spec(nodes, steps, 1e-3, alpha=0.8, hist_mode=mode, u_top=u0[0],
                                       taus=np.full(steps, 1e-3)), u0)
         assert rel_err(a, b) <= 1e-12
+
+
+# ------------------------------------------------- variable time step (step doubling)
+def test_var_tau_steps_equal_prescribed_grid(fpde, oracle):
+    nodes, steps = 250, 40
+    rng = np.random.default_rng(71)
+    taus = 1e-3 * (0.5 + rng.random(steps))
+    u0 = fpde.init_1d(nodes, 72)
+    spec = fpde.make_spec(nodes, steps, 1e-3, alpha=0.7, u_top=u0[0], picard_max=3, picard_tol=1e-12,
+                          var_tau=1)
+    with fpde.Solver(spec) as s:
+        s.set_initial(u0)
+        for t in taus:
+            s.advance_tau(float(t))
+        u = s.solution()
+    grid = fpde.solve(fpde.make_spec(nodes, steps, 1e-3, alpha=0.7, u_top=u0[0], picard_max=3, picard_tol=1e-12,
+                                     taus=taus), u0)
+    assert rel_err(u, grid) <= 1e-13
+    assert rel_err(u, _oracle_grid(oracle, spec, u0, taus, picard_tol=1e-12)) <= TOL_F64
+
+
+def test_varT_step_doubling_matches_oracle_model(fpde, oracle):
+    """The reference's outer loop (fpde_hpc_1d.cpp:8978-9022): the tau and err
+    sequence of the device driver equals the same decisions taken on oracle
+    solves (every solver state is its time grid), and s0 ends on that grid."""
+    nodes, alpha, tau0, eps, tau_max, K = 200, 0.7, 2e-3, 3e-10, 1.6e-2, 10
+    u0 = fpde.init_1d(nodes, 70)
+    mk = lambda cap: fpde.make_spec(nodes, cap, tau0, alpha=alpha, u_top=u0[0], fourier=1.0, var_tau=1)
+    spec = mk(64)
+    ksat = spec.ksat
+
+    def U(grid):
+        if not grid:
+            return u0.copy()
+        p = oracle.make_params(nodes, len(grid), tau0, alpha, ksat=ksat, taus=np.array(grid))
+        return oracle.solve_1d(p, u0)
+
+    # oracle model of the driver
+    m0, m2, m3 = [], [], []
+    tau, model = tau0, []
+    for _ in range(K):
+        def check(t):
+            nonlocal m0, m2, m3
+            m3 = list(m0); m0 = m0 + [t]; m2 = m2 + [t / 2, t / 2]
+            d = U(m0) - U(m2)
+            return float(np.sum(d * d))
+        err = check(tau)
+        if err > eps:
+            first = tau
+            while err > eps:
+                m0 = list(m3); m2 = list(m3); tau /= 2; err = check(tau)
+                if tau < 1e-10:
+                    tau = first; m0 = m0 + [tau]; break
+        elif tau < tau_max:
+            while err <= eps:
+                m0 = list(m3); m2 = list(m3); tau *= 2; err = check(tau)
+                if tau > tau_max:
+                    break
+        model.append((tau, err))
+
+    s0, s2, s3 = fpde.Solver(spec), fpde.Solver(mk(128)), fpde.Solver(mk(64))
+    try:
+        for s in (s0, s2, s3):
+            s.set_initial(u0)
+        tau, dev = tau0, []
+        for _ in range(K):
+            tau, err = fpde.varT_step(s0, s2, s3, tau, eps, tau_max)
+            dev.append((tau, err))
+        u = s0.solution()
+    finally:
+        for s in (s0, s2, s3):
+            s.close()
+    assert [t for t, _ in dev] == [t for t, _ in model]
+    for (_, a), (_, b) in zip(dev, model):
+        assert abs(a - b) <= 1e-6 * b
+    assert rel_err(u, U(m0)) <= TOL_F64
