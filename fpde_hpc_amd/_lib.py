@@ -28,6 +28,7 @@ class Spec(ctypes.Structure):
         ("do_a1", ctypes.c_double), ("hist_mode", ctypes.c_int), ("block", ctypes.c_int),
         ("step_mode", ctypes.c_int), ("sum_alg", ctypes.c_int), ("sum_param", ctypes.c_double),
         ("taus", ctypes.c_void_p), ("var_tau", ctypes.c_int), ("reserved", ctypes.c_int),
+        ("gamma", ctypes.c_double),
     ]
 
 
@@ -75,6 +76,8 @@ SIGNATURES = [
     ("fpde_reference_sum", ctypes.c_int, [_vp, _vp, ctypes.c_int64, ctypes.c_int64, ctypes.c_int64,
                                           _vp, _vp, ctypes.c_int, ctypes.c_double]),
     ("fpde_history_sum_f64", ctypes.c_int, [_vp, _vp, ctypes.c_int64, ctypes.c_int64, ctypes.c_int64, _vp]),
+    ("fpde_space_sum", ctypes.c_int, [_vp, _vp, ctypes.c_int64, ctypes.c_double, ctypes.c_double, ctypes.c_int,
+                                      ctypes.c_double]),
     ("fpde_history_block_gemm", ctypes.c_int, [ctypes.c_int, _vp, ctypes.c_int64, ctypes.c_int64,
                                                ctypes.c_int64, ctypes.c_int, ctypes.c_int, ctypes.c_int, _vp, _vp]),
     ("fpde_solver_footprint", ctypes.c_size_t, [ctypes.POINTER(Spec)]),

@@ -15,7 +15,7 @@ BUILD = os.path.join(HERE, "_build")
 LIB = os.path.join(HERE, "libfpde_hip.so")
 ARCH = "gfx950"
 
-SOURCES = ["weights.hip", "history.hip", "step1d.hip", "lod3d.hip", "solver.cpp", "facade.cpp",
+SOURCES = ["weights.hip", "history.hip", "space.hip", "step1d.hip", "lod3d.hip", "solver.cpp", "facade.cpp",
            "server.cpp"]
 
 # -packed-fp32-ops off: the compiler must not form v_pk_{add,mul,fma}_f32

@@ -332,13 +332,59 @@ fpde_status fpde_history_sum_f64(double* S, const double* D, int64_t n, int64_t 
     int splits = 0;
     fpde_status rc = launch_hist_plain(D, ld, (int)nodes, (int)n, (int)n, omega, part, &splits, 0);
     if (rc == FPDE_OK) {
-        static double zero_corr[4] = {0, 0, 0, 0};
-        (void)zero_corr;
         // combine: S = sum of partial rows (no near levels, no corrections)
         rc = launch_hist_combine(S, part, splits, ld, 0, D, ld, nodes, 0, 0, omega, nullptr, 0);
     }
     hipError_t e = hipDeviceSynchronize();
     hipFree(part);
+    if (rc != FPDE_OK) return rc;
+    if (e != hipSuccess) { set_error(hipGetErrorString(e)); return FPDE_ERR_HIP; }
+    return FPDE_OK;
+}
+
+fpde_status fpde_space_sum(double* V, const double* F, int64_t nodes, double h, double gamma, int weight_mode,
+                           double series_eps) {
+    if (!V || !F || nodes < 4 || nodes > (1 << 24) || !(h > 0.0) || !(gamma > 0.0 && gamma < 1.0) ||
+        (weight_mode != 0 && weight_mode != 1) || !(series_eps > 0.0)) {
+        set_error("bad arguments (nodes >= 4, h > 0, 0 < gamma < 1, weight_mode 0 or 1, series_eps > 0)");
+        return FPDE_ERR_ARG;
+    }
+    fpde_spec sp{};
+    sp.dims = 1;
+    sp.nodes = (int)nodes;
+    sp.gamma = gamma;
+    sp.weight_mode = weight_mode;
+    sp.series_eps = series_eps;
+    const int64_t ldp = (nodes + kSpTile - 1) / kSpTile * kSpTile;
+    const size_t chunks = (size_t)space_chunks((int)(ldp / kSpTile) - 1);
+    DeviceWeights w;
+    double *Fp = nullptr, *P = nullptr;
+    fpde_status rc = FPDE_OK;
+    auto grab = [&](double** p, size_t n) {
+        if (rc == FPDE_OK && hipMalloc(p, sizeof(double) * n) != hipSuccess) {
+            set_error("hipMalloc failed");
+            rc = FPDE_ERR_NOMEM;
+        }
+    };
+    grab(&w.sp_omega, (size_t)nodes);
+    grab(&w.sp_corr, 2 * (size_t)nodes);
+    grab(&Fp, (size_t)ldp);
+    grab(&P, chunks * (size_t)ldp);
+    if (rc == FPDE_OK) {
+        // F on the interior rows, zero elsewhere (calc_v reads F[1..i-1])
+        hipError_t e = hipMemset(Fp, 0, sizeof(double) * ldp);
+        if (e == hipSuccess)
+            e = hipMemcpy(Fp + 1, F + 1, sizeof(double) * (size_t)(nodes - 2), hipMemcpyDeviceToDevice);
+        if (e != hipSuccess) { set_error(hipGetErrorString(e)); rc = FPDE_ERR_HIP; }
+    }
+    if (rc == FPDE_OK) rc = launch_space_weights(sp, w, h, 0);
+    if (rc == FPDE_OK) rc = launch_space_apply(Fp, (int)nodes, w.sp_omega, P, ldp, 0);
+    if (rc == FPDE_OK) rc = launch_space_reduce(V, P, Fp, w.sp_corr, (int)nodes, ldp, 0);
+    hipError_t e = hipDeviceSynchronize();
+    hipFree(w.sp_omega);
+    hipFree(w.sp_corr);
+    hipFree(Fp);
+    hipFree(P);
     if (rc != FPDE_OK) return rc;
     if (e != hipSuccess) { set_error(hipGetErrorString(e)); return FPDE_ERR_HIP; }
     return FPDE_OK;

@@ -94,6 +94,15 @@ struct DeviceWeights {
     double* taus = nullptr;    // [M+1]
     double* Lv = nullptr;      // [M]
     std::vector<double> hLv;
+    // space-fractional scheme (spec.gamma in (0,1)), on the node grid Z_k = k h:
+    // sp_omega[d-1] = Dg kb(gamma, j-1, j, i) for d = i-j >= 1 (closed form,
+    // Toeplitz), sp_corr[2(i-1)+t] = series - Toeplitz weight of the interval
+    // j = t+1 in row i (the reference's series for t0 <= 1), sp_kap = the row
+    // factor Dg kb(gamma, i-1, i, i) = Dg h^(1-gamma)/(1-gamma)
+    bool sp_on = false;
+    double sp_kap = 1.0;
+    double* sp_omega = nullptr;  // [N]
+    double* sp_corr = nullptr;   // [2N]
 };
 
 // Dense weight rows of a prescribed grid (weights.hip): rows r < rows are
@@ -187,10 +196,47 @@ struct Step1DArgs {
     double* H;
     const double* ser_c;
     const double* ser_v;
+    // space-fractional scheme (sp_on): rows scaled by sp_kap and the explicit
+    // sum E_i = sum_y sp_P[y][i] + sp_corr-weighted F_1, F_2 subtracted as
+    // inv_dw_dh E_i; E of step n is kept in sp_E by the launch that forms it
+    int sp_on;
+    double sp_kap;
+    const double* sp_P;       // [chunks][sp_ldp] partial sums of the Toeplitz part
+    int64_t sp_ldp;
+    const double* sp_F;       // F of U^n
+    const double* sp_corr;
+    double* sp_E;
 };
 
-// Kernel launchers (weights.hip, history.hip, step1d.hip, lod3d.hip)
+// ---------------------------------------------------------------------------
+// Space-fractional sum (space.hip): rows are cut into tiles of kSpTile rows
+// (16 blocks of 16) and the 16-wide block diagonals d of the lower-triangular
+// Toeplitz matrix into chunks of kSpDiag; a tile t has block diagonals
+// d < 16 (t + 1) and space_chunks(t) partial rows.
+constexpr int kSpTile = 256;
+constexpr int kSpDiag = 64;
+__host__ __device__ inline int space_chunks(int t) { return (16 * (t + 1) + kSpDiag - 1) / kSpDiag; }
+
+// E_i of an interior row i (1 <= i <= N-2) from the partials and the two
+// surface-interval corrections
+__device__ inline double space_sum(const double* __restrict__ P, int64_t ldp, const double* __restrict__ corr,
+                                   const double* __restrict__ F, int i) {
+    const int ny = space_chunks(i / kSpTile);
+    double e = 0.0;
+    for (int y = 0; y < ny; y++) e += P[(int64_t)y * ldp + i];
+    e = fma(corr[2 * (i - 1)], F[1], e);
+    return fma(corr[2 * (i - 1) + 1], F[2], e);
+}
+
+// Kernel launchers (weights.hip, history.hip, space.hip, step1d.hip, lod3d.hip)
 fpde_status launch_weights(const fpde_spec& s, DeviceWeights& w, hipStream_t st);
+fpde_status launch_space_weights(const fpde_spec& s, DeviceWeights& w, double h, hipStream_t st);
+fpde_status launch_space_flux(const double* U, int N, double h, double ksat, int mode, double* F, int64_t ldp,
+                              hipStream_t st);
+fpde_status launch_space_apply(const double* F, int N, const double* sp_omega, double* P, int64_t ldp,
+                               hipStream_t st);
+fpde_status launch_space_reduce(double* V, const double* P, const double* F, const double* sp_corr, int N,
+                                int64_t ldp, hipStream_t st);
 fpde_status launch_hist_plain(const double* D, int64_t ld, int nodes, int n, int t_hi, const double* omega,
                               double* part, int* splits_out, hipStream_t st);
 fpde_status launch_ref_sum(double* S, const double* old, int64_t old_size, int64_t os, int64_t nodes,

@@ -48,6 +48,11 @@ fpde_status validate(const fpde_spec& s) {
         if (b != 16 && b != 32 && b != 64 && b != 128) { set_error("block must be 16, 32, 64 or 128"); return FPDE_ERR_ARG; }
     }
     if (s.step_mode != 0 && s.step_mode != 1) { set_error("step_mode must be 0 or 1"); return FPDE_ERR_ARG; }
+    if (!(s.gamma >= 0.0 && s.gamma <= 1.0)) { set_error("gamma must be in [0,1] (0 or 1: classical)"); return FPDE_ERR_ARG; }
+    if (s.gamma > 0.0 && s.gamma < 1.0 && s.dims != 1) {
+        set_error("the space-fractional scheme (gamma in (0,1)) is built for 1D solves");
+        return FPDE_ERR_ARG;
+    }
     if (s.dims == 3 && s.picard_max != 1) { set_error("3D LOD uses lagged coefficients (picard_max must be 1)"); return FPDE_ERR_ARG; }
     if (s.sum_alg < 0 || s.sum_alg > 2) {
         set_error("sum_alg must be 0 (full sum), 1 (restricted sum) or 2 (series moments)");
@@ -114,6 +119,10 @@ size_t footprint_bytes(const fpde_spec& s) {
         b += sizeof(double) * P * p.ld;                              // moments
         b += sizeof(double) * 2 * P * (size_t)s.steps;               // coefficient tables
     }
+    if (s.dims == 1 && s.gamma > 0.0 && s.gamma < 1.0) {             // space-fractional sum
+        size_t ldp = (size_t)round_up(s.nodes, kSpTile);
+        b += sizeof(double) * (3 * (size_t)s.nodes + ldp * (1 + space_chunks((int)(ldp / kSpTile) - 1)) + p.ld);
+    }
     if (s.taus || s.var_tau) {                                       // grid, weight rows
         size_t M = (size_t)s.steps, B = (size_t)s.block;
         b += sizeof(double) * (3 * M + 64 + B * (round_up(M + 64, 64) + B + 64));
@@ -152,6 +161,11 @@ struct fpde_solver {
     double* Wnear = nullptr;
     int64_t ldwf = 0, ldwn = 0;
     const double* omega_cur = nullptr;   // weights of the step being enqueued
+    // space-fractional scheme: F of U^n [sp_ldp], partial sums [chunks][sp_ldp], E [ld]
+    double* spF = nullptr;
+    double* spP = nullptr;
+    double* spE = nullptr;
+    int64_t sp_ldp = 0;
     unsigned* bar = nullptr;   // grid barrier of the cooperative 1D block launches
     int* err = nullptr;        // barrier time-out flag
     bool block_mode = false;   // 1D steps run as cooperative block launches
@@ -307,7 +321,18 @@ fpde_status solver_alloc(fpde_solver* s) {
         // the GEMM history forms (one launch per block) and the series form;
         // the plain form needs its history kernel between every two steps
         bool blocked_history = sp.hist_mode != HIST_PLAIN_F64 || sp.sum_alg == 2;
-        s->block_mode = (sp.step_mode == 0) && blocked_history && step1d_block_supported(s->C);
+        const bool space = sp.gamma > 0.0 && sp.gamma < 1.0;
+        if (space) {
+            s->sp_ldp = round_up(sp.nodes, kSpTile);
+            DALLOC(s->w.sp_omega, sizeof(double) * (size_t)sp.nodes);
+            DALLOC(s->w.sp_corr, sizeof(double) * 2 * (size_t)sp.nodes);
+            DALLOC(s->spF, sizeof(double) * (size_t)s->sp_ldp);
+            DALLOC(s->spP, sizeof(double) * (size_t)s->sp_ldp * space_chunks((int)(s->sp_ldp / kSpTile) - 1));
+            DALLOC(s->spE, sizeof(double) * p.ld);
+        }
+        // the space-fractional sum needs the whole U^n between two steps: the
+        // per-iteration launches leave that point free for its kernels
+        s->block_mode = (sp.step_mode == 0) && blocked_history && !space && step1d_block_supported(s->C);
         DALLOC(s->dmax, sizeof(double) * 2 * s->C);
         DALLOC(s->flags, sizeof(int) * 16);
     } else {
@@ -322,6 +347,10 @@ fpde_status solver_alloc(fpde_solver* s) {
     s->h = sp.length / (sp.nodes - 1);
     fpde_status rc = launch_weights(sp, s->w, s->stream);
     if (rc != FPDE_OK) return rc;
+    if (sp.dims == 1) {
+        rc = launch_space_weights(sp, s->w, s->h, s->stream);
+        if (rc != FPDE_OK) return rc;
+    }
     FPDE_HIP_TRY(hipStreamSynchronize(s->stream));
     return FPDE_OK;
 }
@@ -467,6 +496,13 @@ fpde_status launch_1d(fpde_solver* s, int n, int k, int nsplit, int64_t stride, 
     a.extrap = sp.picard_max > 1 ? 1 : 0;
     a.Uit0 = s->Uit0;
     a.uend = s->uend;
+    a.sp_on = s->w.sp_on ? 1 : 0;
+    a.sp_kap = s->w.sp_kap;
+    a.sp_P = s->spP;
+    a.sp_ldp = s->sp_ldp;
+    a.sp_F = s->spF;
+    a.sp_corr = s->w.sp_corr;
+    a.sp_E = s->spE;
     a.ser_P = s->w.ser_P;
     if (a.ser_P > 0) {
         size_t row = (size_t)std::min(n, sp.steps - 1) * a.ser_P;   // the finalize launch reads no row
@@ -486,9 +522,31 @@ fpde_status launch_1d(fpde_solver* s, int n, int k, int nsplit, int64_t stride, 
     return FPDE_OK;
 }
 
+fpde_status finalize_1d(fpde_solver* s);
+
+// space-fractional sum of step n: U^n complete in memory, then F and the
+// partial sums of E (read by the step launches)
+fpde_status space_part(fpde_solver* s) {
+    fpde_status rc = finalize_1d(s);
+    if (rc != FPDE_OK) return rc;
+    const fpde_spec& sp = s->spec;
+    size_t slot = 0;
+    ev_begin(s, 1, &slot);
+    rc = launch_space_flux(s->U, sp.nodes, s->h, sp.ksat, sp.coef_mode, s->spF, s->sp_ldp, s->stream);
+    if (rc == FPDE_OK) rc = launch_space_apply(s->spF, sp.nodes, s->w.sp_omega, s->spP, s->sp_ldp, s->stream);
+    ev_end(s, slot);
+    if (rc != FPDE_OK) return rc;
+    s->stats.step_launches += 2;
+    return FPDE_OK;
+}
+
 fpde_status enqueue_1d_step(fpde_solver* s, int n) {
     int nsplit = 0, near_lo = 0;
     int64_t stride = 0, off = 0;
+    if (s->w.sp_on) {
+        fpde_status rc = space_part(s);
+        if (rc != FPDE_OK) return rc;
+    }
     fpde_status rc = far_part(s, n, &nsplit, &stride, &off, &near_lo);
     if (rc != FPDE_OK) return rc;
     for (int k = 0; k < s->spec.picard_max; k++) {

@@ -1,0 +1,126 @@
+// Space-fractional explicit scheme of the reference (implicit=0, space_der=0,
+// equation 0): per step, the flux differences of U^n
+//   F[j] = (K_j (U_{j+1}-U_j) - K_{j-1} (U_j-U_{j-1})) / h^2   (Om1, :4176-4179)
+// and the lower-triangular Toeplitz sum of calc_v (:2303-2335)
+//   E_i = sum_{j=1}^{i-1} w(i-j) F[j] + corrections of j = 1, 2,
+//   w(d) = Dg kb(gamma, j-1, j, i) (closed form, a function of d only).
+// The sum costs N^2/2 FP64 multiply-adds per step.  It runs on the FP64
+// matrix cores as a block-Toeplitz product: with i = 16 I + p and
+// j = 16 J + q, the 16 x 16 block (I, J) is T_d[p][q] = w(16 d + p - q),
+// d = I - J, so a tile of 256 rows (I = I0 .. I0+15) accumulates, per block
+// diagonal d, T_d (A operand, 16 x 16) times the 16 x 16 matrix of F blocks
+// F_{I-d} (B operand) in four 4-deep v_mfma_f64_16x16x4f64 steps.  Row tiles
+// x chunks of kSpDiag block diagonals form the grid; the chunks' partial
+// sums are added by the consumer (space_sum).
+#include "fpde_internal.h"
+
+namespace fpde {
+
+typedef double sd4_t __attribute__((ext_vector_type(4)));
+
+// F of U (in memory) with K at U; zero outside the interior [1, N-2] up to ldp.
+__global__ __launch_bounds__(256) void k_space_flux(const double* __restrict__ U, int N, double h, double ksat,
+                                                    int mode, double* __restrict__ F, int64_t ldp) {
+    const int64_t j = (int64_t)blockIdx.x * 256 + threadIdx.x;
+    if (j >= ldp) return;
+    double f = 0.0;
+    if (j >= 1 && j <= N - 2) {
+        const double um = U[j - 1], u0 = U[j], up = U[j + 1];
+        double Km, Kj, iC;
+        vgm_coefs(um, (double)(j - 1) * h, ksat, mode, Km, iC);
+        vgm_coefs(u0, (double)j * h, ksat, mode, Kj, iC);
+        const double hh = h * h;
+        f = Kj * up / hh - (Kj / hh + Km / hh) * u0 + Km * um / hh;
+    }
+    F[j] = f;
+}
+
+// Partial sums P[y][i] over the block diagonals d in [kSpDiag y, kSpDiag y +
+// kSpDiag) of row tile t = blockIdx.x (rows i = 256 t + 16 I' + p).  Wave w
+// takes 16 of those diagonals; the four waves' results are added through LDS.
+// Operand layout of v_mfma_f64_16x16x4f64 (as in the history GEMM): A lane
+// l = A[l%16][l/16], B lane l = B[l/16][l%16], result v of lane l =
+// D[4v + l/16][l%16]; here the M index is p, the N index I', the K index q.
+// LDS: the F blocks J in [I0 - d_hi + 1, I0 + 15 - d_lo] with a stride of 17
+// per block (B reads of one 16-lane row fall into distinct banks) and the
+// weights w(m), m in [16 d_lo - 15, 16 d_hi - 1].
+__global__ __launch_bounds__(256) void k_space_apply(const double* __restrict__ F, int N,
+                                                     const double* __restrict__ om, double* __restrict__ P,
+                                                     int64_t ldp) {
+    constexpr int FB = kSpDiag + 15;
+    __shared__ double Fs[FB * 17];
+    __shared__ double Ws[16 * kSpDiag + 15];
+    __shared__ double red[4][4][64];
+    const int t = blockIdx.x, y = blockIdx.y;
+    if (y >= space_chunks(t)) return;                  // uniform over the block
+    const int I0 = 16 * t;
+    const int d_lo = kSpDiag * y;
+    const int d_hi = min(d_lo + kSpDiag, I0 + 16);
+    const int Jb = I0 - d_hi + 1;
+    const int nfb = d_hi - d_lo + 15;
+    const int mb = 16 * d_lo - 15;
+    const int nw = 16 * (d_hi - d_lo) + 15;
+    for (int q = threadIdx.x; q < nfb * 16; q += 256) {
+        const int b = q >> 4, c = q & 15;
+        const int64_t j = (int64_t)(Jb + b) * 16 + c;
+        Fs[b * 17 + c] = (j >= 0 && j < ldp) ? F[j] : 0.0;
+    }
+    for (int q = threadIdx.x; q < nw; q += 256) {
+        const int m = mb + q;
+        Ws[q] = (m >= 1 && m <= N) ? om[m - 1] : 0.0;
+    }
+    __syncthreads();
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    const int r16 = lane & 15, k4 = lane >> 4;
+    sd4_t acc = {0.0, 0.0, 0.0, 0.0};
+    const int dw_lo = d_lo + 16 * wave, dw_hi = min(dw_lo + 16, d_hi);
+    for (int d = dw_lo; d < dw_hi; d++) {
+        const int fb = (r16 + d_hi - 1 - d) * 17 + k4;   // F block I0 + I' - d, element 4 kq + k4
+        const int wb = 16 * d + r16 - k4 - mb;           // w(16 d + p - q), q = 4 kq + k4
+#pragma unroll
+        for (int kq = 0; kq < 4; kq++)
+            acc = __builtin_amdgcn_mfma_f64_16x16x4f64(Ws[wb - 4 * kq], Fs[fb + 4 * kq], acc, 0, 0, 0);
+    }
+#pragma unroll
+    for (int v = 0; v < 4; v++) red[wave][v][lane] = acc[v];
+    __syncthreads();
+    const int v = threadIdx.x >> 6, l = threadIdx.x & 63;
+    const double sum = (red[0][v][l] + red[1][v][l]) + (red[2][v][l] + red[3][v][l]);
+    const int64_t i = (int64_t)kSpTile * t + 16 * (l & 15) + 4 * v + (l >> 4);
+    P[(int64_t)y * ldp + i] = sum;
+}
+
+// V[i] = E_i on the interior rows, 0 on rows 0 and N-1
+__global__ __launch_bounds__(256) void k_space_reduce(double* __restrict__ V, const double* __restrict__ P,
+                                                      const double* __restrict__ F,
+                                                      const double* __restrict__ corr, int N, int64_t ldp) {
+    const int i = blockIdx.x * 256 + threadIdx.x;
+    if (i >= N) return;
+    V[i] = (i >= 1 && i <= N - 2) ? space_sum(P, ldp, corr, F, i) : 0.0;
+}
+
+fpde_status launch_space_flux(const double* U, int N, double h, double ksat, int mode, double* F, int64_t ldp,
+                              hipStream_t st) {
+    if (ldp < N || ldp % kSpTile) { set_error("space flux: bad padded length"); return FPDE_ERR_ARG; }
+    hipLaunchKernelGGL(k_space_flux, dim3((unsigned)(ldp / 256)), dim3(256), 0, st, U, N, h, ksat, mode, F, ldp);
+    FPDE_HIP_TRY(hipGetLastError());
+    return FPDE_OK;
+}
+
+fpde_status launch_space_apply(const double* F, int N, const double* sp_omega, double* P, int64_t ldp,
+                               hipStream_t st) {
+    if (ldp < N || ldp % kSpTile) { set_error("space sum: bad padded length"); return FPDE_ERR_ARG; }
+    const int T = (int)(ldp / kSpTile);
+    hipLaunchKernelGGL(k_space_apply, dim3(T, space_chunks(T - 1)), dim3(256), 0, st, F, N, sp_omega, P, ldp);
+    FPDE_HIP_TRY(hipGetLastError());
+    return FPDE_OK;
+}
+
+fpde_status launch_space_reduce(double* V, const double* P, const double* F, const double* sp_corr, int N,
+                                int64_t ldp, hipStream_t st) {
+    hipLaunchKernelGGL(k_space_reduce, dim3((N + 255) / 256), dim3(256), 0, st, V, P, F, sp_corr, N, ldp);
+    FPDE_HIP_TRY(hipGetLastError());
+    return FPDE_OK;
+}
+
+}  // namespace fpde

@@ -343,16 +343,30 @@ __global__ __launch_bounds__(256) void k_step1d(Step1DArgs g) {
     __syncthreads();
 
     // ---- 4. rows -------------------------------------------------------------
+    // (space-fractional scheme: the diffusion part of A_/B/R scaled by
+    // Dg kb(gamma,i-1,i,i) and Om1's explicit sum -inv_dw_dh E_i, :4182-4189;
+    // E of step n is formed by its first launch and reused by the others)
     double ar = 0.0, br = 1.0, cr = 0.0, dr = 0.0;
     if (act) {
         if (i == 0) { ar = 0.0; br = 1.0; cr = 0.0; dr = g.utop; }
         else if (i == N - 1) { ar = -1.0; br = 1.0; cr = 0.0; dr = 0.0; }
         else {
+            const double kap = g.sp_on ? g.sp_kap : 1.0;
             double Km = sK[tid], Kp = sK[tid + 1];
-            double A = Km * invC * g.h2inv;
-            double B = Kp * invC * g.h2inv;
-            double R = invC * (Kp * g.h2inv + Km * g.h2inv) + g.L;
+            double A = Km * invC * g.h2inv * kap;
+            double B = Kp * invC * g.h2inv * kap;
+            double R = invC * (Kp * g.h2inv + Km * g.h2inv) * kap + g.L;
             ar = A; br = -R; cr = B; dr = -g.L * un + S;
+            if (g.sp_on) {
+                double E;
+                if (first) {
+                    E = space_sum(g.sp_P, g.sp_ldp, g.sp_corr, g.sp_F, i);
+                    g.sp_E[i] = E;
+                } else {
+                    E = g.sp_E[i];
+                }
+                dr -= invC * E;
+            }
         }
     }
 

@@ -370,4 +370,36 @@ fpde_status launch_weights(const fpde_spec& s, DeviceWeights& w, hipStream_t st)
     return FPDE_OK;
 }
 
+// Space-fractional scheme: calc_v (fpde_hpc_1d.cpp:2303-2335) weighs F[j] by
+// Dg kb(gamma, j-1, j, i; Z) on the node grid Z_k = k h.  That is the time
+// weight of interval t = j-1 at step n = i-1 with tau -> h, alpha -> gamma,
+// Da -> Dg = 1/Gamma(1-gamma) (:3602-3603) and no division by a step: the
+// same closed-form lag vector (lag d = i-j) and the same series corrections
+// of the two intervals next to the surface (j = 1, 2; kb's series branch
+// for t0 <= 1, :2265).  The row factor Dg kb(gamma, i-1, i, i) of A_/B/R
+// (:3095-3120) is the closed form of the newest interval (t1 == a).
+fpde_status launch_space_weights(const fpde_spec& s, DeviceWeights& w, double h, hipStream_t st) {
+    w.sp_on = false;
+    w.sp_kap = 1.0;
+    if (!(s.gamma > 0.0 && s.gamma < 1.0)) return FPDE_OK;
+    if (!w.sp_omega || !w.sp_corr) { set_error("space-fractional weights not allocated"); return FPDE_ERR_ARG; }
+    OrderSet os;
+    os.q = 1;
+    os.a[0] = s.gamma;
+    os.c[0] = 1.0 / std::tgamma(1.0 - s.gamma);
+    const int N = s.nodes, tb = 256;
+    hipLaunchKernelGGL(k_weights_omega, dim3((N + tb - 1) / tb), dim3(tb), 0, st, os, h, N, 0.0, w.sp_omega);
+    if (s.weight_mode == 1) {
+        hipLaunchKernelGGL(k_weights_corr, dim3((2 * N + tb - 1) / tb), dim3(tb), 0, st, os, h, N, s.series_eps, 1,
+                           0.0, (const double*)w.sp_omega, w.sp_corr);
+    } else {
+        FPDE_HIP_TRY(hipMemsetAsync(w.sp_corr, 0, sizeof(double) * 2 * (size_t)N, st));
+    }
+    FPDE_HIP_TRY(hipGetLastError());
+    const double e = 1.0 - s.gamma;
+    w.sp_kap = os.c[0] * std::pow(h, e) / e;
+    w.sp_on = true;
+    return FPDE_OK;
+}
+
 }  // namespace fpde

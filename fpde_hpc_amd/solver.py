@@ -24,17 +24,18 @@ def _p(a):
 def make_spec(nodes, steps, tau, alpha=0.8, dims=1, length=1.0, ksat=None, coef_mode=0,
               picard_max=1, picard_tol=0.0, weight_mode=1, series_eps=1e-5, do_q=0, do_rule=0,
               do_a0=0.1, do_a1=1.0, hist_mode="plain", block=64, step_mode=0, fourier=1.0,
-              u_top=-1.0, sum_alg=0, sum_param=1e-5, taus=None, var_tau=0):
+              u_top=-1.0, sum_alg=0, sum_param=1e-5, taus=None, var_tau=0, gamma=0.0):
     """fpde_spec; ksat defaults to the value giving mesh Fourier number `fourier`
     (K*tau/(C*h^2)) at suction -u_top.  `taus` (one length per step) makes the
-    time grid non-uniform."""
+    time grid non-uniform; `gamma` in (0,1) selects the space-fractional
+    scheme (0 or 1: classical diffusion)."""
     if ksat is None:
         h = length / (nodes - 1)
         ksat = lib().fpde_ksat_for_fourier(u_top, h, tau, fourier) if coef_mode == 0 else 1.0
     hm = HIST_MODES[hist_mode] if isinstance(hist_mode, str) else int(hist_mode)
     spec = Spec(dims, nodes, steps, tau, alpha, length, ksat, coef_mode, picard_max, picard_tol,
                 weight_mode, series_eps, do_q, do_rule, do_a0, do_a1, hm, block, step_mode,
-                sum_alg, sum_param, None, var_tau, 0)
+                sum_alg, sum_param, None, var_tau, 0, gamma)
     if taus is not None:
         spec._taus = np.ascontiguousarray(taus, dtype=np.float64)   # alive until the solver copies it
         if spec._taus.size != steps:

@@ -105,6 +105,16 @@ fpde_status fpde_reference_sum(double *S, const double *old, int64_t old_size,
 fpde_status fpde_history_sum_f64(double *S, const double *D, int64_t n, int64_t nodes,
                                  int64_t ld, const double *omega);
 
+/* Space-fractional sum of the explicit scheme, calc_v (fpde_hpc_1d.cpp:2303-2335,
+ * one-sided, with_i = 0), on the uniform node grid Z_k = k h:
+ * V[i] = Dg * sum_{j=1}^{i-1} kb(gamma, j-1, j, i; Z) * F[j], i = 1..nodes-2,
+ * V[0] = V[nodes-1] = 0, Dg = 1/Gamma(1-gamma).  weight_mode 1 takes the
+ * reference's series for the intervals j = 1, 2 (stop at |term| <= series_eps,
+ * kb dispatch :2265), 0 the closed form everywhere.  F, V: device arrays of
+ * `nodes` doubles (F[0] and F[nodes-1] are not read); 0 < gamma < 1. */
+fpde_status fpde_space_sum(double *V, const double *F, int64_t nodes, double h, double gamma,
+                           int weight_mode, double series_eps);
+
 /* Time-block history GEMM of the blocked solver paths, for B future steps:
  * P[r*ld + i] = sum_{t<tc} omega[n0+r-t-1] * D[t*ld + i], r < block, i < nodes,
  * computed in `mode` (1 FP64 MFMA, 2 split bf16, 3 split fp16+scaling,
@@ -163,6 +173,13 @@ typedef struct {
                             (fpde_solver_advance_tau, fpde_varT_step); 1D,
                             hist_mode 0, sum_alg 0, taus NULL; steps = capacity */
     int reserved;
+    double gamma;        /* space-fractional order (CLI "bb", H_solver gamma,
+                            fpde_hpc_1d.cpp:3510): 0 or 1 = classical diffusion;
+                            in (0,1) the reference's explicit one-sided scheme
+                            (implicit=0, space_der=0): the rows carry
+                            Dg kb(gamma,i-1,i,i) (A_/B/R, :3095-3142) and the
+                            right-hand side -Dg inv_dw_dh calc_v(F(U^n))
+                            (Om1 :4165-4189, calc_v :2303-2335); 1D only       */
 } fpde_spec;
 
 typedef struct fpde_solver fpde_solver;

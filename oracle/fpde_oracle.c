@@ -22,6 +22,16 @@
  *   orc_solve_3d        : a 3D field constant in x,y reduces to orc_solve_1d
  *                         (LOD sweeps along x,y are then identities).
  *   DO path             : closed form int rho(a) D^a t^2 da = (t^1.9-t)/ln t.
+ *   orc_space_v         : for F = 1 the closed-form weights telescope to
+ *                         Dg (Z_i^(1-g) - h^(1-g))/(1-g); for random F it equals
+ *                         Dg * int_0^{Z_{i-1}} (Z_i-s)^(-g) f(s) ds of the
+ *                         piecewise-constant f = F[j] on (Z_{j-1}, Z_j) by
+ *                         scipy.integrate.quad; the series branch (j <= 2)
+ *                         agrees with the same integrals to the series tolerance.
+ *   space-fractional solve : one step equals the dense system assembled and
+ *                         solved independently (numpy, quad weights); as
+ *                         gamma -> 1 the scheme tends to the classical one
+ *                         (Dg kb(g,i-1,i,i) -> 1, the explicit sum -> 0).
  *   reference-series mode weights (t0<=1): parity unpinned against the
  *                         compiled reference (it needs fftpack.h, absent in this
  *                         image); pinned only by the series/closed-form limit.
@@ -424,7 +434,7 @@ static inline double c_K(const coef_ctx *c, double u, double z)
 static void line_solve(int n, const double *un, const double *uit, const double *H,
                        const double *zc, double h, double L, int top, double utop,
                        const coef_ctx *cc, double *Al, double *Bt, double *K,
-                       double *iC, double *unew)
+                       double *iC, double *unew, const double *kap, const double *V)
 {
     int N = n - 1;   /* reference N: nodes 0..N */
     for (int i = 0; i < n; i++) {
@@ -434,10 +444,14 @@ static void line_solve(int n, const double *un, const double *uit, const double 
     double hh = h * h;
     if (top == 0) { Al[1] = 0.0; Bt[1] = utop; } else { Al[1] = 1.0; Bt[1] = 0.0; }
     for (int i = 1; i < N; i++) {
-        double A_ = K[i - 1] * iC[i] / hh;
-        double B_ = K[i] * iC[i] / hh;
-        double R_ = iC[i] * ((K[i] / hh) + (K[i - 1] / hh)) + L;
+        /* space-fractional factor Dg kb(gamma,i-1,i,i) of A_/B/R (:3095-3120)
+         * and the explicit sum -Dg inv_dw_dh calc_v of Om1 (:4182-4189) */
+        double kp = kap ? kap[i] : 1.0;
+        double A_ = K[i - 1] * iC[i] / hh * kp;
+        double B_ = K[i] * iC[i] / hh * kp;
+        double R_ = iC[i] * ((K[i] / hh) + (K[i - 1] / hh)) * kp + L;
         double Om = -L * un[i] + H[i];
+        if (V) Om -= iC[i] * V[i];
         Al[i + 1] = B_ / (R_ - A_ * Al[i]);
         Bt[i + 1] = (Al[i + 1] / B_) * (A_ * Bt[i] - Om);
     }
@@ -484,6 +498,58 @@ static void history_term(weights_ctx *w, const orc_params *p, int n, int nodes,
     *Lout = L;
 }
 
+/* ------------------------------------------------------------------------ */
+/* Space-fractional explicit scheme (implicit=0, space_der=0).  The weights
+ * Dg*kb(gamma,j-1,j,i;Z) of calc_v (:2315-2322) over the uniform grid
+ * Z_k = k h (:2386-2387), row-packed: row i (1..n-2) holds j = 1..i-1. */
+static int space_on(const orc_params *p) { return p->gamma > 0.0 && p->gamma < 1.0; }
+
+static double *space_table(int n, double h, double gamma, int weight_mode, double eps)
+{
+    double *Z = (double *)malloc(sizeof(double) * n);
+    size_t rows = n > 2 ? (size_t)(n - 2) : 0;
+    double *T = (double *)malloc(sizeof(double) * (rows * (rows + 1) / 2 + 1));
+    if (!Z || !T) { free(Z); free(T); return NULL; }
+    for (int k = 0; k < n; k++) Z[k] = k * h;
+    double Dg = 1.0 / tgamma(1.0 - gamma);                        /* :3602-3603 */
+    for (int i = 1; i <= n - 2; i++) {
+        double *row = T + (size_t)(i - 1) * i / 2;               /* row i starts at (i-1)i/2 */
+        for (int j = 1; j <= i - 1; j++)
+            row[j - 1] = Dg * orc_kb(gamma, j - 1, i, Z, weight_mode, eps);
+    }
+    free(Z);
+    return T;
+}
+
+static void space_apply(int n, const double *T, const double *F, double *V)
+{
+    for (int i = 0; i < n; i++) V[i] = 0.0;
+    for (int i = 1; i <= n - 2; i++) {
+        const double *row = T + (size_t)(i - 1) * i / 2;
+        double v = 0.0;
+        for (int j = 1; j <= i - 1; j++) v += row[j - 1] * F[j];  /* :2319-2323 */
+        V[i] = v;
+    }
+}
+
+void orc_space_v(int nodes, double h, double gamma, int weight_mode, double eps,
+                 const double *F, double *V)
+{
+    double *T = space_table(nodes, h, gamma, weight_mode, eps);
+    if (!T) { for (int i = 0; i < nodes; i++) V[i] = 0.0; return; }
+    space_apply(nodes, T, F, V);
+    free(T);
+}
+
+void orc_space_flux(int nodes, double h, const double *K, const double *u, double *F)
+{
+    double hh = h * h;
+    F[0] = 0.0;
+    F[nodes - 1] = 0.0;
+    for (int j = 1; j <= nodes - 2; j++)                          /* :4176-4179 */
+        F[j] = K[j] * u[j + 1] / hh - (K[j] / hh + K[j - 1] / hh) * u[j] + K[j - 1] * u[j - 1] / hh;
+}
+
 int orc_solve_1d(const orc_params *p, double *u)
 {
     int n = p->nodes, M = p->steps;
@@ -499,11 +565,29 @@ int orc_solve_1d(const orc_params *p, double *u)
     if (!old || !H || !tmp || !Al || !Bt || !K || !iC || !zc || !uit || !unew) return -1;
     for (int i = 0; i < n; i++) zc[i] = i * h;
     double utop = u[0];
+    /* space-fractional scheme: kap[i] = Dg kb(gamma,i-1,i,i) (the newest
+     * interval, closed form: t1 == a), V = calc_v of F(U^n) with K at U^n */
+    double *kap = NULL, *V = NULL, *Tsp = NULL, *Fsp = NULL, *Ksp = NULL;
+    if (space_on(p)) {
+        kap = (double *)malloc(sizeof(double) * n);
+        V = (double *)malloc(sizeof(double) * n);
+        Fsp = (double *)malloc(sizeof(double) * n);
+        Ksp = (double *)malloc(sizeof(double) * n);
+        Tsp = space_table(n, h, p->gamma, p->weight_mode, p->series_eps);
+        if (!kap || !V || !Fsp || !Ksp || !Tsp) return -1;
+        double Dg = 1.0 / tgamma(1.0 - p->gamma);
+        for (int i = 0; i < n; i++) kap[i] = (i >= 1) ? Dg * orc_kb(p->gamma, i - 1, i, zc, p->weight_mode, p->series_eps) : 1.0;
+    }
     memcpy(old, u, sizeof(double) * n);                        /* :5104-5113 */
     for (int s = 0; s < M; s++) {
         double L;
         history_term(&w, p, s, n, old, H, tmp, &L);
         const double *un = old + (size_t)s * n;
+        if (V) {
+            for (int i = 0; i < n; i++) Ksp[i] = c_K(&cc, un[i], zc[i]);
+            orc_space_flux(n, h, Ksp, un, Fsp);
+            space_apply(n, Tsp, Fsp, V);
+        }
         memcpy(uit, un, sizeof(double) * n);
         /* Picard (picard_max > 1) starts from the linear extrapolation
          * 2 U^n - U^{n-1}; a single pass keeps the lagged coefficients at U^n
@@ -511,7 +595,7 @@ int orc_solve_1d(const orc_params *p, double *u)
         if (p->picard_max > 1 && s >= 1)
             for (int i = 0; i < n; i++) uit[i] = 2.0 * un[i] - un[i - n];
         for (int k = 0; k < p->picard_max; k++) {
-            line_solve(n, un, uit, H, zc, h, L, 0, utop, &cc, Al, Bt, K, iC, unew);
+            line_solve(n, un, uit, H, zc, h, L, 0, utop, &cc, Al, Bt, K, iC, unew, kap, V);
             double dmax = 0.0;
             for (int i = 0; i < n; i++) { double d = fabs(unew[i] - uit[i]); if (d > dmax) dmax = d; }
             memcpy(uit, unew, sizeof(double) * n);
@@ -521,6 +605,7 @@ int orc_solve_1d(const orc_params *p, double *u)
     }
     memcpy(u, old + (size_t)M * n, sizeof(double) * n);
     free(old); free(H); free(tmp); free(Al); free(Bt); free(K); free(iC); free(zc); free(uit); free(unew);
+    free(kap); free(V); free(Tsp); free(Fsp); free(Ksp);
     wctx_free(&w);
     return 0;
 }
@@ -569,7 +654,8 @@ int orc_solve_3d(const orc_params *p, double *u)
                 }
                 memcpy(luit, lun, sizeof(double) * n);
                 double top_val = (axis == 0) ? utop[a1 * n + a0] : 0.0;
-                line_solve(n, lun, luit, lH, lz, h, L, axis == 0 ? 0 : 1, top_val, &cc, lAl, lBt, lK, liC, lnew);
+                line_solve(n, lun, luit, lH, lz, h, L, axis == 0 ? 0 : 1, top_val, &cc, lAl, lBt, lK, liC, lnew,
+                           NULL, NULL);
                 for (int i = 0; i < n; i++) cur[base + stride * i] = lnew[i];
             }
         }

@@ -98,12 +98,24 @@ typedef struct {
     double sum_param;    /* sum_alg 1: threshold on |kb|; sum_alg 2: series terms P  */
     const double *taus;  /* NULL: uniform tau; else the M step lengths tau_0..tau_{M-1}
                             (Time[k+1] = Time[k] + tau_k, init_time_step :3942-3949) */
+    double gamma;        /* space-fractional order (1D): 0 or 1 = classical diffusion,
+                            (0,1) = the explicit one-sided scheme (calc_v :2303-2335) */
 } orc_params;
 
 /* 1D solve: u (nodes) holds u0 on entry, u^M on exit.  u_top = u0[0]. */
 int orc_solve_1d(const orc_params *p, double *u);
 /* 3D LOD solve on nodes^3 (depth index fastest); u holds u0 / u^M. */
 int orc_solve_3d(const orc_params *p, double *u);
+
+/* Space-fractional sum of the explicit scheme (calc_v, :2303-2335, one-sided,
+ * with_i = 0) on the uniform grid Z_k = k h of `nodes` nodes:
+ *   V[i] = Dg * sum_{j=1}^{i-1} kb(gamma, j-1, j, i; Z) * F[j],  i = 1..nodes-2,
+ * Dg = 1/Gamma(1-gamma); V[0] = V[nodes-1] = 0.  kb dispatch as orc_kb. */
+void orc_space_v(int nodes, double h, double gamma, int weight_mode, double eps,
+                 const double *F, double *V);
+/* F[j] of Om1 equation 0 (:4178-4179) on a uniform grid, j = 1..nodes-2:
+ * (K[j] (u[j+1]-u[j]) - K[j-1] (u[j]-u[j-1])) / h^2; F[0] = F[nodes-1] = 0. */
+void orc_space_flux(int nodes, double h, const double *K, const double *u, double *F);
 
 /* Effective (folded) weights used by the product for uniform tau:
  * local = coefficient of (U^{n+1}-U^n); omega[j-1] for j=1..M (Toeplitz),

@@ -25,7 +25,7 @@ class Params(ctypes.Structure):
         ("weight_mode", ctypes.c_int), ("series_eps", ctypes.c_double), ("do_q", ctypes.c_int),
         ("do_rule", ctypes.c_int), ("do_a0", ctypes.c_double), ("do_a1", ctypes.c_double),
         ("nthreads", ctypes.c_int), ("sum_alg", ctypes.c_int), ("sum_param", ctypes.c_double),
-        ("taus", ctypes.c_void_p),
+        ("taus", ctypes.c_void_p), ("gamma", ctypes.c_double),
     ]
 
 
@@ -68,6 +68,9 @@ def lib():
         L.orc_thomas.argtypes = [ctypes.c_int, _dp, _dp, _dp, _dp, _dp]
         L.orc_init_1d.argtypes = [ctypes.c_int, ctypes.c_double, ctypes.c_uint64, _dp]
         L.orc_init_3d.argtypes = [ctypes.c_int, ctypes.c_double, ctypes.c_uint64, _dp]
+        L.orc_space_v.argtypes = [ctypes.c_int, ctypes.c_double, ctypes.c_double, ctypes.c_int, ctypes.c_double,
+                                  _dp, _dp]
+        L.orc_space_flux.argtypes = [ctypes.c_int, ctypes.c_double, _dp, _dp, _dp]
         L.orc_ksat_for_fourier.argtypes = [ctypes.c_double] * 4
         L.orc_ksat_for_fourier.restype = ctypes.c_double
         _lib = L
@@ -174,7 +177,7 @@ def thomas(a, b, c, d):
 def make_params(nodes, steps, tau, alpha=0.8, length=1.0, ksat=None, coef_mode=0,
                 picard_max=1, picard_tol=1e-12, weight_mode=1, series_eps=1e-5,
                 do_q=0, do_rule=0, do_a0=0.1, do_a1=1.0, nthreads=0, fourier=1.0,
-                u_top=None, sum_alg=0, sum_param=1e-5, taus=None):
+                u_top=None, sum_alg=0, sum_param=1e-5, taus=None, gamma=0.0):
     if ksat is None:
         if coef_mode == 0:
             h = length / (nodes - 1)
@@ -182,7 +185,7 @@ def make_params(nodes, steps, tau, alpha=0.8, length=1.0, ksat=None, coef_mode=0
         else:
             ksat = 1.0
     p = Params(nodes, steps, tau, alpha, length, ksat, coef_mode, picard_max, picard_tol,
-               weight_mode, series_eps, do_q, do_rule, do_a0, do_a1, nthreads, sum_alg, sum_param, None)
+               weight_mode, series_eps, do_q, do_rule, do_a0, do_a1, nthreads, sum_alg, sum_param, None, gamma)
     if taus is not None:
         p._taus = np.ascontiguousarray(taus, dtype=np.float64)    # kept alive with the struct
         assert p._taus.size == steps
@@ -225,6 +228,22 @@ def folded_weights(params):
     corr = np.zeros(2 * M)
     lib().orc_folded_weights(ctypes.byref(params), ctypes.byref(local), _p(omega), _p(corr))
     return local.value, omega, corr.reshape(M, 2)
+
+
+def space_v(F, h, gamma, weight_mode=1, eps=1e-5):
+    """calc_v of the explicit space-fractional scheme on a uniform grid."""
+    F = np.ascontiguousarray(F, dtype=np.float64)
+    V = np.zeros_like(F)
+    lib().orc_space_v(F.size, h, gamma, weight_mode, eps, _p(F), _p(V))
+    return V
+
+
+def space_flux(K, u, h):
+    K = np.ascontiguousarray(K, dtype=np.float64)
+    u = np.ascontiguousarray(u, dtype=np.float64)
+    F = np.zeros_like(u)
+    lib().orc_space_flux(u.size, h, _p(K), _p(u), _p(F))
+    return F
 
 
 def ksat_for_fourier(u_top, h, tau, fo=1.0):

@@ -54,6 +54,7 @@ def test_ksat_fourier_matches_oracle(fpde, oracle):
     dict(sum_alg=2, sum_param=0.0, hist_mode=0), dict(sum_alg=2, sum_param=65.0, hist_mode=0),
     dict(sum_alg=2, sum_param=2.5, hist_mode=0), dict(sum_alg=2, sum_param=8.0, hist_mode=1),
     dict(sum_alg=2, sum_param=8.0, hist_mode=0, do_q=4),
+    dict(gamma=1.5), dict(gamma=-0.1), dict(dims=3, nodes=16, gamma=0.5),
 ])
 def test_invalid_specs_rejected(fpde, bad):
     kw = dict(nodes=100, steps=10, tau=1e-3, alpha=0.5, hist_mode=1, block=64)
@@ -105,3 +106,14 @@ def test_struct_layout_matches_binding(fpde, name):
         out = [int(v) for v in subprocess.check_output([exe]).split()]
     assert out[0] == ctypes.sizeof(cls)
     assert out[1:] == [getattr(cls, f).offset for f in fields]
+
+
+@pytest.mark.parametrize("args", [
+    (0, 0, 100, 0.01, 0.5, 1, 1e-5), (8, 8, 3, 0.01, 0.5, 1, 1e-5), (8, 8, 100, 0.0, 0.5, 1, 1e-5),
+    (8, 8, 100, 0.01, 1.0, 1, 1e-5), (8, 8, 100, 0.01, 0.0, 1, 1e-5), (8, 8, 100, 0.01, 0.5, 2, 1e-5),
+])
+def test_space_sum_rejects_bad_arguments(fpde, args):
+    """Argument checks run before any device call (no GPU needed)."""
+    rc = fpde.lib().fpde_space_sum(*args)
+    assert rc == 1
+    assert fpde.lib().fpde_last_error()

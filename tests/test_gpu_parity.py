@@ -732,3 +732,103 @@ def test_varT_step_doubling_matches_oracle_model(fpde, oracle):
     for (_, a), (_, b) in zip(dev, model):
         assert abs(a - b) <= 1e-6 * b
     assert rel_err(u, U(m0)) <= TOL_F64
+
+
+# ------------------------------------------------- space-fractional scheme (gamma < 1)
+@pytest.mark.parametrize("n", [4, 5, 17, 255, 256, 257, 1100, 4100])
+@pytest.mark.parametrize("wm", [0, 1])
+def test_space_sum_kernel_matches_oracle(fpde, oracle, n, wm):
+    """calc_v on the FP64 matrix cores across tile (256 rows) and
+    block-diagonal chunk (1024 columns) boundaries, both weight modes."""
+    rng = np.random.default_rng(3 * n + wm)
+    F = rng.standard_normal(n)
+    h, gam = 1.0 / (n - 1), 0.65
+    ref = oracle.space_v(F, h, gam, weight_mode=wm)
+    L = fpde.lib()
+    dv = Dev(L)
+    try:
+        bF, bV = dv.put(F), dv.put(np.full(n, 7.0))
+        rc = L.fpde_space_sum(dv.ptr(bV), dv.ptr(bF), n, h, gam, wm, 1e-5)
+        assert rc == 0, L.fpde_last_error()
+        V = dv.get(bV, n)
+    finally:
+        dv.close()
+    assert V[0] == 0.0 and V[-1] == 0.0
+    if n > 4:
+        assert rel_err(V, ref) <= 1e-12
+    else:
+        assert V[1] == 0.0 and abs(V[2] - ref[2]) <= 1e-12 * abs(ref[2])
+
+
+def _oracle_space(oracle, spec, u0):
+    p = oracle.make_params(spec.nodes, spec.steps, spec.tau, spec.alpha, ksat=spec.ksat,
+                           coef_mode=spec.coef_mode, picard_max=spec.picard_max, picard_tol=spec.picard_tol,
+                           weight_mode=spec.weight_mode, series_eps=spec.series_eps, sum_alg=spec.sum_alg,
+                           sum_param=spec.sum_param, gamma=spec.gamma)
+    return oracle.solve_1d(p, u0)
+
+
+@pytest.mark.parametrize("mode,tol", [("plain", TOL_F64), ("mfma_f64", TOL_F64), ("split_bf16", TOL_MIXED)])
+@pytest.mark.parametrize("gam", [0.5, 0.8])
+def test_space_fractional_linear_solve_matches_oracle(fpde, oracle, mode, tol, gam):
+    """Linear coefficients, where the space-fractional term changes the
+    solution by 10-30%: every history form with the explicit spatial sum."""
+    nodes, steps = 1100, 60
+    x = np.linspace(0.0, 1.0, nodes)
+    u0 = np.sin(3 * x) + 0.3 * np.cos(17 * x)
+    spec = fpde.make_spec(nodes, steps, 1e-4, alpha=0.8, coef_mode=1, hist_mode=mode, block=32, gamma=gam)
+    u = fpde.solve(spec, u0)
+    ref = _oracle_space(oracle, spec, u0)
+    err = rel_err(u, ref)
+    print(f"space-fractional linear {mode} gamma {gam}: rel err {err:.3e}")
+    assert err <= tol
+
+
+@pytest.mark.parametrize("mode,tol", [("plain", TOL_F64), ("mfma_f64", TOL_F64), ("split_bf16", TOL_MIXED)])
+@pytest.mark.parametrize("picard", [1, 3])
+def test_space_fractional_vgm_solve_matches_oracle(fpde, oracle, mode, tol, picard):
+    """VGM coefficients with Picard: the solution and the space-fractional
+    effect (the difference from the classical solve) both match."""
+    nodes, steps, gam = 700, 80, 0.8
+    u0 = fpde.init_1d(nodes, 81)
+    kw = dict(alpha=0.8, hist_mode=mode, block=32, picard_max=picard, picard_tol=1e-12 if picard > 1 else 0.0,
+              u_top=u0[0])
+    spec = fpde.make_spec(nodes, steps, 1e-3, gamma=gam, **kw)
+    u = fpde.solve(spec, u0)
+    ref = _oracle_space(oracle, spec, u0)
+    classical = _oracle_space(oracle, fpde.make_spec(nodes, steps, 1e-3, **kw), u0)
+    err, eff = rel_err(u, ref), rel_err(u - classical, ref - classical)
+    print(f"space-fractional vgm {mode} picard {picard}: rel err {err:.3e}, effect err {eff:.3e}")
+    assert err <= tol
+    assert eff <= (1e-6 if tol == TOL_F64 else 1e-4)
+
+
+@pytest.mark.parametrize("sum_alg,param", [(1, 1e-3), (2, 16)])
+def test_space_fractional_with_restricted_and_series_history(fpde, oracle, sum_alg, param):
+    nodes, steps = 600, 70
+    x = np.linspace(0.0, 1.0, nodes)
+    u0 = np.cos(2 * x) - 0.2 * np.sin(11 * x)
+    spec = fpde.make_spec(nodes, steps, 1e-4, alpha=0.6, coef_mode=1, hist_mode="plain", gamma=0.7,
+                          sum_alg=sum_alg, sum_param=param)
+    u = fpde.solve(spec, u0)
+    err = rel_err(u, _oracle_space(oracle, spec, u0))
+    print(f"space-fractional sum_alg {sum_alg}: rel err {err:.3e}")
+    assert err <= TOL_F64
+
+
+def test_space_fractional_var_tau_steps(fpde, oracle):
+    """Run-time step lengths with the space-fractional sum: the var_tau solver
+    equals the prescribed-grid oracle solve."""
+    nodes, steps = 400, 30
+    rng = np.random.default_rng(90)
+    taus = 1e-4 * (0.5 + rng.random(steps))
+    x = np.linspace(0.0, 1.0, nodes)
+    u0 = np.sin(4 * x)
+    spec = fpde.make_spec(nodes, steps, 1e-4, alpha=0.7, coef_mode=1, var_tau=1, gamma=0.6)
+    with fpde.Solver(spec) as s:
+        s.set_initial(u0)
+        for t in taus:
+            s.advance_tau(float(t))
+        u = s.solution()
+    p = oracle.make_params(nodes, steps, 1e-4, 0.7, coef_mode=1, taus=taus, gamma=0.6)
+    assert rel_err(u, oracle.solve_1d(p, u0)) <= TOL_F64

@@ -367,3 +367,105 @@ def test_folded_weights_equal_per_order_sums(oracle):
             ref += wq * G(3 - a) / 2 / G(1 - a) * kb / tau
         got = omega[n - t - 1] + (corr[n, t] if t < 2 else 0.0)
         assert abs(got - ref) <= 1e-12 * abs(ref)
+
+
+# ------------------------------------------------- space-fractional operator
+@pytest.mark.parametrize("gam", [0.3, 0.8])
+def test_space_sum_telescopes_for_constant_flux(oracle, gam):
+    """F = 1: the closed-form weights of row i sum to
+    Dg * int_0^{Z_{i-1}} (Z_i - s)^-g ds = Dg (Z_i^(1-g) - h^(1-g)) / (1-g)."""
+    n = 300
+    h = 1.0 / (n - 1)
+    F = np.zeros(n)
+    F[1:n - 1] = 1.0
+    V = oracle.space_v(F, h, gam, weight_mode=0)
+    Z = np.arange(n) * h
+    ref = np.zeros(n)
+    ref[1:n - 1] = (Z[1:n - 1] ** (1 - gam) - h ** (1 - gam)) / (1 - gam) / G(1 - gam)
+    assert np.max(np.abs(V - ref)) <= 1e-13 * np.max(np.abs(ref))
+    assert V[0] == 0.0 and V[-1] == 0.0
+
+
+@pytest.mark.parametrize("gam", [0.25, 0.7])
+def test_space_sum_is_the_integral_of_the_piecewise_constant_flux(oracle, gam):
+    """V_i = Dg int_0^{Z_{i-1}} (Z_i - s)^-g f(s) ds, f = F[j] on (Z_{j-1}, Z_j):
+    the interval left of node j carries F[j] (an index shift fails this)."""
+    n = 24
+    h = 0.5 / (n - 1)
+    rng = np.random.default_rng(17)
+    F = np.zeros(n)
+    F[1:n - 1] = rng.standard_normal(n - 2)
+    Z = np.arange(n) * h
+    ref = np.zeros(n)
+    for i in range(1, n - 1):
+        for j in range(1, i):
+            w, _ = integrate.quad(lambda s: (Z[i] - s) ** -gam, Z[j - 1], Z[j], epsabs=1e-15, epsrel=1e-13)
+            ref[i] += w * F[j]
+    ref /= G(1 - gam)
+    V0 = oracle.space_v(F, h, gam, weight_mode=0)
+    assert np.max(np.abs(V0 - ref)) <= 1e-11 * np.max(np.abs(ref))
+    # the reference series for the two intervals next to the surface (j <= 2)
+    # agrees with the same integrals to the series tolerance
+    V1 = oracle.space_v(F, h, gam, weight_mode=1)
+    assert np.max(np.abs(V1 - ref)) <= 1e-4 * np.max(np.abs(ref))
+    assert not np.array_equal(V0, V1)
+
+
+def test_space_flux_of_quadratic_head_with_linear_conductivity(oracle):
+    """u = x^2, K = x: F_j = (K_j (u_{j+1}-u_j) - K_{j-1} (u_j-u_{j-1})) / h^2
+    = 4 x_j - h exactly (continuum: (x u')' = 4x)."""
+    n = 50
+    h = 1.0 / (n - 1)
+    x = np.arange(n) * h
+    F = oracle.space_flux(x, x ** 2, h)
+    assert np.allclose(F[1:n - 1], 4 * x[1:n - 1] - h, rtol=1e-11, atol=1e-11)
+    assert F[0] == 0.0 and F[-1] == 0.0
+
+
+@pytest.mark.parametrize("gam", [0.4, 0.85])
+def test_space_fractional_step_equals_dense_assembly(oracle, gam):
+    """One step, linear coefficients (K = 1/C = 1): the oracle's Thomas solve
+    equals the dense system assembled from quadrature weights and solved by
+    numpy: L (U_i - U0_i) = kap (U_{i+1} - 2U_i + U_{i-1})/h^2 + V_i(F(U0)),
+    kap = h^(1-g)/Gamma(2-g), L = tau^-a/Gamma(2-a), Dirichlet top, U_N = U_{N-1}."""
+    n, tau, alpha = 20, 1e-3, 0.6
+    h = 1.0 / (n - 1)
+    rng = np.random.default_rng(23)
+    u0 = np.sin(np.arange(n) * 0.4) + 0.1 * rng.standard_normal(n)
+    p = oracle.make_params(n, 1, tau, alpha, coef_mode=1, weight_mode=0, gamma=gam)
+    u1 = oracle.solve_1d(p, u0)
+    Z = np.arange(n) * h
+    F0 = np.zeros(n)
+    F0[1:n - 1] = (u0[2:] - 2 * u0[1:n - 1] + u0[:n - 2]) / h ** 2
+    V = np.zeros(n)
+    for i in range(1, n - 1):
+        for j in range(1, i):
+            w, _ = integrate.quad(lambda s: (Z[i] - s) ** -gam, Z[j - 1], Z[j], epsabs=1e-15, epsrel=1e-13)
+            V[i] += w * F0[j] / G(1 - gam)
+    kap = h ** (1 - gam) / G(2 - gam)
+    L = tau ** -alpha / G(2 - alpha)
+    A = np.zeros((n, n))
+    b = np.zeros(n)
+    A[0, 0] = 1.0
+    b[0] = u0[0]
+    for i in range(1, n - 1):
+        A[i, i - 1] = A[i, i + 1] = kap / h ** 2
+        A[i, i] = -2 * kap / h ** 2 - L
+        b[i] = -L * u0[i] - V[i]
+    A[n - 1, n - 1], A[n - 1, n - 2] = 1.0, -1.0
+    ref = np.linalg.solve(A, b)
+    assert np.max(np.abs(u1 - ref)) <= 1e-11 * np.max(np.abs(ref))
+
+
+def test_space_fractional_tends_to_classical_as_gamma_to_one(oracle):
+    """Dg kb(g,i-1,i,i) = h^(1-g)/Gamma(2-g) -> 1 and Dg = 1/Gamma(1-g) -> 0:
+    the scheme becomes the classical one as g -> 1."""
+    n, M = 80, 30
+    u0 = oracle.init_1d(n, 5)
+    base = oracle.solve_1d(oracle.make_params(n, M, 1e-3, 0.7, u_top=u0[0]), u0)
+    near = oracle.solve_1d(oracle.make_params(n, M, 1e-3, 0.7, u_top=u0[0], gamma=1 - 1e-7), u0)
+    far = oracle.solve_1d(oracle.make_params(n, M, 1e-3, 0.7, u_top=u0[0], gamma=0.8), u0)
+    d_near = np.max(np.abs(near - base)) / np.max(np.abs(base))
+    d_far = np.max(np.abs(far - base)) / np.max(np.abs(base))
+    assert d_near <= 1e-7
+    assert d_far >= 1e3 * d_near
