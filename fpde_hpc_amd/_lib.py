@@ -86,6 +86,7 @@ SIGNATURES = [
     ("fpde_solver_set_initial", ctypes.c_int, [_vp, _dp]),
     ("fpde_solver_reset", ctypes.c_int, [_vp, _dp]),
     ("fpde_solver_advance", ctypes.c_int, [_vp, ctypes.c_int]),
+    ("fpde_solver_advance_graph", ctypes.c_int, [_vp, ctypes.c_int, ctypes.c_int]),
     ("fpde_solver_sync", ctypes.c_int, [_vp]),
     ("fpde_solver_query", ctypes.c_int, [_vp]),
     ("fpde_solver_step", ctypes.c_int, [_vp]),

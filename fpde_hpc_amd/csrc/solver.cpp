@@ -171,6 +171,9 @@ struct fpde_solver {
     bool block_mode = false;   // 1D steps run as cooperative block launches
     int64_t coop_launches = 0; // selects the barrier counter of the next one
     int* flags = nullptr;
+    // graph-captured step loop: the executable graph of the last captured chunk
+    hipGraphExec_t gexec = nullptr;
+    int64_t graph_launches = 0, graph_instantiations = 0;
     void* comp[3] = {nullptr, nullptr, nullptr};
     float* scale = nullptr;
     // 3D
@@ -728,6 +731,7 @@ fpde_status fpde_solver_create(const fpde_spec* spec, fpde_solver** out) {
 fpde_status fpde_solver_destroy(fpde_solver* s) {
     if (!s) return FPDE_OK;
     if (s->stream) hipStreamSynchronize(s->stream);
+    if (s->gexec) hipGraphExecDestroy(s->gexec);
     for (void* p : s->allocs) hipFree(p);
     for (auto& e : s->evs) { hipEventDestroy(e.a); hipEventDestroy(e.b); }
     if (s->stream) hipStreamDestroy(s->stream);
@@ -795,6 +799,57 @@ fpde_status fpde_solver_advance(fpde_solver* s, int nsteps) {
         fpde_status rc = finalize_1d(s);
         if (rc != FPDE_OK) return rc;
     }
+    FPDE_HIP_TRY(hipGetLastError());
+    return FPDE_OK;
+}
+
+fpde_status fpde_solver_advance_graph(fpde_solver* s, int nsteps, int steps_per_graph) {
+    if (!s) { set_error("null solver"); return FPDE_ERR_ARG; }
+    if (!s->initialized) { set_error("set_initial must be called first"); return FPDE_ERR_STATE; }
+    if (steps_per_graph < 1 || steps_per_graph > 4096) { set_error("steps_per_graph must be in [1,4096]"); return FPDE_ERR_ARG; }
+    if (s->spec.dims == 1 && s->block_mode) {
+        set_error("graph capture: the cooperative block launches (step_mode 0) are not captured; use step_mode 1");
+        return FPDE_ERR_STATE;
+    }
+    if (s->spec.var_tau) { set_error("graph capture: var_tau solvers step one length at a time"); return FPDE_ERR_STATE; }
+    const int M = s->spec.steps;
+    const int end = (nsteps <= 0) ? M : std::min(M, s->step + nsteps);
+    const int prof = s->prof;
+    s->prof = 0;                                   // no event records inside a capture
+    fpde_status rc = FPDE_OK;
+    while (rc == FPDE_OK && s->step < end) {
+        const int g_end = std::min(end, s->step + steps_per_graph);
+        hipError_t e = hipStreamBeginCapture(s->stream, hipStreamCaptureModeThreadLocal);
+        if (e != hipSuccess) { set_error(std::string("hipStreamBeginCapture: ") + hipGetErrorString(e)); rc = FPDE_ERR_HIP; break; }
+        for (int n = s->step; n < g_end && rc == FPDE_OK; n++) {
+            rc = (s->spec.dims == 1) ? enqueue_1d_step(s, n) : enqueue_3d_step(s, n);
+            if (rc == FPDE_OK) s->step = n + 1;
+        }
+        if (rc == FPDE_OK && s->spec.dims == 1 && s->step == M) rc = finalize_1d(s);
+        hipGraph_t graph = nullptr;
+        e = hipStreamEndCapture(s->stream, &graph);
+        if (rc != FPDE_OK) { if (graph) hipGraphDestroy(graph); break; }
+        if (e != hipSuccess || !graph) { set_error(std::string("hipStreamEndCapture: ") + hipGetErrorString(e)); rc = FPDE_ERR_HIP; break; }
+        // reuse the executable graph when the chunk has the same topology
+        bool updated = false;
+        if (s->gexec) {
+            hipGraphExecUpdateResult res;
+            hipGraphNode_t bad = nullptr;
+            updated = hipGraphExecUpdate(s->gexec, graph, &bad, &res) == hipSuccess && res == hipGraphExecUpdateSuccess;
+            if (!updated) { hipGetLastError(); hipGraphExecDestroy(s->gexec); s->gexec = nullptr; }
+        }
+        if (!updated) {
+            e = hipGraphInstantiate(&s->gexec, graph, nullptr, nullptr, 0);
+            if (e != hipSuccess) { s->gexec = nullptr; hipGraphDestroy(graph); set_error(std::string("hipGraphInstantiate: ") + hipGetErrorString(e)); rc = FPDE_ERR_HIP; break; }
+            s->graph_instantiations++;
+        }
+        e = hipGraphLaunch(s->gexec, s->stream);
+        hipGraphDestroy(graph);
+        if (e != hipSuccess) { set_error(std::string("hipGraphLaunch: ") + hipGetErrorString(e)); rc = FPDE_ERR_HIP; break; }
+        s->graph_launches++;
+    }
+    s->prof = prof;
+    if (rc != FPDE_OK) return rc;
     FPDE_HIP_TRY(hipGetLastError());
     return FPDE_OK;
 }

@@ -97,6 +97,10 @@ class Solver:
     def advance(self, nsteps=0):
         check(lib().fpde_solver_advance(self._h, int(nsteps)))
 
+    def advance_graph(self, nsteps=0, steps_per_graph=64):
+        """advance() through hipGraphs of steps_per_graph steps each."""
+        check(lib().fpde_solver_advance_graph(self._h, int(nsteps), int(steps_per_graph)))
+
     def sync(self):
         check(lib().fpde_solver_sync(self._h))
 

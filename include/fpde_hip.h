@@ -206,6 +206,17 @@ fpde_status fpde_solver_set_initial(fpde_solver *s, const double *u0);
 fpde_status fpde_solver_reset(fpde_solver *s, const double *u0);
 /* Enqueue nsteps steps (or all remaining if nsteps<=0); asynchronous. */
 fpde_status fpde_solver_advance(fpde_solver *s, int nsteps);
+/* The same steps as fpde_solver_advance, enqueued as hipGraphs: every
+ * `steps_per_graph` steps of launches (history kernels, step launches,
+ * space-fractional sums) are stream-captured into one graph, the solver's
+ * executable graph is updated in place when the topology repeats
+ * (hipGraphExecUpdate) and launched.  Results are identical to
+ * fpde_solver_advance.  For solvers whose steps launch per Picard iteration
+ * (1D with step_mode 1 or hist_mode 0, and 3D); the cooperative block launches
+ * of step_mode 0 are not captured (FPDE_ERR_STATE).  This is the graph-captured
+ * step loop of the device layer; the reference launches kernel by kernel
+ * (fr_ocl_call_sum, fpde_hpc_1d.cpp:1819-1883). */
+fpde_status fpde_solver_advance_graph(fpde_solver *s, int nsteps, int steps_per_graph);
 /* var_tau solvers: one step of length tau (Time[n+1] = Time[n] + tau,
  * init_time_step fpde_hpc_1d.cpp:3942-3949), completed before returning to the
  * stream (U^{n+1} and D[n] in memory). */

@@ -832,3 +832,41 @@ def test_space_fractional_var_tau_steps(fpde, oracle):
         u = s.solution()
     p = oracle.make_params(nodes, steps, 1e-4, 0.7, coef_mode=1, taus=taus, gamma=0.6)
     assert rel_err(u, oracle.solve_1d(p, u0)) <= TOL_F64
+
+
+# ------------------------------------------------- graph-captured step loop
+@pytest.mark.parametrize("case", ["plain", "mfma_f64", "split_bf16", "series", "space", "3d"])
+def test_graph_step_loop_equals_eager(fpde, case):
+    """fpde_solver_advance_graph replays captured chunks of steps (the
+    executable graph updated in place between chunks); every replay is
+    checked against the eager launches, with eager work (the finalize of
+    the solution read) and synchronisation between replays."""
+    kw = dict(alpha=0.75, picard_max=3, picard_tol=1e-12, block=32, step_mode=1)
+    if case == "3d":
+        n, M = 20, 40
+        u0 = fpde.init_3d(n, 5)
+        spec = fpde.make_spec(n, M, 1e-3, dims=3, alpha=0.85, hist_mode="mfma_f64", block=16, u_top=u0[0])
+    else:
+        n, M = 900, 100
+        u0 = fpde.init_1d(n, 6)
+        mode = {"series": "plain", "space": "mfma_f64"}.get(case, case)
+        extra = {"series": dict(sum_alg=2, sum_param=12), "space": dict(gamma=0.7)}.get(case, {})
+        spec = fpde.make_spec(n, M, 1e-3, hist_mode=mode, u_top=u0[0], **kw, **extra)
+    with fpde.Solver(spec) as a, fpde.Solver(spec) as b:
+        a.set_initial(u0)
+        b.set_initial(u0)
+        for chunk in (20, 20, 13, 27, 20):
+            a.advance(chunk)
+            b.advance_graph(chunk, steps_per_graph=8)
+            ua, ub = a.solution(), b.solution()
+            assert a.step == b.step
+            assert np.array_equal(ua, ub), f"{case}: replay differs at step {a.step}: {rel_err(ub, ua):.3e}"
+        assert a.step == M
+
+
+def test_graph_step_loop_refuses_cooperative_solver(fpde):
+    spec = fpde.make_spec(600, 40, 1e-3, hist_mode="split_bf16", block=32, step_mode=0, u_top=-1.0)
+    with fpde.Solver(spec) as s:
+        s.set_initial(fpde.init_1d(600, 2))
+        rc = fpde.lib().fpde_solver_advance_graph(s._h, 10, 8)
+        assert rc == 5 and b"cooperative" in fpde.lib().fpde_last_error()
