@@ -260,9 +260,13 @@ bool step1d_block_supported(int C);
 fpde_status launch_coef3d(const double* U, int n, double h, double ksat, int mode, double* K, double* iC,
                           hipStream_t st);
 fpde_status launch_lateral3d(double* U, const double* K, const double* iC, const double* H, int n, int axis,
-                             double h2inv, double L, double* Al, double* Bt, hipStream_t st);
+                             double h2inv, double L, double* Al, double* Bt, double kap, const double* E,
+                             hipStream_t st);
 fpde_status launch_zline3d(double* U, double* Uprev, const double* utop, int n, double h, double h2inv,
-                           double ksat, int mode, double L, double* Dlev, hipStream_t st);
+                           double ksat, int mode, double L, double* Dlev, double kap, const double* E,
+                           hipStream_t st);
+fpde_status launch_space3d(const double* U, const double* K, int n, int axis, double h, const double* sp_omega,
+                           const double* sp_corr, double* E, hipStream_t st);
 fpde_status launch_series3d(double* S, double* H, int P, const double* dnew, const double* ser_c,
                             const double* ser_v, int64_t nodes, int64_t ld, hipStream_t st);
 fpde_status launch_hist_combine(double* S, const double* part, int nsplit, int64_t part_stride, int64_t part_off,

@@ -11,6 +11,8 @@
 //    ends (Al[1]=1, Bt[1]=0; bcN()=0), coefficients precomputed per node;
 //  - z lines: one workgroup per line, one node per thread, PCR in LDS,
 //    Dirichlet top / zero-flux bottom; it also writes D[n] = U^{n+1}-U^n.
+// With the space-fractional scheme every sweep also takes kap and the line
+// sums E of its own axis (space.hip, k_space3d).
 #include "fpde_internal.h"
 
 namespace fpde {
@@ -34,11 +36,15 @@ __global__ __launch_bounds__(256) void k_coef3d(const double* __restrict__ U, in
 // fetched 16 rows at a time (independent loads) ahead of the recurrence, and
 // the back substitution fetches Al/Bt the same way, so a line costs ~n/16
 // memory round trips instead of n.
+// Space-fractional scheme (E != nullptr): the diffusion part of the rows is
+// scaled by kap = Dg kb(gamma,i-1,i,i) and Om carries -inv_dw_dh E (E: the
+// line sums of k_space3d), as in the 1D rows.
 __global__ __launch_bounds__(256) void k_lateral3d(double* __restrict__ U, const double* __restrict__ K,
                                                    const double* __restrict__ iC,
                                                    const double* __restrict__ H, int n, int axis,
                                                    double h2inv, double L, double* __restrict__ Al,
-                                                   double* __restrict__ Bt) {
+                                                   double* __restrict__ Bt, double kap,
+                                                   const double* __restrict__ E) {
     constexpr int R = 16;
     int line = blockIdx.x * 256 + threadIdx.x;
     int nlines = n * n;
@@ -54,7 +60,7 @@ __global__ __launch_bounds__(256) void k_lateral3d(double* __restrict__ U, const
     Bt[(int64_t)1 * nlines + line] = bt;
     double Kprev = K[base];
     for (int i0 = 1; i0 < N; i0 += R) {
-        double kk[R], ic[R], uu[R], hh[R];
+        double kk[R], ic[R], uu[R], hh[R], ee[R];
 #pragma unroll
         for (int j = 0; j < R; j++) {
             int i = i0 + j;
@@ -64,16 +70,18 @@ __global__ __launch_bounds__(256) void k_lateral3d(double* __restrict__ U, const
             ic[j] = ok ? iC[id] : 0.0;
             uu[j] = ok ? U[id] : 0.0;
             hh[j] = (ok && H) ? H[id] : 0.0;
+            ee[j] = (ok && E) ? E[id] : 0.0;
         }
 #pragma unroll
         for (int j = 0; j < R; j++) {
             int i = i0 + j;
             if (i < N) {
                 double Km = Kprev, Kp = kk[j];
-                double A_ = Km * ic[j] * h2inv;
-                double B_ = Kp * ic[j] * h2inv;
-                double R_ = ic[j] * ((Kp * h2inv) + (Km * h2inv)) + L;
+                double A_ = Km * ic[j] * h2inv * kap;
+                double B_ = Kp * ic[j] * h2inv * kap;
+                double R_ = ic[j] * ((Kp * h2inv) + (Km * h2inv)) * kap + L;
                 double Om = -L * uu[j] + hh[j];
+                if (E) Om -= ic[j] * ee[j];
                 double aln = B_ / (R_ - A_ * al);
                 bt = (aln / B_) * (A_ * bt - Om);
                 al = aln;
@@ -116,7 +124,8 @@ __global__ __launch_bounds__(256) void k_lateral3d(double* __restrict__ U, const
 __global__ __launch_bounds__(256) void k_zline3d(double* __restrict__ U, double* __restrict__ Uprev,
                                                  const double* __restrict__ utop, int n, double h,
                                                  double h2inv, double ksat, int mode, double L,
-                                                 double* __restrict__ Dlev) {
+                                                 double* __restrict__ Dlev, double kap,
+                                                 const double* __restrict__ E) {
     __shared__ double a[2][256], c[2][256], d[2][256], sK[256];
     const int tid = threadIdx.x;
     const int line = blockIdx.x;                   // x + n*y
@@ -137,10 +146,11 @@ __global__ __launch_bounds__(256) void k_zline3d(double* __restrict__ U, double*
         else if (tid == n - 1) { ar = -1.0; br = 1.0; cr = 0.0; dr = 0.0; }
         else {
             double Km = sK[tid - 1], Kp = k;
-            ar = Km * ic * h2inv;
-            cr = Kp * ic * h2inv;
-            br = -(ic * ((Kp * h2inv) + (Km * h2inv)) + L);
+            ar = Km * ic * h2inv * kap;
+            cr = Kp * ic * h2inv * kap;
+            br = -(ic * ((Kp * h2inv) + (Km * h2inv)) * kap + L);
             dr = -L * u;
+            if (E) dr -= ic * E[base + tid];
         }
         double ib = 1.0 / br;
         a[0][tid] = ar * ib; c[0][tid] = cr * ib; d[0][tid] = dr * ib;
@@ -237,17 +247,21 @@ fpde_status launch_coef3d(const double* U, int n, double h, double ksat, int mod
 }
 
 fpde_status launch_lateral3d(double* U, const double* K, const double* iC, const double* H, int n,
-                             int axis, double h2inv, double L, double* Al, double* Bt, hipStream_t st) {
+                             int axis, double h2inv, double L, double* Al, double* Bt, double kap,
+                             const double* E, hipStream_t st) {
     int nl = n * n;
-    hipLaunchKernelGGL(k_lateral3d, dim3((nl + 255) / 256), dim3(256), 0, st, U, K, iC, H, n, axis, h2inv, L, Al, Bt);
+    hipLaunchKernelGGL(k_lateral3d, dim3((nl + 255) / 256), dim3(256), 0, st, U, K, iC, H, n, axis, h2inv, L, Al, Bt,
+                       kap, E);
     return FPDE_OK;
 }
 
 fpde_status launch_zline3d(double* U, double* Uprev, const double* utop, int n, double h, double h2inv,
-                           double ksat, int mode, double L, double* Dlev, hipStream_t st) {
+                           double ksat, int mode, double L, double* Dlev, double kap, const double* E,
+                           hipStream_t st) {
     if (n > 256) { set_error("3D grid: at most 256 nodes per axis"); return FPDE_ERR_ARG; }
     int bs = ((n + 63) / 64) * 64;
-    hipLaunchKernelGGL(k_zline3d, dim3(n * n), dim3(bs), 0, st, U, Uprev, utop, n, h, h2inv, ksat, mode, L, Dlev);
+    hipLaunchKernelGGL(k_zline3d, dim3(n * n), dim3(bs), 0, st, U, Uprev, utop, n, h, h2inv, ksat, mode, L, Dlev,
+                       kap, E);
     return FPDE_OK;
 }
 

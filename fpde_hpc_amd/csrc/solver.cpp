@@ -49,10 +49,7 @@ fpde_status validate(const fpde_spec& s) {
     }
     if (s.step_mode != 0 && s.step_mode != 1) { set_error("step_mode must be 0 or 1"); return FPDE_ERR_ARG; }
     if (!(s.gamma >= 0.0 && s.gamma <= 1.0)) { set_error("gamma must be in [0,1] (0 or 1: classical)"); return FPDE_ERR_ARG; }
-    if (s.gamma > 0.0 && s.gamma < 1.0 && s.dims != 1) {
-        set_error("the space-fractional scheme (gamma in (0,1)) is built for 1D solves");
-        return FPDE_ERR_ARG;
-    }
+
     if (s.dims == 3 && s.picard_max != 1) { set_error("3D LOD uses lagged coefficients (picard_max must be 1)"); return FPDE_ERR_ARG; }
     if (s.sum_alg < 0 || s.sum_alg > 2) {
         set_error("sum_alg must be 0 (full sum), 1 (restricted sum) or 2 (series moments)");
@@ -119,9 +116,10 @@ size_t footprint_bytes(const fpde_spec& s) {
         b += sizeof(double) * P * p.ld;                              // moments
         b += sizeof(double) * 2 * P * (size_t)s.steps;               // coefficient tables
     }
-    if (s.dims == 1 && s.gamma > 0.0 && s.gamma < 1.0) {             // space-fractional sum
+    if (s.gamma > 0.0 && s.gamma < 1.0) {                            // space-fractional sum
+        b += sizeof(double) * (3 * (size_t)s.nodes + p.ld);
         size_t ldp = (size_t)round_up(s.nodes, kSpTile);
-        b += sizeof(double) * (3 * (size_t)s.nodes + ldp * (1 + space_chunks((int)(ldp / kSpTile) - 1)) + p.ld);
+        if (s.dims == 1) b += sizeof(double) * ldp * (1 + space_chunks((int)(ldp / kSpTile) - 1));
     }
     if (s.taus || s.var_tau) {                                       // grid, weight rows
         size_t M = (size_t)s.steps, B = (size_t)s.block;
@@ -340,6 +338,11 @@ fpde_status solver_alloc(fpde_solver* s) {
         DALLOC(s->flags, sizeof(int) * 16);
     } else {
         int n = sp.nodes;
+        if (sp.gamma > 0.0 && sp.gamma < 1.0) {
+            DALLOC(s->w.sp_omega, sizeof(double) * (size_t)n);
+            DALLOC(s->w.sp_corr, sizeof(double) * 2 * (size_t)n);
+            DALLOC(s->spE, sizeof(double) * p.ld);
+        }
         DALLOC(s->Uprev, sizeof(double) * p.ld);
         DALLOC(s->utop, sizeof(double) * (size_t)n * n);
         DALLOC(s->K, sizeof(double) * p.ld);
@@ -350,10 +353,8 @@ fpde_status solver_alloc(fpde_solver* s) {
     s->h = sp.length / (sp.nodes - 1);
     fpde_status rc = launch_weights(sp, s->w, s->stream);
     if (rc != FPDE_OK) return rc;
-    if (sp.dims == 1) {
-        rc = launch_space_weights(sp, s->w, s->h, s->stream);
-        if (rc != FPDE_OK) return rc;
-    }
+    rc = launch_space_weights(sp, s->w, s->h, s->stream);        // along every line of a 3D sweep
+    if (rc != FPDE_OK) return rc;
     FPDE_HIP_TRY(hipStreamSynchronize(s->stream));
     return FPDE_OK;
 }
@@ -676,14 +677,29 @@ fpde_status enqueue_3d_step(fpde_solver* s, int n) {
     ev_end(s, slot);
     if (rc != FPDE_OK) return rc;
     ev_begin(s, 1, &slot);
+    // space-fractional scheme: the line sums of each sweep from the field it
+    // starts from (K of that field: coef3d precedes every sweep then)
+    const bool space = s->w.sp_on;
+    const double kap = space ? s->w.sp_kap : 1.0;
+    double* E = space ? s->spE : nullptr;
+    auto space_sums = [&](int axis) {
+        return space ? launch_space3d(s->U, s->K, nn, axis, s->h, s->w.sp_omega, s->w.sp_corr, s->spE, s->stream)
+                     : FPDE_OK;
+    };
     if ((rc = launch_coef3d(s->U, nn, s->h, sp.ksat, sp.coef_mode, s->K, s->iC, s->stream)) != FPDE_OK) return rc;
-    if ((rc = launch_lateral3d(s->U, s->K, s->iC, s->S, nn, 2, h2inv, s->w.local, s->Al, s->Bt, s->stream)) != FPDE_OK) return rc;
+    if ((rc = space_sums(2)) != FPDE_OK) return rc;
+    if ((rc = launch_lateral3d(s->U, s->K, s->iC, s->S, nn, 2, h2inv, s->w.local, s->Al, s->Bt, kap, E, s->stream)) != FPDE_OK) return rc;
     if ((rc = launch_coef3d(s->U, nn, s->h, sp.ksat, sp.coef_mode, s->K, s->iC, s->stream)) != FPDE_OK) return rc;
-    if ((rc = launch_lateral3d(s->U, s->K, s->iC, nullptr, nn, 1, h2inv, s->w.local, s->Al, s->Bt, s->stream)) != FPDE_OK) return rc;
+    if ((rc = space_sums(1)) != FPDE_OK) return rc;
+    if ((rc = launch_lateral3d(s->U, s->K, s->iC, nullptr, nn, 1, h2inv, s->w.local, s->Al, s->Bt, kap, E, s->stream)) != FPDE_OK) return rc;
+    if (space) {
+        if ((rc = launch_coef3d(s->U, nn, s->h, sp.ksat, sp.coef_mode, s->K, s->iC, s->stream)) != FPDE_OK) return rc;
+        if ((rc = space_sums(0)) != FPDE_OK) return rc;
+    }
     if ((rc = launch_zline3d(s->U, s->Uprev, s->utop, nn, s->h, h2inv, sp.ksat, sp.coef_mode, s->w.local,
-                             s->D + (size_t)(series ? (n & 1) : n) * p.ld, s->stream)) != FPDE_OK) return rc;
+                             s->D + (size_t)(series ? (n & 1) : n) * p.ld, kap, E, s->stream)) != FPDE_OK) return rc;
     ev_end(s, slot);
-    s->stats.step_launches += 5;
+    s->stats.step_launches += space ? 9 : 5;
     s->stats.picard_iters_total++;
     return FPDE_OK;
 }

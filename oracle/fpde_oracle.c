@@ -631,6 +631,17 @@ int orc_solve_3d(const orc_params *p, double *u)
     for (int y = 0; y < n; y++)
         for (int x = 0; x < n; x++) utop[y * n + x] = u[(size_t)n * (x + (size_t)n * y)];
     memcpy(old, u, sizeof(double) * nn);
+    /* space-fractional scheme: every sweep applies the 1D operator along its
+     * own axis (the y/x/z sweeps switch Z, dL, N to that axis, :5262-5265) */
+    double *Tsp = NULL, kap[1024];
+    if (space_on(p)) {
+        double zl[1024];
+        for (int i = 0; i < n; i++) zl[i] = i * h;
+        Tsp = space_table(n, h, p->gamma, p->weight_mode, p->series_eps);
+        if (!Tsp) return -1;
+        double Dg = 1.0 / tgamma(1.0 - p->gamma);
+        for (int i = 0; i < n; i++) kap[i] = (i >= 1) ? Dg * orc_kb(p->gamma, i - 1, i, zl, p->weight_mode, p->series_eps) : 1.0;
+    }
     for (int s = 0; s < M; s++) {
         double L;
         history_term(&w, p, s, (int)nn, old, H, tmp, &L);
@@ -654,15 +665,21 @@ int orc_solve_3d(const orc_params *p, double *u)
                 }
                 memcpy(luit, lun, sizeof(double) * n);
                 double top_val = (axis == 0) ? utop[a1 * n + a0] : 0.0;
+                double lF[1024], lV[1024];
+                if (Tsp) {
+                    for (int i = 0; i < n; i++) lK[i] = c_K(&cc, lun[i], lz[i]);
+                    orc_space_flux(n, h, lK, lun, lF);
+                    space_apply(n, Tsp, lF, lV);
+                }
                 line_solve(n, lun, luit, lH, lz, h, L, axis == 0 ? 0 : 1, top_val, &cc, lAl, lBt, lK, liC, lnew,
-                           NULL, NULL);
+                           Tsp ? kap : NULL, Tsp ? lV : NULL);
                 for (int i = 0; i < n; i++) cur[base + stride * i] = lnew[i];
             }
         }
         memcpy(old + (size_t)(s + 1) * nn, cur, sizeof(double) * nn);
     }
     memcpy(u, old + (size_t)M * nn, sizeof(double) * nn);
-    free(old); free(H); free(tmp); free(cur); free(utop);
+    free(old); free(H); free(tmp); free(cur); free(utop); free(Tsp);
     wctx_free(&w);
     return 0;
 }

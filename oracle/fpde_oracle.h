@@ -98,8 +98,9 @@ typedef struct {
     double sum_param;    /* sum_alg 1: threshold on |kb|; sum_alg 2: series terms P  */
     const double *taus;  /* NULL: uniform tau; else the M step lengths tau_0..tau_{M-1}
                             (Time[k+1] = Time[k] + tau_k, init_time_step :3942-3949) */
-    double gamma;        /* space-fractional order (1D): 0 or 1 = classical diffusion,
-                            (0,1) = the explicit one-sided scheme (calc_v :2303-2335) */
+    double gamma;        /* space-fractional order: 0 or 1 = classical diffusion,
+                            (0,1) = the explicit one-sided scheme (calc_v :2303-2335),
+                            in 3D along every sweep line */
 } orc_params;
 
 /* 1D solve: u (nodes) holds u0 on entry, u^M on exit.  u_top = u0[0]. */

@@ -54,7 +54,7 @@ def test_ksat_fourier_matches_oracle(fpde, oracle):
     dict(sum_alg=2, sum_param=0.0, hist_mode=0), dict(sum_alg=2, sum_param=65.0, hist_mode=0),
     dict(sum_alg=2, sum_param=2.5, hist_mode=0), dict(sum_alg=2, sum_param=8.0, hist_mode=1),
     dict(sum_alg=2, sum_param=8.0, hist_mode=0, do_q=4),
-    dict(gamma=1.5), dict(gamma=-0.1), dict(dims=3, nodes=16, gamma=0.5),
+    dict(gamma=1.5), dict(gamma=-0.1),
 ])
 def test_invalid_specs_rejected(fpde, bad):
     kw = dict(nodes=100, steps=10, tau=1e-3, alpha=0.5, hist_mode=1, block=64)

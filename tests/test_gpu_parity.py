@@ -870,3 +870,30 @@ def test_graph_step_loop_refuses_cooperative_solver(fpde):
         s.set_initial(fpde.init_1d(600, 2))
         rc = fpde.lib().fpde_solver_advance_graph(s._h, 10, 8)
         assert rc == 5 and b"cooperative" in fpde.lib().fpde_last_error()
+
+
+def _oracle_space3d(oracle, spec, u0):
+    p = oracle.make_params(spec.nodes, spec.steps, spec.tau, spec.alpha, ksat=spec.ksat, coef_mode=spec.coef_mode,
+                           weight_mode=spec.weight_mode, series_eps=spec.series_eps, gamma=spec.gamma)
+    return oracle.solve_3d(p, u0)
+
+
+@pytest.mark.parametrize("n,mode,tol", [(17, "mfma_f64", TOL_F64), (40, "plain", TOL_F64), (64, "mfma_f64", TOL_F64),
+                                        (40, "split_bf16", TOL_MIXED)])
+@pytest.mark.parametrize("coef", [0, 1])
+def test_space_fractional_3d_matches_oracle(fpde, oracle, n, mode, tol, coef):
+    """Every LOD sweep with the line sums of its own axis (16-line groups,
+    partial groups and row tiles for n not a multiple of 16)."""
+    steps = 12
+    u0 = fpde.init_3d(n, 7 + n)
+    spec = fpde.make_spec(n, steps, 1e-3, dims=3, alpha=0.85, hist_mode=mode, block=16, coef_mode=coef,
+                          u_top=u0[0], gamma=0.7, ksat=1.0 if coef else None)
+    u = fpde.solve(spec, u0)
+    ref = _oracle_space3d(oracle, spec, u0)
+    spec0 = fpde.make_spec(n, steps, 1e-3, dims=3, alpha=0.85, hist_mode=mode, block=16, coef_mode=coef,
+                           u_top=u0[0], ksat=spec.ksat)
+    classical = _oracle_space3d(oracle, spec0, u0)
+    err, eff = rel_err(u, ref), rel_err(u - classical, ref - classical)
+    print(f"3D space-fractional n={n} {mode} coef {coef}: rel err {err:.3e}, effect err {eff:.3e}")
+    assert err <= tol
+    assert eff <= (1e-6 if tol == TOL_F64 else 1e-3)

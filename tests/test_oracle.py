@@ -322,16 +322,21 @@ def test_vgm_step_satisfies_discrete_equation(oracle):
     assert abs(u1[-1] - u1[-2]) <= 1e-12
 
 
-def test_3d_with_laterally_uniform_field_equals_1d(oracle):
+@pytest.mark.parametrize("gam", [0.0, 0.7])
+def test_3d_with_laterally_uniform_field_equals_1d(oracle, gam):
     """With field-independent coefficients a laterally uniform 3D field
     follows the 1D scheme along depth (the y sweep adds the history term, the
-    x sweep is an identity).  With VGM coefficients the two differ at O(H/L)
-    because the depth sweep evaluates its coefficients after the y sweep."""
-    n, steps = 10, 12
+    x sweep is an identity; the space-fractional sum of a lateral line of a
+    uniform field is 0).  With VGM coefficients the two differ at O(H/L)
+    because the depth sweep evaluates its coefficients after the y sweep; the
+    space-fractional depth sweep likewise forms its explicit sum from the
+    field after the y sweep (U^n + H/L), so with gamma < 1 the reduction is
+    exact at the first step, where H = 0."""
+    n, steps = 10, (12 if gam == 0.0 else 1)
     col = oracle.init_1d(n, 7)
     u3 = np.tile(col, n * n)            # z fastest
-    p3 = oracle.make_params(n, steps, 1e-3, 0.85, coef_mode=1)
-    p1 = oracle.make_params(n, steps, 1e-3, 0.85, coef_mode=1)
+    p3 = oracle.make_params(n, steps, 1e-3, 0.85, coef_mode=1, gamma=gam)
+    p1 = oracle.make_params(n, steps, 1e-3, 0.85, coef_mode=1, gamma=gam)
     r3 = oracle.solve_3d(p3, u3).reshape(n * n, n)
     r1 = oracle.solve_1d(p1, col)
     assert np.max(np.abs(r3 - r1[None, :])) <= 1e-12
@@ -469,3 +474,21 @@ def test_space_fractional_tends_to_classical_as_gamma_to_one(oracle):
     d_far = np.max(np.abs(far - base)) / np.max(np.abs(base))
     assert d_near <= 1e-7
     assert d_far >= 1e3 * d_near
+
+
+def test_3d_space_fractional_lateral_sweeps_are_symmetric(oracle):
+    """One step, linear coefficients: a field varying along x only and the
+    same profile along y only give transposed results (the x and y sweeps
+    apply the same line operator; the history term is 0 at step 0), and the
+    space-fractional sum changes both."""
+    n = 12
+    prof = np.cos(np.arange(n) * 0.5) + 0.2 * np.sin(np.arange(n) * 1.7)
+    z, x, y = np.meshgrid(np.arange(n), np.arange(n), np.arange(n), indexing="ij")
+    fx = prof[x].transpose(2, 1, 0).ravel()      # idx = z + n (x + n y)
+    fy = prof[y].transpose(2, 1, 0).ravel()
+    p = oracle.make_params(n, 1, 1e-3, 0.85, coef_mode=1, gamma=0.7)
+    rx = oracle.solve_3d(p, fx).reshape(n, n, n)  # [y][x][z]
+    ry = oracle.solve_3d(p, fy).reshape(n, n, n)
+    assert np.max(np.abs(rx - ry.transpose(1, 0, 2))) <= 1e-13
+    r0 = oracle.solve_3d(oracle.make_params(n, 1, 1e-3, 0.85, coef_mode=1), fx)
+    assert np.max(np.abs(rx.ravel() - r0)) >= 1e-4
