@@ -32,9 +32,9 @@ VARIANT_DTYPE = {"plain": "f64", "mfma_f64": "f64", "split_bf16": "bf16x2->f32->
 
 
 def run_variant(F, workload, mode, block, steps, warmup, prof_mask_timed=None, nodes=None, msteps=None,
-                picard=(1, 0.0)):
+                picard=(1, 0.0), gamma=0.0):
     kw = {} if workload == "3d" else dict(picard_max=picard[0], picard_tol=picard[1])
-    spec, u0 = F.workload(workload, hist_mode=mode, block=block, nodes=nodes, steps=msteps, **kw)
+    spec, u0 = F.workload(workload, hist_mode=mode, block=block, nodes=nodes, steps=msteps, gamma=gamma, **kw)
     s = F.Solver(spec)
     s.set_initial(u0)
     # profiled warmup: per-class device times with hipEvents on the solver stream
@@ -182,6 +182,8 @@ def main():
     ap.add_argument("--requests", type=int, default=512, help="serving mix size (--workload serve)")
     ap.add_argument("--rate", type=float, default=16.0, help="serving arrival rate, requests/s")
     ap.add_argument("--max-active", type=int, default=16)
+    ap.add_argument("--gamma", type=float, default=0.0,
+                    help="space-fractional order in (0,1) (0: classical diffusion)")
     ap.add_argument("--picard", type=int, default=3,
                     help="Picard iterations per step on the nonlinear coefficient (1D; 3D LOD uses 1)")
     ap.add_argument("--picard-tol", type=float, default=1e-12,
@@ -267,13 +269,14 @@ def main():
     for mode in order:
         K = args.steps if mode == args.headline else 1
         r = run_variant(F, args.workload, mode, args.block, K, args.warmup, prof_mask_timed=None,
-                        nodes=args.nodes, msteps=args.msteps, picard=picard)
+                        nodes=args.nodes, msteps=args.msteps, picard=picard, gamma=args.gamma)
         # decide the dominant kernel class from the profiled warmup, then time
         # it live over the timed region with events on the solver stream
         prof = r["prof"]
         dom = 0 if prof["ms_history"] >= prof["ms_step"] else 1
         r2 = run_variant(F, args.workload, mode, args.block, K, 0, prof_mask_timed=1 << dom,
-                         nodes=args.nodes, msteps=args.msteps, picard=picard) if mode == args.headline else None
+                         nodes=args.nodes, msteps=args.msteps, picard=picard,
+                         gamma=args.gamma) if mode == args.headline else None
         r["dominant"] = dom
         r["timed_live"] = r2
         results[mode] = r
@@ -357,7 +360,9 @@ def main():
             roof["traffic_source"] = "profiles/hbm_counters.json (rocprofv3 --pmc, copy-calibrated)"
     except (OSError, ValueError, KeyError):
         pass
-    cpu = None if (args.no_cpu or spec.dims != 1 or spec.do_q) else cpu_baseline(
+    # the CPU baseline samples the classical scheme (the oracle's spatial sum
+    # is a serial O(N^2) loop per step and is not part of the history path)
+    cpu = None if (args.no_cpu or spec.dims != 1 or spec.do_q or spec.gamma) else cpu_baseline(
         N, M, alpha=spec.alpha, tau=spec.tau, picard=(spec.picard_max, spec.picard_tol))
     out = {
         "metric": "node*steps/s, mixed-MFMA history-sum solver (vs FP64 GPU at <=1e-8 rel err)",
@@ -377,9 +382,10 @@ def main():
                                 + (f"distributed order q={spec.do_q} on [{spec.do_a0},{spec.do_a1}]" if spec.do_q
                                    else f"alpha={spec.alpha}") + f"; history {args.headline} block {args.block}"
                                 + (f"; Picard <= {spec.picard_max} iterations, tol {spec.picard_tol:g}"
-                                   if spec.dims == 1 else "; LOD with lagged coefficients")),
+                                   if spec.dims == 1 else "; LOD with lagged coefficients")
+                                + (f"; space-fractional explicit scheme gamma={spec.gamma:g}" if spec.gamma else "")),
                    "nodes": N, "time_steps": M, "history_mode": args.headline, "block": args.block,
-                   "picard_max": spec.picard_max, "picard_tol": spec.picard_tol,
+                   "picard_max": spec.picard_max, "picard_tol": spec.picard_tol, "gamma": spec.gamma,
                    "parallelism": "single GPU"},
         "roofline": roof,
         "cpu_baseline": cpu,
