@@ -152,6 +152,12 @@ enum HistMode : int {
 constexpr int kGroup = 32;     // history levels per packed group (K of one 16x16x32 MFMA)
 
 
+// 1D chunks of the partitioned solve (256 nodes each, N <= 65536) and the
+// Picard barrier flags of the cooperative launches: one 128-B flag per chunk
+// and launch parity
+constexpr int kMaxChunks = 256;
+constexpr size_t kFlagBarBytes = 2 * (size_t)kMaxChunks * 32 * sizeof(unsigned);
+
 // Arguments of one 1D step launch (step1d.hip)
 struct Step1DArgs {
     int N, C;
@@ -187,7 +193,7 @@ struct Step1DArgs {
     int n_end;
     int picard_max;
     int n0;                   // first step of the history block (far partial row = n - n0)
-    unsigned* bar;            // grid barrier counters, one per launch parity (bar[0], bar[16])
+    unsigned* flagbar;        // Picard barrier flags [2 parities][kMaxChunks][32] (128 B per flag)
     int epar;                 // parity of this cooperative launch
     int* err;                 // set when a grid barrier times out
     // series-compressed history (ser_P > 0): moments H [P][ld], coefficient
@@ -201,9 +207,9 @@ struct Step1DArgs {
     // inv_dw_dh E_i; E of step n is kept in sp_E by the launch that forms it
     int sp_on;
     double sp_kap;
-    const double* sp_P;       // [chunks][sp_ldp] partial sums of the Toeplitz part
+    double* sp_P;             // [chunks][sp_ldp] partial sums of the Toeplitz part
     int64_t sp_ldp;
-    const double* sp_F;       // F of U^n
+    double* sp_F;             // F of U^n
     const double* sp_corr;
     double* sp_E;
 };
@@ -216,6 +222,68 @@ struct Step1DArgs {
 constexpr int kSpTile = 256;
 constexpr int kSpDiag = 64;
 __host__ __device__ inline int space_chunks(int t) { return (16 * (t + 1) + kSpDiag - 1) / kSpDiag; }
+
+typedef double sd4_t __attribute__((ext_vector_type(4)));
+
+// LDS of one work unit: the F blocks J in [I0 - d_hi + 1, I0 + 15 - d_lo]
+// with a stride of 17 per block (the B reads of one 16-lane row fall into
+// distinct banks), the weights w(m), m in [16 d_lo - 15, 16 d_hi - 1], and
+// the four waves' results.
+struct SpaceUnitLds {
+    double Fs[(kSpDiag + 15) * 17];
+    double Ws[16 * kSpDiag + 15];
+    double red[4][4][64];
+};
+
+// One work unit of the spatial sum: P[y][i] = sum over the block diagonals d
+// in [kSpDiag y, kSpDiag y + kSpDiag) of row tile t (rows i = 256 t + 16 I'
+// + p) of sum_q w(16 d + p - q) F[16 (I0 + I' - d) + q], I0 = 16 t, with
+// w(m) = om[m-1] for 1 <= m <= N, else 0 (om: the Toeplitz lag vector).
+// Wave w takes 16 of the diagonals; the four waves' results are added
+// through LDS.  v_mfma_f64_16x16x4f64 layout (as in the history GEMM): A
+// lane l = A[l%16][l/16], B lane l = B[l/16][l%16], result v of lane l =
+// D[4v + l/16][l%16]; M index p, N index I', K index q.  Called by all 256
+// threads of a workgroup (it synchronises the workgroup).
+__device__ inline void space_unit(const double* __restrict__ F, int N, const double* __restrict__ om,
+                                  double* __restrict__ P, int64_t ldp, int t, int y, SpaceUnitLds& L) {
+    const int tid = threadIdx.x;
+    const int I0 = 16 * t;
+    const int d_lo = kSpDiag * y;
+    const int d_hi = min(d_lo + kSpDiag, I0 + 16);
+    const int Jb = I0 - d_hi + 1;
+    const int nfb = d_hi - d_lo + 15;
+    const int mb = 16 * d_lo - 15;
+    const int nw = 16 * (d_hi - d_lo) + 15;
+    __syncthreads();                                   // LDS free (a previous unit may still read it)
+    for (int q = tid; q < nfb * 16; q += 256) {
+        const int b = q >> 4, c = q & 15;
+        const int64_t j = (int64_t)(Jb + b) * 16 + c;
+        L.Fs[b * 17 + c] = (j >= 0 && j < ldp) ? F[j] : 0.0;
+    }
+    for (int q = tid; q < nw; q += 256) {
+        const int m = mb + q;
+        L.Ws[q] = (m >= 1 && m <= N) ? om[m - 1] : 0.0;
+    }
+    __syncthreads();
+    const int lane = tid & 63, wave = tid >> 6;
+    const int r16 = lane & 15, k4 = lane >> 4;
+    sd4_t acc = {0.0, 0.0, 0.0, 0.0};
+    const int dw_lo = d_lo + 16 * wave, dw_hi = min(dw_lo + 16, d_hi);
+    for (int d = dw_lo; d < dw_hi; d++) {
+        const int fb = (r16 + d_hi - 1 - d) * 17 + k4;   // F block I0 + I' - d, element 4 kq + k4
+        const int wb = 16 * d + r16 - k4 - mb;           // w(16 d + p - q), q = 4 kq + k4
+#pragma unroll
+        for (int kq = 0; kq < 4; kq++)
+            acc = __builtin_amdgcn_mfma_f64_16x16x4f64(L.Ws[wb - 4 * kq], L.Fs[fb + 4 * kq], acc, 0, 0, 0);
+    }
+#pragma unroll
+    for (int v = 0; v < 4; v++) L.red[wave][v][lane] = acc[v];
+    __syncthreads();
+    const int v = tid >> 6, l = tid & 63;
+    const double sum = (L.red[0][v][l] + L.red[1][v][l]) + (L.red[2][v][l] + L.red[3][v][l]);
+    const int64_t i = (int64_t)kSpTile * t + 16 * (l & 15) + 4 * v + (l >> 4);
+    P[(int64_t)y * ldp + i] = sum;
+}
 
 // E_i of an interior row i (1 <= i <= N-2) from the partials and the two
 // surface-interval corrections

@@ -164,7 +164,7 @@ struct fpde_solver {
     double* spP = nullptr;
     double* spE = nullptr;
     int64_t sp_ldp = 0;
-    unsigned* bar = nullptr;   // grid barrier of the cooperative 1D block launches
+    unsigned* flagbar = nullptr; // grid-barrier flags of the cooperative 1D block launches
     int* err = nullptr;        // barrier time-out flag
     bool block_mode = false;   // 1D steps run as cooperative block launches
     int64_t coop_launches = 0; // selects the barrier counter of the next one
@@ -314,9 +314,9 @@ fpde_status solver_alloc(fpde_solver* s) {
         DALLOC(s->Uit, sizeof(double) * p.ld);
         DALLOC(s->Uit0, sizeof(double) * p.ld);
         DALLOC(s->uend, sizeof(double) * 2 * 2 * s->C);
-        DALLOC(s->bar, 128);
+        DALLOC(s->flagbar, kFlagBarBytes);
+        FPDE_HIP_TRY(hipMemsetAsync(s->flagbar, 0, kFlagBarBytes, s->stream));
         DALLOC(s->err, 64);
-        FPDE_HIP_TRY(hipMemsetAsync(s->bar, 0, 128, s->stream));
         FPDE_HIP_TRY(hipMemsetAsync(s->err, 0, 64, s->stream));
         // cooperative block launches pay off where a launch covers many steps:
         // the GEMM history forms (one launch per block) and the series form;
@@ -331,8 +331,9 @@ fpde_status solver_alloc(fpde_solver* s) {
             DALLOC(s->spP, sizeof(double) * (size_t)s->sp_ldp * space_chunks((int)(s->sp_ldp / kSpTile) - 1));
             DALLOC(s->spE, sizeof(double) * p.ld);
         }
-        // the space-fractional sum needs the whole U^n between two steps: the
-        // per-iteration launches leave that point free for its kernels
+        // the space-fractional sum needs the whole U^n between two steps: its
+        // kernels run between the per-iteration launches (a cooperative form
+        // with helper workgroups for the sum measured no faster)
         s->block_mode = (sp.step_mode == 0) && blocked_history && !space && step1d_block_supported(s->C);
         DALLOC(s->dmax, sizeof(double) * 2 * s->C);
         DALLOC(s->flags, sizeof(int) * 16);
@@ -593,7 +594,7 @@ fpde_status launch_1d_block(fpde_solver* s, int n, int n_end, int n0, int nsplit
     a.near_lo = near_lo;
     a.red = s->red;
     a.dmax = s->dmax;
-    a.bar = s->bar;
+    a.flagbar = s->flagbar;
     a.epar = (int)(s->coop_launches & 1);
     a.err = s->err;
     a.ser_P = s->w.ser_P;
@@ -789,7 +790,7 @@ fpde_status fpde_solver_reset(fpde_solver* s, const double* u0) {
     s->tc = 0;
     s->gsplits = 0;
     s->packed_groups = 0;
-    if (s->bar) FPDE_HIP_TRY(hipMemsetAsync(s->bar, 0, 128, s->stream));
+    if (s->flagbar) FPDE_HIP_TRY(hipMemsetAsync(s->flagbar, 0, kFlagBarBytes, s->stream));
     if (s->err) FPDE_HIP_TRY(hipMemsetAsync(s->err, 0, 64, s->stream));
     s->coop_launches = 0;
     return fpde_solver_set_initial(s, u0);

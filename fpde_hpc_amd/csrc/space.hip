@@ -16,8 +16,6 @@
 
 namespace fpde {
 
-typedef double sd4_t __attribute__((ext_vector_type(4)));
-
 // F of U (in memory) with K at U; zero outside the interior [1, N-2] up to ldp.
 __global__ __launch_bounds__(256) void k_space_flux(const double* __restrict__ U, int N, double h, double ksat,
                                                     int mode, double* __restrict__ F, int64_t ldp) {
@@ -35,59 +33,16 @@ __global__ __launch_bounds__(256) void k_space_flux(const double* __restrict__ U
     F[j] = f;
 }
 
-// Partial sums P[y][i] over the block diagonals d in [kSpDiag y, kSpDiag y +
-// kSpDiag) of row tile t = blockIdx.x (rows i = 256 t + 16 I' + p).  Wave w
-// takes 16 of those diagonals; the four waves' results are added through LDS.
-// Operand layout of v_mfma_f64_16x16x4f64 (as in the history GEMM): A lane
-// l = A[l%16][l/16], B lane l = B[l/16][l%16], result v of lane l =
-// D[4v + l/16][l%16]; here the M index is p, the N index I', the K index q.
-// LDS: the F blocks J in [I0 - d_hi + 1, I0 + 15 - d_lo] with a stride of 17
-// per block (B reads of one 16-lane row fall into distinct banks) and the
-// weights w(m), m in [16 d_lo - 15, 16 d_hi - 1].
+// Partial sums of the spatial sum on the matrix cores: one work unit (row
+// tile t, block-diagonal chunk y) per workgroup (space_unit, shared with the
+// cooperative step kernel).
 __global__ __launch_bounds__(256) void k_space_apply(const double* __restrict__ F, int N,
                                                      const double* __restrict__ om, double* __restrict__ P,
                                                      int64_t ldp) {
-    constexpr int FB = kSpDiag + 15;
-    __shared__ double Fs[FB * 17];
-    __shared__ double Ws[16 * kSpDiag + 15];
-    __shared__ double red[4][4][64];
+    __shared__ SpaceUnitLds lds;
     const int t = blockIdx.x, y = blockIdx.y;
     if (y >= space_chunks(t)) return;                  // uniform over the block
-    const int I0 = 16 * t;
-    const int d_lo = kSpDiag * y;
-    const int d_hi = min(d_lo + kSpDiag, I0 + 16);
-    const int Jb = I0 - d_hi + 1;
-    const int nfb = d_hi - d_lo + 15;
-    const int mb = 16 * d_lo - 15;
-    const int nw = 16 * (d_hi - d_lo) + 15;
-    for (int q = threadIdx.x; q < nfb * 16; q += 256) {
-        const int b = q >> 4, c = q & 15;
-        const int64_t j = (int64_t)(Jb + b) * 16 + c;
-        Fs[b * 17 + c] = (j >= 0 && j < ldp) ? F[j] : 0.0;
-    }
-    for (int q = threadIdx.x; q < nw; q += 256) {
-        const int m = mb + q;
-        Ws[q] = (m >= 1 && m <= N) ? om[m - 1] : 0.0;
-    }
-    __syncthreads();
-    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
-    const int r16 = lane & 15, k4 = lane >> 4;
-    sd4_t acc = {0.0, 0.0, 0.0, 0.0};
-    const int dw_lo = d_lo + 16 * wave, dw_hi = min(dw_lo + 16, d_hi);
-    for (int d = dw_lo; d < dw_hi; d++) {
-        const int fb = (r16 + d_hi - 1 - d) * 17 + k4;   // F block I0 + I' - d, element 4 kq + k4
-        const int wb = 16 * d + r16 - k4 - mb;           // w(16 d + p - q), q = 4 kq + k4
-#pragma unroll
-        for (int kq = 0; kq < 4; kq++)
-            acc = __builtin_amdgcn_mfma_f64_16x16x4f64(Ws[wb - 4 * kq], Fs[fb + 4 * kq], acc, 0, 0, 0);
-    }
-#pragma unroll
-    for (int v = 0; v < 4; v++) red[wave][v][lane] = acc[v];
-    __syncthreads();
-    const int v = threadIdx.x >> 6, l = threadIdx.x & 63;
-    const double sum = (red[0][v][l] + red[1][v][l]) + (red[2][v][l] + red[3][v][l]);
-    const int64_t i = (int64_t)kSpTile * t + 16 * (l & 15) + 4 * v + (l >> 4);
-    P[(int64_t)y * ldp + i] = sum;
+    space_unit(F, N, om, P, ldp, t, y, lds);
 }
 
 // V[i] = E_i on the interior rows, 0 on rows 0 and N-1

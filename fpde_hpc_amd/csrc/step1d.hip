@@ -135,6 +135,19 @@ __device__ inline double hist_mem(const Step1DArgs& g, int i, int n, int64_t par
     return far + (nr0 + nr1);
 }
 
+// The newest level and the two-level corrections of the oldest intervals,
+// written with explicit fma so that every kernel rounds them alike:
+// om0 dnew + corr[2n] d0 + corr[2n+1] d1, with d0 = dnew at n = 1 and
+// d1 = dnew at n = 2.
+__device__ inline double hist_tail(double om0, const double* __restrict__ corr, int n, int near_lo, double dnew,
+                                   double d0m, double d1m) {
+    const double w0 = (near_lo <= n - 1) ? om0 : 0.0;
+    double cr = (n == 1) ? corr[2] * dnew : corr[2 * n] * (n >= 2 ? d0m : 0.0);
+    if (n == 2) cr = fma(corr[2 * n + 1], dnew, cr);
+    else if (n >= 3) cr = fma(corr[2 * n + 1], d1m, cr);
+    return fma(w0, dnew, cr);
+}
+
 // max over the 256 threads of a block (wave shuffles + LDS across the 4 waves)
 __device__ inline double block_max(double v, double* sh) {
     for (int off = 32; off > 0; off >>= 1) v = fmax(v, __shfl_xor(v, off));
@@ -271,11 +284,7 @@ __global__ __launch_bounds__(256) void k_step1d(Step1DArgs g) {
                 S = series_update(g.H, g.ld, i, g.ser_P, dnew, g.ser_c, g.ser_v);
             } else if (g.k >= 0 && n >= 1) {
                 // S_n = far + near (in memory) + newest level + two-level corrections
-                double newest = (g.near_lo <= n - 1) ? g.omega[0] * dnew : 0.0;
-                double cr = (n == 1) ? g.corr[2] * dnew : g.corr[2 * n] * d0m;
-                if (n == 2) cr = fma(g.corr[2 * n + 1], dnew, cr);
-                else if (n >= 3) cr = fma(g.corr[2 * n + 1], d1m, cr);
-                S = hm + newest + cr;
+                S = hm + hist_tail(g.omega[0], g.corr, n, g.near_lo, dnew, d0m, d1m);
             }
             if (g.k >= 0) g.Sbuf[i] = S;
             // Picard starts from the extrapolation 2 U^n - U^{n-1}; the
@@ -450,33 +459,55 @@ __global__ __launch_bounds__(256) void k_step1d(Step1DArgs g) {
 
 constexpr unsigned kSpinLimit = 1u << 18;     // bounded wait of a grid barrier
 
-// Grid barrier over the co-resident workgroups.  Each launch counts its
-// arrivals on its own counter (cnt, zeroed by the previous launch), so the
-// j-th barrier of a launch waits for the count (j+1)*nwg: one atomic add per
-// workgroup and relaxed polling, with a single agent-scope release before the
-// arrival and a single acquire after it (on thread 0; the workgroup barriers
-// order the other threads' accesses around them).  Returns false (and sets
-// *err) after a bounded wait.
-__device__ inline bool grid_sync(unsigned* cnt, unsigned target, int* err, int* sh_ok) {
+// Grid barrier of the C chunk workgroups on per-workgroup flags: workgroup
+// c stores the barrier's epoch e to its own flag (128 B apart) after an
+// agent-scope release, and thread j polls flag j (the C flags in parallel,
+// no read-modify-write on a shared counter), then one agent-scope acquire.
+// Each launch uses the flag array of its parity (epochs 1, 2, ... within
+// the launch); its workgroups zero their flags of the other parity for the
+// next launch.  Returns false (and sets *err) after a bounded wait.
+// flag_arrive and flag_wait are its two halves: work placed between them
+// overlaps the wait (the workgroup's own writes made there are not covered
+// by the barrier).
+__device__ inline void flag_arrive(unsigned* fl, int c, unsigned e) {
     __syncthreads();
     if (threadIdx.x == 0) {
-        int ok = 1;
         __builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent");
-        __hip_atomic_fetch_add(cnt, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        __hip_atomic_store(fl + 32 * c, e, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    }
+}
+
+__device__ inline bool flag_wait(unsigned* fl, int C, unsigned e, int* err, int* sh_ok) {
+    if (threadIdx.x == 0) *sh_ok = 1;
+    __syncthreads();
+    for (int j = threadIdx.x; j < C; j += blockDim.x) {
         unsigned spins = 0;
-        while (__hip_atomic_load(cnt, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) < target) {
+        while (__hip_atomic_load(fl + 32 * j, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) < e) {
             __builtin_amdgcn_s_sleep(1);
             if (++spins > kSpinLimit) {
-                ok = 0;
+                *sh_ok = 0;
                 __hip_atomic_store(err, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
                 break;
             }
         }
-        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
-        *sh_ok = ok;
     }
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
     __syncthreads();
     return *sh_ok != 0;
+}
+
+__device__ inline bool flag_sync(unsigned* fl, int c, int C, unsigned e, int* err, int* sh_ok) {
+    flag_arrive(fl, c, e);
+    return flag_wait(fl, C, e, err, sh_ok);
+}
+
+// History term of step n at node i (levels t < n-1 in memory, the newest
+// difference dnew in a register, the two-level corrections of the oldest
+// intervals from d0m, d1m; Toeplitz weights or the step's own row).
+__device__ inline double hist_step(const Step1DArgs& g, int i, int n, double dnew, double d0m, double d1m) {
+    const double* om = g.omega + (int64_t)(n - g.n0) * g.omega_stride;
+    double hm = hist_mem(g, i, n, (int64_t)(n - g.n0) * g.ld, om);
+    return hm + hist_tail(om[0], g.corr, n, g.near_lo, dnew, d0m, d1m);
 }
 
 __global__ __launch_bounds__(256) void k_step1d_block(Step1DArgs g) {
@@ -514,32 +545,36 @@ __global__ __launch_bounds__(256) void k_step1d_block(Step1DArgs g) {
     // written by an earlier launch, else taken when this launch makes them
     double d0m = (!series && act && n_first >= 1) ? g.D[i] : 0.0;
     double d1m = (!series && act && n_first >= 2) ? g.D[g.ld + i] : 0.0;
-    int bk = 0;                                                    // barriers passed in this launch
-    // this launch counts on bar[16*epar]; it zeroes the other counter for the
-    // next launch (the previous launch, its last user, has finished)
-    unsigned* cnt = g.bar + 16 * g.epar;
-    if (c == 0 && tid == 0) __hip_atomic_store(g.bar + 16 * (1 - g.epar), 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    int bk = 0;                                                    // Picard barriers passed in this launch
+    // this launch synchronises on the flags of its parity and zeroes its own
+    // flag of the other parity for the next launch (the previous launch,
+    // their last user, has finished)
+    unsigned* fl = g.flagbar + (size_t)g.epar * 32 * kMaxChunks;
+    if (tid == 0)
+        __hip_atomic_store(g.flagbar + (size_t)(1 - g.epar) * 32 * kMaxChunks + 32 * c, 0u, __ATOMIC_RELAXED,
+                           __HIP_MEMORY_SCOPE_AGENT);
 
+    // history term of the next step, formed while the convergence barrier
+    // of the current one is pending (used when that barrier ends the step)
+    double S_next = 0.0;
+    bool have_next = false;
     for (int n = n_first; n < g.n_end; n++) {
         // ---- history term of step n -------------------------------------
         // weights of step n: one Toeplitz vector, or the step's own row
         // (prescribed grid); local coefficient likewise
-        const double* om = g.omega + (int64_t)(n - g.n0) * g.omega_stride;
         const double Ln = g.Lv ? g.Lv[n] : g.L;
         double S = 0.0;
-        if (n >= 1 && act) {
+        if (have_next) {
+            S = S_next;
+        } else if (n >= 1 && act) {
             if (series) {
                 S = series_update(g.H, g.ld, i, g.ser_P, dlast, g.ser_c + (size_t)n * g.ser_P,
                                   g.ser_v + (size_t)n * g.ser_P);
             } else {
-                double hm = hist_mem(g, i, n, (int64_t)(n - g.n0) * g.ld, om);
-                double newest = (g.near_lo <= n - 1) ? om[0] * dlast : 0.0;
-                double cr = (n == 1) ? g.corr[2] * dlast : g.corr[2 * n] * (n >= 2 ? d0m : 0.0);
-                if (n == 2) cr = fma(g.corr[2 * n + 1], dlast, cr);
-                else if (n >= 3) cr = fma(g.corr[2 * n + 1], d1m, cr);
-                S = hm + newest + cr;
+                S = hist_step(g, i, n, dlast, d0m, d1m);
             }
         }
+        have_next = false;
         // Picard starts from 2 U^n - U^{n-1} (a single pass: lagged at U^n)
         double uit = un, uith = uh;
         if (g.extrap && n >= 1) { uit = un + dlast; uith = uh + dh; }
@@ -551,11 +586,23 @@ __global__ __launch_bounds__(256) void k_step1d_block(Step1DArgs g) {
             const bool check_first = k >= 1 && k == g.picard_max - 1;
             if (check_first) {
                 if (tid == 0) g.dmax[(size_t)(bk & 1) * C + c] = dpub;
-                if (!grid_sync(cnt, (unsigned)(bk + 1) * (unsigned)C, g.err, &sh_ok)) return;
+                flag_arrive(fl, c, (unsigned)(bk + 1));
+                // while the other workgroups arrive: the history term of step
+                // n+1 if U^k ends this step (D[n] = U^k - U^n; series moments
+                // are updated in place and wait for the decision)
+                const bool spec = !series && n + 1 < g.n_end && act;
+                if (spec) {
+                    const double dn = uit - un;
+                    S_next = hist_step(g, i, n + 1, dn, n == 0 ? dn : d0m, n == 1 ? dn : d1m);
+                }
+                if (!flag_wait(fl, C, (unsigned)(bk + 1), g.err, &sh_ok)) return;
                 bk++;
                 double dv = 0.0;
                 for (int j = tid; j < C; j += 256) dv = fmax(dv, g.dmax[(size_t)((bk - 1) & 1) * C + j]);
-                if (block_max(dv, sred) <= g.tol) break;      // U^k is the result (uniform decision)
+                if (block_max(dv, sred) <= g.tol) {           // U^k is the result (uniform decision)
+                    have_next = !series && n + 1 < g.n_end;
+                    break;
+                }
             }
             // ---- coefficients at the iterate, rows, chunk elimination ----
             double K = 0.0, invC = 0.0;
@@ -636,7 +683,7 @@ __global__ __launch_bounds__(256) void k_step1d_block(Step1DArgs g) {
                 redw[4 * (2 * c + 1) + 2] = cr;
                 redw[4 * (2 * c + 1) + 3] = dr - ar * x;
             }
-            if (!grid_sync(cnt, (unsigned)(bk + 1) * (unsigned)C, g.err, &sh_ok)) return;
+            if (!flag_sync(fl, c, C, (unsigned)(bk + 1), g.err, &sh_ok)) return;
             // ---- every workgroup: convergence of U^k, then the reduced solve
             const double* redr = g.red + (size_t)(bk & 1) * R2 * 4;
             bk++;
@@ -693,6 +740,7 @@ __global__ __launch_bounds__(256) void k_step1d_block(Step1DArgs g) {
     }
 }
 
+// Whether the C workgroups of a cooperative block launch can be co-resident.
 bool step1d_block_supported(int C) {
     int dev = 0, coop = 0, cus = 0, per_cu = 0;
     if (hipGetDevice(&dev) != hipSuccess) return false;
@@ -703,7 +751,8 @@ bool step1d_block_supported(int C) {
 }
 
 fpde_status launch_step1d_block(const Step1DArgs& a, hipStream_t st) {
-    if (2 * a.C > kMaxRed) { set_error("1D line too long for the reduced system (N <= 65536)"); return FPDE_ERR_ARG; }
+    if (2 * a.C > kMaxRed || a.C > kMaxChunks) { set_error("1D line too long for the reduced system (N <= 65536)"); return FPDE_ERR_ARG; }
+    if (a.sp_on) { set_error("cooperative step: the space-fractional sum runs with per-iteration launches"); return FPDE_ERR_ARG; }
     if (a.n_end <= a.n) return FPDE_OK;
     Step1DArgs arg = a;
     void* params[] = {&arg};

@@ -257,8 +257,12 @@ def test_cooperative_block_launch_equals_per_iteration_launch(fpde, mode, picard
     """The two 1D launch forms (one cooperative launch per history block with
     grid barriers between Picard iterations; one launch per iteration) run
     the same arithmetic: results agree to rounding, also when advanced in
-    pieces that cut blocks.  (The split forms place their GEMM cut one level
-    apart in the two forms, so they agree to the split precision.)"""
+    pieces that cut blocks.  The reduced systems are solved by different
+    PCR variants (one wave for small C per iteration, the workgroup in the
+    cooperative form), and at mesh Fourier number 10 their rounding grows to
+    ~2e-13 over 150 steps, hence the 1e-12 bound.  (The split forms place
+    their GEMM cut one level apart in the two forms, so they agree to the
+    split precision.)"""
     nodes, steps = 900, 150
     u0 = fpde.init_1d(nodes, 91)
     out = []
@@ -270,7 +274,9 @@ def test_cooperative_block_launch_equals_per_iteration_launch(fpde, mode, picard
             for chunk in (7, 50, 1000):
                 s.advance(chunk)
             out.append(s.solution())
-    assert rel_err(out[0], out[1]) <= (TOL_MIXED if mode.startswith("split") else 1e-13)
+    err = rel_err(out[0], out[1])
+    print(f"coop vs per-iteration {mode} picard {picard} tol {tol}: {err:.3e}")
+    assert err <= (TOL_MIXED if mode.startswith("split") else 1e-12)
 
 
 @pytest.mark.parametrize("block", [16, 32, 128])
@@ -897,3 +903,21 @@ def test_space_fractional_3d_matches_oracle(fpde, oracle, n, mode, tol, coef):
     print(f"3D space-fractional n={n} {mode} coef {coef}: rel err {err:.3e}, effect err {eff:.3e}")
     assert err <= tol
     assert eff <= (1e-6 if tol == TOL_F64 else 1e-3)
+
+
+@pytest.mark.parametrize("mode,tol", [("mfma_f64", 1e-12), ("split_bf16", TOL_MIXED)])
+@pytest.mark.parametrize("nodes", [300, 2900])
+def test_space_fractional_cooperative_equals_per_iteration(fpde, oracle, mode, tol, nodes):
+    """The cooperative block launches (chunk workgroups + helper workgroups
+    sharing the spatial sum between two grid barriers per step) against the
+    per-iteration launches and the oracle."""
+    steps = 150
+    u0 = fpde.init_1d(nodes, 91)
+    kw = dict(alpha=0.75, hist_mode=mode, block=32, picard_max=3, picard_tol=1e-12, u_top=u0[0], gamma=0.75)
+    a = fpde.solve(fpde.make_spec(nodes, steps, 1e-3, step_mode=0, **kw), u0)
+    b = fpde.solve(fpde.make_spec(nodes, steps, 1e-3, step_mode=1, **kw), u0)
+    spec = fpde.make_spec(nodes, steps, 1e-3, **kw)
+    ref = _oracle_space(oracle, spec, u0)
+    print(f"space coop {mode} N={nodes}: coop vs launch {rel_err(a, b):.3e}, coop vs oracle {rel_err(a, ref):.3e}")
+    assert rel_err(a, b) <= tol
+    assert rel_err(a, ref) <= (TOL_F64 if mode == "mfma_f64" else TOL_MIXED)
