@@ -554,19 +554,175 @@ __global__ __launch_bounds__(256) void k_step1d_block(Step1DArgs g) {
         __hip_atomic_store(g.flagbar + (size_t)(1 - g.epar) * 32 * kMaxChunks + 32 * c, 0u, __ATOMIC_RELAXED,
                            __HIP_MEMORY_SCOPE_AGENT);
 
-    // history term of the next step, formed while the convergence barrier
-    // of the current one is pending (used when that barrier ends the step)
-    double S_next = 0.0;
-    bool have_next = false;
-    for (int n = n_first; n < g.n_end; n++) {
+    // ---- one Picard build: coefficients at the iterate (uit, uith), rows,
+    // chunk elimination (chunk factors into fx, fy, fz), the two reduced rows
+    // published with dval in the convergence slot, then the grid barrier
+    double fx = 0.0, fy = 0.0, fz = 0.0;
+    auto build = [&](double uit, double uith, double S, double Ln, double un_, double dval) -> bool {
+        double K = 0.0, invC = 0.0;
+        if (halo) {
+            double Kl, iCl;
+            vgm_coefs(uith, (s - 1) * g.h, g.ksat, g.coef_mode, Kl, iCl);
+            sK[0] = Kl;
+        }
+        if (act) {
+            vgm_coefs(uit, i * g.h, g.ksat, g.coef_mode, K, invC);
+            sK[tid + 1] = K;
+        }
+        __syncthreads();
+        double ar = 0.0, br = 1.0, cr = 0.0, dr = 0.0;
+        if (act) {
+            if (i == 0) { ar = 0.0; br = 1.0; cr = 0.0; dr = g.utop; }
+            else if (i == N - 1) { ar = -1.0; br = 1.0; cr = 0.0; dr = 0.0; }
+            else {
+                double Km = sK[tid], Kp = sK[tid + 1];
+                double A = Km * invC * g.h2inv;
+                double B = Kp * invC * g.h2inv;
+                double R = invC * (Kp * g.h2inv + Km * g.h2inv) + Ln;
+                ar = A; br = -R; cr = B; dr = -Ln * un_ + S;
+            }
+        }
+        if (inner) {
+            double ib = rcp_nr(br);
+            pA[0][q] = (tid == 1) ? 0.0 : ar * ib;
+            pC[0][q] = (tid == nc - 2) ? 0.0 : cr * ib;
+            pX[0][q] = dr * ib;
+            pY[0][q] = (tid == 1) ? -ar * ib : 0.0;
+            pZ[0][q] = (tid == nc - 2) ? -cr * ib : 0.0;
+        }
+        __syncthreads();
+        int pp = 0;
+        for (int delta = 1; delta < ni; delta <<= 1) {
+            if (inner) {
+                double aq = pA[pp][q], cq = pC[pp][q];
+                double am = 0, cm = 0, xm = 0, ym = 0, zm = 0, ap = 0, cp = 0, xp = 0, yp = 0, zp = 0;
+                if (q - delta >= 0) {
+                    am = pA[pp][q - delta]; cm = pC[pp][q - delta];
+                    xm = pX[pp][q - delta]; ym = pY[pp][q - delta]; zm = pZ[pp][q - delta];
+                }
+                if (q + delta < ni) {
+                    ap = pA[pp][q + delta]; cp = pC[pp][q + delta];
+                    xp = pX[pp][q + delta]; yp = pY[pp][q + delta]; zp = pZ[pp][q + delta];
+                }
+                double inv = rcp_nr(1.0 - aq * cm - cq * ap);
+                pA[1 - pp][q] = -aq * am * inv;
+                pC[1 - pp][q] = -cq * cp * inv;
+                pX[1 - pp][q] = (pX[pp][q] - aq * xm - cq * xp) * inv;
+                pY[1 - pp][q] = (pY[pp][q] - aq * ym - cq * yp) * inv;
+                pZ[1 - pp][q] = (pZ[pp][q] - aq * zm - cq * zp) * inv;
+            }
+            __syncthreads();
+            pp = 1 - pp;
+        }
+        fx = 0.0; fy = 0.0; fz = 0.0;
+        if (inner) { fx = pX[pp][q]; fy = pY[pp][q]; fz = pZ[pp][q]; }
+        double* redw = g.red + (size_t)(bk & 1) * R2 * 4;
+        if (tid == 0) {
+            double x, y, z;
+            if (nc == 2) { x = 0.0; y = 0.0; z = 1.0; }
+            else { x = pX[pp][0]; y = pY[pp][0]; z = pZ[pp][0]; }
+            redw[4 * (2 * c) + 0] = ar;
+            redw[4 * (2 * c) + 1] = br + cr * y;
+            redw[4 * (2 * c) + 2] = cr * z;
+            redw[4 * (2 * c) + 3] = dr - cr * x;
+            g.dmax[(size_t)(bk & 1) * C + c] = dval;
+        }
+        if (tid == nc - 1) {
+            double x, y, z;
+            if (nc == 2) { x = 0.0; y = 1.0; z = 0.0; }
+            else { x = pX[pp][ni - 1]; y = pY[pp][ni - 1]; z = pZ[pp][ni - 1]; }
+            redw[4 * (2 * c + 1) + 0] = ar * y;
+            redw[4 * (2 * c + 1) + 1] = br + ar * z;
+            redw[4 * (2 * c + 1) + 2] = cr;
+            redw[4 * (2 * c + 1) + 3] = dr - ar * x;
+        }
+        if (!flag_sync(fl, c, C, (unsigned)(bk + 1), g.err, &sh_ok)) return false;
+        bk++;
+        return true;
+    };
+    // ---- after a build's barrier: the reduced rows; with check != 0 first
+    // the decision on the published convergence slots (check 1: U^k met the
+    // tolerance -> return 1, nothing solved; check 2: the tentatively ended
+    // previous step did not -> return 2, nothing solved); then the reduced
+    // solve and back-substitution (uit, uith, dpub updated), return 0
+    auto solve = [&](int check, double& uit, double& uith, double& dpub) -> int {
+        const double* redr = g.red + (size_t)((bk - 1) & 1) * R2 * 4;
+        double rrow[2][4];
+#pragma unroll
+        for (int j = 0; j < 2; j++) {
+            int r = tid + 256 * j;
+            bool ok = r < R2;
+#pragma unroll
+            for (int qq = 0; qq < 4; qq++) rrow[j][qq] = ok ? redr[4 * r + qq] : 0.0;
+        }
+        if (check) {
+            double dv = 0.0;
+            for (int j = tid; j < C; j += 256) dv = fmax(dv, g.dmax[(size_t)((bk - 1) & 1) * C + j]);
+            const bool met = block_max(dv, sred) <= g.tol;            // uniform decision
+            if (check == 1 && met) return 1;
+            if (check == 2 && !met) return 2;
+        }
+#pragma unroll
+        for (int j = 0; j < 2; j++) {
+            int r = tid + 256 * j;
+            if (r < R2) {
+                double ib = rcp_nr(rrow[j][1]);
+                rA[0][r] = rrow[j][0] * ib;
+                rC[0][r] = rrow[j][2] * ib;
+                rD[0][r] = rrow[j][3] * ib;
+            }
+        }
+        __syncthreads();
+        int pr = pcr_norm(rA, rC, rD, R2);
+        double unew = 0.0;
+        if (act) {
+            if (i == s) unew = rD[pr][2 * c];
+            else if (i == e) unew = rD[pr][2 * c + 1];
+            else unew = fx + fy * rD[pr][2 * c] + fz * rD[pr][2 * c + 1];
+        }
+        double unewh = halo ? rD[pr][2 * c - 1] : 0.0;
+        double d = act ? fabs(unew - uit) : 0.0;
+        dpub = block_max(d, sred);               // also orders the LDS reads before reuse
+        uit = unew;
+        uith = unewh;
+        return 0;
+    };
+    // ---- end of step m with U^{m+1} = (res, resh): D[m], U, carried state
+    auto finish = [&](int m, double res, double resh) {
+        double dn = res - un;
+        if (act) {
+            g.D[(series ? (m & 1) : m) * g.ld + i] = dn;
+            g.U[i] = res;
+        }
+        dh = resh - uh;
+        uh = resh;
+        dlast = dn;
+        un = res;
+        if (m == 0) d0m = dn;
+        if (m == 1) d1m = dn;
+    };
+
+    // A step whose last Picard iteration is due (k = picard_max - 1) ends
+    // tentatively with U^k when the launch has a next step: its convergence
+    // measure goes out in the convergence slots of the next step's first
+    // build, so the barrier that decides it is the one that exchanges those
+    // rows.  If it did not converge, every workgroup (the decision is
+    // uniform) restores the step, runs its last iteration, rewrites D[n] and
+    // U, and starts the next step again.  Series moments are updated in
+    // place, so sum_alg 2 decides convergence before the step ends.
+    bool pending = false;          // the previous step ended tentatively
+    double pend_d = 0.0;           // its convergence measure (this chunk)
+    int sv_n = 0;                  // that step's state, for a rollback
+    double sv_un = 0.0, sv_uh = 0.0, sv_dh = 0.0, sv_dlast = 0.0, sv_d0m = 0.0, sv_d1m = 0.0;
+    double sv_S = 0.0, sv_Ln = 0.0, sv_uit = 0.0, sv_uith = 0.0;
+    int n = n_first;
+    while (n < g.n_end) {
         // ---- history term of step n -------------------------------------
         // weights of step n: one Toeplitz vector, or the step's own row
         // (prescribed grid); local coefficient likewise
         const double Ln = g.Lv ? g.Lv[n] : g.L;
         double S = 0.0;
-        if (have_next) {
-            S = S_next;
-        } else if (n >= 1 && act) {
+        if (n >= 1 && act) {
             if (series) {
                 S = series_update(g.H, g.ld, i, g.ser_P, dlast, g.ser_c + (size_t)n * g.ser_P,
                                   g.ser_v + (size_t)n * g.ser_P);
@@ -574,169 +730,54 @@ __global__ __launch_bounds__(256) void k_step1d_block(Step1DArgs g) {
                 S = hist_step(g, i, n, dlast, d0m, d1m);
             }
         }
-        have_next = false;
         // Picard starts from 2 U^n - U^{n-1} (a single pass: lagged at U^n)
         double uit = un, uith = uh;
         if (g.extrap && n >= 1) { uit = un + dlast; uith = uh + dh; }
         double dpub = 0.0;                   // max |U^k - U^{k-1}| of the chunk, k >= 1
+        bool tentative = false, rollback = false;
         for (int k = 0; k < g.picard_max; k++) {
-            // the last build is skipped when U^k already met the tolerance:
-            // decide that first (one barrier on the convergence measure);
-            // earlier iterations publish it with their reduced rows
-            const bool check_first = k >= 1 && k == g.picard_max - 1;
-            if (check_first) {
+            const bool last_due = k >= 1 && k == g.picard_max - 1;
+            if (last_due && !series && n + 1 < g.n_end) {
+                tentative = true;            // U^k ends the step, decided with the next build
+                break;
+            }
+            if (last_due) {
+                // the last build is skipped when U^k already met the
+                // tolerance: decide that first (one barrier on the measure)
                 if (tid == 0) g.dmax[(size_t)(bk & 1) * C + c] = dpub;
-                flag_arrive(fl, c, (unsigned)(bk + 1));
-                // while the other workgroups arrive: the history term of step
-                // n+1 if U^k ends this step (D[n] = U^k - U^n; series moments
-                // are updated in place and wait for the decision)
-                const bool spec = !series && n + 1 < g.n_end && act;
-                if (spec) {
-                    const double dn = uit - un;
-                    S_next = hist_step(g, i, n + 1, dn, n == 0 ? dn : d0m, n == 1 ? dn : d1m);
-                }
-                if (!flag_wait(fl, C, (unsigned)(bk + 1), g.err, &sh_ok)) return;
+                if (!flag_sync(fl, c, C, (unsigned)(bk + 1), g.err, &sh_ok)) return;
                 bk++;
-                double dv = 0.0;
-                for (int j = tid; j < C; j += 256) dv = fmax(dv, g.dmax[(size_t)((bk - 1) & 1) * C + j]);
-                if (block_max(dv, sred) <= g.tol) {           // U^k is the result (uniform decision)
-                    have_next = !series && n + 1 < g.n_end;
-                    break;
-                }
-            }
-            // ---- coefficients at the iterate, rows, chunk elimination ----
-            double K = 0.0, invC = 0.0;
-            if (halo) {
-                double Kl, iCl;
-                vgm_coefs(uith, (s - 1) * g.h, g.ksat, g.coef_mode, Kl, iCl);
-                sK[0] = Kl;
-            }
-            if (act) {
-                vgm_coefs(uit, i * g.h, g.ksat, g.coef_mode, K, invC);
-                sK[tid + 1] = K;
-            }
-            __syncthreads();
-            double ar = 0.0, br = 1.0, cr = 0.0, dr = 0.0;
-            if (act) {
-                if (i == 0) { ar = 0.0; br = 1.0; cr = 0.0; dr = g.utop; }
-                else if (i == N - 1) { ar = -1.0; br = 1.0; cr = 0.0; dr = 0.0; }
-                else {
-                    double Km = sK[tid], Kp = sK[tid + 1];
-                    double A = Km * invC * g.h2inv;
-                    double B = Kp * invC * g.h2inv;
-                    double R = invC * (Kp * g.h2inv + Km * g.h2inv) + Ln;
-                    ar = A; br = -R; cr = B; dr = -Ln * un + S;
-                }
-            }
-            if (inner) {
-                double ib = rcp_nr(br);
-                pA[0][q] = (tid == 1) ? 0.0 : ar * ib;
-                pC[0][q] = (tid == nc - 2) ? 0.0 : cr * ib;
-                pX[0][q] = dr * ib;
-                pY[0][q] = (tid == 1) ? -ar * ib : 0.0;
-                pZ[0][q] = (tid == nc - 2) ? -cr * ib : 0.0;
-            }
-            __syncthreads();
-            int pp = 0;
-            for (int delta = 1; delta < ni; delta <<= 1) {
-                if (inner) {
-                    double aq = pA[pp][q], cq = pC[pp][q];
-                    double am = 0, cm = 0, xm = 0, ym = 0, zm = 0, ap = 0, cp = 0, xp = 0, yp = 0, zp = 0;
-                    if (q - delta >= 0) {
-                        am = pA[pp][q - delta]; cm = pC[pp][q - delta];
-                        xm = pX[pp][q - delta]; ym = pY[pp][q - delta]; zm = pZ[pp][q - delta];
-                    }
-                    if (q + delta < ni) {
-                        ap = pA[pp][q + delta]; cp = pC[pp][q + delta];
-                        xp = pX[pp][q + delta]; yp = pY[pp][q + delta]; zp = pZ[pp][q + delta];
-                    }
-                    double inv = rcp_nr(1.0 - aq * cm - cq * ap);
-                    pA[1 - pp][q] = -aq * am * inv;
-                    pC[1 - pp][q] = -cq * cp * inv;
-                    pX[1 - pp][q] = (pX[pp][q] - aq * xm - cq * xp) * inv;
-                    pY[1 - pp][q] = (pY[pp][q] - aq * ym - cq * yp) * inv;
-                    pZ[1 - pp][q] = (pZ[pp][q] - aq * zm - cq * zp) * inv;
-                }
-                __syncthreads();
-                pp = 1 - pp;
-            }
-            double fx = 0.0, fy = 0.0, fz = 0.0;
-            if (inner) { fx = pX[pp][q]; fy = pY[pp][q]; fz = pZ[pp][q]; }
-            // reduced rows of the chunk ends and the convergence measure
-            double* redw = g.red + (size_t)(bk & 1) * R2 * 4;
-            if (tid == 0) {
-                double x, y, z;
-                if (nc == 2) { x = 0.0; y = 0.0; z = 1.0; }
-                else { x = pX[pp][0]; y = pY[pp][0]; z = pZ[pp][0]; }
-                redw[4 * (2 * c) + 0] = ar;
-                redw[4 * (2 * c) + 1] = br + cr * y;
-                redw[4 * (2 * c) + 2] = cr * z;
-                redw[4 * (2 * c) + 3] = dr - cr * x;
-                g.dmax[(size_t)(bk & 1) * C + c] = dpub;
-            }
-            if (tid == nc - 1) {
-                double x, y, z;
-                if (nc == 2) { x = 0.0; y = 1.0; z = 0.0; }
-                else { x = pX[pp][ni - 1]; y = pY[pp][ni - 1]; z = pZ[pp][ni - 1]; }
-                redw[4 * (2 * c + 1) + 0] = ar * y;
-                redw[4 * (2 * c + 1) + 1] = br + ar * z;
-                redw[4 * (2 * c + 1) + 2] = cr;
-                redw[4 * (2 * c + 1) + 3] = dr - ar * x;
-            }
-            if (!flag_sync(fl, c, C, (unsigned)(bk + 1), g.err, &sh_ok)) return;
-            // ---- every workgroup: convergence of U^k, then the reduced solve
-            const double* redr = g.red + (size_t)(bk & 1) * R2 * 4;
-            bk++;
-            double rrow[2][4];
-#pragma unroll
-            for (int j = 0; j < 2; j++) {
-                int r = tid + 256 * j;
-                bool ok = r < R2;
-#pragma unroll
-                for (int qq = 0; qq < 4; qq++) rrow[j][qq] = ok ? redr[4 * r + qq] : 0.0;
-            }
-            if (k >= 1 && !check_first) {
                 double dv = 0.0;
                 for (int j = tid; j < C; j += 256) dv = fmax(dv, g.dmax[(size_t)((bk - 1) & 1) * C + j]);
                 if (block_max(dv, sred) <= g.tol) break;      // U^k is the result (uniform decision)
             }
-#pragma unroll
-            for (int j = 0; j < 2; j++) {
-                int r = tid + 256 * j;
-                if (r < R2) {
-                    double ib = rcp_nr(rrow[j][1]);
-                    rA[0][r] = rrow[j][0] * ib;
-                    rC[0][r] = rrow[j][2] * ib;
-                    rD[0][r] = rrow[j][3] * ib;
-                }
-            }
-            __syncthreads();
-            int pr = pcr_norm(rA, rC, rD, R2);
-            double unew = 0.0;
-            if (act) {
-                if (i == s) unew = rD[pr][2 * c];
-                else if (i == e) unew = rD[pr][2 * c + 1];
-                else unew = fx + fy * rD[pr][2 * c] + fz * rD[pr][2 * c + 1];
-            }
-            double unewh = halo ? rD[pr][2 * c - 1] : 0.0;
-            double d = act ? fabs(unew - uit) : 0.0;
-            dpub = block_max(d, sred);               // also orders the LDS reads before reuse
-            uit = unew;
-            uith = unewh;
+            const bool pend_check = k == 0 && pending;
+            if (!build(uit, uith, S, Ln, un, pend_check ? pend_d : dpub)) return;
+            const int r = solve(pend_check ? 2 : ((k >= 1 && !last_due) ? 1 : 0), uit, uith, dpub);
+            if (r == 1) break;                                 // U^k is the result
+            if (r == 2) { rollback = true; break; }
+            pending = false;
+        }
+        if (rollback) {
+            // the tentatively ended step sv_n needs its last iteration
+            un = sv_un; uh = sv_uh; dh = sv_dh; dlast = sv_dlast; d0m = sv_d0m; d1m = sv_d1m;
+            double ruit = sv_uit, ruith = sv_uith, rd = 0.0;
+            if (!build(ruit, ruith, sv_S, sv_Ln, un, 0.0)) return;
+            solve(0, ruit, ruith, rd);
+            finish(sv_n, ruit, ruith);
+            pending = false;
+            n = sv_n + 1;
+            continue;
+        }
+        if (tentative) {
+            sv_n = n; sv_un = un; sv_uh = uh; sv_dh = dh; sv_dlast = dlast; sv_d0m = d0m; sv_d1m = d1m;
+            sv_S = S; sv_Ln = Ln; sv_uit = uit; sv_uith = uith;
+            pending = true;
+            pend_d = dpub;
         }
         // ---- step n is done: the last iterate is U^{n+1} -------------------
-        const double result = uit, resulth = uith;
-        double dn = result - un;
-        if (act) {
-            g.D[(series ? (n & 1) : n) * g.ld + i] = dn;
-            g.U[i] = result;
-        }
-        dh = resulth - uh;
-        uh = resulth;
-        dlast = dn;
-        un = result;
-        if (n == 0) d0m = dn;
-        if (n == 1) d1m = dn;
+        finish(n, uit, uith);
+        n++;
     }
 }
 

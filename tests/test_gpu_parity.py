@@ -252,12 +252,16 @@ def test_picard_chunk_halo_tight(fpde, oracle, tol, picard):
 
 
 @pytest.mark.parametrize("mode", ["plain", "mfma_f64", "split_bf16"])
-@pytest.mark.parametrize("picard,tol", [(1, 0.0), (3, 1e-12), (3, 1e-5)])
+@pytest.mark.parametrize("picard,tol", [(1, 0.0), (3, 1e-12), (3, 1e-5), (2, 1e-12), (3, 1e-15), (4, 1e-14)])
 def test_cooperative_block_launch_equals_per_iteration_launch(fpde, mode, picard, tol):
     """The two 1D launch forms (one cooperative launch per history block with
     grid barriers between Picard iterations; one launch per iteration) run
     the same arithmetic: results agree to rounding, also when advanced in
-    pieces that cut blocks.  The reduced systems are solved by different
+    pieces that cut blocks.  The cooperative form ends a step tentatively
+    when its last Picard iteration is due and decides convergence with the
+    next step's first rows; tolerances that are not met (1e-15, 1e-14, and
+    picard_max 2) exercise the rollback to the skipped iteration.  The
+    reduced systems are solved by different
     PCR variants (one wave for small C per iteration, the workgroup in the
     cooperative form), and at mesh Fourier number 10 their rounding grows to
     ~2e-13 over 150 steps, hence the 1e-12 bound.  (The split forms place
