@@ -558,7 +558,10 @@ __global__ __launch_bounds__(256) void k_step1d_block(Step1DArgs g) {
     // chunk elimination (chunk factors into fx, fy, fz), the two reduced rows
     // published with dval in the convergence slot, then the grid barrier
     double fx = 0.0, fy = 0.0, fz = 0.0;
-    auto build = [&](double uit, double uith, double S, double Ln, double un_, double dval) -> bool {
+    // the part of the next step's history sum that is already in memory
+    // (far partials and near levels t < n), formed while a barrier is pending
+    double hm_next = 0.0;
+    auto build = [&](double uit, double uith, double S, double Ln, double un_, double dval, int pre_n) -> bool {
         double K = 0.0, invC = 0.0;
         if (halo) {
             double Kl, iCl;
@@ -636,7 +639,10 @@ __global__ __launch_bounds__(256) void k_step1d_block(Step1DArgs g) {
             redw[4 * (2 * c + 1) + 2] = cr;
             redw[4 * (2 * c + 1) + 3] = dr - ar * x;
         }
-        if (!flag_sync(fl, c, C, (unsigned)(bk + 1), g.err, &sh_ok)) return false;
+        flag_arrive(fl, c, (unsigned)(bk + 1));
+        if (pre_n >= 0 && act)
+            hm_next = hist_mem(g, i, pre_n, (int64_t)(pre_n - g.n0) * g.ld, g.omega + (int64_t)(pre_n - g.n0) * g.omega_stride);
+        if (!flag_wait(fl, C, (unsigned)(bk + 1), g.err, &sh_ok)) return false;
         bk++;
         return true;
     };
@@ -715,6 +721,8 @@ __global__ __launch_bounds__(256) void k_step1d_block(Step1DArgs g) {
     int sv_n = 0;                  // that step's state, for a rollback
     double sv_un = 0.0, sv_uh = 0.0, sv_dh = 0.0, sv_dlast = 0.0, sv_d0m = 0.0, sv_d1m = 0.0;
     double sv_S = 0.0, sv_Ln = 0.0, sv_uit = 0.0, sv_uith = 0.0;
+    double hm_cur = 0.0;           // that part for the current step
+    bool have_hm = false;
     int n = n_first;
     while (n < g.n_end) {
         // ---- history term of step n -------------------------------------
@@ -726,10 +734,16 @@ __global__ __launch_bounds__(256) void k_step1d_block(Step1DArgs g) {
             if (series) {
                 S = series_update(g.H, g.ld, i, g.ser_P, dlast, g.ser_c + (size_t)n * g.ser_P,
                                   g.ser_v + (size_t)n * g.ser_P);
+            } else if (have_hm) {
+                const double* om = g.omega + (int64_t)(n - g.n0) * g.omega_stride;
+                S = hm_cur + hist_tail(om[0], g.corr, n, g.near_lo, dlast, d0m, d1m);
             } else {
                 S = hist_step(g, i, n, dlast, d0m, d1m);
             }
         }
+        // the next step's in-memory history part goes with this step's first
+        // barrier (levels t < n are written; D[n] enters through the tail)
+        const int pre_n = (!series && n + 1 < g.n_end) ? n + 1 : -1;
         // Picard starts from 2 U^n - U^{n-1} (a single pass: lagged at U^n)
         double uit = un, uith = uh;
         if (g.extrap && n >= 1) { uit = un + dlast; uith = uh + dh; }
@@ -752,7 +766,7 @@ __global__ __launch_bounds__(256) void k_step1d_block(Step1DArgs g) {
                 if (block_max(dv, sred) <= g.tol) break;      // U^k is the result (uniform decision)
             }
             const bool pend_check = k == 0 && pending;
-            if (!build(uit, uith, S, Ln, un, pend_check ? pend_d : dpub)) return;
+            if (!build(uit, uith, S, Ln, un, pend_check ? pend_d : dpub, k == 0 ? pre_n : -1)) return;
             const int r = solve(pend_check ? 2 : ((k >= 1 && !last_due) ? 1 : 0), uit, uith, dpub);
             if (r == 1) break;                                 // U^k is the result
             if (r == 2) { rollback = true; break; }
@@ -762,7 +776,7 @@ __global__ __launch_bounds__(256) void k_step1d_block(Step1DArgs g) {
             // the tentatively ended step sv_n needs its last iteration
             un = sv_un; uh = sv_uh; dh = sv_dh; dlast = sv_dlast; d0m = sv_d0m; d1m = sv_d1m;
             double ruit = sv_uit, ruith = sv_uith, rd = 0.0;
-            if (!build(ruit, ruith, sv_S, sv_Ln, un, 0.0)) return;
+            if (!build(ruit, ruith, sv_S, sv_Ln, un, 0.0, -1)) return;
             solve(0, ruit, ruith, rd);
             finish(sv_n, ruit, ruith);
             pending = false;
@@ -777,6 +791,8 @@ __global__ __launch_bounds__(256) void k_step1d_block(Step1DArgs g) {
         }
         // ---- step n is done: the last iterate is U^{n+1} -------------------
         finish(n, uit, uith);
+        hm_cur = hm_next;
+        have_hm = pre_n >= 0;
         n++;
     }
 }
