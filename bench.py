@@ -139,6 +139,7 @@ def serve_mix(F, n_req, rate, seed=1004, quantum=64, budget_gb=200.0, max_active
     ids = []
     t0 = time.perf_counter()
     k = 0
+    t_report = t0
     while k < len(reqs) or srv.poll(quantum) > 0:
         now = time.perf_counter() - t0
         while k < len(reqs) and reqs[k][0] <= now:
@@ -146,6 +147,10 @@ def serve_mix(F, n_req, rate, seed=1004, quantum=64, budget_gb=200.0, max_active
             k += 1
         if k < len(reqs) and srv.poll(quantum) == 0:
             time.sleep(max(0.0, min(0.01, reqs[k][0] - (time.perf_counter() - t0))))
+        if time.perf_counter() - t_report > 20.0:          # progress for long mixes
+            t_report = time.perf_counter()
+            done = sum(1 for rid, _ in ids if srv.info(rid)["status"] in (0, 1))
+            print(f"[serve] {t_report - t0:.0f}s: {k} submitted, {done} finished", file=sys.stderr, flush=True)
     makespan = time.perf_counter() - t0
     lat, done_work, status = [], 0, []
     for rid, work in ids:
