@@ -866,7 +866,12 @@ fpde_status fpde_solver_advance_graph(fpde_solver* s, int nsteps, int steps_per_
         s->graph_launches++;
     }
     s->prof = prof;
-    if (rc != FPDE_OK) return rc;
+    if (rc != FPDE_OK) {
+        // the host state has advanced over steps whose launches were not run:
+        // the solver needs fpde_solver_reset before further steps
+        s->initialized = false;
+        return rc;
+    }
     FPDE_HIP_TRY(hipGetLastError());
     return FPDE_OK;
 }

@@ -34,8 +34,7 @@ __global__ __launch_bounds__(256) void k_space_flux(const double* __restrict__ U
 }
 
 // Partial sums of the spatial sum on the matrix cores: one work unit (row
-// tile t, block-diagonal chunk y) per workgroup (space_unit, shared with the
-// cooperative step kernel).
+// tile t, block-diagonal chunk y, space_unit) per workgroup.
 __global__ __launch_bounds__(256) void k_space_apply(const double* __restrict__ F, int N,
                                                      const double* __restrict__ om, double* __restrict__ P,
                                                      int64_t ldp) {

@@ -213,7 +213,8 @@ fpde_status fpde_solver_advance(fpde_solver *s, int nsteps);
  * (hipGraphExecUpdate) and launched.  Results are identical to
  * fpde_solver_advance.  For solvers whose steps launch per Picard iteration
  * (1D with step_mode 1 or hist_mode 0, and 3D); the cooperative block launches
- * of step_mode 0 are not captured (FPDE_ERR_STATE).  This is the graph-captured
+ * of step_mode 0 are not captured (FPDE_ERR_STATE).  After a failure inside a
+ * capture the solver needs fpde_solver_reset.  This is the graph-captured
  * step loop of the device layer; the reference launches kernel by kernel
  * (fr_ocl_call_sum, fpde_hpc_1d.cpp:1819-1883). */
 fpde_status fpde_solver_advance_graph(fpde_solver *s, int nsteps, int steps_per_graph);
