@@ -967,3 +967,22 @@ def test_3d_and_distributed_order_full_size_vs_plain(fpde):
     e3, ed = rel_err(out3["split_bf16"], out3["plain"]), rel_err(outd["split_f16"], outd["plain"])
     print(f"3d 128^3 split_bf16 {e3:.3e}; do N=4000 split_f16 {ed:.3e}")
     assert e3 <= TOL_MIXED and ed <= TOL_MIXED
+
+
+@pytest.mark.parametrize("nodes", [65536, 40001])
+def test_longest_lines_both_launch_forms(fpde, oracle, nodes):
+    """The largest 1D lines (256 chunk workgroups, a 512-row reduced system,
+    256 barrier flags) and a ragged one: both launch forms against the oracle,
+    across two history blocks."""
+    steps = 130
+    u0 = fpde.init_1d(nodes, 33)
+    outs = []
+    for step_mode in (0, 1):
+        spec = fpde.make_spec(nodes, steps, 1e-3, alpha=0.7, hist_mode="mfma_f64", block=64, picard_max=3,
+                              picard_tol=1e-12, u_top=u0[0], step_mode=step_mode)
+        outs.append(fpde.solve(spec, u0))
+    p = oracle.make_params(nodes, steps, 1e-3, 0.7, ksat=spec.ksat, picard_max=3, picard_tol=1e-12)
+    ref = oracle.solve_1d(p, u0)
+    e0, e1 = rel_err(outs[0], ref), rel_err(outs[1], ref)
+    print(f"N={nodes}: cooperative {e0:.3e}, per-iteration {e1:.3e}")
+    assert e0 <= TOL_F64 and e1 <= TOL_F64
