@@ -35,6 +35,13 @@ def run_variant(F, workload, mode, block, steps, warmup, prof_mask_timed=None, n
                 picard=(1, 0.0), gamma=0.0):
     kw = {} if workload == "3d" else dict(picard_max=picard[0], picard_tol=picard[1])
     spec, u0 = F.workload(workload, hist_mode=mode, block=block, nodes=nodes, steps=msteps, gamma=gamma, **kw)
+    tuned = None
+    if block == 0 and mode != "plain":
+        # untimed: every candidate block runs the whole solve once
+        b, ms = F.autotune_block(spec, u0, trial_steps=spec.steps)
+        spec.block = b
+        tuned = {"block": b, "trial_ms_per_step": ms}
+        print(f"[bench] {mode}: autotuned block {b} ({ms * 1e3:.1f} us/step)", file=sys.stderr, flush=True)
     s = F.Solver(spec)
     s.set_initial(u0)
     # profiled warmup: per-class device times with hipEvents on the solver stream
@@ -61,7 +68,7 @@ def run_variant(F, workload, mode, block, steps, warmup, prof_mask_timed=None, n
         timed_stats.append(s.stats())
     u_last = s.solution()
     s.close()
-    return dict(spec=spec, u=u_last, u_first=u, times=times, prof=prof, timed=timed_stats)
+    return dict(spec=spec, u=u_last, u_first=u, times=times, prof=prof, timed=timed_stats, tuned=tuned)
 
 
 def cpu_baseline(N, M, seconds_hint=20.0, alpha=0.7, tau=5e-4, seed=1001, picard=(1, 0.0)):
@@ -180,7 +187,8 @@ def main():
     ap.add_argument("--workload", default="1d_large")
     ap.add_argument("--headline", default="split_bf16")
     ap.add_argument("--compare", default="plain,mfma_f64")
-    ap.add_argument("--block", type=int, default=64)
+    ap.add_argument("--block", type=int, default=64,
+                    help="history block size 16/32/64/128; 0: autotune (timed trial of each, untimed)")
     ap.add_argument("--nodes", type=int, default=None)
     ap.add_argument("--msteps", type=int, default=None)
     ap.add_argument("--no-cpu", action="store_true")
@@ -279,7 +287,8 @@ def main():
         # it live over the timed region with events on the solver stream
         prof = r["prof"]
         dom = 0 if prof["ms_history"] >= prof["ms_step"] else 1
-        r2 = run_variant(F, args.workload, mode, args.block, K, 0, prof_mask_timed=1 << dom,
+        r2 = run_variant(F, args.workload, mode, r["spec"].block if args.block == 0 else args.block, K, 0,
+                         prof_mask_timed=1 << dom,
                          nodes=args.nodes, msteps=args.msteps, picard=picard,
                          gamma=args.gamma) if mode == args.headline else None
         r["dominant"] = dom
@@ -385,17 +394,20 @@ def main():
         "config": {"workload": (f"{args.workload}: {'3D LOD ' + str(N) + '^3' if spec.dims == 3 else '1D N=' + str(N)} "
                                 f"time-fractional moisture transport M={M} tau={spec.tau} "
                                 + (f"distributed order q={spec.do_q} on [{spec.do_a0},{spec.do_a1}]" if spec.do_q
-                                   else f"alpha={spec.alpha}") + f"; history {args.headline} block {args.block}"
+                                   else f"alpha={spec.alpha}") + f"; history {args.headline} block {spec.block}"
+                                + (" (autotuned)" if args.block == 0 else "")
                                 + (f"; Picard <= {spec.picard_max} iterations, tol {spec.picard_tol:g}"
                                    if spec.dims == 1 else "; LOD with lagged coefficients")
                                 + (f"; space-fractional explicit scheme gamma={spec.gamma:g}" if spec.gamma else "")),
-                   "nodes": N, "time_steps": M, "history_mode": args.headline, "block": args.block,
+                   "nodes": N, "time_steps": M, "history_mode": args.headline, "block": spec.block,
                    "picard_max": spec.picard_max, "picard_tol": spec.picard_tol, "gamma": spec.gamma,
                    "parallelism": "single GPU"},
         "roofline": roof,
         "cpu_baseline": cpu,
         "variants": variants,
     }
+    if results[args.headline].get("tuned"):
+        out["config"]["block_autotune"] = results[args.headline]["tuned"]
     if "plain" in variants:
         out["speedup_vs_plain_fp64"] = variants["plain"]["ms_per_solve"] / (t_mean * 1e3)
         if ref_u is not None:

@@ -820,6 +820,41 @@ fpde_status fpde_solver_advance(fpde_solver* s, int nsteps) {
     return FPDE_OK;
 }
 
+fpde_status fpde_autotune_block(const fpde_spec* spec, const double* u0, int trial_steps, int* block_out,
+                                double* ms_per_step_out) {
+    if (!spec || !u0 || !block_out) { set_error("null argument"); return FPDE_ERR_ARG; }
+    fpde_spec base = *spec;
+    if (base.hist_mode != HIST_PLAIN_F64 && base.block == 0) base.block = 64;   // 0: "pick one"
+    fpde_status rc = validate(base);
+    if (rc != FPDE_OK) return rc;
+    *block_out = base.block;
+    if (ms_per_step_out) *ms_per_step_out = 0.0;
+    if (base.hist_mode == HIST_PLAIN_F64) return FPDE_OK;   // no history GEMM, no block
+    const int cands[4] = {16, 32, 64, 128};
+    int steps = trial_steps > 0 ? trial_steps : 2048;
+    steps = std::min(base.steps, std::max(steps, 2 * 128 + 1));
+    double best = -1.0;
+    for (int b : cands) {
+        fpde_spec t = base;
+        t.block = b;
+        t.steps = steps;                        // a prescribed grid runs its first `steps` entries
+        fpde_solver* sv = nullptr;
+        rc = fpde_solver_create(&t, &sv);
+        if (rc != FPDE_OK) return rc;
+        rc = fpde_solver_set_initial(sv, u0);
+        if (rc == FPDE_OK) rc = fpde_solver_sync(sv);
+        auto t0 = std::chrono::steady_clock::now();
+        if (rc == FPDE_OK) rc = fpde_solver_advance(sv, 0);
+        if (rc == FPDE_OK) rc = fpde_solver_sync(sv);
+        double ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
+        fpde_solver_destroy(sv);
+        if (rc != FPDE_OK) return rc;
+        if (best < 0.0 || ms < best) { best = ms; *block_out = b; }
+    }
+    if (ms_per_step_out) *ms_per_step_out = best / steps;
+    return FPDE_OK;
+}
+
 fpde_status fpde_solver_advance_graph(fpde_solver* s, int nsteps, int steps_per_graph) {
     if (!s) { set_error("null solver"); return FPDE_ERR_ARG; }
     if (!s->initialized) { set_error("set_initial must be called first"); return FPDE_ERR_STATE; }

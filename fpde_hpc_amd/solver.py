@@ -209,6 +209,16 @@ def varT_step(s0, s2, s3, tau, eps, tau_max):
     return t.value, e.value
 
 
+def autotune_block(spec, u0, trial_steps=0):
+    """History block size B picked by timed device trials (fpde_autotune_block);
+    returns (B, ms per step of its trial)."""
+    u0 = np.ascontiguousarray(u0, dtype=np.float64)
+    b = ctypes.c_int(0)
+    ms = ctypes.c_double(0.0)
+    check(lib().fpde_autotune_block(ctypes.byref(spec), _p(u0), int(trial_steps), ctypes.byref(b), ctypes.byref(ms)))
+    return b.value, ms.value
+
+
 def footprint(spec):
     return lib().fpde_solver_footprint(ctypes.byref(spec))
 

@@ -204,6 +204,22 @@ fpde_status fpde_solver_destroy(fpde_solver *s);
 fpde_status fpde_solver_set_initial(fpde_solver *s, const double *u0);
 /* Restart the solve from u0 at step 0, keeping every device allocation. */
 fpde_status fpde_solver_reset(fpde_solver *s, const double *u0);
+/* Autotuner of the history block size B (the level count of one history
+ * GEMM block and of the near field summed inside the step: 16, 32, 64 or 128)
+ * for a GEMM-form spec (hist_mode 1-4; spec->block may be 0 here): each
+ * candidate runs a trial solve from u0 of the first `trial_steps` steps
+ * (<= 0: 2048; at least 257, at most spec->steps) on the device, timed from
+ * enqueue to completion; the fastest B goes to *block_out and its trial time
+ * per step (ms) to *ms_per_step_out (may be NULL).  Small B costs more
+ * launches and GEMM blocks, large B a longer near-field sum per step, and the
+ * history share grows with the step count: a trial of the whole run
+ * (trial_steps = spec->steps) measures the real balance.  hist_mode 0 has no
+ * block: FPDE_OK without a trial.  No counterpart in the reference: its
+ * OpenCL block size is a command-line option (oclBS, fpde_hpc_1d.cpp:9713-9714)
+ * compiled into the kernel text (:1771-1775). */
+fpde_status fpde_autotune_block(const fpde_spec *spec, const double *u0, int trial_steps,
+                                int *block_out, double *ms_per_step_out);
+
 /* Enqueue nsteps steps (or all remaining if nsteps<=0); asynchronous. */
 fpde_status fpde_solver_advance(fpde_solver *s, int nsteps);
 /* The same steps as fpde_solver_advance, enqueued as hipGraphs: every

@@ -117,3 +117,20 @@ def test_space_sum_rejects_bad_arguments(fpde, args):
     rc = fpde.lib().fpde_space_sum(*args)
     assert rc == 1
     assert fpde.lib().fpde_last_error()
+
+
+def test_autotune_block_argument_checks_and_plain_form(fpde):
+    """Null arguments and invalid specs are refused before any device call;
+    hist_mode 0 has no history block, so no trial runs and the block stays."""
+    L = fpde.lib()
+    spec = fpde.make_spec(64, 300, 1e-3, hist_mode="split_bf16", block=0)
+    u0 = np.zeros(64)
+    b, ms = ctypes.c_int(-1), ctypes.c_double(-1.0)
+    assert L.fpde_autotune_block(None, fpde.solver._p(u0), 0, ctypes.byref(b), None) == 1
+    assert L.fpde_autotune_block(ctypes.byref(spec), None, 0, ctypes.byref(b), None) == 1
+    assert L.fpde_autotune_block(ctypes.byref(spec), fpde.solver._p(u0), 0, None, None) == 1
+    bad = fpde.make_spec(64, 300, 1e-3, hist_mode="split_bf16", block=48)
+    assert L.fpde_autotune_block(ctypes.byref(bad), fpde.solver._p(u0), 0, ctypes.byref(b), None) == 1
+    plain = fpde.make_spec(64, 300, 1e-3, hist_mode="plain", block=0)
+    assert L.fpde_autotune_block(ctypes.byref(plain), fpde.solver._p(u0), 0, ctypes.byref(b), ctypes.byref(ms)) == 0
+    assert b.value == 0 and ms.value == 0.0

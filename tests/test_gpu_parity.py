@@ -986,3 +986,24 @@ def test_longest_lines_both_launch_forms(fpde, oracle, nodes):
     e0, e1 = rel_err(outs[0], ref), rel_err(outs[1], ref)
     print(f"N={nodes}: cooperative {e0:.3e}, per-iteration {e1:.3e}")
     assert e0 <= TOL_F64 and e1 <= TOL_F64
+
+
+def test_autotune_block_picks_a_candidate_that_solves_right(fpde, oracle):
+    """The autotuner times every candidate block on the device and returns one
+    of them; a solve with the picked block matches the oracle (1D, split and
+    FP64 MFMA forms) and the picked block is stable on a 3D spec too."""
+    nodes, steps = 900, 300
+    u0 = fpde.init_1d(nodes, 77)
+    for mode, tol in (("mfma_f64", TOL_F64), ("split_bf16", TOL_MIXED)):
+        spec = fpde.make_spec(nodes, steps, 1e-3, alpha=0.7, hist_mode=mode, block=0, picard_max=3,
+                              picard_tol=1e-12, u_top=u0[0])
+        b, ms = fpde.autotune_block(spec, u0)
+        print(f"autotune {mode} N={nodes} M={steps}: block {b}, {ms * 1e3:.1f} us/step")
+        assert b in (16, 32, 64, 128) and ms > 0.0
+        spec.block = b
+        ref = _oracle_1d(oracle, spec, u0, picard_tol=1e-12)
+        assert rel_err(fpde.solve(spec, u0), ref) <= tol
+    spec3, v0 = fpde.workload("3d", hist_mode="split_f16", nodes=24, steps=280, block=0)
+    b3, ms3 = fpde.autotune_block(spec3, v0, trial_steps=280)
+    print(f"autotune 3d 24^3: block {b3}, {ms3 * 1e3:.1f} us/step")
+    assert b3 in (16, 32, 64, 128) and ms3 > 0.0
