@@ -99,6 +99,45 @@ __device__ inline int pcr_wave(double* A, double* Cc, double* Dd, int n) {
     return p;
 }
 
+// Chunk elimination: the interior rows q in [0, ni) of a chunk, normalised
+// to b = 1 (a_q x_{q-1} + x_q + c_q x_{q+1} = d_q) and held by thread q + 1
+// in registers, for three right-hand sides (x: the row's own; y, z: unit
+// values at the chunk ends), by parallel cyclic reduction: each thread keeps
+// its own row in registers and reads its two partners of the level from the
+// LDS ping-pong buffers.  Returns the buffer holding the solution of every
+// row.
+__device__ inline int chunk_solve(double (*pA)[256], double (*pC)[256], double (*pX)[256], double (*pY)[256],
+                                  double (*pZ)[256], int ni, int q, bool inner, double aq, double cq, double xq,
+                                  double yq, double zq) {
+    if (inner) { pA[0][q] = aq; pC[0][q] = cq; pX[0][q] = xq; pY[0][q] = yq; pZ[0][q] = zq; }
+    __syncthreads();
+    int pp = 0;
+    for (int delta = 1; delta < ni; delta <<= 1) {
+        if (inner) {
+            double am = 0, cm = 0, xm = 0, ym = 0, zm = 0, ap = 0, cp = 0, xp = 0, yp = 0, zp = 0;
+            if (q - delta >= 0) {
+                am = pA[pp][q - delta]; cm = pC[pp][q - delta];
+                xm = pX[pp][q - delta]; ym = pY[pp][q - delta]; zm = pZ[pp][q - delta];
+            }
+            if (q + delta < ni) {
+                ap = pA[pp][q + delta]; cp = pC[pp][q + delta];
+                xp = pX[pp][q + delta]; yp = pY[pp][q + delta]; zp = pZ[pp][q + delta];
+            }
+            const double inv = rcp_nr(1.0 - aq * cm - cq * ap);
+            const double na = -aq * am * inv, nc = -cq * cp * inv;
+            xq = (xq - aq * xm - cq * xp) * inv;
+            yq = (yq - aq * ym - cq * yp) * inv;
+            zq = (zq - aq * zm - cq * zp) * inv;
+            aq = na;
+            cq = nc;
+            pA[1 - pp][q] = aq; pC[1 - pp][q] = cq; pX[1 - pp][q] = xq; pY[1 - pp][q] = yq; pZ[1 - pp][q] = zq;
+        }
+        __syncthreads();
+        pp = 1 - pp;
+    }
+    return pp;
+}
+
 // Far partial sums plus the near levels t in [near_lo, n-2] already in
 // memory, gathered in 32-wide batches of independent loads (a sum over m
 // terms costs ~m/32 memory round trips).  Every load is guarded: omega is
@@ -146,6 +185,30 @@ __device__ inline double hist_tail(double om0, const double* __restrict__ corr, 
     if (n == 2) cr = fma(corr[2 * n + 1], dnew, cr);
     else if (n >= 3) cr = fma(corr[2 * n + 1], d1m, cr);
     return fma(w0, dnew, cr);
+}
+
+// one DPP move of a double (both 32-bit halves) inside the rows of 16 lanes
+template <int CTRL>
+__device__ inline double dpp_move(double v) {
+    const int lo = __builtin_amdgcn_update_dpp(__double2loint(v), __double2loint(v), CTRL, 0xf, 0xf, false);
+    const int hi = __builtin_amdgcn_update_dpp(__double2hiint(v), __double2hiint(v), CTRL, 0xf, 0xf, false);
+    return __hiloint2double(hi, lo);
+}
+
+// max over the 64 lanes of a wave, in every lane (called with all lanes
+// active): rotations inside each row of 16 lanes (DPP row_ror 8, 4, 2, 1)
+// leave every lane with its row's maximum; the four row maxima are then read
+// from lanes 0, 16, 32 and 48
+__device__ inline double wave_max(double v) {
+    v = fmax(v, dpp_move<0x128>(v));
+    v = fmax(v, dpp_move<0x124>(v));
+    v = fmax(v, dpp_move<0x122>(v));
+    v = fmax(v, dpp_move<0x121>(v));
+    auto lane_val = [&](int l) {
+        return __hiloint2double(__builtin_amdgcn_readlane(__double2hiint(v), l),
+                                __builtin_amdgcn_readlane(__double2loint(v), l));
+    };
+    return fmax(fmax(lane_val(0), lane_val(16)), fmax(lane_val(32), lane_val(48)));
 }
 
 // max over the 256 threads of a block (wave shuffles + LDS across the 4 waves)
@@ -384,38 +447,10 @@ __global__ __launch_bounds__(256) void k_step1d(Step1DArgs g) {
     const int ni = nc - 2;
     const int q = tid - 1;
     const bool inner = (tid >= 1) && (tid <= nc - 2);
-    if (inner) {
-        double ib = rcp_nr(br);
-        pA[0][q] = (tid == 1) ? 0.0 : ar * ib;
-        pC[0][q] = (tid == nc - 2) ? 0.0 : cr * ib;
-        pX[0][q] = dr * ib;
-        pY[0][q] = (tid == 1) ? -ar * ib : 0.0;
-        pZ[0][q] = (tid == nc - 2) ? -cr * ib : 0.0;
-    }
-    __syncthreads();
-    int pp = 0;
-    for (int delta = 1; delta < ni; delta <<= 1) {
-        if (inner) {
-            double aq = pA[pp][q], cq = pC[pp][q];
-            double am = 0, cm = 0, xm = 0, ym = 0, zm = 0, ap = 0, cp = 0, xp = 0, yp = 0, zp = 0;
-            if (q - delta >= 0) {
-                am = pA[pp][q - delta]; cm = pC[pp][q - delta];
-                xm = pX[pp][q - delta]; ym = pY[pp][q - delta]; zm = pZ[pp][q - delta];
-            }
-            if (q + delta < ni) {
-                ap = pA[pp][q + delta]; cp = pC[pp][q + delta];
-                xp = pX[pp][q + delta]; yp = pY[pp][q + delta]; zp = pZ[pp][q + delta];
-            }
-            double inv = rcp_nr(1.0 - aq * cm - cq * ap);
-            pA[1 - pp][q] = -aq * am * inv;
-            pC[1 - pp][q] = -cq * cp * inv;
-            pX[1 - pp][q] = (pX[pp][q] - aq * xm - cq * xp) * inv;
-            pY[1 - pp][q] = (pY[pp][q] - aq * ym - cq * yp) * inv;
-            pZ[1 - pp][q] = (pZ[pp][q] - aq * zm - cq * zp) * inv;
-        }
-        __syncthreads();
-        pp = 1 - pp;
-    }
+    double ib = inner ? rcp_nr(br) : 0.0;
+    const int pp = chunk_solve(pA, pC, pX, pY, pZ, ni, q, inner, (tid == 1) ? 0.0 : ar * ib,
+                               (tid == nc - 2) ? 0.0 : cr * ib, dr * ib, (tid == 1) ? -ar * ib : 0.0,
+                               (tid == nc - 2) ? -cr * ib : 0.0);
     if (inner) {
         g.xyz[i] = pX[pp][q];
         g.xyz[g.ld + i] = pY[pp][q];
@@ -514,7 +549,7 @@ __global__ __launch_bounds__(256) void k_step1d_block(Step1DArgs g) {
     __shared__ double rA[2][kMaxRed], rC[2][kMaxRed], rD[2][kMaxRed];
     __shared__ double pA[2][256], pC[2][256], pX[2][256], pY[2][256], pZ[2][256];
     __shared__ double sK[258];
-    __shared__ double sred[4];
+    __shared__ double sdp[4];                  // wave maxima of the last solve's |U^{k+1} - U^k|
     __shared__ int sh_ok;
     const int tid = threadIdx.x;
     const int c = blockIdx.x;
@@ -528,6 +563,7 @@ __global__ __launch_bounds__(256) void k_step1d_block(Step1DArgs g) {
     const bool series = g.ser_P > 0;
     const int halo_tid = (nc < 256) ? 255 : 0;
     const bool halo = tid == halo_tid && c > 0;
+    const int wave = tid >> 6, lane = tid & 63;
     const int ni = nc - 2;
     const int q = tid - 1;
     const bool inner = (tid >= 1) && (tid <= nc - 2);
@@ -554,14 +590,20 @@ __global__ __launch_bounds__(256) void k_step1d_block(Step1DArgs g) {
         __hip_atomic_store(g.flagbar + (size_t)(1 - g.epar) * 32 * kMaxChunks + 32 * c, 0u, __ATOMIC_RELAXED,
                            __HIP_MEMORY_SCOPE_AGENT);
 
+    if (tid < 4) sdp[tid] = 0.0;
+    // the chunk's max |U^{k+1} - U^k| of the last solve (thread 0, after a
+    // workgroup barrier that follows that solve)
+    auto last_measure = [&]() { return fmax(fmax(sdp[0], sdp[1]), fmax(sdp[2], sdp[3])); };
+
     // ---- one Picard build: coefficients at the iterate (uit, uith), rows,
     // chunk elimination (chunk factors into fx, fy, fz), the two reduced rows
-    // published with dval in the convergence slot, then the grid barrier
+    // published with the last solve's measure in the convergence slot, then
+    // the grid barrier
     double fx = 0.0, fy = 0.0, fz = 0.0;
     // the part of the next step's history sum that is already in memory
     // (far partials and near levels t < n), formed while a barrier is pending
     double hm_next = 0.0;
-    auto build = [&](double uit, double uith, double S, double Ln, double un_, double dval, int pre_n) -> bool {
+    auto build = [&](double uit, double uith, double S, double Ln, double un_, int pre_n) -> bool {
         double K = 0.0, invC = 0.0;
         if (halo) {
             double Kl, iCl;
@@ -585,38 +627,10 @@ __global__ __launch_bounds__(256) void k_step1d_block(Step1DArgs g) {
                 ar = A; br = -R; cr = B; dr = -Ln * un_ + S;
             }
         }
-        if (inner) {
-            double ib = rcp_nr(br);
-            pA[0][q] = (tid == 1) ? 0.0 : ar * ib;
-            pC[0][q] = (tid == nc - 2) ? 0.0 : cr * ib;
-            pX[0][q] = dr * ib;
-            pY[0][q] = (tid == 1) ? -ar * ib : 0.0;
-            pZ[0][q] = (tid == nc - 2) ? -cr * ib : 0.0;
-        }
-        __syncthreads();
-        int pp = 0;
-        for (int delta = 1; delta < ni; delta <<= 1) {
-            if (inner) {
-                double aq = pA[pp][q], cq = pC[pp][q];
-                double am = 0, cm = 0, xm = 0, ym = 0, zm = 0, ap = 0, cp = 0, xp = 0, yp = 0, zp = 0;
-                if (q - delta >= 0) {
-                    am = pA[pp][q - delta]; cm = pC[pp][q - delta];
-                    xm = pX[pp][q - delta]; ym = pY[pp][q - delta]; zm = pZ[pp][q - delta];
-                }
-                if (q + delta < ni) {
-                    ap = pA[pp][q + delta]; cp = pC[pp][q + delta];
-                    xp = pX[pp][q + delta]; yp = pY[pp][q + delta]; zp = pZ[pp][q + delta];
-                }
-                double inv = rcp_nr(1.0 - aq * cm - cq * ap);
-                pA[1 - pp][q] = -aq * am * inv;
-                pC[1 - pp][q] = -cq * cp * inv;
-                pX[1 - pp][q] = (pX[pp][q] - aq * xm - cq * xp) * inv;
-                pY[1 - pp][q] = (pY[pp][q] - aq * ym - cq * yp) * inv;
-                pZ[1 - pp][q] = (pZ[pp][q] - aq * zm - cq * zp) * inv;
-            }
-            __syncthreads();
-            pp = 1 - pp;
-        }
+        double ib = inner ? rcp_nr(br) : 0.0;
+        const int pp = chunk_solve(pA, pC, pX, pY, pZ, ni, q, inner, (tid == 1) ? 0.0 : ar * ib,
+                                   (tid == nc - 2) ? 0.0 : cr * ib, dr * ib, (tid == 1) ? -ar * ib : 0.0,
+                                   (tid == nc - 2) ? -cr * ib : 0.0);
         fx = 0.0; fy = 0.0; fz = 0.0;
         if (inner) { fx = pX[pp][q]; fy = pY[pp][q]; fz = pZ[pp][q]; }
         double* redw = g.red + (size_t)(bk & 1) * R2 * 4;
@@ -628,7 +642,7 @@ __global__ __launch_bounds__(256) void k_step1d_block(Step1DArgs g) {
             redw[4 * (2 * c) + 1] = br + cr * y;
             redw[4 * (2 * c) + 2] = cr * z;
             redw[4 * (2 * c) + 3] = dr - cr * x;
-            g.dmax[(size_t)(bk & 1) * C + c] = dval;
+            g.dmax[(size_t)(bk & 1) * C + c] = last_measure();
         }
         if (tid == nc - 1) {
             double x, y, z;
@@ -650,8 +664,11 @@ __global__ __launch_bounds__(256) void k_step1d_block(Step1DArgs g) {
     // the decision on the published convergence slots (check 1: U^k met the
     // tolerance -> return 1, nothing solved; check 2: the tentatively ended
     // previous step did not -> return 2, nothing solved); then the reduced
-    // solve and back-substitution (uit, uith, dpub updated), return 0
-    auto solve = [&](int check, double& uit, double& uith, double& dpub) -> int {
+    // solve and back-substitution (uit, uith updated), return 0.  Every wave
+    // forms the decision's maximum itself, so it is uniform without a
+    // workgroup barrier.  The chunk's max |U^{k+1} - U^k| is left as four
+    // wave maxima in sdp for thread 0, which publishes it with the next build.
+    auto solve = [&](int check, double& uit, double& uith) -> int {
         const double* redr = g.red + (size_t)((bk - 1) & 1) * R2 * 4;
         double rrow[2][4];
 #pragma unroll
@@ -663,8 +680,8 @@ __global__ __launch_bounds__(256) void k_step1d_block(Step1DArgs g) {
         }
         if (check) {
             double dv = 0.0;
-            for (int j = tid; j < C; j += 256) dv = fmax(dv, g.dmax[(size_t)((bk - 1) & 1) * C + j]);
-            const bool met = block_max(dv, sred) <= g.tol;            // uniform decision
+            for (int j = lane; j < C; j += 64) dv = fmax(dv, g.dmax[(size_t)((bk - 1) & 1) * C + j]);
+            const bool met = wave_max(dv) <= g.tol;
             if (check == 1 && met) return 1;
             if (check == 2 && !met) return 2;
         }
@@ -679,16 +696,16 @@ __global__ __launch_bounds__(256) void k_step1d_block(Step1DArgs g) {
             }
         }
         __syncthreads();
-        int pr = pcr_norm(rA, rC, rD, R2);
+        const double* xs = &rD[pcr_norm(rA, rC, rD, R2)][0];
         double unew = 0.0;
         if (act) {
-            if (i == s) unew = rD[pr][2 * c];
-            else if (i == e) unew = rD[pr][2 * c + 1];
-            else unew = fx + fy * rD[pr][2 * c] + fz * rD[pr][2 * c + 1];
+            if (i == s) unew = xs[2 * c];
+            else if (i == e) unew = xs[2 * c + 1];
+            else unew = fx + fy * xs[2 * c] + fz * xs[2 * c + 1];
         }
-        double unewh = halo ? rD[pr][2 * c - 1] : 0.0;
-        double d = act ? fabs(unew - uit) : 0.0;
-        dpub = block_max(d, sred);               // also orders the LDS reads before reuse
+        double unewh = halo ? xs[2 * c - 1] : 0.0;
+        const double d = wave_max(act ? fabs(unew - uit) : 0.0);
+        if (lane == 0) sdp[wave] = d;
         uit = unew;
         uith = unewh;
         return 0;
@@ -717,7 +734,6 @@ __global__ __launch_bounds__(256) void k_step1d_block(Step1DArgs g) {
     // U, and starts the next step again.  Series moments are updated in
     // place, so sum_alg 2 decides convergence before the step ends.
     bool pending = false;          // the previous step ended tentatively
-    double pend_d = 0.0;           // its convergence measure (this chunk)
     int sv_n = 0;                  // that step's state, for a rollback
     double sv_un = 0.0, sv_uh = 0.0, sv_dh = 0.0, sv_dlast = 0.0, sv_d0m = 0.0, sv_d1m = 0.0;
     double sv_S = 0.0, sv_Ln = 0.0, sv_uit = 0.0, sv_uith = 0.0;
@@ -747,7 +763,6 @@ __global__ __launch_bounds__(256) void k_step1d_block(Step1DArgs g) {
         // Picard starts from 2 U^n - U^{n-1} (a single pass: lagged at U^n)
         double uit = un, uith = uh;
         if (g.extrap && n >= 1) { uit = un + dlast; uith = uh + dh; }
-        double dpub = 0.0;                   // max |U^k - U^{k-1}| of the chunk, k >= 1
         bool tentative = false, rollback = false;
         for (int k = 0; k < g.picard_max; k++) {
             const bool last_due = k >= 1 && k == g.picard_max - 1;
@@ -758,16 +773,17 @@ __global__ __launch_bounds__(256) void k_step1d_block(Step1DArgs g) {
             if (last_due) {
                 // the last build is skipped when U^k already met the
                 // tolerance: decide that first (one barrier on the measure)
-                if (tid == 0) g.dmax[(size_t)(bk & 1) * C + c] = dpub;
+                __syncthreads();
+                if (tid == 0) g.dmax[(size_t)(bk & 1) * C + c] = last_measure();
                 if (!flag_sync(fl, c, C, (unsigned)(bk + 1), g.err, &sh_ok)) return;
                 bk++;
                 double dv = 0.0;
-                for (int j = tid; j < C; j += 256) dv = fmax(dv, g.dmax[(size_t)((bk - 1) & 1) * C + j]);
-                if (block_max(dv, sred) <= g.tol) break;      // U^k is the result (uniform decision)
+                for (int j = lane; j < C; j += 64) dv = fmax(dv, g.dmax[(size_t)((bk - 1) & 1) * C + j]);
+                if (wave_max(dv) <= g.tol) break;             // U^k is the result (uniform decision)
             }
             const bool pend_check = k == 0 && pending;
-            if (!build(uit, uith, S, Ln, un, pend_check ? pend_d : dpub, k == 0 ? pre_n : -1)) return;
-            const int r = solve(pend_check ? 2 : ((k >= 1 && !last_due) ? 1 : 0), uit, uith, dpub);
+            if (!build(uit, uith, S, Ln, un, k == 0 ? pre_n : -1)) return;
+            const int r = solve(pend_check ? 2 : ((k >= 1 && !last_due) ? 1 : 0), uit, uith);
             if (r == 1) break;                                 // U^k is the result
             if (r == 2) { rollback = true; break; }
             pending = false;
@@ -775,9 +791,9 @@ __global__ __launch_bounds__(256) void k_step1d_block(Step1DArgs g) {
         if (rollback) {
             // the tentatively ended step sv_n needs its last iteration
             un = sv_un; uh = sv_uh; dh = sv_dh; dlast = sv_dlast; d0m = sv_d0m; d1m = sv_d1m;
-            double ruit = sv_uit, ruith = sv_uith, rd = 0.0;
-            if (!build(ruit, ruith, sv_S, sv_Ln, un, 0.0, -1)) return;
-            solve(0, ruit, ruith, rd);
+            double ruit = sv_uit, ruith = sv_uith;
+            if (!build(ruit, ruith, sv_S, sv_Ln, un, -1)) return;
+            solve(0, ruit, ruith);
             finish(sv_n, ruit, ruith);
             pending = false;
             n = sv_n + 1;
@@ -787,7 +803,6 @@ __global__ __launch_bounds__(256) void k_step1d_block(Step1DArgs g) {
             sv_n = n; sv_un = un; sv_uh = uh; sv_dh = dh; sv_dlast = dlast; sv_d0m = d0m; sv_d1m = d1m;
             sv_S = S; sv_Ln = Ln; sv_uit = uit; sv_uith = uith;
             pending = true;
-            pend_d = dpub;
         }
         // ---- step n is done: the last iterate is U^{n+1} -------------------
         finish(n, uit, uith);
