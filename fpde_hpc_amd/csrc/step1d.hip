@@ -27,25 +27,36 @@ namespace fpde {
 constexpr int kMaxRed = 512;   // reduced rows (2C) per launch: N <= 65536
 
 // Parallel cyclic reduction on rows normalised to b = 1 (row r: a_r x_{r-1}
-// + x_r + c_r x_{r+1} = d_r), 256 threads, two rows per thread, ping-pong LDS
-// buffers so each level costs one barrier and one reciprocal per row.
-// Rows outside [0,n) are identity rows.  Returns the buffer holding x in D.
+// + x_r + c_r x_{r+1} = d_r), 256 threads, two rows per thread kept in
+// registers, ping-pong LDS buffers for the partners, so each level costs one
+// barrier and one reciprocal per row.  Rows outside [0,n) are identity rows.
+// Returns the buffer holding x in D.
 __device__ inline int pcr_norm(double (*A)[kMaxRed], double (*Cc)[kMaxRed], double (*Dd)[kMaxRed], int n) {
     const int tid = threadIdx.x;
+    double ar[2], cr[2], dr[2];
+#pragma unroll
+    for (int j = 0; j < 2; j++) {
+        const int r = tid + 256 * j;
+        ar[j] = r < n ? A[0][r] : 0.0;
+        cr[j] = r < n ? Cc[0][r] : 0.0;
+        dr[j] = r < n ? Dd[0][r] : 0.0;
+    }
     int p = 0;
     for (int delta = 1; delta < n; delta <<= 1) {
 #pragma unroll
         for (int j = 0; j < 2; j++) {
             int r = tid + 256 * j;
             if (r < n) {
-                double ar = A[p][r], cr = Cc[p][r], dr = Dd[p][r];
                 double am = 0.0, cm = 0.0, dm = 0.0, ap = 0.0, cp = 0.0, dp = 0.0;
                 if (r - delta >= 0) { am = A[p][r - delta]; cm = Cc[p][r - delta]; dm = Dd[p][r - delta]; }
                 if (r + delta < n) { ap = A[p][r + delta]; cp = Cc[p][r + delta]; dp = Dd[p][r + delta]; }
-                double inv = rcp_nr(1.0 - ar * cm - cr * ap);
-                A[1 - p][r] = -ar * am * inv;
-                Cc[1 - p][r] = -cr * cp * inv;
-                Dd[1 - p][r] = (dr - ar * dm - cr * dp) * inv;
+                double inv = rcp_nr(1.0 - ar[j] * cm - cr[j] * ap);
+                dr[j] = (dr[j] - ar[j] * dm - cr[j] * dp) * inv;
+                ar[j] = -ar[j] * am * inv;
+                cr[j] = -cr[j] * cp * inv;
+                A[1 - p][r] = ar[j];
+                Cc[1 - p][r] = cr[j];
+                Dd[1 - p][r] = dr[j];
             }
         }
         __syncthreads();
@@ -178,12 +189,13 @@ __device__ inline double hist_mem(const Step1DArgs& g, int i, int n, int64_t par
 // written with explicit fma so that every kernel rounds them alike:
 // om0 dnew + corr[2n] d0 + corr[2n+1] d1, with d0 = dnew at n = 1 and
 // d1 = dnew at n = 2.
-__device__ inline double hist_tail(double om0, const double* __restrict__ corr, int n, int near_lo, double dnew,
-                                   double d0m, double d1m) {
+// c0 = corr[2n], c1 = corr[2n+1].
+__device__ inline double hist_tail(double om0, double c0, double c1, int n, int near_lo, double dnew, double d0m,
+                                   double d1m) {
     const double w0 = (near_lo <= n - 1) ? om0 : 0.0;
-    double cr = (n == 1) ? corr[2] * dnew : corr[2 * n] * (n >= 2 ? d0m : 0.0);
-    if (n == 2) cr = fma(corr[2 * n + 1], dnew, cr);
-    else if (n >= 3) cr = fma(corr[2 * n + 1], d1m, cr);
+    double cr = (n == 1) ? c0 * dnew : c0 * (n >= 2 ? d0m : 0.0);
+    if (n == 2) cr = fma(c1, dnew, cr);
+    else if (n >= 3) cr = fma(c1, d1m, cr);
     return fma(w0, dnew, cr);
 }
 
@@ -211,9 +223,9 @@ __device__ inline double wave_max(double v) {
     return fmax(fmax(lane_val(0), lane_val(16)), fmax(lane_val(32), lane_val(48)));
 }
 
-// max over the 256 threads of a block (wave shuffles + LDS across the 4 waves)
+// max over the 256 threads of a block (wave maxima + LDS across the 4 waves)
 __device__ inline double block_max(double v, double* sh) {
-    for (int off = 32; off > 0; off >>= 1) v = fmax(v, __shfl_xor(v, off));
+    v = wave_max(v);
     if ((threadIdx.x & 63) == 0) sh[threadIdx.x >> 6] = v;
     __syncthreads();
     v = fmax(fmax(sh[0], sh[1]), fmax(sh[2], sh[3]));
@@ -347,7 +359,7 @@ __global__ __launch_bounds__(256) void k_step1d(Step1DArgs g) {
                 S = series_update(g.H, g.ld, i, g.ser_P, dnew, g.ser_c, g.ser_v);
             } else if (g.k >= 0 && n >= 1) {
                 // S_n = far + near (in memory) + newest level + two-level corrections
-                S = hm + hist_tail(g.omega[0], g.corr, n, g.near_lo, dnew, d0m, d1m);
+                S = hm + hist_tail(g.omega[0], g.corr[2 * n], g.corr[2 * n + 1], n, g.near_lo, dnew, d0m, d1m);
             }
             if (g.k >= 0) g.Sbuf[i] = S;
             // Picard starts from the extrapolation 2 U^n - U^{n-1}; the
@@ -542,7 +554,7 @@ __device__ inline bool flag_sync(unsigned* fl, int c, int C, unsigned e, int* er
 __device__ inline double hist_step(const Step1DArgs& g, int i, int n, double dnew, double d0m, double d1m) {
     const double* om = g.omega + (int64_t)(n - g.n0) * g.omega_stride;
     double hm = hist_mem(g, i, n, (int64_t)(n - g.n0) * g.ld, om);
-    return hm + hist_tail(om[0], g.corr, n, g.near_lo, dnew, d0m, d1m);
+    return hm + hist_tail(om[0], g.corr[2 * n], g.corr[2 * n + 1], n, g.near_lo, dnew, d0m, d1m);
 }
 
 __global__ __launch_bounds__(256) void k_step1d_block(Step1DArgs g) {
@@ -603,6 +615,7 @@ __global__ __launch_bounds__(256) void k_step1d_block(Step1DArgs g) {
     // the part of the next step's history sum that is already in memory
     // (far partials and near levels t < n), formed while a barrier is pending
     double hm_next = 0.0;
+    double tl_next[3] = {0.0, 0.0, 0.0};       // and its tail weights: omega[0], corr[2n], corr[2n+1]
     auto build = [&](double uit, double uith, double S, double Ln, double un_, int pre_n) -> bool {
         double K = 0.0, invC = 0.0;
         if (halo) {
@@ -654,8 +667,13 @@ __global__ __launch_bounds__(256) void k_step1d_block(Step1DArgs g) {
             redw[4 * (2 * c + 1) + 3] = dr - ar * x;
         }
         flag_arrive(fl, c, (unsigned)(bk + 1));
-        if (pre_n >= 0 && act)
-            hm_next = hist_mem(g, i, pre_n, (int64_t)(pre_n - g.n0) * g.ld, g.omega + (int64_t)(pre_n - g.n0) * g.omega_stride);
+        if (pre_n >= 0 && act) {
+            const double* om = g.omega + (int64_t)(pre_n - g.n0) * g.omega_stride;
+            tl_next[0] = om[0];
+            tl_next[1] = g.corr[2 * pre_n];
+            tl_next[2] = g.corr[2 * pre_n + 1];
+            hm_next = hist_mem(g, i, pre_n, (int64_t)(pre_n - g.n0) * g.ld, om);
+        }
         if (!flag_wait(fl, C, (unsigned)(bk + 1), g.err, &sh_ok)) return false;
         bk++;
         return true;
@@ -738,6 +756,7 @@ __global__ __launch_bounds__(256) void k_step1d_block(Step1DArgs g) {
     double sv_un = 0.0, sv_uh = 0.0, sv_dh = 0.0, sv_dlast = 0.0, sv_d0m = 0.0, sv_d1m = 0.0;
     double sv_S = 0.0, sv_Ln = 0.0, sv_uit = 0.0, sv_uith = 0.0;
     double hm_cur = 0.0;           // that part for the current step
+    double tl_cur[3] = {0.0, 0.0, 0.0};
     bool have_hm = false;
     int n = n_first;
     while (n < g.n_end) {
@@ -751,8 +770,7 @@ __global__ __launch_bounds__(256) void k_step1d_block(Step1DArgs g) {
                 S = series_update(g.H, g.ld, i, g.ser_P, dlast, g.ser_c + (size_t)n * g.ser_P,
                                   g.ser_v + (size_t)n * g.ser_P);
             } else if (have_hm) {
-                const double* om = g.omega + (int64_t)(n - g.n0) * g.omega_stride;
-                S = hm_cur + hist_tail(om[0], g.corr, n, g.near_lo, dlast, d0m, d1m);
+                S = hm_cur + hist_tail(tl_cur[0], tl_cur[1], tl_cur[2], n, g.near_lo, dlast, d0m, d1m);
             } else {
                 S = hist_step(g, i, n, dlast, d0m, d1m);
             }
@@ -807,6 +825,7 @@ __global__ __launch_bounds__(256) void k_step1d_block(Step1DArgs g) {
         // ---- step n is done: the last iterate is U^{n+1} -------------------
         finish(n, uit, uith);
         hm_cur = hm_next;
+        tl_cur[0] = tl_next[0]; tl_cur[1] = tl_next[1]; tl_cur[2] = tl_next[2];
         have_hm = pre_n >= 0;
         n++;
     }
