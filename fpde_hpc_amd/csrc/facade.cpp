@@ -414,7 +414,7 @@ fpde_status fpde_history_block_gemm(int mode, const double* D, int64_t ld, int64
     if (e != hipSuccess) { rc = FPDE_ERR_NOMEM; set_error("hipMalloc failed"); }
     if (rc == FPDE_OK && split) rc = launch_pack_split(mode, D, ld, (int)nodes, 0, tc / kGroup, comp[0], comp[1], comp[2], scale, 0);
     if (rc == FPDE_OK) {
-        if (split) rc = launch_gemm_split(mode, comp[0], comp[1], comp[2], scale, ld, (int)nodes, n0, tc, block, omega, part, &splits, 0);
+        if (split) rc = launch_gemm_split(mode, comp[0], comp[1], comp[2], scale, ld, (int)nodes, n0, tc, block, omega, nullptr, 0, part, &splits, 0);
         else rc = launch_gemm_f64(D, ld, (int)nodes, n0, tc, block, omega, part, &splits, 0);
     }
     if (rc == FPDE_OK) {

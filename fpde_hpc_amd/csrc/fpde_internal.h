@@ -70,6 +70,30 @@ __device__ inline double rcp_nr(double x) {
     return fma(r, e, r);
 }
 
+// one DPP move of a double (both 32-bit halves) inside the rows of 16 lanes
+template <int CTRL>
+__device__ inline double dpp_move(double v) {
+    const int lo = __builtin_amdgcn_update_dpp(__double2loint(v), __double2loint(v), CTRL, 0xf, 0xf, false);
+    const int hi = __builtin_amdgcn_update_dpp(__double2hiint(v), __double2hiint(v), CTRL, 0xf, 0xf, false);
+    return __hiloint2double(hi, lo);
+}
+
+// max over the 64 lanes of a wave, in every lane (called with all lanes
+// active): rotations inside each row of 16 lanes (DPP row_ror 8, 4, 2, 1)
+// leave every lane with its row's maximum; the four row maxima are then read
+// from lanes 0, 16, 32 and 48
+__device__ inline double wave_max(double v) {
+    v = fmax(v, dpp_move<0x128>(v));
+    v = fmax(v, dpp_move<0x124>(v));
+    v = fmax(v, dpp_move<0x122>(v));
+    v = fmax(v, dpp_move<0x121>(v));
+    auto lane_val = [&](int l) {
+        return __hiloint2double(__builtin_amdgcn_readlane(__double2hiint(v), l),
+                                __builtin_amdgcn_readlane(__double2loint(v), l));
+    };
+    return fmax(fmax(lane_val(0), lane_val(16)), fmax(lane_val(32), lane_val(48)));
+}
+
 // ---------------------------------------------------------------------------
 // Weight sets on the device for a uniform step (see weights.hip):
 //   local   : coefficient of (U^{n+1}-U^n) in the discrete derivative
@@ -319,9 +343,11 @@ fpde_status launch_pack_split(int mode, const double* D, int64_t ld, int nodes, 
                               void* c1, void* c2, float* scale, hipStream_t st);
 fpde_status launch_gemm_f64_dense(const double* D, int64_t ld, int nodes, int tc, int block, const double* W,
                                   int64_t ldw, double* P, int* splits_out, hipStream_t st);
+// W != nullptr: dense weight rows W[r][t] (leading dimension ldw >= tc) of a
+// prescribed grid in place of the Toeplitz vector omega.
 fpde_status launch_gemm_split(int mode, const void* c0, const void* c1, const void* c2, const float* scale,
-                              int64_t ld, int nodes, int n0, int tc, int block, const double* omega, double* P,
-                              int* splits_out, hipStream_t st);
+                              int64_t ld, int nodes, int n0, int tc, int block, const double* omega,
+                              const double* W, int64_t ldw, double* P, int* splits_out, hipStream_t st);
 fpde_status launch_step1d(const Step1DArgs& a, hipStream_t st);
 fpde_status launch_step1d_block(const Step1DArgs& a, hipStream_t st);
 bool step1d_block_supported(int C);

@@ -322,15 +322,17 @@ __global__ __launch_bounds__(256) void k_pack_split(const double* __restrict__ D
 
 // Split GEMM.  Grid: x = 64-node tiles, y = K splits.  Per 32-level group:
 // wave w (nodes 16w..16w+15) loads its B fragments (one 16-B load per
-// component), the A fragments (Toeplitz weights, split the same way) come
-// from LDS where the block builds them for 4 groups at a time, and the
-// products are accumulated in fp32 by magnitude class, then flushed to fp64.
-template <typename T, int NS, int BR>
+// component), the A fragments (Toeplitz weights omega[n0+r-t-1], or with
+// DENSE the rows W[r][t] of a prescribed grid, split the same way) come from
+// LDS where the block builds them for 4 groups at a time, and the products
+// are accumulated in fp32 by magnitude class, then flushed to fp64.
+template <typename T, int NS, int BR, bool DENSE>
 __global__ __launch_bounds__(256) void k_gemm_split(const T* __restrict__ c0, const T* __restrict__ c1,
                                                     const T* __restrict__ c2,
                                                     const float* __restrict__ scale, int64_t ld,
                                                     int nodes, int n0, int tc, int kchunk,
                                                     const double* __restrict__ omega,
+                                                    const double* __restrict__ W, int64_t ldw,
                                                     double* __restrict__ P) {
     typedef typename SplitTraits<T>::vec vec_t;
     constexpr int RT = BR / 16;
@@ -364,17 +366,31 @@ __global__ __launch_bounds__(256) void k_gemm_split(const T* __restrict__ c0, co
             if (SplitTraits<T>::scaled) bsc[gi] = node_ok ? (double)scale[(int64_t)g * ld + node] : 0.0;
         }
         __syncthreads();
-        // build A fragments: entries (gb, rt, lane) -> 8 weights
+        // build A fragments: entries (gb, rt, lane) -> 8 weights.  Each wave
+        // builds one whole fragment per pass (GB * RT * 64 is a multiple of
+        // 256), so the dense form's fragment maximum is a wave maximum.
         for (int f = threadIdx.x; f < GB * RT * 64; f += 256) {
             int fl = f & 63, frt = (f >> 6) % RT, fgb = f / (64 * RT);
             int g = gb0 + fgb;
             int r = frt * 16 + (fl & 15);
             int tbase = g * kGroup + 8 * (fl >> 4);
+            double wd[8];
+            if constexpr (DENSE) {
+#pragma unroll
+                for (int e = 0; e < 8; e++) wd[e] = (g < ghi) ? W[(int64_t)r * ldw + tbase + e] : 0.0;
+            }
             // fp16: power-of-two scale per fragment so that its largest weight
-            // sits in [2^14, 2^15) and the lo parts stay normal.  The weights
-            // decrease with the lag n0+r-t, so the extremes are at the corners.
+            // sits in [2^14, 2^15) and the lo parts stay normal.  Toeplitz
+            // weights decrease with the lag n0+r-t, so the extremes are at the
+            // corners; dense rows take the maximum over the fragment.
             double sa = 1.0;
-            if (SplitTraits<T>::scaled && g < ghi) {
+            if constexpr (SplitTraits<T>::scaled && DENSE) {
+                double mx = 0.0;
+#pragma unroll
+                for (int e = 0; e < 8; e++) mx = fmax(mx, fabs(wd[e]));
+                mx = wave_max(mx);
+                if (mx > 0.0) sa = ldexp(1.0, 14 - ilogb(mx));
+            } else if (SplitTraits<T>::scaled && g < ghi) {
                 int lag_lo = n0 + frt * 16 - (g * kGroup + kGroup - 1);
                 int lag_hi = n0 + frt * 16 + 15 - g * kGroup;
                 double wmax = fmax(fabs(omega[lag_lo - 1]), fabs(omega[lag_hi - 1]));
@@ -386,7 +402,8 @@ __global__ __launch_bounds__(256) void k_gemm_split(const T* __restrict__ c0, co
             for (int e = 0; e < 8; e++) {
                 double wv = 0.0;
                 int t = tbase + e;
-                if (g < ghi) wv = omega[n0 + r - t - 1] * sa;
+                if constexpr (DENSE) wv = wd[e] * sa;
+                else if (g < ghi) wv = omega[n0 + r - t - 1] * sa;
                 T hh = (T)(float)wv;
                 double rr = wv - (double)(float)hh;
                 T mm = (T)(float)rr;
@@ -581,20 +598,25 @@ fpde_status launch_pack_split(int mode, const double* D, int64_t ld, int nodes, 
 template <typename T, int NS, int BR>
 static void gemm_split_t(dim3 grid, hipStream_t st, const void* c0, const void* c1, const void* c2,
                          const float* scale, int64_t ld, int nodes, int n0, int tc, int kchunk,
-                         const double* omega, double* P) {
-    hipLaunchKernelGGL((k_gemm_split<T, NS, BR>), grid, dim3(256), 0, st, (const T*)c0, (const T*)c1,
-                       (const T*)c2, scale, ld, nodes, n0, tc, kchunk, omega, P);
+                         const double* omega, const double* W, int64_t ldw, double* P) {
+    if (W)
+        hipLaunchKernelGGL((k_gemm_split<T, NS, BR, true>), grid, dim3(256), 0, st, (const T*)c0, (const T*)c1,
+                           (const T*)c2, scale, ld, nodes, n0, tc, kchunk, omega, W, ldw, P);
+    else
+        hipLaunchKernelGGL((k_gemm_split<T, NS, BR, false>), grid, dim3(256), 0, st, (const T*)c0, (const T*)c1,
+                           (const T*)c2, scale, ld, nodes, n0, tc, kchunk, omega, W, ldw, P);
 }
 
 template <typename T, int NS>
 static fpde_status gemm_split_b(int block, dim3 grid, hipStream_t st, const void* c0, const void* c1,
                                 const void* c2, const float* scale, int64_t ld, int nodes, int n0,
-                                int tc, int kchunk, const double* omega, double* P) {
+                                int tc, int kchunk, const double* omega, const double* W, int64_t ldw,
+                                double* P) {
     switch (block) {
-        case 16: gemm_split_t<T, NS, 16>(grid, st, c0, c1, c2, scale, ld, nodes, n0, tc, kchunk, omega, P); break;
-        case 32: gemm_split_t<T, NS, 32>(grid, st, c0, c1, c2, scale, ld, nodes, n0, tc, kchunk, omega, P); break;
-        case 64: gemm_split_t<T, NS, 64>(grid, st, c0, c1, c2, scale, ld, nodes, n0, tc, kchunk, omega, P); break;
-        case 128: gemm_split_t<T, NS, 128>(grid, st, c0, c1, c2, scale, ld, nodes, n0, tc, kchunk, omega, P); break;
+        case 16: gemm_split_t<T, NS, 16>(grid, st, c0, c1, c2, scale, ld, nodes, n0, tc, kchunk, omega, W, ldw, P); break;
+        case 32: gemm_split_t<T, NS, 32>(grid, st, c0, c1, c2, scale, ld, nodes, n0, tc, kchunk, omega, W, ldw, P); break;
+        case 64: gemm_split_t<T, NS, 64>(grid, st, c0, c1, c2, scale, ld, nodes, n0, tc, kchunk, omega, W, ldw, P); break;
+        case 128: gemm_split_t<T, NS, 128>(grid, st, c0, c1, c2, scale, ld, nodes, n0, tc, kchunk, omega, W, ldw, P); break;
         default: set_error("block must be 16, 32, 64 or 128"); return FPDE_ERR_ARG;
     }
     return FPDE_OK;
@@ -602,17 +624,19 @@ static fpde_status gemm_split_b(int block, dim3 grid, hipStream_t st, const void
 
 fpde_status launch_gemm_split(int mode, const void* c0, const void* c1, const void* c2,
                               const float* scale, int64_t ld, int nodes, int n0, int tc, int block,
-                              const double* omega, double* P, int* splits_out, hipStream_t st) {
+                              const double* omega, const double* W, int64_t ldw, double* P, int* splits_out,
+                              hipStream_t st) {
     if (tc <= 0) { *splits_out = 0; return FPDE_OK; }
     if (tc % kGroup != 0) { set_error("split GEMM needs a group-aligned cut"); return FPDE_ERR_ARG; }
+    if (W && ldw < tc) { set_error("split GEMM: dense weight rows shorter than the cut"); return FPDE_ERR_ARG; }
     int kchunk, splits;
     gemm_split_plan(tc, nodes, &kchunk, &splits);
     dim3 grid((nodes + 63) / 64, splits);
     fpde_status rc;
     switch (mode) {
-        case HIST_SPLIT_BF16: rc = gemm_split_b<__bf16, 2>(block, grid, st, c0, c1, c2, scale, ld, nodes, n0, tc, kchunk, omega, P); break;
-        case HIST_SPLIT_F16: rc = gemm_split_b<_Float16, 2>(block, grid, st, c0, c1, c2, scale, ld, nodes, n0, tc, kchunk, omega, P); break;
-        case HIST_SPLIT_BF16X3: rc = gemm_split_b<__bf16, 3>(block, grid, st, c0, c1, c2, scale, ld, nodes, n0, tc, kchunk, omega, P); break;
+        case HIST_SPLIT_BF16: rc = gemm_split_b<__bf16, 2>(block, grid, st, c0, c1, c2, scale, ld, nodes, n0, tc, kchunk, omega, W, ldw, P); break;
+        case HIST_SPLIT_F16: rc = gemm_split_b<_Float16, 2>(block, grid, st, c0, c1, c2, scale, ld, nodes, n0, tc, kchunk, omega, W, ldw, P); break;
+        case HIST_SPLIT_BF16X3: rc = gemm_split_b<__bf16, 3>(block, grid, st, c0, c1, c2, scale, ld, nodes, n0, tc, kchunk, omega, W, ldw, P); break;
         default: set_error("bad split mode"); return FPDE_ERR_ARG;
     }
     *splits_out = splits;

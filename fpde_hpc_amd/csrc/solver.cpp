@@ -67,10 +67,6 @@ fpde_status validate(const fpde_spec& s) {
     }
     if (s.taus) {
         if (s.dims != 1) { set_error("a prescribed time grid (taus) is for 1D solves"); return FPDE_ERR_ARG; }
-        if (s.hist_mode != HIST_PLAIN_F64 && s.hist_mode != HIST_MFMA_F64) {
-            set_error("a prescribed time grid needs hist_mode 0 (plain) or 1 (FP64 MFMA with dense weights)");
-            return FPDE_ERR_ARG;
-        }
         if (s.sum_alg != 0) { set_error("a prescribed time grid needs sum_alg 0"); return FPDE_ERR_ARG; }
         for (int k = 0; k < s.steps; k++)
             if (!(s.taus[k] > 0.0 && std::isfinite(s.taus[k]))) { set_error("taus must be positive and finite"); return FPDE_ERR_ARG; }
@@ -278,7 +274,7 @@ fpde_status solver_alloc(fpde_solver* s) {
         FPDE_HIP_TRY(hipMemcpyAsync(s->w.Time, T.data(), sizeof(double) * (M + 2), hipMemcpyHostToDevice, s->stream));
         FPDE_HIP_TRY(hipMemcpyAsync(s->w.taus, tt.data(), sizeof(double) * (M + 1), hipMemcpyHostToDevice, s->stream));
         DALLOC(s->Wrow, sizeof(double) * (size_t)(M + 64));
-        if (sp.hist_mode == HIST_MFMA_F64) {
+        if (sp.hist_mode != HIST_PLAIN_F64) {      // dense GEMM rows: FP64 MFMA and split forms
             s->ldwf = round_up(M + 64, 64);
             s->ldwn = sp.block + 64;
             DALLOC(s->Wfar, sizeof(double) * (size_t)sp.block * s->ldwf);
@@ -428,10 +424,18 @@ fpde_status far_part(fpde_solver* s, int n, int* nsplit, int64_t* stride, int64_
             }
             size_t cb = 2 * (size_t)t_lo * p.ld;     // 2-byte components, group-major
             auto cp = [&](int c) { return s->comp[c] ? (const void*)((const char*)s->comp[c] + cb) : nullptr; };
+            if (s->w.grid) {
+                // prescribed grid (t_lo = 0: every lag kept): dense far rows
+                // for the GEMM, reversed near rows for the step kernels
+                if (rc == FPDE_OK)
+                    rc = launch_weights_grid(sp, s->w, n, B, std::max(tc, 1), 0, 0, s->Wfar, s->ldwf, s->stream);
+                if (rc == FPDE_OK)
+                    rc = launch_weights_grid(sp, s->w, n, B, B + 32, tc, 1, s->Wnear, s->ldwn, s->stream);
+            }
             if (rc == FPDE_OK)
                 rc = launch_gemm_split(sp.hist_mode, cp(0), cp(1), cp(2), s->scale + (size_t)(t_lo / kGroup) * p.ld,
-                                       p.ld, (int)p.nodes, n - t_lo, tc - t_lo, B, s->w.omega, s->part, &s->gsplits,
-                                       s->stream);
+                                       p.ld, (int)p.nodes, n - t_lo, tc - t_lo, B, s->w.omega,
+                                       s->w.grid ? s->Wfar : nullptr, s->ldwf, s->part, &s->gsplits, s->stream);
         } else if (s->w.grid) {
             // prescribed grid: dense weights of the block, far (t < tc) and
             // near (t in [tc, n), one reversed row per step)

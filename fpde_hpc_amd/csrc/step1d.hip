@@ -199,30 +199,6 @@ __device__ inline double hist_tail(double om0, double c0, double c1, int n, int 
     return fma(w0, dnew, cr);
 }
 
-// one DPP move of a double (both 32-bit halves) inside the rows of 16 lanes
-template <int CTRL>
-__device__ inline double dpp_move(double v) {
-    const int lo = __builtin_amdgcn_update_dpp(__double2loint(v), __double2loint(v), CTRL, 0xf, 0xf, false);
-    const int hi = __builtin_amdgcn_update_dpp(__double2hiint(v), __double2hiint(v), CTRL, 0xf, 0xf, false);
-    return __hiloint2double(hi, lo);
-}
-
-// max over the 64 lanes of a wave, in every lane (called with all lanes
-// active): rotations inside each row of 16 lanes (DPP row_ror 8, 4, 2, 1)
-// leave every lane with its row's maximum; the four row maxima are then read
-// from lanes 0, 16, 32 and 48
-__device__ inline double wave_max(double v) {
-    v = fmax(v, dpp_move<0x128>(v));
-    v = fmax(v, dpp_move<0x124>(v));
-    v = fmax(v, dpp_move<0x122>(v));
-    v = fmax(v, dpp_move<0x121>(v));
-    auto lane_val = [&](int l) {
-        return __hiloint2double(__builtin_amdgcn_readlane(__double2hiint(v), l),
-                                __builtin_amdgcn_readlane(__double2loint(v), l));
-    };
-    return fmax(fmax(lane_val(0), lane_val(16)), fmax(lane_val(32), lane_val(48)));
-}
-
 // max over the 256 threads of a block (wave maxima + LDS across the 4 waves)
 __device__ inline double block_max(double v, double* sh) {
     v = wave_max(v);

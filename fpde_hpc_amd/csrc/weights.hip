@@ -176,15 +176,17 @@ __device__ inline double kb_grid(double a, double p0, double p1, double pa, int 
 }
 
 __global__ void k_weights_grid(OrderSet os, const double* __restrict__ Time, const double* __restrict__ taus,
-                               int n_first, int rows, int len, int t_lo, int reversed, int series, double eps,
-                               double* __restrict__ W, int64_t ldw) {
+                               int M, int n_first, int rows, int len, int t_lo, int reversed, int series,
+                               double eps, double* __restrict__ W, int64_t ldw) {
     int64_t idx = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (idx >= (int64_t)rows * len) return;
     int r = (int)(idx / len), j = (int)(idx % len);
     int n = n_first + r;
     int t = reversed ? n - 1 - j : j;
     double w = 0.0;
-    if (t >= t_lo && t >= 0 && t < n) {
+    // rows of a block past the grid (n > M) stay zero: Time holds the M + 2
+    // points Time[0..M+1]
+    if (n <= M && t >= t_lo && t >= 0 && t < n) {
         for (int q = 0; q < os.q; q++) w += os.c[q] * kb_grid(os.a[q], Time[t], Time[t + 1], Time[n + 1], t, series, eps);
         w /= taus[t];
     }
@@ -292,7 +294,7 @@ fpde_status launch_weights_grid(const fpde_spec& s, const DeviceWeights& w, int 
     if (rc != FPDE_OK) return rc;
     int64_t total = (int64_t)rows * len;
     hipLaunchKernelGGL(k_weights_grid, dim3((unsigned)((total + 255) / 256)), dim3(256), 0, st, os,
-                       (const double*)w.Time, (const double*)w.taus, n_first, rows, len, t_lo, reversed,
+                       (const double*)w.Time, (const double*)w.taus, w.M, n_first, rows, len, t_lo, reversed,
                        s.weight_mode == 1 ? 1 : 0, s.series_eps, W, ldw);
     FPDE_HIP_TRY(hipGetLastError());
     return FPDE_OK;

@@ -168,7 +168,7 @@ typedef struct {
                             (Time[k+1] = Time[k] + tau_k, init_time_step
                             fpde_hpc_1d.cpp:3942-3949), copied at create; the
                             weights are then per step (not Toeplitz): 1D,
-                            hist_mode 0 or 1, sum_alg 0                         */
+                            sum_alg 0; the GEMM forms take dense weight rows  */
     int var_tau;         /* 1: the step length is given step by step at run time
                             (fpde_solver_advance_tau, fpde_varT_step); 1D,
                             hist_mode 0, sum_alg 0, taus NULL; steps = capacity */
