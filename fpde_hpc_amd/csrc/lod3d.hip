@@ -254,6 +254,245 @@ __global__ __launch_bounds__(256) void k_zline3d(double* __restrict__ U, double*
     }
 }
 
+// Lateral (y or x) sweep as a partitioned solve: each line's n nodes are cut
+// into kLatSeg segments of at most S rows, one thread each (32 lines x 8
+// segments per workgroup; the 32 lines are consecutive z, so every row load
+// is coalesced).  The line's system is the reference one: zero flux at both
+// ends (U_0 = U_1: Al_1 = 1, Bt_1 = 0, fpde_hpc_1d.cpp:4021, :4511; U_N =
+// U_{N-1}: the bottom of U1, :4704-4720) and the rows of A_/B/R/Om between.
+//  1. A segment [lo, hi] moves its couplings to U_{lo-1} and U_{hi+1} to two
+//     right-hand sides (a_lo e_1, c_hi e_last) and runs the Thomas
+//     recurrences for the three: U_i = x_i + y_i U_{lo-1} + z_i U_{hi+1}.
+//  2. One barrier publishes the segments' end rows; every thread runs the
+//     recursion over the segments (f_s = U_lo, l_s = U_hi: l_{s-1} = P + Q f_s
+//     forward, f_s = F_s + G_s f_{s+1} back) and back-substitutes its rows.
+// Same system as the one-thread-per-line sweeps (k_lateral3d*); the reference's
+// 1e-15 guards on the bottom row only act on singular lines (L = 0), which
+// this sweep does not take (L > 0 keeps every row diagonally dominant).
+constexpr int kLatSeg = 8;
+template <int S>
+__global__ __launch_bounds__(256, 2) void k_lateral3d_seg(double* __restrict__ U, const double* __restrict__ K,
+                                                      const double* __restrict__ iC,
+                                                      const double* __restrict__ H, int n, int axis,
+                                                      double h2inv, double L, double kap,
+                                                      const double* __restrict__ E) {
+    __shared__ double sEnd[kLatSeg][32][7];    // x, y, z of the first and the last row
+    const int tid = threadIdx.x;
+    const int l = tid & 31, sg = tid >> 5;
+    const int line = blockIdx.x * 32 + l;
+    const bool ok_line = line < n * n;
+    const int ln = ok_line ? line : 0;
+    const int z0 = ln % n, o = ln / n;
+    int64_t base, stride;
+    if (axis == 2) { base = z0 + (int64_t)n * o; stride = (int64_t)n * n; }   // (z,x=o), along y
+    else { base = z0 + (int64_t)n * n * o; stride = n; }                        // (z,y=o), along x
+    const int N = n - 1;
+    const int Sg = (n + kLatSeg - 1) / kLatSeg;      // rows per segment (<= S)
+    const int W = (n + Sg - 1) / Sg;                 // segments present
+    const int lo = sg * Sg, hi = min(n, lo + Sg) - 1;
+    const bool act = ok_line && lo <= hi;
+    // 1. Thomas on the segment: cp = c', x/y the modified right-hand sides
+    // (straight-line code: rows past hi and the two end rows by selects)
+    double cp[S], xs[S], ys[S];
+    double zl = 0.0;                                 // z of the last row
+    double cprev = 0.0, xprev = 0.0, yprev = 0.0;
+    double Km = (act && lo >= 1) ? K[base + stride * (lo - 1)] : 0.0;
+#pragma unroll
+    for (int j = 0; j < S; j++) {
+        const int i = lo + j;
+        const bool valid = act && i <= hi;
+        const bool inner = valid && i >= 1 && i <= N - 1;
+        const int64_t id = base + stride * (inner ? i : 0);
+        const double Kp = inner ? K[id] : 0.0;
+        const double icv = inner ? iC[id] : 0.0;
+        const double uv = inner ? U[id] : 0.0;
+        const double hv = (inner && H) ? H[id] : 0.0;
+        const double ev = (inner && E) ? E[id] : 0.0;
+        const double a = inner ? Km * icv * h2inv * kap : (i == N ? -1.0 : 0.0);
+        const double c = inner ? Kp * icv * h2inv * kap : (i == 0 ? -1.0 : 0.0);
+        const double b = inner ? -(icv * ((Kp * h2inv) + (Km * h2inv)) * kap + L) : 1.0;
+        const double d = inner ? -L * uv + hv - icv * ev : 0.0;
+        Km = Kp;
+        const double inv = rcp_nr(j == 0 ? b : b - a * cprev);
+        cp[j] = (valid && i < hi) ? c * inv : 0.0;
+        xs[j] = valid ? (j == 0 ? d : d - a * xprev) * inv : 0.0;
+        ys[j] = valid ? (j == 0 ? -a : -a * yprev) * inv : 0.0;
+        if (valid && i == hi) zl = -c * inv;
+        cprev = cp[j]; xprev = xs[j]; yprev = ys[j];
+    }
+    // back: x, y in place, z_i = -c'_i z_{i+1} into cp (rows past hi stay 0)
+    double xn = 0.0, yn = 0.0, zn = 0.0;
+#pragma unroll
+    for (int j = S - 1; j >= 0; j--) {
+        const int i = lo + j;
+        const bool valid = act && i <= hi;
+        const bool last = valid && i == hi;
+        xn = last ? xs[j] : xs[j] - cp[j] * xn;
+        yn = last ? ys[j] : ys[j] - cp[j] * yn;
+        zn = last ? zl : -cp[j] * zn;
+        xs[j] = xn; ys[j] = yn; cp[j] = zn;
+        if (valid && j == 0) { sEnd[sg][l][0] = xn; sEnd[sg][l][1] = yn; sEnd[sg][l][2] = zn; }
+        if (last) { sEnd[sg][l][3] = xn; sEnd[sg][l][4] = yn; sEnd[sg][l][5] = zn; }
+    }
+    __syncthreads();
+    if (!act) return;
+    // 2. the segments' end values: forward l_{s-1} = P + Q f_s, back f_s = F_s + G_s f_{s+1}
+    double Fv[kLatSeg], Gv[kLatSeg], Pv[kLatSeg], Qv[kLatSeg];
+    double pv = 0.0, qv = 0.0;
+#pragma unroll
+    for (int k = 0; k < kLatSeg; k++) {
+        if (k < W) {
+            const double Xf = sEnd[k][l][0], Yf = sEnd[k][l][1], Zf = sEnd[k][l][2];
+            const double Xl = sEnd[k][l][3], Yl = sEnd[k][l][4], Zl = sEnd[k][l][5];
+            const double inv = rcp_nr(1.0 - Yf * qv);
+            Fv[k] = (Xf + Yf * pv) * inv;
+            Gv[k] = Zf * inv;
+            const double yq = Yl * qv;
+            pv = Xl + Yl * pv + yq * Fv[k];
+            qv = yq * Gv[k] + Zl;
+            Pv[k] = pv; Qv[k] = qv;
+        }
+    }
+    double fr = 0.0, vr = 0.0, vl = 0.0;             // f_{k+1}; this segment's right and left values
+#pragma unroll
+    for (int k = kLatSeg - 1; k >= 0; k--) {
+        if (k < W) {
+            if (k == sg) vr = fr;
+            if (k == sg - 1) vl = Pv[k] + Qv[k] * fr;    // l_{sg-1}
+            fr = Fv[k] + Gv[k] * fr;
+        }
+    }
+#pragma unroll
+    for (int j = 0; j < S; j++) {
+        const int i = lo + j;
+        if (i <= hi) U[base + stride * i] = xs[j] + ys[j] * vl + cp[j] * vr;
+    }
+}
+
+// Depth (z) sweep as a partitioned solve over 16 contiguous lines per
+// workgroup: the lines' U, their VGM coefficients at U and the right-hand
+// sides are staged in LDS by coalesced loads, then each line's n nodes are
+// cut into kZSeg segments of at most 8 rows (one thread each) and solved as
+// in k_lateral3d_seg (Thomas per segment with the two coupling right-hand
+// sides, one barrier, the recursion over the segments).  Rows: Dirichlet top
+// U_0 = u_top, the rows of A_/B/R/Om, zero-flux bottom U_{n-1} = U_{n-2};
+// the new U, D[n] = U - Uprev and Uprev go out through LDS, coalesced.
+constexpr int kZSeg = 16, kZLines = 16;
+__global__ __launch_bounds__(256, 2) void k_zline3d_seg(double* __restrict__ U, double* __restrict__ Uprev,
+                                                       const double* __restrict__ utop, int n, double h,
+                                                       double h2inv, double ksat, int mode, double L,
+                                                       double* __restrict__ Dlev, double kap,
+                                                       const double* __restrict__ E) {
+    constexpr int LD = 129;                      // odd row stride: the 16 lines hit distinct banks
+    __shared__ double sK[kZLines][LD], sIC[kZLines][LD], sD[kZLines][LD];
+    __shared__ double sEnd[kZSeg][kZLines][7];
+    const int tid = threadIdx.x;
+    const int nlines = n * n;
+    const int line0 = blockIdx.x * kZLines;
+    const int nl = min(kZLines, nlines - line0);        // lines of this group
+    const int64_t base0 = (int64_t)n * line0;
+    // stage: K, 1/C and the right-hand side -L U - E/C of every node (four
+    // nodes in flight per thread: the VGM chains are latency bound)
+#pragma unroll 4
+    for (int q = tid; q < nl * n; q += 256) {
+        const int l = q / n, i = q % n;
+        const double u = U[base0 + q];
+        double k, ic;
+        vgm_coefs(u, i * h, ksat, mode, k, ic);
+        sK[l][i] = k;
+        sIC[l][i] = ic;
+        sD[l][i] = -L * u - (E ? ic * E[base0 + q] : 0.0);
+    }
+    __syncthreads();
+    const int l = tid & (kZLines - 1), sg = tid / kZLines;
+    const int N = n - 1;
+    const int Sg = (n + kZSeg - 1) / kZSeg;             // rows per segment (<= 8)
+    const int W = (n + Sg - 1) / Sg;
+    const int lo = sg * Sg, hi = min(n, lo + Sg) - 1;
+    const bool act = l < nl && lo <= hi;
+    const double top = (act && lo == 0) ? utop[line0 + l] : 0.0;
+    constexpr int S = 8;
+    double cp[S], xs[S], ys[S];
+    double zl = 0.0, cprev = 0.0, xprev = 0.0, yprev = 0.0;
+#pragma unroll
+    for (int j = 0; j < S; j++) {
+        const int i = lo + j;
+        const bool valid = act && i <= hi;
+        const bool inner = valid && i >= 1 && i <= N - 1;
+        const int ii = inner ? i : 1;
+        const double Km = inner ? sK[l][ii - 1] : 0.0, Kp = inner ? sK[l][ii] : 0.0;
+        const double icv = inner ? sIC[l][ii] : 0.0;
+        const double a = inner ? Km * icv * h2inv * kap : (i == N ? -1.0 : 0.0);
+        const double c = inner ? Kp * icv * h2inv * kap : 0.0;
+        const double b = inner ? -(icv * ((Kp * h2inv) + (Km * h2inv)) * kap + L) : 1.0;
+        const double d = inner ? sD[l][ii] : (i == 0 ? top : 0.0);
+        const double inv = rcp_nr(j == 0 ? b : b - a * cprev);
+        cp[j] = (valid && i < hi) ? c * inv : 0.0;
+        xs[j] = valid ? (j == 0 ? d : d - a * xprev) * inv : 0.0;
+        ys[j] = valid ? (j == 0 ? -a : -a * yprev) * inv : 0.0;
+        if (valid && i == hi) zl = -c * inv;
+        cprev = cp[j]; xprev = xs[j]; yprev = ys[j];
+    }
+    double xn = 0.0, yn = 0.0, zn = 0.0;
+#pragma unroll
+    for (int j = S - 1; j >= 0; j--) {
+        const int i = lo + j;
+        const bool valid = act && i <= hi;
+        const bool last = valid && i == hi;
+        xn = last ? xs[j] : xs[j] - cp[j] * xn;
+        yn = last ? ys[j] : ys[j] - cp[j] * yn;
+        zn = last ? zl : -cp[j] * zn;
+        xs[j] = xn; ys[j] = yn; cp[j] = zn;
+        if (valid && j == 0) { sEnd[sg][l][0] = xn; sEnd[sg][l][1] = yn; sEnd[sg][l][2] = zn; }
+        if (last) { sEnd[sg][l][3] = xn; sEnd[sg][l][4] = yn; sEnd[sg][l][5] = zn; }
+    }
+    __syncthreads();
+    if (act) {
+        double Fv[kZSeg], Gv[kZSeg], Pv[kZSeg], Qv[kZSeg];
+        double pv = 0.0, qv = 0.0;
+#pragma unroll
+        for (int k = 0; k < kZSeg; k++) {
+            if (k < W) {
+                const double Xf = sEnd[k][l][0], Yf = sEnd[k][l][1], Zf = sEnd[k][l][2];
+                const double Xl = sEnd[k][l][3], Yl = sEnd[k][l][4], Zl = sEnd[k][l][5];
+                const double inv = rcp_nr(1.0 - Yf * qv);
+                Fv[k] = (Xf + Yf * pv) * inv;
+                Gv[k] = Zf * inv;
+                const double yq = Yl * qv;
+                pv = Xl + Yl * pv + yq * Fv[k];
+                qv = yq * Gv[k] + Zl;
+                Pv[k] = pv; Qv[k] = qv;
+            }
+        }
+        double fr = 0.0, vr = 0.0, vl = 0.0;
+#pragma unroll
+        for (int k = kZSeg - 1; k >= 0; k--) {
+            if (k < W) {
+                if (k == sg) vr = fr;
+                if (k == sg - 1) vl = Pv[k] + Qv[k] * fr;
+                fr = Fv[k] + Gv[k] * fr;
+            }
+        }
+        // this thread's rows of sD are read by no other thread: overwrite
+        // them with the solution
+#pragma unroll
+        for (int j = 0; j < S; j++) {
+            const int i = lo + j;
+            if (i <= hi) sD[l][i] = xs[j] + ys[j] * vl + cp[j] * vr;
+        }
+    }
+    __syncthreads();
+    for (int q = tid; q < nl * n; q += 256) {
+        const int ll = q / n, i = q % n;
+        const double un = sD[ll][i];
+        const int64_t id = base0 + q;
+        const double up = Uprev[id];
+        U[id] = un;
+        Dlev[id] = un - up;
+        Uprev[id] = un;
+    }
+}
+
 // S[i] = sum of far partials + near levels [near_lo, n) + two-level corrections.
 __global__ __launch_bounds__(256) void k_hist_combine(double* __restrict__ S, const double* __restrict__ part,
                                                       int nsplit, int64_t part_stride, int64_t part_off,
@@ -324,6 +563,11 @@ fpde_status launch_lateral3d(double* U, const double* K, const double* iC, const
                              int axis, double h2inv, double L, double* Al, double* Bt, double kap,
                              const double* E, hipStream_t st) {
     int nl = n * n;
+    if (n <= 16 * kLatSeg && L > 0.0) {   // partitioned: 8 segments of <= 16 rows per line
+        hipLaunchKernelGGL(k_lateral3d_seg<16>, dim3((nl + 31) / 32), dim3(256), 0, st, U, K, iC, H, n, axis, h2inv,
+                           L, kap, E);
+        return FPDE_OK;
+    }
     if (n <= 128) {            // recurrence coefficients in LDS, one wave per 64 lines
         hipLaunchKernelGGL(k_lateral3d_lds<128>, dim3((nl + 63) / 64), dim3(64), 0, st, U, K, iC, H, n, axis, h2inv,
                            L, kap, E);
@@ -338,6 +582,11 @@ fpde_status launch_zline3d(double* U, double* Uprev, const double* utop, int n, 
                            double ksat, int mode, double L, double* Dlev, double kap, const double* E,
                            hipStream_t st) {
     if (n > 256) { set_error("3D grid: at most 256 nodes per axis"); return FPDE_ERR_ARG; }
+    if (n <= 8 * kZSeg && L > 0.0) {        // partitioned: 16 lines per workgroup, 16 segments of <= 8 rows
+        hipLaunchKernelGGL(k_zline3d_seg, dim3((n * n + kZLines - 1) / kZLines), dim3(256), 0, st, U, Uprev, utop, n,
+                           h, h2inv, ksat, mode, L, Dlev, kap, E);
+        return FPDE_OK;
+    }
     int bs = ((n + 63) / 64) * 64;
     hipLaunchKernelGGL(k_zline3d, dim3(n * n), dim3(bs), 0, st, U, Uprev, utop, n, h, h2inv, ksat, mode, L, Dlev,
                        kap, E);
