@@ -71,12 +71,13 @@ def run_variant(F, workload, mode, block, steps, warmup, prof_mask_timed=None, n
     return dict(spec=spec, u=u_last, u_first=u, times=times, prof=prof, timed=timed_stats, tuned=tuned)
 
 
-def cpu_baseline(N, M, seconds_hint=20.0, alpha=0.7, tau=5e-4, seed=1001, picard=(1, 0.0)):
+def cpu_baseline(N, M, seconds_hint=10.0, alpha=0.7, tau=5e-4, seed=1001, picard=(1, 0.0)):
     """FP64 oracle (oracle/, the reference loop order: node-outer, t-inner,
     OpenMP over nodes) timed on this host: the history sum at the run-average
-    history length n = M/2 over all N nodes, plus one oracle step (VGM
-    coefficients + Thomas), median of repetitions; node*steps/s =
-    N / (t_sum + t_step)."""
+    history length n = M/2 over all N nodes, repeated for about seconds_hint
+    seconds of CPU work (median of the repetitions), plus one oracle step (VGM
+    coefficients + Thomas, median over repeated 4-step solves);
+    node*steps/s = N / (t_sum + t_step)."""
     import numpy as np
     from oracle import oracle as O
     O.lib()
@@ -89,27 +90,31 @@ def cpu_baseline(N, M, seconds_hint=20.0, alpha=0.7, tau=5e-4, seed=1001, picard
     taus = np.full(n + 2, tau)
     reps = []
     t_start = time.perf_counter()
-    while len(reps) < 3 or (time.perf_counter() - t_start < seconds_hint and len(reps) < 10):
+    while len(reps) < 3 or time.perf_counter() - t_start < seconds_hint:
         t0 = time.perf_counter()
         O.time_summing(old, kb, taus, nthreads=cores)
         reps.append(time.perf_counter() - t0)
-        if time.perf_counter() - t_start > seconds_hint * 2:
-            break
     t_sum = statistics.median(reps)
+    t_sum_total = time.perf_counter() - t_start
     del old
     u0 = O.init_1d(N, seed)
     p = O.make_params(N, 4, tau, alpha, u_top=u0[0], nthreads=cores, picard_max=picard[0], picard_tol=picard[1])
-    t0 = time.perf_counter()
-    O.solve_1d(p, u0)
-    t_step = (time.perf_counter() - t0) / 4
+    steps = []
+    t_start = time.perf_counter()
+    while len(steps) < 3 or (time.perf_counter() - t_start < 1.0 and len(steps) < 50):
+        t0 = time.perf_counter()
+        O.solve_1d(p, u0)
+        steps.append((time.perf_counter() - t0) / 4)
+    t_step = statistics.median(steps)
     return {
         "value": N / (t_sum + t_step),
         "unit": "node*steps/s",
         "cores": cores,
         "kind": "port",
         "sample": (f"oracle time_summing over n={n} levels x N={N} nodes (run-average history "
-                   f"length for M={M}), {len(reps)} reps median {t_sum:.3f}s, + oracle step "
-                   f"{t_step * 1e3:.2f} ms; extrapolated to the whole run"),
+                   f"length for M={M}), {len(reps)} reps in {t_sum_total:.1f} s, median {t_sum * 1e3:.3f} ms; "
+                   f"+ oracle step {t_step * 1e3:.3f} ms (median of {len(steps)} 4-step solves); "
+                   f"extrapolated to the whole run"),
     }
 
 

@@ -957,6 +957,21 @@ def _full(fpde, name, modes, **kw):
     return out
 
 
+def test_small_configuration_full_size_against_oracle(fpde, oracle):
+    """The CPU-runnable configuration at full size (1D N=1000, M=2000,
+    tau=1e-3, alpha=0.8, Picard <= 3 with tol 1e-12): every history form
+    against the FP64 oracle run to the end."""
+    ref = None
+    for mode, tol in (("plain", TOL_F64), ("mfma_f64", TOL_F64), ("split_bf16", TOL_MIXED),
+                      ("split_f16", TOL_MIXED)):
+        spec, u0 = fpde.workload("1d_small", hist_mode=mode, picard_max=3, picard_tol=1e-12)
+        if ref is None:
+            ref = _oracle_1d(oracle, spec, u0, picard_tol=1e-12)
+        err = rel_err(fpde.solve(spec, u0), ref)
+        print(f"1d_small {mode}: rel err vs oracle {err:.3e}")
+        assert err <= tol
+
+
 def test_headline_size_mixed_and_fp64_mfma_vs_plain(fpde):
     """1D N=10^4, M=2*10^4, Picard <= 3 (tol 1e-12): split bf16 within 1e-8 and
     FP64 MFMA within 1e-10 of the plain FP64 stream."""
