@@ -403,6 +403,8 @@ fpde_status fpde_history_block_gemm(int mode, const double* D, int64_t ld, int64
     double* part = nullptr;
     void* comp[3] = {nullptr, nullptr, nullptr};
     float* scale = nullptr;
+    void* afrag = nullptr;
+    double* ainv = nullptr;
     fpde_status rc = FPDE_OK;
     int splits = 0;
     int64_t lv = ((levels + kGroup - 1) / kGroup) * kGroup;
@@ -410,11 +412,13 @@ fpde_status fpde_history_block_gemm(int mode, const double* D, int64_t ld, int64
     if (e == hipSuccess && split) {
         for (int c = 0; c < 3 && e == hipSuccess; c++) e = hipMalloc(&comp[c], 2 * (size_t)lv * ld);
         if (e == hipSuccess) e = hipMalloc(&scale, sizeof(float) * (size_t)(lv / kGroup) * ld);
+        if (e == hipSuccess) e = hipMalloc(&afrag, split_afrag_bytes(mode, block, (int)(lv / kGroup)));
+        if (e == hipSuccess) e = hipMalloc(&ainv, sizeof(double) * (size_t)(lv / kGroup) * (block / 16));
     }
     if (e != hipSuccess) { rc = FPDE_ERR_NOMEM; set_error("hipMalloc failed"); }
     if (rc == FPDE_OK && split) rc = launch_pack_split(mode, D, ld, (int)nodes, 0, tc / kGroup, comp[0], comp[1], comp[2], scale, 0);
     if (rc == FPDE_OK) {
-        if (split) rc = launch_gemm_split(mode, comp[0], comp[1], comp[2], scale, ld, (int)nodes, n0, tc, block, omega, nullptr, 0, part, &splits, 0);
+        if (split) rc = launch_gemm_split(mode, comp[0], comp[1], comp[2], scale, ld, (int)nodes, n0, tc, block, omega, nullptr, 0, afrag, ainv, part, &splits, 0);
         else rc = launch_gemm_f64(D, ld, (int)nodes, n0, tc, block, omega, part, &splits, 0);
     }
     if (rc == FPDE_OK) {
@@ -428,6 +432,8 @@ fpde_status fpde_history_block_gemm(int mode, const double* D, int64_t ld, int64
     hipFree(part);
     for (int c = 0; c < 3; c++) hipFree(comp[c]);
     hipFree(scale);
+    hipFree(afrag);
+    hipFree(ainv);
     return rc;
 }
 

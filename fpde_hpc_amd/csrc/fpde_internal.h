@@ -344,10 +344,14 @@ fpde_status launch_pack_split(int mode, const double* D, int64_t ld, int nodes, 
 fpde_status launch_gemm_f64_dense(const double* D, int64_t ld, int nodes, int tc, int block, const double* W,
                                   int64_t ldw, double* P, int* splits_out, hipStream_t st);
 // W != nullptr: dense weight rows W[r][t] (leading dimension ldw >= tc) of a
-// prescribed grid in place of the Toeplitz vector omega.
+// prescribed grid in place of the Toeplitz vector omega.  afrag / ainv:
+// workspace of split_afrag_bytes(mode, block, tc / 32) bytes and
+// (tc / 32) * (block / 16) doubles for the block's A fragments.
+size_t split_afrag_bytes(int mode, int block, int groups);
 fpde_status launch_gemm_split(int mode, const void* c0, const void* c1, const void* c2, const float* scale,
                               int64_t ld, int nodes, int n0, int tc, int block, const double* omega,
-                              const double* W, int64_t ldw, double* P, int* splits_out, hipStream_t st);
+                              const double* W, int64_t ldw, void* afrag, double* ainv, double* P,
+                              int* splits_out, hipStream_t st);
 fpde_status launch_step1d(const Step1DArgs& a, hipStream_t st);
 fpde_status launch_step1d_block(const Step1DArgs& a, hipStream_t st);
 bool step1d_block_supported(int C);

@@ -320,19 +320,79 @@ __global__ __launch_bounds__(256) void k_pack_split(const double* __restrict__ D
     }
 }
 
+// A fragments of the split GEMM for the groups [0, tc/32) of one block: the
+// weights of rows r < BR against the levels of a group (Toeplitz
+// omega[n0+r-t-1], or with DENSE the rows W[r][t] of a prescribed grid),
+// split hi/lo(/lo2) into the MFMA A-operand layout (lane l of fragment rt:
+// row 16 rt + l%16, levels 8(l/16)..8(l/16)+7 of the group).  They depend on
+// the block only, not on the node tile or the K split, so they are formed
+// once here instead of in every GEMM workgroup.  Layout: afrag[((g NS + c)
+// RT + rt) 64 + lane], ainv[g RT + rt] = 1 / the fragment's power-of-two
+// scale (fp16; 1 for bf16).  Grid: one workgroup per group; each wave builds
+// whole fragments, so the dense form's fragment maximum is a wave maximum.
+template <typename T, int NS, int BR, bool DENSE>
+__global__ __launch_bounds__(256) void k_afrag_split(int n0, const double* __restrict__ omega,
+                                                     const double* __restrict__ W, int64_t ldw,
+                                                     typename SplitTraits<T>::vec* __restrict__ afrag,
+                                                     double* __restrict__ ainv) {
+    typedef typename SplitTraits<T>::vec vec_t;
+    constexpr int RT = BR / 16;
+    const int g = blockIdx.x;
+    for (int f = threadIdx.x; f < RT * 64; f += 256) {
+        const int fl = f & 63, frt = f >> 6;
+        const int r = frt * 16 + (fl & 15);
+        const int tbase = g * kGroup + 8 * (fl >> 4);
+        double wd[8];
+#pragma unroll
+        for (int e = 0; e < 8; e++) wd[e] = DENSE ? W[(int64_t)r * ldw + tbase + e] : omega[n0 + r - (tbase + e) - 1];
+        // fp16: power-of-two scale per fragment so that its largest weight
+        // sits in [2^14, 2^15) and the lo parts stay normal.  Toeplitz
+        // weights decrease with the lag n0+r-t, so the extremes are at the
+        // corners; dense rows take the maximum over the fragment.
+        double sa = 1.0;
+        if constexpr (SplitTraits<T>::scaled && DENSE) {
+            double mx = 0.0;
+#pragma unroll
+            for (int e = 0; e < 8; e++) mx = fmax(mx, fabs(wd[e]));
+            mx = wave_max(mx);
+            if (mx > 0.0) sa = ldexp(1.0, 14 - ilogb(mx));
+        } else if (SplitTraits<T>::scaled) {
+            const int lag_lo = n0 + frt * 16 - (g * kGroup + kGroup - 1);
+            const int lag_hi = n0 + frt * 16 + 15 - g * kGroup;
+            const double wmax = fmax(fabs(omega[lag_lo - 1]), fabs(omega[lag_hi - 1]));
+            if (wmax > 0.0) sa = ldexp(1.0, 14 - ilogb(wmax));
+        }
+        if (fl == 0) ainv[g * RT + frt] = 1.0 / sa;
+        vec_t h, m, l;
+#pragma unroll
+        for (int e = 0; e < 8; e++) {
+            const double wv = wd[e] * sa;
+            T hh = (T)(float)wv;
+            double rr = wv - (double)(float)hh;
+            T mm = (T)(float)rr;
+            h[e] = hh;
+            m[e] = mm;
+            if (NS == 3) l[e] = (T)(float)(rr - (double)(float)mm);
+        }
+        const size_t base = ((size_t)g * NS * RT + frt) * 64 + fl;
+        afrag[base] = h;
+        afrag[base + RT * 64] = m;
+        if (NS == 3) afrag[base + 2 * RT * 64] = l;
+    }
+}
+
 // Split GEMM.  Grid: x = 64-node tiles, y = K splits.  Per 32-level group:
 // wave w (nodes 16w..16w+15) loads its B fragments (one 16-B load per
-// component), the A fragments (Toeplitz weights omega[n0+r-t-1], or with
-// DENSE the rows W[r][t] of a prescribed grid, split the same way) come from
-// LDS where the block builds them for 4 groups at a time, and the products
-// are accumulated in fp32 by magnitude class, then flushed to fp64.
-template <typename T, int NS, int BR, bool DENSE>
+// component), the A fragments (k_afrag_split) come from LDS, where the block
+// stages them for 4 groups at a time, and the products are accumulated in
+// fp32 by magnitude class, then flushed to fp64.
+template <typename T, int NS, int BR>
 __global__ __launch_bounds__(256) void k_gemm_split(const T* __restrict__ c0, const T* __restrict__ c1,
                                                     const T* __restrict__ c2,
                                                     const float* __restrict__ scale, int64_t ld,
-                                                    int nodes, int n0, int tc, int kchunk,
-                                                    const double* __restrict__ omega,
-                                                    const double* __restrict__ W, int64_t ldw,
+                                                    int nodes, int tc, int kchunk,
+                                                    const typename SplitTraits<T>::vec* __restrict__ afrag_g,
+                                                    const double* __restrict__ ainv_g,
                                                     double* __restrict__ P) {
     typedef typename SplitTraits<T>::vec vec_t;
     constexpr int RT = BR / 16;
@@ -366,54 +426,17 @@ __global__ __launch_bounds__(256) void k_gemm_split(const T* __restrict__ c0, co
             if (SplitTraits<T>::scaled) bsc[gi] = node_ok ? (double)scale[(int64_t)g * ld + node] : 0.0;
         }
         __syncthreads();
-        // build A fragments: entries (gb, rt, lane) -> 8 weights.  Each wave
-        // builds one whole fragment per pass (GB * RT * 64 is a multiple of
-        // 256), so the dense form's fragment maximum is a wave maximum.
+        // A fragments of the batch (formed once per block by k_afrag_split)
         for (int f = threadIdx.x; f < GB * RT * 64; f += 256) {
-            int fl = f & 63, frt = (f >> 6) % RT, fgb = f / (64 * RT);
-            int g = gb0 + fgb;
-            int r = frt * 16 + (fl & 15);
-            int tbase = g * kGroup + 8 * (fl >> 4);
-            double wd[8];
-            if constexpr (DENSE) {
-#pragma unroll
-                for (int e = 0; e < 8; e++) wd[e] = (g < ghi) ? W[(int64_t)r * ldw + tbase + e] : 0.0;
+            const int fl = f & 63, frt = (f >> 6) % RT, fgb = f / (64 * RT);
+            const int g = gb0 + fgb;
+            if (g < ghi) {
+                const size_t base = ((size_t)g * NS * RT + frt) * 64 + fl;
+                afrag[0][fgb][frt][fl] = afrag_g[base];
+                afrag[1][fgb][frt][fl] = afrag_g[base + RT * 64];
+                if (NS == 3) afrag[NS - 1][fgb][frt][fl] = afrag_g[base + 2 * RT * 64];
+                if (fl == 0) ainv[fgb][frt] = ainv_g[g * RT + frt];
             }
-            // fp16: power-of-two scale per fragment so that its largest weight
-            // sits in [2^14, 2^15) and the lo parts stay normal.  Toeplitz
-            // weights decrease with the lag n0+r-t, so the extremes are at the
-            // corners; dense rows take the maximum over the fragment.
-            double sa = 1.0;
-            if constexpr (SplitTraits<T>::scaled && DENSE) {
-                double mx = 0.0;
-#pragma unroll
-                for (int e = 0; e < 8; e++) mx = fmax(mx, fabs(wd[e]));
-                mx = wave_max(mx);
-                if (mx > 0.0) sa = ldexp(1.0, 14 - ilogb(mx));
-            } else if (SplitTraits<T>::scaled && g < ghi) {
-                int lag_lo = n0 + frt * 16 - (g * kGroup + kGroup - 1);
-                int lag_hi = n0 + frt * 16 + 15 - g * kGroup;
-                double wmax = fmax(fabs(omega[lag_lo - 1]), fabs(omega[lag_hi - 1]));
-                if (wmax > 0.0) sa = ldexp(1.0, 14 - ilogb(wmax));
-            }
-            if (fl == 0) ainv[fgb][frt] = 1.0 / sa;
-            vec_t h, m, l;
-#pragma unroll
-            for (int e = 0; e < 8; e++) {
-                double wv = 0.0;
-                int t = tbase + e;
-                if constexpr (DENSE) wv = wd[e] * sa;
-                else if (g < ghi) wv = omega[n0 + r - t - 1] * sa;
-                T hh = (T)(float)wv;
-                double rr = wv - (double)(float)hh;
-                T mm = (T)(float)rr;
-                h[e] = hh;
-                m[e] = mm;
-                if (NS == 3) l[e] = (T)(float)(rr - (double)(float)mm);
-            }
-            afrag[0][fgb][frt][fl] = h;
-            afrag[1][fgb][frt][fl] = m;
-            if (NS == 3) afrag[NS - 1][fgb][frt][fl] = l;
         }
         __syncthreads();
 #pragma unroll
@@ -598,45 +621,54 @@ fpde_status launch_pack_split(int mode, const double* D, int64_t ld, int nodes, 
 template <typename T, int NS, int BR>
 static void gemm_split_t(dim3 grid, hipStream_t st, const void* c0, const void* c1, const void* c2,
                          const float* scale, int64_t ld, int nodes, int n0, int tc, int kchunk,
-                         const double* omega, const double* W, int64_t ldw, double* P) {
+                         const double* omega, const double* W, int64_t ldw, void* afrag, double* ainv, double* P) {
+    typedef typename SplitTraits<T>::vec vec_t;
     if (W)
-        hipLaunchKernelGGL((k_gemm_split<T, NS, BR, true>), grid, dim3(256), 0, st, (const T*)c0, (const T*)c1,
-                           (const T*)c2, scale, ld, nodes, n0, tc, kchunk, omega, W, ldw, P);
+        hipLaunchKernelGGL((k_afrag_split<T, NS, BR, true>), dim3(tc / kGroup), dim3(256), 0, st, n0, omega, W, ldw,
+                           (vec_t*)afrag, ainv);
     else
-        hipLaunchKernelGGL((k_gemm_split<T, NS, BR, false>), grid, dim3(256), 0, st, (const T*)c0, (const T*)c1,
-                           (const T*)c2, scale, ld, nodes, n0, tc, kchunk, omega, W, ldw, P);
+        hipLaunchKernelGGL((k_afrag_split<T, NS, BR, false>), dim3(tc / kGroup), dim3(256), 0, st, n0, omega, W, ldw,
+                           (vec_t*)afrag, ainv);
+    hipLaunchKernelGGL((k_gemm_split<T, NS, BR>), grid, dim3(256), 0, st, (const T*)c0, (const T*)c1, (const T*)c2,
+                       scale, ld, nodes, tc, kchunk, (const vec_t*)afrag, (const double*)ainv, P);
 }
 
 template <typename T, int NS>
 static fpde_status gemm_split_b(int block, dim3 grid, hipStream_t st, const void* c0, const void* c1,
                                 const void* c2, const float* scale, int64_t ld, int nodes, int n0,
                                 int tc, int kchunk, const double* omega, const double* W, int64_t ldw,
-                                double* P) {
+                                void* afrag, double* ainv, double* P) {
     switch (block) {
-        case 16: gemm_split_t<T, NS, 16>(grid, st, c0, c1, c2, scale, ld, nodes, n0, tc, kchunk, omega, W, ldw, P); break;
-        case 32: gemm_split_t<T, NS, 32>(grid, st, c0, c1, c2, scale, ld, nodes, n0, tc, kchunk, omega, W, ldw, P); break;
-        case 64: gemm_split_t<T, NS, 64>(grid, st, c0, c1, c2, scale, ld, nodes, n0, tc, kchunk, omega, W, ldw, P); break;
-        case 128: gemm_split_t<T, NS, 128>(grid, st, c0, c1, c2, scale, ld, nodes, n0, tc, kchunk, omega, W, ldw, P); break;
+        case 16: gemm_split_t<T, NS, 16>(grid, st, c0, c1, c2, scale, ld, nodes, n0, tc, kchunk, omega, W, ldw, afrag, ainv, P); break;
+        case 32: gemm_split_t<T, NS, 32>(grid, st, c0, c1, c2, scale, ld, nodes, n0, tc, kchunk, omega, W, ldw, afrag, ainv, P); break;
+        case 64: gemm_split_t<T, NS, 64>(grid, st, c0, c1, c2, scale, ld, nodes, n0, tc, kchunk, omega, W, ldw, afrag, ainv, P); break;
+        case 128: gemm_split_t<T, NS, 128>(grid, st, c0, c1, c2, scale, ld, nodes, n0, tc, kchunk, omega, W, ldw, afrag, ainv, P); break;
         default: set_error("block must be 16, 32, 64 or 128"); return FPDE_ERR_ARG;
     }
     return FPDE_OK;
 }
 
+size_t split_afrag_bytes(int mode, int block, int groups) {
+    const size_t ns = (mode == HIST_SPLIT_BF16X3) ? 3 : 2;
+    return (size_t)groups * ns * (size_t)(block / 16) * 64 * 16;
+}
+
 fpde_status launch_gemm_split(int mode, const void* c0, const void* c1, const void* c2,
                               const float* scale, int64_t ld, int nodes, int n0, int tc, int block,
-                              const double* omega, const double* W, int64_t ldw, double* P, int* splits_out,
-                              hipStream_t st) {
+                              const double* omega, const double* W, int64_t ldw, void* afrag, double* ainv,
+                              double* P, int* splits_out, hipStream_t st) {
     if (tc <= 0) { *splits_out = 0; return FPDE_OK; }
     if (tc % kGroup != 0) { set_error("split GEMM needs a group-aligned cut"); return FPDE_ERR_ARG; }
     if (W && ldw < tc) { set_error("split GEMM: dense weight rows shorter than the cut"); return FPDE_ERR_ARG; }
+    if (!afrag || !ainv) { set_error("split GEMM: no fragment workspace"); return FPDE_ERR_ARG; }
     int kchunk, splits;
     gemm_split_plan(tc, nodes, &kchunk, &splits);
     dim3 grid((nodes + 63) / 64, splits);
     fpde_status rc;
     switch (mode) {
-        case HIST_SPLIT_BF16: rc = gemm_split_b<__bf16, 2>(block, grid, st, c0, c1, c2, scale, ld, nodes, n0, tc, kchunk, omega, W, ldw, P); break;
-        case HIST_SPLIT_F16: rc = gemm_split_b<_Float16, 2>(block, grid, st, c0, c1, c2, scale, ld, nodes, n0, tc, kchunk, omega, W, ldw, P); break;
-        case HIST_SPLIT_BF16X3: rc = gemm_split_b<__bf16, 3>(block, grid, st, c0, c1, c2, scale, ld, nodes, n0, tc, kchunk, omega, W, ldw, P); break;
+        case HIST_SPLIT_BF16: rc = gemm_split_b<__bf16, 2>(block, grid, st, c0, c1, c2, scale, ld, nodes, n0, tc, kchunk, omega, W, ldw, afrag, ainv, P); break;
+        case HIST_SPLIT_F16: rc = gemm_split_b<_Float16, 2>(block, grid, st, c0, c1, c2, scale, ld, nodes, n0, tc, kchunk, omega, W, ldw, afrag, ainv, P); break;
+        case HIST_SPLIT_BF16X3: rc = gemm_split_b<__bf16, 3>(block, grid, st, c0, c1, c2, scale, ld, nodes, n0, tc, kchunk, omega, W, ldw, afrag, ainv, P); break;
         default: set_error("bad split mode"); return FPDE_ERR_ARG;
     }
     *splits_out = splits;

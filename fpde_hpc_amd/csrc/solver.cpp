@@ -101,8 +101,11 @@ size_t footprint_bytes(const fpde_spec& s) {
     Plan p = make_plan(s);
     size_t b = 0;
     b += sizeof(double) * (size_t)p.levels * p.ld;               // FP64 history
-    if (is_split(s.hist_mode))
+    if (is_split(s.hist_mode)) {
         b += (size_t)n_comp(s.hist_mode) * 2 * p.levels * p.ld + sizeof(float) * (p.levels / kGroup) * p.ld;
+        b += split_afrag_bytes(s.hist_mode, s.block, (int)(p.levels / kGroup)) +
+             sizeof(double) * (size_t)(p.levels / kGroup) * (s.block / 16);
+    }
     b += sizeof(double) * (size_t)p.part_rows * p.ld;            // partial sums
     b += sizeof(double) * (size_t)p.ld * 8;                      // U, S, xyz, ...
     if (s.dims == 3) b += sizeof(double) * (size_t)p.ld * 6;     // Uprev, K, iC, Al, Bt
@@ -170,6 +173,8 @@ struct fpde_solver {
     int64_t graph_launches = 0, graph_instantiations = 0;
     void* comp[3] = {nullptr, nullptr, nullptr};
     float* scale = nullptr;
+    void* afrag = nullptr;       // A fragments of a split GEMM block
+    double* ainv = nullptr;
     // 3D
     double* Uprev = nullptr;
     double* utop = nullptr;
@@ -302,6 +307,8 @@ fpde_status solver_alloc(fpde_solver* s) {
             DALLOC(s->comp[c], 2 * (size_t)p.levels * p.ld);
         }
         DALLOC(s->scale, sizeof(float) * (size_t)(p.levels / kGroup) * p.ld);
+        DALLOC(s->afrag, split_afrag_bytes(sp.hist_mode, sp.block, (int)(p.levels / kGroup)));
+        DALLOC(s->ainv, sizeof(double) * (size_t)(p.levels / kGroup) * (sp.block / 16));
     }
     if (sp.dims == 1) {
         s->C = (int)((sp.nodes + 255) / 256);
@@ -435,7 +442,8 @@ fpde_status far_part(fpde_solver* s, int n, int* nsplit, int64_t* stride, int64_
             if (rc == FPDE_OK)
                 rc = launch_gemm_split(sp.hist_mode, cp(0), cp(1), cp(2), s->scale + (size_t)(t_lo / kGroup) * p.ld,
                                        p.ld, (int)p.nodes, n - t_lo, tc - t_lo, B, s->w.omega,
-                                       s->w.grid ? s->Wfar : nullptr, s->ldwf, s->part, &s->gsplits, s->stream);
+                                       s->w.grid ? s->Wfar : nullptr, s->ldwf, s->afrag, s->ainv, s->part,
+                                       &s->gsplits, s->stream);
         } else if (s->w.grid) {
             // prescribed grid: dense weights of the block, far (t < tc) and
             // near (t in [tc, n), one reversed row per step)
