@@ -410,34 +410,54 @@ __global__ __launch_bounds__(256) void k_gemm_split(const T* __restrict__ c0, co
 #pragma unroll
     for (int rt = 0; rt < RT; rt++) acc[rt] = (d4_t){0.0, 0.0, 0.0, 0.0};
 
+    // software pipeline: the next batch's B fragments (history) and A
+    // fragments (weights) load into registers while this batch computes
+    constexpr int AE = GB * RT * 64 / 256;                // A entries per thread per batch
+    vec_t bn[GB][NS], an[AE][NS];
+    double sn[GB], ain[AE];
+    auto load_batch = [&](int gb0) {
+#pragma unroll
+        for (int gi = 0; gi < GB; gi++) {
+            const int g = min(gb0 + gi, ghi - 1);
+            const int64_t off = ((int64_t)g * ld + (node_ok ? node : 0)) * kGroup + 8 * (lane >> 4);
+            bn[gi][0] = *reinterpret_cast<const vec_t*>(c0 + off);
+            bn[gi][1] = *reinterpret_cast<const vec_t*>(c1 + off);
+            if (NS == 3) bn[gi][NS - 1] = *reinterpret_cast<const vec_t*>(c2 + off);
+            sn[gi] = 1.0;
+            if (SplitTraits<T>::scaled) sn[gi] = node_ok ? (double)scale[(int64_t)g * ld + node] : 0.0;
+        }
+#pragma unroll
+        for (int k = 0; k < AE; k++) {
+            const int f = threadIdx.x + 256 * k;
+            const int fl = f & 63, frt = (f >> 6) % RT, fgb = f / (64 * RT);
+            const int g = min(gb0 + fgb, ghi - 1);          // groups past ghi are never used
+            const size_t base = ((size_t)g * NS * RT + frt) * 64 + fl;
+            an[k][0] = afrag_g[base];
+            an[k][1] = afrag_g[base + RT * 64];
+            if (NS == 3) an[k][NS - 1] = afrag_g[base + 2 * RT * 64];
+            ain[k] = ainv_g[g * RT + frt];
+        }
+    };
+    if (glo < ghi) load_batch(glo);
     for (int gb0 = glo; gb0 < ghi; gb0 += GB) {
-        // B fragments of the batch first (global loads in flight while the
-        // block builds the A fragments)
         vec_t bfr[GB][NS];
         double bsc[GB];
 #pragma unroll
         for (int gi = 0; gi < GB; gi++) {
-            int g = min(gb0 + gi, ghi - 1);
-            int64_t off = ((int64_t)g * ld + (node_ok ? node : 0)) * kGroup + 8 * (lane >> 4);
-            bfr[gi][0] = *reinterpret_cast<const vec_t*>(c0 + off);
-            bfr[gi][1] = *reinterpret_cast<const vec_t*>(c1 + off);
-            if (NS == 3) bfr[gi][NS - 1] = *reinterpret_cast<const vec_t*>(c2 + off);
-            bsc[gi] = 1.0;
-            if (SplitTraits<T>::scaled) bsc[gi] = node_ok ? (double)scale[(int64_t)g * ld + node] : 0.0;
+#pragma unroll
+            for (int c = 0; c < NS; c++) bfr[gi][c] = bn[gi][c];
+            bsc[gi] = sn[gi];
         }
-        __syncthreads();
-        // A fragments of the batch (formed once per block by k_afrag_split)
-        for (int f = threadIdx.x; f < GB * RT * 64; f += 256) {
+        __syncthreads();                       // the previous batch is done with the LDS fragments
+#pragma unroll
+        for (int k = 0; k < AE; k++) {
+            const int f = threadIdx.x + 256 * k;
             const int fl = f & 63, frt = (f >> 6) % RT, fgb = f / (64 * RT);
-            const int g = gb0 + fgb;
-            if (g < ghi) {
-                const size_t base = ((size_t)g * NS * RT + frt) * 64 + fl;
-                afrag[0][fgb][frt][fl] = afrag_g[base];
-                afrag[1][fgb][frt][fl] = afrag_g[base + RT * 64];
-                if (NS == 3) afrag[NS - 1][fgb][frt][fl] = afrag_g[base + 2 * RT * 64];
-                if (fl == 0) ainv[fgb][frt] = ainv_g[g * RT + frt];
-            }
+#pragma unroll
+            for (int c = 0; c < NS; c++) afrag[c][fgb][frt][fl] = an[k][c];
+            if (fl == 0) ainv[fgb][frt] = ain[k];
         }
+        if (gb0 + GB < ghi) load_batch(gb0 + GB);
         __syncthreads();
 #pragma unroll
         for (int gi = 0; gi < GB; gi++) {
