@@ -170,14 +170,22 @@ __global__ __launch_bounds__(256) void k_gemm_f64(const double* __restrict__ D, 
         __syncthreads();
         for (int q = threadIdx.x; q < W; q += 256) win[q] = omega[n0 - s1 + q];
         __syncthreads();
-        // four 4-deep K steps per iteration: the four B loads are issued
-        // before the first MFMA that needs them
+        // four 4-deep K steps per iteration; the next iteration's four B
+        // loads are in flight while this one's MFMAs run
+        double bn[4];
+#pragma unroll
+        for (int q = 0; q < 4; q++) {
+            const int tt = s0 + 4 * q + kk;
+            bn[q] = (tt < s1 && node_ok) ? D[(int64_t)tt * ld + node] : 0.0;
+        }
         for (int t = s0; t < s1; t += 16) {
             double b[4];
 #pragma unroll
+            for (int q = 0; q < 4; q++) b[q] = bn[q];
+#pragma unroll
             for (int q = 0; q < 4; q++) {
-                int tt = t + 4 * q + kk;
-                b[q] = (tt < s1 && node_ok) ? D[(int64_t)tt * ld + node] : 0.0;
+                const int tt = t + 16 + 4 * q + kk;
+                bn[q] = (tt < s1 && node_ok) ? D[(int64_t)tt * ld + node] : 0.0;
             }
 #pragma unroll
             for (int q = 0; q < 4; q++) {
