@@ -266,9 +266,11 @@ __global__ __launch_bounds__(256) void k_zline3d(double* __restrict__ U, double*
 //  2. One barrier publishes the segments' end rows; every thread runs the
 //     recursion over the segments (f_s = U_lo, l_s = U_hi: l_{s-1} = P + Q f_s
 //     forward, f_s = F_s + G_s f_{s+1} back) and back-substitutes its rows.
-// Same system as the one-thread-per-line sweeps (k_lateral3d*); the reference's
-// 1e-15 guards on the bottom row only act on singular lines (L = 0), which
-// this sweep does not take (L > 0 keeps every row diagonally dominant).
+// Same system as the one-thread-per-line sweeps (k_lateral3d*).  The
+// reference's 1e-15 guards on the bottom row act on singular lines (|1 -
+// Al_N| <= 1e-15, which L > 0 rules out: every row is diagonally dominant)
+// and on a bottom value within 1e-15 of zero, where the two differ by less
+// than that absolute amount.
 constexpr int kLatSeg = 8;
 template <int S>
 __global__ __launch_bounds__(256, 2) void k_lateral3d_seg(double* __restrict__ U, const double* __restrict__ K,
