@@ -544,16 +544,20 @@ fpde_status finalize_1d(fpde_solver* s);
 // space-fractional sum of step n: U^n complete in memory, then F and the
 // partial sums of E (read by the step launches)
 fpde_status space_part(fpde_solver* s) {
+    // a pending solve's finalize launch also forms F of U^n; otherwise (the
+    // first step, or U^n finished by an earlier call) the flux kernel does,
+    // which also keeps the padding past N at zero
+    const bool fused = s->pending;
     fpde_status rc = finalize_1d(s);
     if (rc != FPDE_OK) return rc;
     const fpde_spec& sp = s->spec;
     size_t slot = 0;
     ev_begin(s, 1, &slot);
-    rc = launch_space_flux(s->U, sp.nodes, s->h, sp.ksat, sp.coef_mode, s->spF, s->sp_ldp, s->stream);
+    if (!fused) rc = launch_space_flux(s->U, sp.nodes, s->h, sp.ksat, sp.coef_mode, s->spF, s->sp_ldp, s->stream);
     if (rc == FPDE_OK) rc = launch_space_apply(s->spF, sp.nodes, s->w.sp_omega, s->spP, s->sp_ldp, s->stream);
     ev_end(s, slot);
     if (rc != FPDE_OK) return rc;
-    s->stats.step_launches += 2;
+    s->stats.step_launches += fused ? 1 : 2;
     return FPDE_OK;
 }
 

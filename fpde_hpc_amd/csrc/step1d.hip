@@ -355,7 +355,45 @@ __global__ __launch_bounds__(256) void k_step1d(Step1DArgs g) {
         S = Sk;
         uit = val;
     }
-    if (g.k < 0) return;                         // finalize only (uniform)
+    if (g.k < 0) {                               // finalize only (uniform)
+        // space-fractional scheme: F of U^n for this chunk's rows (Om1
+        // equation 0, fpde_hpc_1d.cpp:4176-4179), the arithmetic of
+        // k_space_flux.  The neighbours s-1 and e+1 are chunk ends of the
+        // reduced solution (or of the converged iterate), so the next step
+        // needs no flux launch.
+        if (g.sp_on && g.recover) {
+            double* sU = &pA[0][0];              // U^n of nodes s-1 .. e+1
+            double* sKf = &pC[0][0];             // K at U^n of nodes s-1 .. e
+            if (act) {
+                double k, icv;
+                vgm_coefs(un, i * g.h, g.ksat, g.coef_mode, k, icv);
+                sU[tid + 1] = un;
+                sKf[tid + 1] = k;
+            }
+            if (tid == halo_tid) {
+                if (c > 0) {
+                    const double ul = conv ? uit_halo : xs[2 * c - 1];
+                    double k, icv;
+                    vgm_coefs(ul, (s - 1) * g.h, g.ksat, g.coef_mode, k, icv);
+                    sU[0] = ul;
+                    sKf[0] = k;
+                }
+                if (c < C - 1) sU[nc + 1] = conv ? g.Uit[e + 1] : xs[2 * c + 2];
+            }
+            __syncthreads();
+            if (act) {
+                double f = 0.0;
+                if (i >= 1 && i <= N - 2) {
+                    const double um = sU[tid], u0 = sU[tid + 1], up = sU[tid + 2];
+                    const double Km = sKf[tid], Kj = sKf[tid + 1];
+                    const double hh = g.h * g.h;
+                    f = Kj * up / hh - (Kj / hh + Km / hh) * u0 + Km * um / hh;
+                }
+                g.sp_F[i] = f;
+            }
+        }
+        return;
+    }
 
     if (!first) {
         double* dm = g.dmax + (size_t)g.parity * C;
