@@ -360,9 +360,11 @@ fpde_status launch_coef3d(const double* U, int n, double h, double ksat, int mod
 fpde_status launch_lateral3d(double* U, const double* K, const double* iC, const double* H, int n, int axis,
                              double h2inv, double L, double* Al, double* Bt, double kap, const double* E,
                              hipStream_t st);
+// K, iC: the coefficients at U when a coefficient pass already formed them
+// (else nullptr: the sweep evaluates them itself)
 fpde_status launch_zline3d(double* U, double* Uprev, const double* utop, int n, double h, double h2inv,
                            double ksat, int mode, double L, double* Dlev, double kap, const double* E,
-                           hipStream_t st);
+                           const double* K, const double* iC, hipStream_t st);
 fpde_status launch_space3d(const double* U, const double* K, int n, int axis, double h, const double* sp_omega,
                            const double* sp_corr, double* E, hipStream_t st);
 fpde_status launch_series3d(double* S, double* H, int P, const double* dnew, const double* ser_c,

@@ -384,7 +384,9 @@ __global__ __launch_bounds__(256, 2) void k_zline3d_seg(double* __restrict__ U, 
                                                        const double* __restrict__ utop, int n, double h,
                                                        double h2inv, double ksat, int mode, double L,
                                                        double* __restrict__ Dlev, double kap,
-                                                       const double* __restrict__ E) {
+                                                       const double* __restrict__ E,
+                                                       const double* __restrict__ Kin,
+                                                       const double* __restrict__ iCin) {
     constexpr int LD = 129;                      // odd row stride: the 16 lines hit distinct banks
     __shared__ double sK[kZLines][LD], sIC[kZLines][LD], sD[kZLines][LD];
     __shared__ double sEnd[kZSeg][kZLines][7];
@@ -394,13 +396,15 @@ __global__ __launch_bounds__(256, 2) void k_zline3d_seg(double* __restrict__ U, 
     const int nl = min(kZLines, nlines - line0);        // lines of this group
     const int64_t base0 = (int64_t)n * line0;
     // stage: K, 1/C and the right-hand side -L U - E/C of every node (four
-    // nodes in flight per thread: the VGM chains are latency bound)
+    // nodes in flight per thread: the VGM chains are latency bound); K and
+    // 1/C come from memory when a coefficient pass already formed them at U
 #pragma unroll 4
     for (int q = tid; q < nl * n; q += 256) {
         const int l = q / n, i = q % n;
         const double u = U[base0 + q];
         double k, ic;
-        vgm_coefs(u, i * h, ksat, mode, k, ic);
+        if (Kin) { k = Kin[base0 + q]; ic = iCin[base0 + q]; }
+        else vgm_coefs(u, i * h, ksat, mode, k, ic);
         sK[l][i] = k;
         sIC[l][i] = ic;
         sD[l][i] = -L * u - (E ? ic * E[base0 + q] : 0.0);
@@ -582,11 +586,11 @@ fpde_status launch_lateral3d(double* U, const double* K, const double* iC, const
 
 fpde_status launch_zline3d(double* U, double* Uprev, const double* utop, int n, double h, double h2inv,
                            double ksat, int mode, double L, double* Dlev, double kap, const double* E,
-                           hipStream_t st) {
+                           const double* K, const double* iC, hipStream_t st) {
     if (n > 256) { set_error("3D grid: at most 256 nodes per axis"); return FPDE_ERR_ARG; }
     if (n <= 8 * kZSeg && L > 0.0) {        // partitioned: 16 lines per workgroup, 16 segments of <= 8 rows
         hipLaunchKernelGGL(k_zline3d_seg, dim3((n * n + kZLines - 1) / kZLines), dim3(256), 0, st, U, Uprev, utop, n,
-                           h, h2inv, ksat, mode, L, Dlev, kap, E);
+                           h, h2inv, ksat, mode, L, Dlev, kap, E, K, iC);
         return FPDE_OK;
     }
     int bs = ((n + 63) / 64) * 64;

@@ -713,8 +713,10 @@ fpde_status enqueue_3d_step(fpde_solver* s, int n) {
         if ((rc = launch_coef3d(s->U, nn, s->h, sp.ksat, sp.coef_mode, s->K, s->iC, s->stream)) != FPDE_OK) return rc;
         if ((rc = space_sums(0)) != FPDE_OK) return rc;
     }
+    // the depth sweep reuses the coefficient pass of the space-fractional sums
     if ((rc = launch_zline3d(s->U, s->Uprev, s->utop, nn, s->h, h2inv, sp.ksat, sp.coef_mode, s->w.local,
-                             s->D + (size_t)(series ? (n & 1) : n) * p.ld, kap, E, s->stream)) != FPDE_OK) return rc;
+                             s->D + (size_t)(series ? (n & 1) : n) * p.ld, kap, E, space ? s->K : nullptr,
+                             space ? s->iC : nullptr, s->stream)) != FPDE_OK) return rc;
     ev_end(s, slot);
     s->stats.step_launches += space ? 9 : 5;
     s->stats.picard_iters_total++;
