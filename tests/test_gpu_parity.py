@@ -319,6 +319,28 @@ def test_solve_3d_matches_oracle(fpde, oracle, mode):
     assert err <= (TOL_MIXED if mode.startswith("split") else TOL_F64)
 
 
+@pytest.mark.parametrize("n", [129, 150])
+def test_solve_3d_long_lines_matches_oracle(fpde, oracle, n):
+    """Lines longer than 128 nodes take the one-thread-per-line lateral sweep
+    (Thomas coefficients in HBM) and the PCR depth sweep: FP64 and split
+    forms against the oracle, across a history GEMM block, with and without
+    the space-fractional sums."""
+    steps = 36
+    u0 = fpde.init_3d(n, 1003)
+    cases = [("plain", TOL_F64, 0.0), ("split_bf16", TOL_MIXED, 0.0)]
+    if n == 129:
+        cases.append(("mfma_f64", TOL_F64, 0.7))
+    for mode, tol, gamma in cases:
+        spec = fpde.make_spec(n, steps, 1e-3, alpha=0.85, dims=3, hist_mode=mode, block=16, u_top=u0[0],
+                              gamma=gamma)
+        u = fpde.solve(spec, u0)
+        ref = _oracle_space3d(oracle, spec, u0) if gamma else oracle.solve_3d(
+            oracle.make_params(n, steps, 1e-3, 0.85, ksat=spec.ksat), u0)
+        err = rel_err(u, ref)
+        print(f"3D n={n} {mode} gamma {gamma}: rel err {err:.3e}")
+        assert err <= tol
+
+
 def test_advance_in_pieces_equals_one_shot(fpde):
     nodes, steps = 400, 150
     u0 = fpde.init_1d(nodes, 3)
@@ -930,10 +952,11 @@ def test_space_fractional_3d_matches_oracle(fpde, oracle, n, mode, tol, coef):
 
 @pytest.mark.parametrize("mode,tol", [("mfma_f64", 1e-12), ("split_bf16", TOL_MIXED)])
 @pytest.mark.parametrize("nodes", [300, 2900])
-def test_space_fractional_cooperative_equals_per_iteration(fpde, oracle, mode, tol, nodes):
-    """The cooperative block launches (chunk workgroups + helper workgroups
-    sharing the spatial sum between two grid barriers per step) against the
-    per-iteration launches and the oracle."""
+def test_space_fractional_step_modes_agree(fpde, oracle, mode, tol, nodes):
+    """With gamma < 1 a step_mode 0 request (cooperative block launches where
+    they apply) runs the per-iteration launches, since the spatial sum needs
+    U^n complete before each step: both requests give the same solve, and it
+    matches the oracle."""
     steps = 150
     u0 = fpde.init_1d(nodes, 91)
     kw = dict(alpha=0.75, hist_mode=mode, block=32, picard_max=3, picard_tol=1e-12, u_top=u0[0], gamma=0.75)
