@@ -117,79 +117,6 @@ __global__ __launch_bounds__(256) void k_lateral3d(double* __restrict__ U, const
     }
 }
 
-// Lateral sweep with the recurrence coefficients in LDS: one wave per 64
-// lines (one line per lane, adjacent lanes = adjacent z), Al/Bt of the wave's
-// lines kept in LDS ([NMAX+1][64] each, 132 KB for NMAX = 128), so a sweep
-// reads K, 1/C, U (and H, E) once and writes U once, and the n^2 lines are
-// spread over n^2/64 workgroups (every CU) instead of n^2/256.  Same
-// arithmetic as k_lateral3d.  (Evaluating K and 1/C here instead of reading
-// them measured slower: one wave per CU cannot hide the VGM latency.)
-template <int NMAX>
-__global__ __launch_bounds__(64) void k_lateral3d_lds(double* __restrict__ U, const double* __restrict__ K,
-                                                     const double* __restrict__ iC,
-                                                     const double* __restrict__ H, int n, int axis,
-                                                     double h2inv, double L, double kap,
-                                                     const double* __restrict__ E) {
-    constexpr int R = 16;
-    __shared__ double sAl[NMAX + 1][64], sBt[NMAX + 1][64];
-    const int lane = threadIdx.x;
-    const int line = blockIdx.x * 64 + lane;
-    const int nlines = n * n;
-    const bool ok_line = line < nlines;
-    const int ln = ok_line ? line : 0;
-    const int z = ln % n, o = ln / n;
-    int64_t base, stride;
-    if (axis == 2) { base = z + (int64_t)n * o; stride = (int64_t)n * n; }   // (z,x=o), along y
-    else { base = z + (int64_t)n * n * o; stride = n; }                        // (z,y=o), along x
-    const int N = n - 1;
-    double al = 1.0, bt = 0.0;                 // al1 with Al[1]=1 (:4021), Bt[1]=bc0()=0 (:4511)
-    sAl[1][lane] = al;
-    sBt[1][lane] = bt;
-    double Kprev = ok_line ? K[base] : 0.0;
-    for (int i0 = 1; i0 < N; i0 += R) {
-        double kk[R], ic[R], uu[R], hh[R], ee[R];
-#pragma unroll
-        for (int j = 0; j < R; j++) {
-            int i = i0 + j;
-            bool ok = ok_line && i < N;
-            int64_t id = base + stride * (ok ? i : 0);
-            kk[j] = ok ? K[id] : 0.0;
-            ic[j] = ok ? iC[id] : 0.0;
-            uu[j] = ok ? U[id] : 0.0;
-            hh[j] = (ok && H) ? H[id] : 0.0;
-            ee[j] = (ok && E) ? E[id] : 0.0;
-        }
-#pragma unroll
-        for (int j = 0; j < R; j++) {
-            int i = i0 + j;
-            if (i < N) {
-                double Km = Kprev, Kp = kk[j];
-                double A_ = Km * ic[j] * h2inv * kap;
-                double B_ = Kp * ic[j] * h2inv * kap;
-                double R_ = ic[j] * ((Kp * h2inv) + (Km * h2inv)) * kap + L;
-                double Om = -L * uu[j] + hh[j];
-                if (E) Om -= ic[j] * ee[j];
-                double aln = B_ / (R_ - A_ * al);
-                bt = (aln / B_) * (A_ * bt - Om);
-                al = aln;
-                sAl[i + 1][lane] = al;
-                sBt[i + 1][lane] = bt;
-                Kprev = Kp;
-            }
-        }
-    }
-    if (!ok_line) return;
-    // zero-flux end, U1 :4704-4720 with bcN()=0
-    double uN;
-    if (fabs(bt) > 1e-15) uN = (fabs(1.0 - al) > 1e-15) ? bt / (1.0 - al) : U[base + stride * (N - 1)];
-    else uN = (fabs(1.0 - al) > 1e-15) ? 0.0 : U[base + stride * (N - 1)];
-    U[base + stride * N] = uN;
-    double u = uN;
-    for (int i = N - 1; i >= 0; i--) {
-        u = sAl[i + 1][lane] * u + sBt[i + 1][lane];
-        U[base + stride * i] = u;
-    }
-}
 
 // Depth sweep: block = one z line, one node per thread (blockDim = n rounded
 // up to 64, n <= 256).  PCR on rows normalised to b = 1 with ping-pong LDS
@@ -571,11 +498,6 @@ fpde_status launch_lateral3d(double* U, const double* K, const double* iC, const
     int nl = n * n;
     if (n <= 16 * kLatSeg && L > 0.0) {   // partitioned: 8 segments of <= 16 rows per line
         hipLaunchKernelGGL(k_lateral3d_seg<16>, dim3((nl + 31) / 32), dim3(256), 0, st, U, K, iC, H, n, axis, h2inv,
-                           L, kap, E);
-        return FPDE_OK;
-    }
-    if (n <= 128) {            // recurrence coefficients in LDS, one wave per 64 lines
-        hipLaunchKernelGGL(k_lateral3d_lds<128>, dim3((nl + 63) / 64), dim3(64), 0, st, U, K, iC, H, n, axis, h2inv,
                            L, kap, E);
         return FPDE_OK;
     }
