@@ -306,8 +306,8 @@ __global__ __launch_bounds__(256, 2) void k_lateral3d_seg(double* __restrict__ U
 // sides, one barrier, the recursion over the segments).  Rows: Dirichlet top
 // U_0 = u_top, the rows of A_/B/R/Om, zero-flux bottom U_{n-1} = U_{n-2};
 // the new U, D[n] = U - Uprev and Uprev go out through LDS, coalesced.
-constexpr int kZSeg = 16, kZLines = 16;
-__global__ __launch_bounds__(256, 2) void k_zline3d_seg(double* __restrict__ U, double* __restrict__ Uprev,
+constexpr int kZSeg = 16, kZLines = 16, kZThreads = kZSeg * kZLines;
+__global__ __launch_bounds__(kZThreads, 2) void k_zline3d_seg(double* __restrict__ U, double* __restrict__ Uprev,
                                                        const double* __restrict__ utop, int n, double h,
                                                        double h2inv, double ksat, int mode, double L,
                                                        double* __restrict__ Dlev, double kap,
@@ -326,7 +326,7 @@ __global__ __launch_bounds__(256, 2) void k_zline3d_seg(double* __restrict__ U, 
     // nodes in flight per thread: the VGM chains are latency bound); K and
     // 1/C come from memory when a coefficient pass already formed them at U
 #pragma unroll 4
-    for (int q = tid; q < nl * n; q += 256) {
+    for (int q = tid; q < nl * n; q += kZThreads) {
         const int l = q / n, i = q % n;
         const double u = U[base0 + q];
         double k, ic;
@@ -415,7 +415,7 @@ __global__ __launch_bounds__(256, 2) void k_zline3d_seg(double* __restrict__ U, 
         }
     }
     __syncthreads();
-    for (int q = tid; q < nl * n; q += 256) {
+    for (int q = tid; q < nl * n; q += kZThreads) {
         const int ll = q / n, i = q % n;
         const double un = sD[ll][i];
         const int64_t id = base0 + q;
@@ -511,7 +511,7 @@ fpde_status launch_zline3d(double* U, double* Uprev, const double* utop, int n, 
                            const double* K, const double* iC, hipStream_t st) {
     if (n > 256) { set_error("3D grid: at most 256 nodes per axis"); return FPDE_ERR_ARG; }
     if (n <= 8 * kZSeg && L > 0.0) {        // partitioned: 16 lines per workgroup, 16 segments of <= 8 rows
-        hipLaunchKernelGGL(k_zline3d_seg, dim3((n * n + kZLines - 1) / kZLines), dim3(256), 0, st, U, Uprev, utop, n,
+        hipLaunchKernelGGL(k_zline3d_seg, dim3((n * n + kZLines - 1) / kZLines), dim3(kZThreads), 0, st, U, Uprev, utop, n,
                            h, h2inv, ksat, mode, L, Dlev, kap, E, K, iC);
         return FPDE_OK;
     }
