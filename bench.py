@@ -236,7 +236,7 @@ def main():
     if args.workload == "counters":
         # Short program for rocprofv3 --pmc passes: a known-size device copy
         # (counter calibration), the plain FP64 history stream (n=2000 levels,
-        # N=10000) and a split-bf16 solve (N=10000, M=2000).
+        # N=10000), and a split-bf16 and an FP64-MFMA solve (N=10000, M=2000).
         import ctypes
         L = F.lib()
         ctx = ctypes.c_void_p()
@@ -256,10 +256,11 @@ def main():
         p = L.fpde_cl_buf_device_ptr
         F.check(L.fpde_history_sum_f64(p(bufs[4]), p(bufs[2]), n, nodes, ld, p(bufs[3])))
         print("[counters] plain history sum n=2000 N=10000", file=sys.stderr, flush=True)
-        spec, u0 = F.workload("1d_large", hist_mode="split_bf16", steps=2000, picard_max=args.picard,
-                              picard_tol=args.picard_tol)
-        F.solve(spec, u0)
-        print("[counters] split_bf16 solve N=10000 M=2000", file=sys.stderr, flush=True)
+        for mode in ("split_bf16", "mfma_f64"):
+            spec, u0 = F.workload("1d_large", hist_mode=mode, steps=2000, picard_max=args.picard,
+                                  picard_tol=args.picard_tol)
+            F.solve(spec, u0)
+            print(f"[counters] {mode} solve N=10000 M=2000", file=sys.stderr, flush=True)
         for bb in bufs:
             L.fpde_cl_buf_free(bb)
         L.fpde_cl_ctx_destroy(ctx)

@@ -21,35 +21,47 @@ namespace fpde {
 
 // ---------------------------------------------------------------------------
 // Plain FP64 streaming kernel: part[ks*ld + i] = sum_{t in chunk ks, t < t_hi}
-// omega[n-t-1] * D[t*ld + i].  256 threads, 2 nodes per thread (16-B loads).
+// omega[n-t-1] * D[t*ld + i].  256 threads, 2 nodes per thread (16-B loads);
+// the weights of the chunk are staged in LDS windows of kPlainWin levels
+// (wave-uniform reads, one barrier pair per window).
+constexpr int kPlainWin = 1024;
 __global__ __launch_bounds__(256) void k_hist_plain(const double* __restrict__ D, int64_t ld,
                                                     int nodes, int n, int t_hi, int chunk,
                                                     const double* __restrict__ omega,
                                                     double* __restrict__ part) {
-    int i2 = (blockIdx.x * 256 + threadIdx.x) * 2;
-    if (i2 >= nodes) return;
-    int ks = blockIdx.y;
-    int t0 = ks * chunk;
-    int t1 = min(t_hi, t0 + chunk);
-    const double* __restrict__ w = omega + (n - 1);   // w[-t] = omega[n-t-1]
+    __shared__ double wwin[kPlainWin];                // wwin[t - w0] = omega[n-t-1]
+    const int i2 = (blockIdx.x * 256 + threadIdx.x) * 2;
+    const bool ok = i2 < nodes;                       // threads past the nodes still stage weights
+    const int ks = blockIdx.y;
+    const int t0 = ks * chunk;
+    const int t1 = min(t_hi, t0 + chunk);
     double a0 = 0.0, a1 = 0.0, b0 = 0.0, b1 = 0.0;
-    int t = t0;
-    for (; t + 4 <= t1; t += 4) {
-        double2 d0 = *reinterpret_cast<const double2*>(D + (int64_t)(t + 0) * ld + i2);
-        double2 d1 = *reinterpret_cast<const double2*>(D + (int64_t)(t + 1) * ld + i2);
-        double2 d2 = *reinterpret_cast<const double2*>(D + (int64_t)(t + 2) * ld + i2);
-        double2 d3 = *reinterpret_cast<const double2*>(D + (int64_t)(t + 3) * ld + i2);
-        double w0 = w[-t], w1 = w[-t - 1], w2 = w[-t - 2], w3 = w[-t - 3];
-        a0 = fma(w0, d0.x, a0); a1 = fma(w0, d0.y, a1);
-        b0 = fma(w1, d1.x, b0); b1 = fma(w1, d1.y, b1);
-        a0 = fma(w2, d2.x, a0); a1 = fma(w2, d2.y, a1);
-        b0 = fma(w3, d3.x, b0); b1 = fma(w3, d3.y, b1);
+    for (int w0 = t0; w0 < t1; w0 += kPlainWin) {
+        const int w1 = min(t1, w0 + kPlainWin);
+        __syncthreads();
+        for (int q = threadIdx.x; q < w1 - w0; q += 256) wwin[q] = omega[n - (w0 + q) - 1];
+        __syncthreads();
+        if (!ok) continue;
+        int t = w0;
+        for (; t + 4 <= w1; t += 4) {
+            double2 d0 = *reinterpret_cast<const double2*>(D + (int64_t)(t + 0) * ld + i2);
+            double2 d1 = *reinterpret_cast<const double2*>(D + (int64_t)(t + 1) * ld + i2);
+            double2 d2 = *reinterpret_cast<const double2*>(D + (int64_t)(t + 2) * ld + i2);
+            double2 d3 = *reinterpret_cast<const double2*>(D + (int64_t)(t + 3) * ld + i2);
+            const double* w = wwin + (t - w0);
+            double x0 = w[0], x1 = w[1], x2 = w[2], x3 = w[3];
+            a0 = fma(x0, d0.x, a0); a1 = fma(x0, d0.y, a1);
+            b0 = fma(x1, d1.x, b0); b1 = fma(x1, d1.y, b1);
+            a0 = fma(x2, d2.x, a0); a1 = fma(x2, d2.y, a1);
+            b0 = fma(x3, d3.x, b0); b1 = fma(x3, d3.y, b1);
+        }
+        for (; t < w1; t++) {
+            double2 d0 = *reinterpret_cast<const double2*>(D + (int64_t)t * ld + i2);
+            double x0 = wwin[t - w0];
+            a0 = fma(x0, d0.x, a0); a1 = fma(x0, d0.y, a1);
+        }
     }
-    for (; t < t1; t++) {
-        double2 d0 = *reinterpret_cast<const double2*>(D + (int64_t)t * ld + i2);
-        double w0 = w[-t];
-        a0 = fma(w0, d0.x, a0); a1 = fma(w0, d0.y, a1);
-    }
+    if (!ok) return;
     double2 r;
     r.x = a0 + b0;
     r.y = a1 + b1;
