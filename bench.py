@@ -118,6 +118,62 @@ def cpu_baseline(N, M, seconds_hint=10.0, alpha=0.7, tau=5e-4, seed=1001, picard
     }
 
 
+def ensemble_run(F, count, nodes, msteps, hist_mode, picard, reps=3):
+    """`count` independent 1D solves (alpha spread over [0.5, 0.9], own
+    initial states) timed three ways: one batch (fpde_solver_advance_batch:
+    one cooperative launch per history block for all of them), one after
+    another, and each on its own stream at once.  Returns the medians and
+    the largest difference of the batch results from the solo ones."""
+    import numpy as np
+    alphas = np.linspace(0.5, 0.9, count)
+    u0s = [F.init_1d(nodes, 700 + i) for i in range(count)]
+    specs = [F.make_spec(nodes, msteps, 1e-3, alpha=float(a), hist_mode=hist_mode, block=64,
+                         picard_max=picard[0], picard_tol=picard[1], u_top=u0[0]) for a, u0 in zip(alphas, u0s)]
+    solvers = [F.Solver(sp) for sp in specs]
+
+    def reset_all():
+        for s, u0 in zip(solvers, u0s):
+            s.reset(u0)
+        for s in solvers:
+            s.sync()
+
+    def batch():
+        F.advance_batch(solvers, 0)
+
+    def sequential():
+        for s in solvers:
+            s.advance(0)
+            s.sync()
+
+    def streams():
+        for s in solvers:
+            s.advance(0)
+        for s in solvers:
+            s.sync()
+
+    times, sols = {}, {}
+    for name, fn in (("batch", batch), ("sequential", sequential), ("streams", streams)):
+        ts = []
+        for _ in range(reps):
+            reset_all()
+            t0 = time.perf_counter()
+            fn()
+            ts.append(time.perf_counter() - t0)
+        times[name] = statistics.median(ts)
+        sols[name] = [s.solution() for s in solvers]
+        print(f"[ensemble] {name}: {times[name]:.3f} s for {count} solves", file=sys.stderr, flush=True)
+    for s in solvers:
+        s.close()
+    diff = max(float(np.abs(b - a).max() / np.abs(a).max()) for a, b in zip(sols["sequential"], sols["batch"]))
+    work = count * nodes * msteps
+    return {"count": count, "nodes": nodes, "steps": msteps, "alphas": [float(a) for a in alphas],
+            "hist_mode": hist_mode, "seconds": times,
+            "node_steps_per_s": {k: work / v for k, v in times.items()},
+            "batch_speedup_vs_sequential": times["sequential"] / times["batch"],
+            "batch_speedup_vs_streams": times["streams"] / times["batch"],
+            "batch_max_rel_diff_vs_solo": diff}
+
+
 def serve_mix(F, n_req, rate, seed=1004, quantum=64, budget_gb=200.0, max_active=16, hist_mode="split_f16"):
     """The serving mix: Poisson arrivals at `rate` req/s; 80% 1D (N~U[500,20000],
     M~U[1000,40000]), 20% 3D (32^3..96^3, M~U[200,1000]); alpha~U(0.3,0.95);
@@ -200,6 +256,7 @@ def main():
     ap.add_argument("--requests", type=int, default=512, help="serving mix size (--workload serve)")
     ap.add_argument("--rate", type=float, default=16.0, help="serving arrival rate, requests/s")
     ap.add_argument("--max-active", type=int, default=16)
+    ap.add_argument("--count", type=int, default=6, help="solves in the batch (--workload ensemble)")
     ap.add_argument("--gamma", type=float, default=0.0,
                     help="space-fractional order in (0,1) (0: classical diffusion)")
     ap.add_argument("--picard", type=int, default=3,
@@ -265,6 +322,18 @@ def main():
             L.fpde_cl_buf_free(bb)
         L.fpde_cl_ctx_destroy(ctx)
         print(json.dumps({"counters_program": "done"}))
+        return
+    if args.workload == "ensemble":
+        r = ensemble_run(F, args.count, args.nodes or 10000, args.msteps or 2000, args.headline,
+                         (args.picard, args.picard_tol), reps=max(1, args.steps))
+        print(json.dumps({
+            "metric": "node*steps/s, ensemble of independent 1D solves in one batch",
+            "value": r["node_steps_per_s"]["batch"], "unit": "node*steps/s", "n_gpus": 1, "steps": max(1, args.steps),
+            "warmup": 0, "ms_per_step": r["seconds"]["batch"] * 1e3, "higher_is_better": True, "scaling": "weak",
+            "vs_baseline": None, "dtype": VARIANT_DTYPE.get(args.headline, "f64"), "data": "synthetic",
+            "config": {"workload": f"{r['count']} x 1D N={r['nodes']}, M={r['steps']}, alpha 0.5..0.9",
+                       "parallelism": "single GPU, one cooperative launch per history block for the batch"},
+            "ensemble": r}))
         return
     if args.workload == "serve":
         r = serve_mix(F, args.requests, args.rate, max_active=args.max_active, hist_mode=args.headline)

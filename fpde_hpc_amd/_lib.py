@@ -87,6 +87,7 @@ SIGNATURES = [
     ("fpde_solver_reset", ctypes.c_int, [_vp, _dp]),
     ("fpde_solver_advance", ctypes.c_int, [_vp, ctypes.c_int]),
     ("fpde_solver_advance_graph", ctypes.c_int, [_vp, ctypes.c_int, ctypes.c_int]),
+    ("fpde_solver_advance_batch", ctypes.c_int, [ctypes.POINTER(_vp), ctypes.c_int, ctypes.c_int]),
     ("fpde_autotune_block", ctypes.c_int, [_vp, _dp, ctypes.c_int, ctypes.POINTER(ctypes.c_int), _dp]),
     ("fpde_solver_sync", ctypes.c_int, [_vp]),
     ("fpde_solver_query", ctypes.c_int, [_vp]),

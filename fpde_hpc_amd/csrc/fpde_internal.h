@@ -238,6 +238,16 @@ struct Step1DArgs {
     double* sp_E;
 };
 
+// The fields of a batched cooperative launch that change from block to block:
+// solve j runs steps [n[j], n_end[j]) on workgroups [first[j], first[j+1]).
+constexpr int kMaxBatch = 32;
+struct BatchTick {
+    int count;
+    int first[kMaxBatch + 1];
+    int n[kMaxBatch], n_end[kMaxBatch], n0[kMaxBatch], nsplit[kMaxBatch], near_lo[kMaxBatch], epar[kMaxBatch];
+    int64_t part_stride[kMaxBatch];
+};
+
 // ---------------------------------------------------------------------------
 // Space-fractional sum (space.hip): rows are cut into tiles of kSpTile rows
 // (16 blocks of 16) and the 16-wide block diagonals d of the lower-triangular
@@ -355,6 +365,8 @@ fpde_status launch_gemm_split(int mode, const void* c0, const void* c1, const vo
 fpde_status launch_step1d(const Step1DArgs& a, hipStream_t st);
 fpde_status launch_step1d_block(const Step1DArgs& a, hipStream_t st);
 bool step1d_block_supported(int C);
+bool step1d_batch_supported(int64_t wgs);
+fpde_status launch_step1d_batch(const Step1DArgs* base, const BatchTick& t, hipStream_t st);
 fpde_status launch_coef3d(const double* U, int n, double h, double ksat, int mode, double* K, double* iC,
                           hipStream_t st);
 fpde_status launch_lateral3d(double* U, const double* K, const double* iC, const double* H, int n, int axis,

@@ -578,9 +578,8 @@ fpde_status enqueue_1d_step(fpde_solver* s, int n) {
     return FPDE_OK;
 }
 
-// Cooperative block launch for the steps [n, n_end) (no pending solve
-// between launches: each launch leaves U^{n_end} and D[n_end-1] in memory).
-fpde_status launch_1d_block(fpde_solver* s, int n, int n_end, int n0, int nsplit, int64_t stride, int near_lo) {
+// Arguments of the cooperative block launch for the steps [n, n_end).
+Step1DArgs block_args(fpde_solver* s, int n, int n_end, int n0, int nsplit, int64_t stride, int near_lo) {
     const fpde_spec& sp = s->spec;
     Step1DArgs a{};
     a.N = sp.nodes;
@@ -619,6 +618,14 @@ fpde_status launch_1d_block(fpde_solver* s, int n, int n_end, int n0, int nsplit
         a.ser_c = s->w.ser_c;
         a.ser_v = s->w.ser_v;
     }
+    return a;
+}
+
+// Cooperative block launch for the steps [n, n_end) (no pending solve
+// between launches: each launch leaves U^{n_end} and D[n_end-1] in memory).
+fpde_status launch_1d_block(fpde_solver* s, int n, int n_end, int n0, int nsplit, int64_t stride, int near_lo) {
+    const fpde_spec& sp = s->spec;
+    const Step1DArgs a = block_args(s, n, n_end, n0, nsplit, stride, near_lo);
     size_t slot = 0;
     ev_begin(s, 1, &slot);
     fpde_status rc = launch_step1d_block(a, s->stream);
@@ -631,20 +638,26 @@ fpde_status launch_1d_block(fpde_solver* s, int n, int n_end, int n0, int nsplit
     return FPDE_OK;
 }
 
-// 1D steps [n, end) in cooperative block launches: one launch per history
-// block (GEMM forms), per step (plain: its history kernel runs between
-// steps), or per 256 steps (series moments, no history kernels).
-fpde_status enqueue_1d_blocks(fpde_solver* s, int n, int end) {
+// The step range [n, n_end) of the next cooperative launch and the first
+// step n0 of its history block: one launch per history block (GEMM forms),
+// per step (plain: its history kernel runs between steps), or per 256 steps
+// (series moments, no history kernels).
+void next_block(const fpde_solver* s, int n, int end, int* n_end, int* n0) {
     const fpde_spec& sp = s->spec;
+    if (sp.sum_alg == 2) { *n_end = std::min(end, n + 256); *n0 = n; }
+    else if (sp.hist_mode == HIST_PLAIN_F64) { *n_end = n + 1; *n0 = n; }   // step_mode 0 never selects this
+    else { *n_end = std::min(end, s->n0 + sp.block); *n0 = s->n0; }
+}
+
+// 1D steps [n, end) in cooperative block launches.
+fpde_status enqueue_1d_blocks(fpde_solver* s, int n, int end) {
     while (n < end) {
         int nsplit = 0, near_lo = 0;
         int64_t stride = 0, off = 0;
         fpde_status rc = far_part(s, n, &nsplit, &stride, &off, &near_lo);
         if (rc != FPDE_OK) return rc;
         int n_end, n0;
-        if (sp.sum_alg == 2) { n_end = std::min(end, n + 256); n0 = n; }
-        else if (sp.hist_mode == HIST_PLAIN_F64) { n_end = n + 1; n0 = n; }   // step_mode 0 never selects this
-        else { n_end = std::min(end, s->n0 + sp.block); n0 = s->n0; }
+        next_block(s, n, end, &n_end, &n0);
         rc = launch_1d_block(s, n, n_end, n0, nsplit, stride, near_lo);
         if (rc != FPDE_OK) return rc;
         n = n_end;
@@ -835,6 +848,120 @@ fpde_status fpde_solver_advance(fpde_solver* s, int nsteps) {
         if (rc != FPDE_OK) return rc;
     }
     FPDE_HIP_TRY(hipGetLastError());
+    return FPDE_OK;
+}
+
+// Many solvers advanced together.  Per history block: the far part of each
+// 1D block-mode solver on its own stream (they overlap), then one cooperative
+// launch of k_step1d_batch that runs the block's steps of all of them.  The
+// other solvers advance on their own streams meanwhile.  Returns when every
+// solver has reached its end step.
+fpde_status fpde_solver_advance_batch(fpde_solver* const* solvers, int count, int nsteps) {
+    if (!solvers || count < 1 || count > kMaxBatch) { set_error("advance_batch: 1 to 32 solvers"); return FPDE_ERR_ARG; }
+    for (int i = 0; i < count; i++) {
+        if (!solvers[i]) { set_error("advance_batch: null solver"); return FPDE_ERR_ARG; }
+        if (!solvers[i]->initialized) { set_error("set_initial must be called first"); return FPDE_ERR_STATE; }
+        for (int j = 0; j < i; j++)
+            if (solvers[j] == solvers[i]) { set_error("advance_batch: a solver appears twice"); return FPDE_ERR_ARG; }
+    }
+    std::vector<fpde_solver*> blk;
+    std::vector<int> ends;
+    int64_t wgs = 0;
+    for (int i = 0; i < count; i++) {
+        fpde_solver* s = solvers[i];
+        if (s->spec.dims == 1 && s->block_mode) {
+            blk.push_back(s);
+            ends.push_back(nsteps <= 0 ? s->spec.steps : std::min(s->spec.steps, s->step + nsteps));
+            wgs += s->C;
+        }
+    }
+    if (!blk.empty() && !step1d_batch_supported(wgs)) {
+        set_error("advance_batch: the batch's " + std::to_string(wgs) +
+                  " chunk workgroups exceed what the GPU holds at once; split the batch");
+        return FPDE_ERR_ARG;
+    }
+    for (int i = 0; i < count; i++) {
+        fpde_solver* s = solvers[i];
+        if (s->spec.dims == 1 && s->block_mode) continue;
+        fpde_status rc = fpde_solver_advance(s, nsteps);
+        if (rc != FPDE_OK) return rc;
+    }
+    if (blk.empty()) {
+        for (int i = 0; i < count; i++) FPDE_HIP_TRY(hipStreamSynchronize(solvers[i]->stream));
+        return FPDE_OK;
+    }
+    const int B = (int)blk.size();
+    // the argument rows of the batch (pointers and constants of each solve)
+    std::vector<Step1DArgs> rows(B);
+    for (int j = 0; j < B; j++) rows[j] = block_args(blk[j], blk[j]->step, blk[j]->step, 0, 0, 0, 0);
+    for (fpde_solver* s : blk) FPDE_HIP_TRY(hipStreamSynchronize(s->stream));
+    Step1DArgs* drows = nullptr;
+    std::vector<hipEvent_t> ev(B + 1, nullptr);
+    fpde_status rc = FPDE_OK;
+    hipStream_t bst = blk[0]->stream;
+    auto hip_ok = [&](hipError_t e) {
+        if (e == hipSuccess) return true;
+        set_error(std::string("advance_batch: ") + hipGetErrorString(e));
+        rc = FPDE_ERR_HIP;
+        return false;
+    };
+    if (hip_ok(hipMalloc((void**)&drows, sizeof(Step1DArgs) * B)) &&
+        hip_ok(hipMemcpy(drows, rows.data(), sizeof(Step1DArgs) * B, hipMemcpyHostToDevice))) {
+        for (int j = 0; j <= B && rc == FPDE_OK; j++) hip_ok(hipEventCreateWithFlags(&ev[j], hipEventDisableTiming));
+    }
+    while (rc == FPDE_OK) {
+        BatchTick t{};
+        t.count = B;
+        int first = 0;
+        for (int j = 0; j < B && rc == FPDE_OK; j++) {
+            fpde_solver* s = blk[j];
+            t.first[j] = first;
+            t.n[j] = t.n_end[j] = s->step;
+            if (s->step >= ends[j]) continue;                // done: no workgroups
+            int nsplit = 0, near_lo = 0, n_end = 0, n0 = 0;
+            int64_t stride = 0, off = 0;
+            rc = far_part(s, s->step, &nsplit, &stride, &off, &near_lo);
+            if (rc != FPDE_OK) break;
+            next_block(s, s->step, ends[j], &n_end, &n0);
+            t.n_end[j] = n_end;
+            t.n0[j] = n0;
+            t.nsplit[j] = nsplit;
+            t.near_lo[j] = near_lo;
+            t.part_stride[j] = stride;
+            t.epar[j] = (int)(s->coop_launches & 1);
+            first += s->C;
+            if (s->stream != bst && hip_ok(hipEventRecord(ev[j], s->stream))) hip_ok(hipStreamWaitEvent(bst, ev[j], 0));
+        }
+        if (rc != FPDE_OK) break;
+        t.first[B] = first;
+        if (first == 0) break;
+        rc = launch_step1d_batch(drows, t, bst);
+        if (rc != FPDE_OK) break;
+        if (!hip_ok(hipEventRecord(ev[B], bst))) break;
+        for (int j = 0; j < B; j++) {
+            fpde_solver* s = blk[j];
+            if (t.n_end[j] == t.n[j]) continue;
+            if (s->stream != bst && !hip_ok(hipStreamWaitEvent(s->stream, ev[B], 0))) break;
+            s->launches++;
+            s->coop_launches++;
+            s->stats.step_launches++;
+            s->stats.picard_iters_total += (int)((int64_t)(t.n_end[j] - t.n[j]) * s->spec.picard_max);
+            s->step = t.n_end[j];
+        }
+    }
+    // the argument rows stay alive until the last launch has run
+    for (int i = 0; i < count; i++) {
+        hipError_t e = hipStreamSynchronize(solvers[i]->stream);
+        if (rc == FPDE_OK) hip_ok(e);
+    }
+    for (hipEvent_t e : ev)
+        if (e) hipEventDestroy(e);
+    if (drows) hipFree(drows);
+    if (rc != FPDE_OK) return rc;
+    for (fpde_solver* s : blk) {
+        rc = check_barrier(s);
+        if (rc != FPDE_OK) return rc;
+    }
     return FPDE_OK;
 }
 

@@ -571,14 +571,15 @@ __device__ inline double hist_step(const Step1DArgs& g, int i, int n, double dne
     return hm + hist_tail(om[0], g.corr[2 * n], g.corr[2 * n + 1], n, g.near_lo, dnew, d0m, d1m);
 }
 
-__global__ __launch_bounds__(256) void k_step1d_block(Step1DArgs g) {
+// Steps [g.n, g.n_end) of one solve by its chunk workgroup c (0 <= c < g.C):
+// the body of the single-solve launch and of the batched one.
+__device__ __forceinline__ void step_block(const Step1DArgs& g, const int c) {
     __shared__ double rA[2][kMaxRed], rC[2][kMaxRed], rD[2][kMaxRed];
     __shared__ double pA[2][256], pC[2][256], pX[2][256], pY[2][256], pZ[2][256];
     __shared__ double sK[258];
     __shared__ double sdp[4];                  // wave maxima of the last solve's |U^{k+1} - U^k|
     __shared__ int sh_ok;
     const int tid = threadIdx.x;
-    const int c = blockIdx.x;
     const int N = g.N, C = g.C;
     const int s = (int)(((int64_t)c * N) / C);
     const int e = (int)(((int64_t)(c + 1) * N) / C) - 1;
@@ -845,14 +846,52 @@ __global__ __launch_bounds__(256) void k_step1d_block(Step1DArgs g) {
     }
 }
 
-// Whether the C workgroups of a cooperative block launch can be co-resident.
-bool step1d_block_supported(int C) {
+__global__ __launch_bounds__(256) void k_step1d_block(Step1DArgs g) { step_block(g, (int)blockIdx.x); }
+
+// Many independent solves in one cooperative launch: workgroups
+// [t.first[j], t.first[j+1]) are the chunks of solve j.  Each solve keeps its
+// own barrier flags, convergence slots and Picard decisions, so its arithmetic
+// is that of its single-solve launch; the fields that change from block to
+// block come with the launch, the rest from the solve's row of `base`.
+__global__ __launch_bounds__(256) void k_step1d_batch(const Step1DArgs* __restrict__ base, BatchTick t) {
+    const int wg = (int)blockIdx.x;
+    int j = 0;
+    while (j + 1 < t.count && wg >= t.first[j + 1]) j++;
+    Step1DArgs g = base[j];
+    g.n = t.n[j];
+    g.n_end = t.n_end[j];
+    g.n0 = t.n0[j];
+    g.nsplit = t.nsplit[j];
+    g.near_lo = t.near_lo[j];
+    g.epar = t.epar[j];
+    g.part_stride = t.part_stride[j];
+    step_block(g, wg - t.first[j]);
+}
+
+static bool coop_fits(const void* kern, int64_t wgs) {
     int dev = 0, coop = 0, cus = 0, per_cu = 0;
     if (hipGetDevice(&dev) != hipSuccess) return false;
     if (hipDeviceGetAttribute(&coop, hipDeviceAttributeCooperativeLaunch, dev) != hipSuccess || !coop) return false;
     if (hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess) return false;
-    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_step1d_block, 256, 0) != hipSuccess) return false;
-    return (int64_t)per_cu * cus >= C;
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, kern, 256, 0) != hipSuccess) return false;
+    return (int64_t)per_cu * cus >= wgs;
+}
+
+// Whether the C workgroups of a cooperative block launch can be co-resident.
+bool step1d_block_supported(int C) { return coop_fits((const void*)k_step1d_block, C); }
+bool step1d_batch_supported(int64_t wgs) { return coop_fits((const void*)k_step1d_batch, wgs); }
+
+fpde_status launch_step1d_batch(const Step1DArgs* base, const BatchTick& t, hipStream_t st) {
+    if (t.count < 1 || t.count > kMaxBatch) { set_error("batched step: 1 to 32 solves"); return FPDE_ERR_ARG; }
+    for (int j = 0; j < t.count; j++)
+        if (t.first[j + 1] < t.first[j]) { set_error("batched step: bad workgroup table"); return FPDE_ERR_ARG; }
+    const int wgs = t.first[t.count];
+    if (wgs == 0) return FPDE_OK;
+    const Step1DArgs* b = base;
+    BatchTick tick = t;
+    void* params[] = {&b, &tick};
+    FPDE_HIP_TRY(hipLaunchCooperativeKernel((const void*)k_step1d_batch, dim3(wgs), dim3(256), params, 0, st));
+    return FPDE_OK;
 }
 
 fpde_status launch_step1d_block(const Step1DArgs& a, hipStream_t st) {

@@ -152,6 +152,28 @@ def solve(spec, u0):
         return s.solution()
 
 
+def advance_batch(solvers, nsteps=0):
+    """Advance up to 32 solvers together (fpde_solver_advance_batch): the 1D
+    cooperative-block solvers share one cooperative launch per history block.
+    Returns when all of them are done."""
+    solvers = list(solvers)
+    arr = (ctypes.c_void_p * len(solvers))(*[s._h.value for s in solvers])
+    check(lib().fpde_solver_advance_batch(arr, len(solvers), int(nsteps)))
+
+
+def solve_batch(specs, u0s):
+    """Solutions of independent solves run as one batch (advance_batch)."""
+    solvers = [Solver(sp) for sp in specs]
+    try:
+        for s, u0 in zip(solvers, u0s):
+            s.set_initial(u0)
+        advance_batch(solvers, 0)
+        return [s.solution() for s in solvers]
+    finally:
+        for s in solvers:
+            s.close()
+
+
 class Server:
     """Continuous-batching request server (fpde_server_*)."""
 

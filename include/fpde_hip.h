@@ -220,6 +220,20 @@ fpde_status fpde_solver_reset(fpde_solver *s, const double *u0);
 fpde_status fpde_autotune_block(const fpde_spec *spec, const double *u0, int trial_steps,
                                 int *block_out, double *ms_per_step_out);
 
+/* Advance `count` (1..32) independent solvers together, each by nsteps
+ * (<= 0: to its last step), and return when all are done.  Solvers may differ
+ * in N, M, alpha, weights, history form and block size.  The 1D solvers that
+ * run cooperative block launches (step_mode 0 with a GEMM or series history)
+ * share ONE cooperative launch per history block: their far-part kernels run
+ * on their own streams, then a single k_step1d_batch launch runs the block's
+ * steps of every one of them (each with its own barriers and Picard
+ * decisions, so results equal those of fpde_solver_advance).  The others
+ * advance on their own streams meanwhile.  FPDE_ERR_ARG when the batch's chunk
+ * workgroups do not fit on the GPU at once.  For ensemble runs and parameter
+ * sweeps, which the reference runs as one process per solve, configured on
+ * the command line (mpi_test.sh:12-23). */
+fpde_status fpde_solver_advance_batch(fpde_solver *const *solvers, int count, int nsteps);
+
 /* Enqueue nsteps steps (or all remaining if nsteps<=0); asynchronous. */
 fpde_status fpde_solver_advance(fpde_solver *s, int nsteps);
 /* The same steps as fpde_solver_advance, enqueued as hipGraphs: every

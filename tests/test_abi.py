@@ -134,3 +134,16 @@ def test_autotune_block_argument_checks_and_plain_form(fpde):
     plain = fpde.make_spec(64, 300, 1e-3, hist_mode="plain", block=0)
     assert L.fpde_autotune_block(ctypes.byref(plain), fpde.solver._p(u0), 0, ctypes.byref(b), ctypes.byref(ms)) == 0
     assert b.value == 0 and ms.value == 0.0
+
+
+def test_advance_batch_argument_checks(fpde):
+    """A null array, a count outside 1..32 and a null solver are refused
+    before any device call."""
+    L = fpde.lib()
+    null1 = (ctypes.c_void_p * 1)(None)
+    assert L.fpde_solver_advance_batch(None, 1, 0) == 1
+    assert L.fpde_solver_advance_batch(null1, 0, 0) == 1
+    many = (ctypes.c_void_p * 33)()
+    assert L.fpde_solver_advance_batch(many, 33, 0) == 1
+    assert L.fpde_solver_advance_batch(null1, 1, 0) == 1
+    assert b"null solver" in L.fpde_last_error()

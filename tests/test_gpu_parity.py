@@ -385,6 +385,72 @@ def test_server_results_same_alone_and_in_mix(fpde):
     srv.close()
 
 
+def test_advance_batch_matches_solo(fpde):
+    """Solves of uneven N, M, alpha, history form and block size advanced as
+    one batch (one cooperative launch per history block for the block-mode
+    ones; the plain form on its own stream) give each solo result exactly:
+    every solve keeps its own barriers and Picard decisions.  One solver
+    starts the batch 37 steps ahead, so the batch's blocks are not aligned."""
+    cases = [
+        dict(n=3000, m=300, alpha=0.6, hist_mode="split_bf16", block=64, picard_max=3, picard_tol=1e-10),
+        dict(n=10000, m=200, alpha=0.85, hist_mode="mfma_f64", block=32, picard_max=3, picard_tol=1e-10),
+        dict(n=777, m=513, alpha=0.35, hist_mode="split_f16", block=128),
+        dict(n=2048, m=150, alpha=0.7, hist_mode="split_bf16", block=16, do_q=4),
+        dict(n=1200, m=400, alpha=0.75, hist_mode="plain", sum_alg=2, sum_param=10),
+        dict(n=1500, m=120, alpha=0.5, hist_mode="plain"),
+    ]
+    specs, u0s = [], []
+    for i, c in enumerate(cases):
+        u0 = fpde.init_1d(c["n"], 3000 + i)
+        kw = {k: v for k, v in c.items() if k not in ("n", "m")}
+        specs.append(fpde.make_spec(c["n"], c["m"], 1e-3, u_top=u0[0], **kw))
+        u0s.append(u0)
+    pre = [37 if i == 2 else 0 for i in range(len(cases))]
+
+    def solo(sp, u0, p):                           # the batch's schedule, alone
+        with fpde.Solver(sp) as s:
+            s.set_initial(u0)
+            if p:
+                s.advance(p)
+            s.advance(150)
+            s.advance(0)
+            return s.solution()
+
+    alone = [solo(sp, u0, p) for sp, u0, p in zip(specs, u0s, pre)]
+    solvers = [fpde.Solver(sp) for sp in specs]
+    try:
+        for s, u0 in zip(solvers, u0s):
+            s.set_initial(u0)
+        solvers[2].advance(pre[2])
+        fpde.advance_batch(solvers, 150)           # part of the way
+        assert [s.step for s in solvers] == [min(c["m"], 150 + p) for c, p in zip(cases, pre)]
+        fpde.advance_batch(solvers, 0)             # the rest
+        got = [s.solution() for s in solvers]
+    finally:
+        for s in solvers:
+            s.close()
+    for c, a, b in zip(cases, alone, got):
+        d = float(np.abs(b - a).max())
+        print(f"batch vs solo {c['hist_mode']} N={c['n']}: max |diff| {d:.3e}", flush=True)
+        np.testing.assert_array_equal(b, a)
+
+
+def test_advance_batch_refuses_what_does_not_fit(fpde):
+    """Seven N=10000 solves need 280 chunk workgroups at once, more than the
+    GPU holds for this kernel: refused before any launch."""
+    solvers = [fpde.Solver(fpde.make_spec(10000, 10, 1e-3, hist_mode="split_bf16")) for _ in range(7)]
+    try:
+        for s in solvers:
+            s.set_initial(fpde.init_1d(10000, 5))
+        with pytest.raises(fpde.FpdeError, match="exceed"):
+            fpde.advance_batch(solvers, 0)
+        fpde.advance_batch(solvers[:2], 0)          # a batch that fits still runs
+        assert solvers[0].step == 10 and solvers[1].step == 10
+    finally:
+        for s in solvers:
+            s.close()
+
+
 # ------------------------------------------------------------ restricted summing
 def _lag_weights(alpha, tau, M):
     j = np.arange(1, M + 1, dtype=np.float64)
