@@ -343,7 +343,8 @@ def main():
             "ms_per_step": r["makespan_s"] * 1e3, "higher_is_better": True, "scaling": "weak",
             "vs_baseline": None, "dtype": VARIANT_DTYPE.get(args.headline, "f64"), "data": "synthetic",
             "config": {"workload": "serving mix: Poisson arrivals, 80% 1D / 20% 3D, 25% distributed order",
-                       "parallelism": "single GPU, one stream per active request"},
+                       "parallelism": "single GPU; 1D block-form requests share batched cooperative launches, "
+                                      "the others one stream per request"},
             "serving": r}))
         return
 

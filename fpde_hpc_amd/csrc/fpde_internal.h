@@ -366,6 +366,8 @@ fpde_status launch_step1d(const Step1DArgs& a, hipStream_t st);
 fpde_status launch_step1d_block(const Step1DArgs& a, hipStream_t st);
 bool step1d_block_supported(int C);
 bool step1d_batch_supported(int64_t wgs);
+bool solver_batchable(const fpde_solver* s);
+int solver_chunks(const fpde_solver* s);
 fpde_status launch_step1d_batch(const Step1DArgs* base, const BatchTick& t, hipStream_t st);
 fpde_status launch_coef3d(const double* U, int n, double h, double ksat, int mode, double* K, double* iC,
                           hipStream_t st);

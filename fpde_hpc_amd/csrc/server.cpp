@@ -4,10 +4,13 @@
 // arrival order.  A scheduling round admits queued requests while the
 // device-memory budget (projected history bytes, fpde_solver_footprint) and
 // the active-slot limit allow, advances every active solve by a quantum of
-// steps on its own HIP stream (so the work of different requests overlaps on
-// the device), and retires solves whose streams have drained, freeing their
-// memory and slot for the next requests.  State stays on the device for the
-// whole life of a request; only u0 and the final field cross PCIe.
+// steps, and retires solves whose streams have drained, freeing their memory
+// and slot for the next requests.  The 1D solves of the cooperative block
+// form advance together: as many as fit on the GPU at once share one
+// cooperative launch per history block (fpde_solver_advance_batch).  The
+// others (3D, per-iteration 1D) advance on their own HIP streams, so their
+// work overlaps on the device.  State stays on the device for the whole life
+// of a request; only u0 and the final field cross PCIe.
 #include "fpde_internal.h"
 
 #include <chrono>
@@ -117,9 +120,6 @@ fpde_status fpde_server_submit(fpde_server* srv, const fpde_spec* spec, const do
     auto r = std::make_unique<Request>();
     r->id = srv->next_id++;
     r->spec = *spec;
-    // requests share the device: the cooperative block form needs its
-    // workgroups co-resident, so served solves launch per Picard iteration
-    r->spec.step_mode = 1;
     r->submit_s = srv->now();
     r->bytes = fpde_solver_footprint(spec);
     if (id_out) *id_out = r->id;
@@ -156,10 +156,30 @@ fpde_status fpde_server_cancel(fpde_server* srv, int64_t id) {
 int fpde_server_poll(fpde_server* srv, int quantum) {
     if (!srv) return -1;
     admit(srv);
+    std::vector<Request*> coop;
     for (Request* r : srv->active) {
-        if (fpde_solver_step(r->solver) < r->spec.steps) {
-            if (fpde_solver_advance(r->solver, quantum) != FPDE_OK) r->status = 2;
+        if (fpde_solver_step(r->solver) >= r->spec.steps) continue;
+        if (solver_batchable(r->solver)) { coop.push_back(r); continue; }
+        if (fpde_solver_advance(r->solver, quantum) != FPDE_OK) r->status = 2;
+    }
+    // the cooperative 1D solves, in groups whose chunk workgroups fit on the
+    // GPU together: one batched launch per history block for each group (the
+    // call returns when the group's quantum is done; the other requests'
+    // work, enqueued above, runs meanwhile)
+    size_t g0 = 0;
+    while (g0 < coop.size()) {
+        std::vector<fpde_solver*> grp;
+        int64_t wgs = 0;
+        size_t g1 = g0;
+        while (g1 < coop.size() && (int)grp.size() < kMaxBatch &&
+               (grp.empty() || step1d_batch_supported(wgs + solver_chunks(coop[g1]->solver)))) {
+            wgs += solver_chunks(coop[g1]->solver);
+            grp.push_back(coop[g1]->solver);
+            g1++;
         }
+        if (fpde_solver_advance_batch(grp.data(), (int)grp.size(), quantum) != FPDE_OK)
+            for (size_t i = g0; i < g1; i++) coop[i]->status = 2;
+        g0 = g1;
     }
     // retire drained solves
     std::vector<Request*> keep;

@@ -12,7 +12,9 @@
 #include <chrono>
 #include <cmath>
 #include <cstring>
+#include <map>
 #include <memory>
+#include <mutex>
 #include <random>
 #include <vector>
 
@@ -738,6 +740,13 @@ fpde_status enqueue_3d_step(fpde_solver* s, int n) {
 
 }  // namespace
 
+namespace fpde {
+// Whether a solver advances in the batched cooperative launches, and its
+// chunk workgroups (the server groups such solvers into batches).
+bool solver_batchable(const fpde_solver* s) { return s && s->spec.dims == 1 && s->block_mode; }
+int solver_chunks(const fpde_solver* s) { return s ? s->C : 0; }
+}  // namespace fpde
+
 // ===========================================================================
 // C ABI
 extern "C" {
@@ -891,11 +900,20 @@ fpde_status fpde_solver_advance_batch(fpde_solver* const* solvers, int count, in
         return FPDE_OK;
     }
     const int B = (int)blk.size();
-    // the argument rows of the batch (pointers and constants of each solve)
+    // the argument rows of the batch (pointers and constants of each solve),
+    // in a per-device table that lives until the process ends (a hipFree per
+    // call would wait for every stream of the device); one batch at a time
+    // uses it
+    static std::mutex rows_mu;
+    static std::map<int, Step1DArgs*> rows_tab;
+    std::lock_guard<std::mutex> lock(rows_mu);
+    int dev = 0;
+    FPDE_HIP_TRY(hipGetDevice(&dev));
+    Step1DArgs*& drows = rows_tab[dev];
+    if (!drows) FPDE_HIP_TRY(hipMalloc((void**)&drows, sizeof(Step1DArgs) * kMaxBatch));
     std::vector<Step1DArgs> rows(B);
     for (int j = 0; j < B; j++) rows[j] = block_args(blk[j], blk[j]->step, blk[j]->step, 0, 0, 0, 0);
     for (fpde_solver* s : blk) FPDE_HIP_TRY(hipStreamSynchronize(s->stream));
-    Step1DArgs* drows = nullptr;
     std::vector<hipEvent_t> ev(B + 1, nullptr);
     fpde_status rc = FPDE_OK;
     hipStream_t bst = blk[0]->stream;
@@ -905,8 +923,9 @@ fpde_status fpde_solver_advance_batch(fpde_solver* const* solvers, int count, in
         rc = FPDE_ERR_HIP;
         return false;
     };
-    if (hip_ok(hipMalloc((void**)&drows, sizeof(Step1DArgs) * B)) &&
-        hip_ok(hipMemcpy(drows, rows.data(), sizeof(Step1DArgs) * B, hipMemcpyHostToDevice))) {
+    // the previous batch's launches have finished: that call synchronised
+    // its streams before returning
+    if (hip_ok(hipMemcpy(drows, rows.data(), sizeof(Step1DArgs) * B, hipMemcpyHostToDevice))) {
         for (int j = 0; j <= B && rc == FPDE_OK; j++) hip_ok(hipEventCreateWithFlags(&ev[j], hipEventDisableTiming));
     }
     while (rc == FPDE_OK) {
@@ -956,7 +975,6 @@ fpde_status fpde_solver_advance_batch(fpde_solver* const* solvers, int count, in
     }
     for (hipEvent_t e : ev)
         if (e) hipEventDestroy(e);
-    if (drows) hipFree(drows);
     if (rc != FPDE_OK) return rc;
     for (fpde_solver* s : blk) {
         rc = check_barrier(s);

@@ -451,6 +451,41 @@ def test_advance_batch_refuses_what_does_not_fit(fpde):
             s.close()
 
 
+def test_server_batches_block_form_requests(fpde):
+    """1D requests in the cooperative block form (step_mode 0) advance in the
+    server's batched launches, beside 3D requests on their own streams; the
+    server's quantum (64 steps) cuts the 1D steps at the block bounds of
+    blocks 16/32/64, so each 1D result equals the solve alone exactly."""
+    rng = np.random.default_rng(77)
+    specs = []
+    for i in range(10):
+        if i % 4 == 3:
+            n = int(rng.integers(8, 14))
+            u0 = fpde.init_3d(n, 900 + i)
+            sp = fpde.make_spec(n, int(rng.integers(20, 60)), 1e-3, alpha=float(rng.uniform(0.3, 0.95)),
+                                dims=3, hist_mode="split_f16", block=16, u_top=u0[0])
+        else:
+            n = int(rng.integers(500, 6000))
+            u0 = fpde.init_1d(n, 900 + i)
+            kw = dict(do_q=4) if i == 5 else {}
+            sp = fpde.make_spec(n, int(rng.integers(100, 400)), 1e-3, alpha=float(rng.uniform(0.3, 0.95)),
+                                hist_mode=("split_bf16", "mfma_f64", "split_f16")[i % 3], block=(16, 32, 64)[i % 3],
+                                picard_max=2, picard_tol=1e-10, u_top=u0[0], **kw)
+        specs.append((sp, u0))
+    alone = [fpde.solve(sp, u0) for sp, u0 in specs]
+    fps = [fpde.footprint(sp) for sp, _ in specs]
+    srv = fpde.Server(mem_budget_bytes=max(sum(fps) // 2, max(fps)), max_active=6)
+    ids = [srv.submit(sp, u0) for sp, u0 in specs]
+    srv.run()
+    for (sp, _), rid, a in zip(specs, ids, alone):
+        assert srv.info(rid)["status"] == 0
+        if sp.dims == 1:
+            np.testing.assert_array_equal(srv.result(rid), a)
+        else:
+            assert rel_err(srv.result(rid), a) <= 1e-14
+    srv.close()
+
+
 # ------------------------------------------------------------ restricted summing
 def _lag_weights(alpha, tau, M):
     j = np.arange(1, M + 1, dtype=np.float64)
