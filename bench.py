@@ -257,6 +257,8 @@ def main():
     ap.add_argument("--rate", type=float, default=16.0, help="serving arrival rate, requests/s")
     ap.add_argument("--max-active", type=int, default=16)
     ap.add_argument("--count", type=int, default=6, help="solves in the batch (--workload ensemble)")
+    ap.add_argument("--no-ensemble", action="store_true",
+                    help="skip the six-solve batch measurement added to the headline line")
     ap.add_argument("--gamma", type=float, default=0.0,
                     help="space-fractional order in (0,1) (0: classical diffusion)")
     ap.add_argument("--picard", type=int, default=3,
@@ -488,6 +490,10 @@ def main():
         out["speedup_vs_plain_fp64"] = variants["plain"]["ms_per_solve"] / (t_mean * 1e3)
         if ref_u is not None:
             out["rel_err_vs_fp64"] = variants[args.headline].get("rel_err_vs_fp64")
+    if not args.no_ensemble and args.workload == "1d_large" and args.nodes is None and spec.dims == 1:
+        # independent solves of the same column (alpha 0.5..0.9, M=2000), one
+        # batch against one after another: not part of the headline figure
+        out["ensemble"] = ensemble_run(F, 6, N, 2000, args.headline, picard, reps=3)
     print(json.dumps(out))
 
 

@@ -51,3 +51,6 @@ def test_bench_json_line():
     assert roof["bound"] in ("hbm", "mfma") and 0 < roof["frac"] <= 1.0
     assert abs(roof["frac"] - roof["achieved"] / roof["peak"]) < 1e-9
     assert out["rel_err_vs_fp64"] <= 1e-8
+    ens = out["ensemble"]                          # six solves as one batch vs one after another
+    assert ens["count"] == 6 and ens["batch_max_rel_diff_vs_solo"] == 0.0
+    assert ens["batch_speedup_vs_sequential"] > 1.0
