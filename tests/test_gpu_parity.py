@@ -435,6 +435,27 @@ def test_advance_batch_matches_solo(fpde):
         np.testing.assert_array_equal(b, a)
 
 
+def test_advance_batch_of_32_small_solves(fpde):
+    """The largest batch: 32 solves of 5..700 nodes (one to three chunk
+    workgroups each, so the workgroup table has 33 uneven bounds), M from 40 to
+    300 (solves leave the batch at different blocks), blocks 16..128, all
+    equal to their solo results."""
+    rng = np.random.default_rng(3232)
+    modes = ("split_bf16", "split_f16", "mfma_f64")
+    specs, u0s = [], []
+    for i in range(32):
+        n = 5 if i < 2 else int(rng.integers(6, 701))
+        u0 = fpde.init_1d(n, 4000 + i)
+        specs.append(fpde.make_spec(n, int(rng.integers(40, 301)), 1e-3, alpha=float(rng.uniform(0.2, 0.95)),
+                                    hist_mode=modes[i % 3], block=(16, 32, 64, 128)[i % 4], picard_max=1 + i % 3,
+                                    picard_tol=1e-10, u_top=u0[0]))
+        u0s.append(u0)
+    alone = [fpde.solve(sp, u0) for sp, u0 in zip(specs, u0s)]
+    got = fpde.solve_batch(specs, u0s)
+    for a, b in zip(alone, got):
+        np.testing.assert_array_equal(b, a)
+
+
 def test_advance_batch_refuses_what_does_not_fit(fpde):
     """Seven N=10000 solves need 280 chunk workgroups at once, more than the
     GPU holds for this kernel: refused before any launch."""
