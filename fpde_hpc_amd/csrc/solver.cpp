@@ -879,6 +879,10 @@ fpde_status fpde_solver_advance_batch(fpde_solver* const* solvers, int count, in
     for (int i = 0; i < count; i++) {
         fpde_solver* s = solvers[i];
         if (s->spec.dims == 1 && s->block_mode) {
+            if (s->C < 1 || s->C > kMaxChunks || s->w.sp_on) {   // what the block launch itself refuses
+                set_error("advance_batch: solver outside the cooperative block form's limits");
+                return FPDE_ERR_ARG;
+            }
             blk.push_back(s);
             ends.push_back(nsteps <= 0 ? s->spec.steps : std::min(s->spec.steps, s->step + nsteps));
             wgs += s->C;
