@@ -457,16 +457,22 @@ def test_advance_batch_of_32_small_solves(fpde):
 
 
 def test_advance_batch_refuses_what_does_not_fit(fpde):
-    """Seven N=10000 solves need 280 chunk workgroups at once, more than the
-    GPU holds for this kernel: refused before any launch."""
-    solvers = [fpde.Solver(fpde.make_spec(10000, 10, 1e-3, hist_mode="split_bf16")) for _ in range(7)]
+    """Nine N=65536 solves need 9 x 256 chunk workgroups at once, more than
+    the GPU holds for this kernel: refused before any launch.  One of them
+    alone (256 workgroups, the longest line of the block form) still runs as
+    a batch and equals its solo result."""
+    u0 = fpde.init_1d(65536, 5)
+    spec = fpde.make_spec(65536, 6, 1e-3, hist_mode="split_bf16", picard_max=2, picard_tol=1e-10, u_top=u0[0])
+    solvers = [fpde.Solver(spec) for _ in range(9)]
     try:
         for s in solvers:
-            s.set_initial(fpde.init_1d(10000, 5))
+            s.set_initial(u0)
         with pytest.raises(fpde.FpdeError, match="exceed"):
             fpde.advance_batch(solvers, 0)
-        fpde.advance_batch(solvers[:2], 0)          # a batch that fits still runs
-        assert solvers[0].step == 10 and solvers[1].step == 10
+        assert all(s.step == 0 for s in solvers)
+        fpde.advance_batch(solvers[:1], 0)
+        assert solvers[0].step == 6
+        np.testing.assert_array_equal(solvers[0].solution(), fpde.solve(spec, u0))
     finally:
         for s in solvers:
             s.close()
