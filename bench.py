@@ -118,17 +118,25 @@ def cpu_baseline(N, M, seconds_hint=10.0, alpha=0.7, tau=5e-4, seed=1001, picard
     }
 
 
-def ensemble_run(F, count, nodes, msteps, hist_mode, picard, reps=3):
+def ensemble_run(F, count, nodes, msteps, hist_mode, picard, reps=3, uneven=False):
     """`count` independent 1D solves (alpha spread over [0.5, 0.9], own
-    initial states) timed three ways: one batch (fpde_solver_advance_batch:
-    one cooperative launch per history block for all of them), one after
-    another, and each on its own stream at once.  Returns the medians and
-    the largest difference of the batch results from the solo ones."""
+    initial states; with `uneven`, N spread over [nodes/4, nodes] and M over
+    [msteps/2, msteps] as well) timed three ways: one batch
+    (fpde_solver_advance_batch: one cooperative launch per history block for
+    all of them), one after another, and each on its own stream at once.
+    Returns the medians and the largest difference of the batch results from
+    the solo ones."""
     import numpy as np
     alphas = np.linspace(0.5, 0.9, count)
-    u0s = [F.init_1d(nodes, 700 + i) for i in range(count)]
-    specs = [F.make_spec(nodes, msteps, 1e-3, alpha=float(a), hist_mode=hist_mode, block=64,
-                         picard_max=picard[0], picard_tol=picard[1], u_top=u0[0]) for a, u0 in zip(alphas, u0s)]
+    if uneven:
+        ns = [int(v) for v in np.linspace(nodes // 4, nodes, count)[::-1]]
+        ms = [int(v) for v in np.linspace(msteps // 2, msteps, count)]
+    else:
+        ns, ms = [nodes] * count, [msteps] * count
+    u0s = [F.init_1d(n, 700 + i) for i, n in enumerate(ns)]
+    specs = [F.make_spec(n, m, 1e-3, alpha=float(a), hist_mode=hist_mode, block=64,
+                         picard_max=picard[0], picard_tol=picard[1], u_top=u0[0])
+             for n, m, a, u0 in zip(ns, ms, alphas, u0s)]
     solvers = [F.Solver(sp) for sp in specs]
 
     def reset_all():
@@ -165,8 +173,9 @@ def ensemble_run(F, count, nodes, msteps, hist_mode, picard, reps=3):
     for s in solvers:
         s.close()
     diff = max(float(np.abs(b - a).max() / np.abs(a).max()) for a, b in zip(sols["sequential"], sols["batch"]))
-    work = count * nodes * msteps
-    return {"count": count, "nodes": nodes, "steps": msteps, "alphas": [float(a) for a in alphas],
+    work = sum(n * m for n, m in zip(ns, ms))
+    return {"count": count, "nodes": ns if uneven else nodes, "steps": ms if uneven else msteps,
+            "alphas": [float(a) for a in alphas],
             "hist_mode": hist_mode, "seconds": times,
             "node_steps_per_s": {k: work / v for k, v in times.items()},
             "batch_speedup_vs_sequential": times["sequential"] / times["batch"],
@@ -257,6 +266,8 @@ def main():
     ap.add_argument("--rate", type=float, default=16.0, help="serving arrival rate, requests/s")
     ap.add_argument("--max-active", type=int, default=16)
     ap.add_argument("--count", type=int, default=6, help="solves in the batch (--workload ensemble)")
+    ap.add_argument("--uneven", action="store_true",
+                    help="--workload ensemble: spread N over [nodes/4, nodes] and M over [M/2, M] too")
     ap.add_argument("--no-ensemble", action="store_true",
                     help="skip the six-solve batch measurement added to the headline line")
     ap.add_argument("--gamma", type=float, default=0.0,
@@ -327,13 +338,14 @@ def main():
         return
     if args.workload == "ensemble":
         r = ensemble_run(F, args.count, args.nodes or 10000, args.msteps or 2000, args.headline,
-                         (args.picard, args.picard_tol), reps=max(1, args.steps))
+                         (args.picard, args.picard_tol), reps=max(1, args.steps), uneven=args.uneven)
         print(json.dumps({
             "metric": "node*steps/s, ensemble of independent 1D solves in one batch",
             "value": r["node_steps_per_s"]["batch"], "unit": "node*steps/s", "n_gpus": 1, "steps": max(1, args.steps),
             "warmup": 0, "ms_per_step": r["seconds"]["batch"] * 1e3, "higher_is_better": True, "scaling": "weak",
             "vs_baseline": None, "dtype": VARIANT_DTYPE.get(args.headline, "f64"), "data": "synthetic",
             "config": {"workload": f"{r['count']} x 1D N={r['nodes']}, M={r['steps']}, alpha 0.5..0.9",
+                       "uneven": bool(args.uneven),
                        "parallelism": "single GPU, one cooperative launch per history block for the batch"},
             "ensemble": r}))
         return
