@@ -183,7 +183,7 @@ def ensemble_run(F, count, nodes, msteps, hist_mode, picard, reps=3, uneven=Fals
             "batch_max_rel_diff_vs_solo": diff}
 
 
-def serve_mix(F, n_req, rate, seed=1004, quantum=64, budget_gb=200.0, max_active=16, hist_mode="split_f16"):
+def serve_mix(F, n_req, rate, seed=1004, quantum=256, budget_gb=200.0, max_active=16, hist_mode="split_f16"):
     """The serving mix: Poisson arrivals at `rate` req/s; 80% 1D (N~U[500,20000],
     M~U[1000,40000]), 20% 3D (32^3..96^3, M~U[200,1000]); alpha~U(0.3,0.95);
     25% distributed order (16 Gauss-Legendre nodes on [0.1,1]); seeded heads."""
@@ -266,6 +266,7 @@ def main():
     ap.add_argument("--rate", type=float, default=16.0, help="serving arrival rate, requests/s")
     ap.add_argument("--max-active", type=int, default=16)
     ap.add_argument("--count", type=int, default=6, help="solves in the batch (--workload ensemble)")
+    ap.add_argument("--quantum", type=int, default=256, help="steps per scheduling round (--workload serve)")
     ap.add_argument("--uneven", action="store_true",
                     help="--workload ensemble: spread N over [nodes/4, nodes] and M over [M/2, M] too")
     ap.add_argument("--no-ensemble", action="store_true",
@@ -350,7 +351,8 @@ def main():
             "ensemble": r}))
         return
     if args.workload == "serve":
-        r = serve_mix(F, args.requests, args.rate, max_active=args.max_active, hist_mode=args.headline)
+        r = serve_mix(F, args.requests, args.rate, quantum=args.quantum, max_active=args.max_active,
+                      hist_mode=args.headline)
         print(json.dumps({
             "metric": "p50/p99 solve latency and aggregate node*steps/s, serving mix (continuous batching)",
             "value": r["node_steps_per_s"], "unit": "node*steps/s", "n_gpus": 1, "steps": 1, "warmup": 0,

@@ -203,7 +203,7 @@ class Server:
     def cancel(self, rid):
         check(lib().fpde_server_cancel(self._h, rid))
 
-    def poll(self, quantum=64):
+    def poll(self, quantum=256):
         return lib().fpde_server_poll(self._h, quantum)
 
     def run(self, stop_after=0):

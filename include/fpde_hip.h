@@ -312,8 +312,9 @@ fpde_status fpde_server_destroy(fpde_server *srv);
 fpde_status fpde_server_submit(fpde_server *srv, const fpde_spec *spec, const double *u0,
                                int64_t *id_out);
 fpde_status fpde_server_cancel(fpde_server *srv, int64_t id);
-/* Run the scheduler until every queued request has finished (or until the
- * number of finished requests reaches stop_after, if > 0). */
+/* Run the scheduler, in rounds of 256 steps, until every queued request has
+ * finished (or until the number of finished requests reaches stop_after, if
+ * > 0). */
 fpde_status fpde_server_run(fpde_server *srv, int64_t stop_after);
 /* One scheduling round: admit, advance each active solve by up to `quantum`
  * steps, retire finished ones.  Returns number of active+queued requests. */

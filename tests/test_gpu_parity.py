@@ -481,7 +481,7 @@ def test_advance_batch_refuses_what_does_not_fit(fpde):
 def test_server_batches_block_form_requests(fpde):
     """1D requests in the cooperative block form (step_mode 0) advance in the
     server's batched launches, beside 3D requests on their own streams; the
-    server's quantum (64 steps) cuts the 1D steps at the block bounds of
+    server's quantum (256 steps) cuts the 1D steps at the block bounds of
     blocks 16/32/64, so each 1D result equals the solve alone exactly."""
     rng = np.random.default_rng(77)
     specs = []
