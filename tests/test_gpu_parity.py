@@ -386,9 +386,9 @@ def test_server_results_same_alone_and_in_mix(fpde):
 
 
 def test_advance_batch_matches_solo(fpde):
-    """Solves of uneven N, M, alpha, history form and block size advanced as
-    one batch (one cooperative launch per history block for the block-mode
-    ones; the plain form on its own stream) give each solo result exactly:
+    """Solves of uneven N, M, alpha, history form, block size and time grid
+    advanced as one batch (one cooperative launch per history block for the
+    block-mode ones; the plain form on its own stream) give each solo result exactly:
     every solve keeps its own barriers and Picard decisions.  One solver
     starts the batch 37 steps ahead, so the batch's blocks are not aligned."""
     cases = [
@@ -398,6 +398,8 @@ def test_advance_batch_matches_solo(fpde):
         dict(n=2048, m=150, alpha=0.7, hist_mode="split_bf16", block=16, do_q=4),
         dict(n=1200, m=400, alpha=0.75, hist_mode="plain", sum_alg=2, sum_param=10),
         dict(n=1500, m=120, alpha=0.5, hist_mode="plain"),
+        dict(n=900, m=200, alpha=0.6, hist_mode="split_f16", block=32, picard_max=2, picard_tol=1e-10,
+             taus=0.2 * np.diff((np.arange(201) / 200.0) ** 2)),     # graded prescribed grid
     ]
     specs, u0s = [], []
     for i, c in enumerate(cases):
