@@ -400,10 +400,12 @@ def test_advance_batch_matches_solo(fpde):
         dict(n=1500, m=120, alpha=0.5, hist_mode="plain"),
         dict(n=900, m=200, alpha=0.6, hist_mode="split_f16", block=32, picard_max=2, picard_tol=1e-10,
              taus=0.2 * np.diff((np.arange(201) / 200.0) ** 2)),     # graded prescribed grid
+        dict(n=600, m=90, alpha=0.7, hist_mode="split_bf16", gamma=0.8),   # space-fractional: own stream
+        dict(n=12, m=40, alpha=0.8, hist_mode="split_f16", block=16, dims=3),  # 3D: own stream
     ]
     specs, u0s = [], []
     for i, c in enumerate(cases):
-        u0 = fpde.init_1d(c["n"], 3000 + i)
+        u0 = (fpde.init_3d if c.get("dims") == 3 else fpde.init_1d)(c["n"], 3000 + i)
         kw = {k: v for k, v in c.items() if k not in ("n", "m")}
         specs.append(fpde.make_spec(c["n"], c["m"], 1e-3, u_top=u0[0], **kw))
         u0s.append(u0)
