@@ -12,15 +12,25 @@
  *                         the series' own tail bound.
  *   orc_time_summing    : L1 is exact for u linear in t (D^a t = t^(1-a)/G(2-a)),
  *                         O(tau^(2-a)) convergence for u=t^2, brute force loop.
- *   orc_wetness/inv_dw_dh/orc_kr : VGM closed forms; 1/C checked against a
- *                         numerical derivative of the wetness curve.
+ *   orc_wetness         : the van Genuchten retention curve evaluated in numpy.
+ *   orc_inv_dw_dh       : a numerical derivative of the wetness curve and the
+ *                         analytic derivative of the van Genuchten curve.
+ *   orc_kr              : the Mualem integral Se^(1/2) [int_0^Se dx/h /
+ *                         int_0^1 dx/h]^2 over the inverted retention curve,
+ *                         by scipy.integrate.quad.
  *   orc_thomas          : numpy.linalg.solve on random diagonally dominant systems.
  *   orc_gauss_legendre  : numpy.polynomial.legendre.leggauss.
  *   orc_solve_1d        : linear mode vs the Mittag-Leffler eigen-solution
- *                         (convergence in tau); VGM mode checked for the
- *                         discrete equation residual of every step.
+ *                         (convergence in tau); a lagged VGM step satisfies
+ *                         the rows at U^n; a Picard step run to tol 1e-14
+ *                         satisfies the rows assembled in numpy with K and 1/C
+ *                         at its own result (and the lagged step does not).
  *   orc_solve_3d        : a 3D field constant in x,y reduces to orc_solve_1d
- *                         (LOD sweeps along x,y are then identities).
+ *                         (LOD sweeps along x,y are then identities); every
+ *                         sweep of a VGM step with a non-zero history term
+ *                         satisfies its own rows assembled in numpy (which
+ *                         sweep carries the history, split3d even and odd, the
+ *                         field each sweep takes its coefficients from).
  *   DO path             : closed form int rho(a) D^a t^2 da = (t^1.9-t)/ln t.
  *   orc_space_v         : for F = 1 the closed-form weights telescope to
  *                         Dg (Z_i^(1-g) - h^(1-g))/(1-g); for random F it equals
@@ -616,7 +626,9 @@ int orc_solve_1d(const orc_params *p, double *u)
  * zero-flux bottom); the lateral sweeps have zero flux at both ends.  Every
  * sweep has the L*(U - U_prev) term (R at :3120; implicit3d=0) and evaluates
  * the coefficients at the field left by the previous sweep. */
-int orc_solve_3d(const orc_params *p, double *u)
+int orc_solve_3d(const orc_params *p, double *u) { return orc_solve_3d_trace(p, u, NULL); }
+
+int orc_solve_3d_trace(const orc_params *p, double *u, double *sweeps)
 {
     int n = p->nodes, M = p->steps;
     size_t nn = (size_t)n * n * n;
@@ -660,7 +672,10 @@ int orc_solve_3d(const orc_params *p, double *u)
                 for (int i = 0; i < n; i++) {
                     size_t id = base + stride * i;
                     lun[i] = cur[id];
-                    lH[i] = (axis == 2) ? H[id] : 0.0;
+                    /* Om += mult Da time_summing: mult = 1 in the m3d_c=2 sweep
+                     * only, or 1/3 in every sweep when split3d is odd (the
+                     * cached per-node sum, :4106-4108, :4172-4173, :4258-4263) */
+                    lH[i] = (p->split3d & 1) ? H[id] / 3.0 : ((axis == 2) ? H[id] : 0.0);
                     lz[i] = (axis == 0) ? i * h : a0 * h;
                 }
                 memcpy(luit, lun, sizeof(double) * n);
@@ -675,6 +690,7 @@ int orc_solve_3d(const orc_params *p, double *u)
                            Tsp ? kap : NULL, Tsp ? lV : NULL);
                 for (int i = 0; i < n; i++) cur[base + stride * i] = lnew[i];
             }
+            if (sweeps && s == M - 1) memcpy(sweeps + (size_t)(2 - axis) * nn, cur, sizeof(double) * nn);
         }
         memcpy(old + (size_t)(s + 1) * nn, cur, sizeof(double) * nn);
     }

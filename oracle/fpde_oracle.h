@@ -101,12 +101,23 @@ typedef struct {
     double gamma;        /* space-fractional order: 0 or 1 = classical diffusion,
                             (0,1) = the explicit one-sided scheme (calc_v :2303-2335),
                             in 3D along every sweep line */
+    int    split3d;      /* 3D, the reference's split3d flag (rp_split_3d): even = the
+                            history term only in the first (y, m3d_c=2) sweep; odd =
+                            a third of it in each of the three sweeps (Om1 :4172-4173,
+                            :4258-4263) */
+    int    fk;           /* kernel function g of the generalized Caputo derivative
+                            (func_in_kernel, g() :1889-1898): 0 t, 1 sqrt(t), 2 t^2,
+                            3 t^fp */
+    double fp;           /* func_power of fk 3 */
 } orc_params;
 
 /* 1D solve: u (nodes) holds u0 on entry, u^M on exit.  u_top = u0[0]. */
 int orc_solve_1d(const orc_params *p, double *u);
 /* 3D LOD solve on nodes^3 (depth index fastest); u holds u0 / u^M. */
 int orc_solve_3d(const orc_params *p, double *u);
+/* The same solve; sweeps (3 * nodes^3, may be NULL) receives the field after
+ * the y, the x and the z sweep of the last step, in that order. */
+int orc_solve_3d_trace(const orc_params *p, double *u, double *sweeps);
 
 /* Space-fractional sum of the explicit scheme (calc_v, :2303-2335, one-sided,
  * with_i = 0) on the uniform grid Z_k = k h of `nodes` nodes:

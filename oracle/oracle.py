@@ -25,7 +25,8 @@ class Params(ctypes.Structure):
         ("weight_mode", ctypes.c_int), ("series_eps", ctypes.c_double), ("do_q", ctypes.c_int),
         ("do_rule", ctypes.c_int), ("do_a0", ctypes.c_double), ("do_a1", ctypes.c_double),
         ("nthreads", ctypes.c_int), ("sum_alg", ctypes.c_int), ("sum_param", ctypes.c_double),
-        ("taus", ctypes.c_void_p), ("gamma", ctypes.c_double),
+        ("taus", ctypes.c_void_p), ("gamma", ctypes.c_double), ("split3d", ctypes.c_int),
+        ("fk", ctypes.c_int), ("fp", ctypes.c_double),
     ]
 
 
@@ -64,6 +65,7 @@ def lib():
         L.orc_gauss_legendre.argtypes = [ctypes.c_int, ctypes.c_double, ctypes.c_double, _dp, _dp]
         L.orc_solve_1d.argtypes = [ctypes.POINTER(Params), _dp]
         L.orc_solve_3d.argtypes = [ctypes.POINTER(Params), _dp]
+        L.orc_solve_3d_trace.argtypes = [ctypes.POINTER(Params), _dp, _dp]
         L.orc_folded_weights.argtypes = [ctypes.POINTER(Params), _dp, _dp, _dp]
         L.orc_thomas.argtypes = [ctypes.c_int, _dp, _dp, _dp, _dp, _dp]
         L.orc_init_1d.argtypes = [ctypes.c_int, ctypes.c_double, ctypes.c_uint64, _dp]
@@ -177,7 +179,7 @@ def thomas(a, b, c, d):
 def make_params(nodes, steps, tau, alpha=0.8, length=1.0, ksat=None, coef_mode=0,
                 picard_max=1, picard_tol=1e-12, weight_mode=1, series_eps=1e-5,
                 do_q=0, do_rule=0, do_a0=0.1, do_a1=1.0, nthreads=0, fourier=1.0,
-                u_top=None, sum_alg=0, sum_param=1e-5, taus=None, gamma=0.0):
+                u_top=None, sum_alg=0, sum_param=1e-5, taus=None, gamma=0.0, split3d=0, fk=0, fp=0.6):
     if ksat is None:
         if coef_mode == 0:
             h = length / (nodes - 1)
@@ -185,7 +187,8 @@ def make_params(nodes, steps, tau, alpha=0.8, length=1.0, ksat=None, coef_mode=0
         else:
             ksat = 1.0
     p = Params(nodes, steps, tau, alpha, length, ksat, coef_mode, picard_max, picard_tol,
-               weight_mode, series_eps, do_q, do_rule, do_a0, do_a1, nthreads, sum_alg, sum_param, None, gamma)
+               weight_mode, series_eps, do_q, do_rule, do_a0, do_a1, nthreads, sum_alg, sum_param, None, gamma,
+               split3d, fk, fp)
     if taus is not None:
         p._taus = np.ascontiguousarray(taus, dtype=np.float64)    # kept alive with the struct
         assert p._taus.size == steps
@@ -219,6 +222,16 @@ def solve_3d(params, u0):
     if rc != 0:
         raise RuntimeError("orc_solve_3d failed")
     return u
+
+
+def solve_3d_trace(params, u0):
+    """(u^M, fields after the y, x and z sweeps of the last step)."""
+    u = np.array(u0, dtype=np.float64, copy=True)
+    sw = np.zeros(3 * u.size)
+    rc = lib().orc_solve_3d_trace(ctypes.byref(params), _p(u), _p(sw))
+    if rc != 0:
+        raise RuntimeError("orc_solve_3d failed")
+    return u, sw.reshape(3, u.size)
 
 
 def folded_weights(params):

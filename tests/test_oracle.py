@@ -263,6 +263,171 @@ def test_vgm_limits(oracle):
     assert all(0 < k < 1 for k in ks) and all(a > b for a, b in zip(ks, ks[1:]))
 
 
+# Independent van Genuchten-Mualem evaluation in numpy, from the published
+# model (van Genuchten 1980, m = 1 - 1/n) with the reference's constants
+# n = 1.25, s0 = 0.0736, s1 = 0.55, a = 0.066 1/cm (fpde_hpc_1d.cpp:2513-2518);
+# P is the suction head in metres (a * 100 P in 1/cm * cm).
+_VG_N, _VG_S0, _VG_S1, _VG_A = 1.25, 0.0736, 0.55, 0.066
+_VG_M = 1.0 - 1.0 / _VG_N
+
+
+def _vg_se(P):
+    x = _VG_A * 100.0 * np.maximum(P, 0.0)
+    return (1.0 + x ** _VG_N) ** -_VG_M
+
+
+def _vg_capacity(P):
+    """C = -d theta / dP of theta = s0 + (s1 - s0) Se(P)."""
+    x = _VG_A * 100.0 * P
+    return (_VG_S1 - _VG_S0) * _VG_M * _VG_N * _VG_A * 100.0 * x ** (_VG_N - 1) * (1 + x ** _VG_N) ** (-_VG_M - 1)
+
+
+def _mualem_kr(se):
+    """Mualem (1976): Kr(Se) = Se^(1/2) [int_0^Se dx/h(x) / int_0^1 dx/h(x)]^2,
+    h(x) the suction of the retention curve inverted: (x^(-1/m) - 1)^(1/n) / a.
+    Integrated numerically (the (1-x)^(-1/n) end point of the denominator
+    through quad's algebraic weight)."""
+    inv_h = lambda x: _VG_A / (x ** (-1.0 / _VG_M) - 1.0) ** (1.0 / _VG_N)
+    smooth = lambda x: inv_h(x) * (1.0 - x) ** (1.0 / _VG_N) if x < 1.0 else _VG_A * _VG_M ** (1.0 / _VG_N)
+    den, _ = integrate.quad(smooth, 0.0, 1.0, weight="alg", wvar=(0.0, -1.0 / _VG_N), epsabs=0, epsrel=1e-13,
+                            limit=200)
+    num, _ = integrate.quad(inv_h, 0.0, se, epsabs=0, epsrel=1e-13, limit=200)
+    return se ** 0.5 * (num / den) ** 2
+
+
+@pytest.mark.parametrize("P", [0.003, 0.05, 0.5, 1.0, 4.0, 20.0])
+def test_kr_is_the_mualem_integral(oracle, P):
+    """orc_kr (k_form 1, fpde_hpc_1d.cpp:2634-2635) against the Mualem integral
+    over the retention curve, and the saturation orc_wetness gives against the
+    van Genuchten curve (cv_form 4, :2513-2524)."""
+    se = (oracle.wetness(P) - _VG_S0) / (_VG_S1 - _VG_S0)
+    assert abs(se - _vg_se(P)) <= 1e-14
+    ref = _mualem_kr(se)
+    assert abs(oracle.kr(P) - ref) <= 1e-12 * ref
+    assert abs(_mualem_kr_closed(P) - ref) <= 1e-12 * ref
+
+
+def test_inverse_capacity_matches_van_genuchten_derivative(oracle):
+    """1/C of inv_dw_dh (:2558-2571) against the analytic derivative of the
+    van Genuchten curve written in numpy."""
+    for P in (0.01, 0.3, 1.0, 5.0):
+        assert abs(oracle.inv_dw_dh(P) * _vg_capacity(P) - 1.0) <= 1e-12
+
+
+def test_picard_step_is_a_fixed_point_of_its_own_coefficients(oracle):
+    """A Picard step run to tol 1e-14 satisfies the tridiagonal rows (A_/B/R,
+    :3095-3142) assembled in numpy with K and 1/C at U^{n+1} itself, the
+    closed-form history term of the two older intervals and the Dirichlet top
+    / zero-flux bottom.  The lagged step (coefficients at U^n, picard_max 1,
+    the reference scheme) does not: the check sees the coefficient update."""
+    nodes, tau, alpha, fo = 60, 1e-3, 0.7, 10.0
+    u0 = oracle.init_1d(nodes, 11)
+    h = 1.0 / (nodes - 1)
+    z = np.arange(nodes) * h
+    Da = 1.0 / G(1 - alpha)
+    L = Da * tau ** (1 - alpha) / (1 - alpha) / tau
+    n = 2                                            # the step that makes U^3
+
+    def b(t):                                        # b(t, t+1; n+1), g(t) = t
+        return tau ** (1 - alpha) * ((n + 1 - t) ** (1 - alpha) - (n - t) ** (1 - alpha)) / (1 - alpha)
+
+    def run(picard_max):
+        out = [u0]
+        for steps in (1, 2, 3):
+            p = oracle.make_params(nodes, steps, tau, alpha, u_top=u0[0], fourier=fo, picard_max=picard_max,
+                                   picard_tol=1e-14, weight_mode=0)
+            out.append(oracle.solve_1d(p, u0))
+        return out, p.ksat
+
+    def residual(U, ksat):
+        H = Da * sum(b(t) * (U[t + 1] - U[t]) / tau for t in range(n))
+        un, unew = U[n], U[n + 1]
+        P = z - unew
+        K = ksat * _mualem_kr_closed(P)
+        iC = 1.0 / _vg_capacity(P)
+        i = np.arange(1, nodes - 1)
+        lhs = (K[i - 1] * iC[i] * unew[i - 1] - iC[i] * (K[i] + K[i - 1]) * unew[i] + K[i] * iC[i] * unew[i + 1]) \
+            / h ** 2 - L * unew[i]
+        return np.max(np.abs(lhs - (-L * un[i] + H[i]))) / (L * np.max(np.abs(unew)))
+
+    U, ksat = run(40)
+    assert residual(U, ksat) <= 1e-11
+    assert U[3][0] == u0[0] and abs(U[3][-1] - U[3][-2]) <= 1e-14
+    Ul, ksat_l = run(1)
+    assert residual(Ul, ksat_l) >= 1e-6
+
+
+def _mualem_kr_closed(P):
+    """Kr at suction P by the Mualem integral, vectorised over P (used for
+    residual checks: tabulated once per distinct P would be slow, so the
+    closed form that the integral test above pins is evaluated)."""
+    se = _vg_se(P)
+    return np.sqrt(se) * (1 - (1 - se ** (1 / _VG_M)) ** _VG_M) ** 2
+
+
+@pytest.mark.parametrize("split3d", [0, 1])
+def test_3d_sweeps_satisfy_their_rows(oracle, split3d):
+    """Each LOD sweep of a VGM 3D step (step 2, so the history term H is not
+    0) satisfies its own tridiagonal rows assembled in numpy: the y sweep
+    (m3d_c=2, fpde_hpc_1d.cpp:5211-5325) with coefficients at U^n and
+    Om = -L U^n + H; the x sweep (:5362-5476) at the field the y sweep left,
+    H absent; the z sweep (:5513-5626) at the field the x sweep left, Dirichlet
+    top.  With split3d odd (Om1 :4172-4173, :4258-4263) every sweep carries
+    H/3.  Zero flux at both ends of the lateral lines, at the bottom of the
+    depth lines."""
+    n, tau, alpha = 9, 1e-3, 0.75
+    u0 = oracle.init_3d(n, 21)
+    h = 1.0 / (n - 1)
+
+    def params(M):
+        return oracle.make_params(n, M, tau, alpha, u_top=u0[0], fourier=3.0, weight_mode=0, split3d=split3d)
+
+    U = [u0] + [oracle.solve_3d(params(k), u0) for k in (1, 2)]
+    u3, sw = oracle.solve_3d_trace(params(3), u0)
+    assert np.array_equal(u3, sw[2])
+    ksat = params(1).ksat
+    Da = 1.0 / G(1 - alpha)
+    L = Da * tau ** (1 - alpha) / (1 - alpha) / tau
+    step = 2
+    b = lambda t: tau ** (1 - alpha) * ((step + 1 - t) ** (1 - alpha) - (step - t) ** (1 - alpha)) / (1 - alpha)
+    H = Da * sum(b(t) * (U[t + 1] - U[t]) / tau for t in range(step))
+
+    def lines(F, axis):                       # idx = z + n (x + n y): F3[y][x][z]
+        F3 = F.reshape(n, n, n)
+        if axis == "y":
+            return F3.transpose(2, 1, 0).reshape(n * n, n)
+        if axis == "x":
+            return F3.transpose(2, 0, 1).reshape(n * n, n)
+        return F3.reshape(n * n, n)
+
+    def depth(axis):
+        zz = np.arange(n) * h
+        if axis == "z":
+            return np.ones((n * n, 1)) * zz[None, :]
+        return np.repeat(zz, n)[:, None] * np.ones((1, n))     # lateral lines lie at one depth
+
+    def residual(axis, before, after, Hs):
+        ub, ua, Hl, Z = lines(before, axis), lines(after, axis), lines(Hs, axis), depth(axis)
+        P = Z - ub
+        K = ksat * _mualem_kr_closed(P)
+        iC = 1.0 / _vg_capacity(P)
+        i = np.arange(1, n - 1)
+        lhs = (K[:, i - 1] * iC[:, i] * ua[:, i - 1] - iC[:, i] * (K[:, i] + K[:, i - 1]) * ua[:, i]
+               + K[:, i] * iC[:, i] * ua[:, i + 1]) / h ** 2 - L * ua[:, i]
+        top = ua[:, 1] if axis != "z" else lines(u0, "z")[:, 0]
+        assert np.max(np.abs(ua[:, 0] - top)) <= 1e-14
+        assert np.max(np.abs(ua[:, -1] - ua[:, -2])) <= 1e-14
+        return np.max(np.abs(lhs - (-L * ub[:, i] + Hl[:, i]))) / (L * np.max(np.abs(ua)))
+
+    share = H / 3.0 if split3d else H
+    rest = H / 3.0 if split3d else 0.0 * H
+    assert residual("y", U[2], sw[0], share) <= 1e-12
+    assert residual("x", sw[0], sw[1], rest) <= 1e-12
+    assert residual("z", sw[1], sw[2], rest) <= 1e-12
+    # the check has teeth: H is visible against the rows' scale
+    assert np.max(np.abs(H)) >= 1e-4 * L * np.max(np.abs(u3))
+
+
 # ---------------------------------------------------------------- linear algebra
 def test_thomas_matches_dense_solve(oracle):
     rng = np.random.default_rng(5)
