@@ -71,51 +71,163 @@ def run_variant(F, workload, mode, block, steps, warmup, prof_mask_timed=None, n
     return dict(spec=spec, u=u_last, u_first=u, times=times, prof=prof, timed=timed_stats, tuned=tuned)
 
 
-def cpu_baseline(N, M, seconds_hint=10.0, alpha=0.7, tau=5e-4, seed=1001, picard=(1, 0.0)):
-    """FP64 oracle (oracle/, the reference loop order: node-outer, t-inner,
-    OpenMP over nodes) timed on this host: the history sum at the run-average
-    history length n = M/2 over all N nodes, repeated for about seconds_hint
-    seconds of CPU work (median of the repetitions), plus one oracle step (VGM
-    coefficients + Thomas, median over repeated 4-step solves);
-    node*steps/s = N / (t_sum + t_step)."""
-    import numpy as np
-    from oracle import oracle as O
-    O.lib()
-    cores = min(16, os.cpu_count() or 1)
-    n = M // 2
-    rng = np.random.default_rng(0)
-    old = np.cumsum(rng.standard_normal((n + 1, N)) * 1e-4, axis=0)
-    Time = O.time_grid(n + 1, tau)
-    _, kb = O.l1_weights(alpha, n, Time, weight_mode=1)
-    taus = np.full(n + 2, tau)
+def cpu_info():
+    """Threads for the OpenMP oracle and the host CPU model.  The threads are
+    the host share this process is given (OMP_NUM_THREADS; on the GPU box
+    that is the CPU share of one GPU), else every core of the host."""
+    ncpu = os.cpu_count() or 1
+    try:
+        cores = max(1, min(ncpu, int(os.environ.get("OMP_NUM_THREADS", ncpu))))
+    except ValueError:
+        cores = ncpu
+    model = "unknown"
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    model = line.split(":", 1)[1].strip()
+                    break
+    except OSError:
+        pass
+    return cores, model, ncpu
+
+
+def _median_time(fn, min_reps, seconds):
     reps = []
     t_start = time.perf_counter()
-    while len(reps) < 3 or time.perf_counter() - t_start < seconds_hint:
+    while len(reps) < min_reps or time.perf_counter() - t_start < seconds:
         t0 = time.perf_counter()
-        O.time_summing(old, kb, taus, nthreads=cores)
+        fn()
         reps.append(time.perf_counter() - t0)
-    t_sum = statistics.median(reps)
-    t_sum_total = time.perf_counter() - t_start
-    del old
-    u0 = O.init_1d(N, seed)
-    p = O.make_params(N, 4, tau, alpha, u_top=u0[0], nthreads=cores, picard_max=picard[0], picard_tol=picard[1])
-    steps = []
-    t_start = time.perf_counter()
-    while len(steps) < 3 or (time.perf_counter() - t_start < 1.0 and len(steps) < 50):
-        t0 = time.perf_counter()
-        O.solve_1d(p, u0)
-        steps.append((time.perf_counter() - t0) / 4)
-    t_step = statistics.median(steps)
-    return {
-        "value": N / (t_sum + t_step),
-        "unit": "node*steps/s",
-        "cores": cores,
-        "kind": "port",
-        "sample": (f"oracle time_summing over n={n} levels x N={N} nodes (run-average history "
-                   f"length for M={M}), {len(reps)} reps in {t_sum_total:.1f} s, median {t_sum * 1e3:.3f} ms; "
-                   f"+ oracle step {t_step * 1e3:.3f} ms (median of {len(steps)} 4-step solves); "
-                   f"extrapolated to the whole run"),
-    }
+    return statistics.median(reps), len(reps), time.perf_counter() - t_start
+
+
+def cpu_history_step(O, cores, nodes, n, tau, orders, node_frac=1.0, seconds=5.0):
+    """Oracle time of the history term of one step at history length n (the
+    run average n = M/2): per order the weights kb_3_cache of that step
+    (precalc_kb_3) and time_summing over the nodes (reference loop order:
+    node-outer, t-inner, OpenMP over nodes).  With node_frac < 1 the sum runs
+    on that share of the nodes and is scaled up (the sum is independent per
+    node).  Returns (seconds per step, repetitions, sampled seconds)."""
+    import numpy as np
+    ns = max(1, int(round(nodes * node_frac)))
+    rng = np.random.default_rng(0)
+    old = np.cumsum(rng.standard_normal((n + 1, ns)) * 1e-4, axis=0)
+    Time = O.time_grid(n + 1, tau)
+    taus = np.full(n + 2, tau)
+
+    def one():
+        for a in orders:
+            _, kb = O.l1_weights(a, n, Time, weight_mode=1)
+            O.time_summing(old, kb, taus, nthreads=cores)
+    t, reps, total = _median_time(one, 3, seconds)
+    return t / node_frac, reps, total
+
+
+def cpu_baselines(F, picard):
+    """The FP64 oracle (oracle/, restating the reference path) timed on this
+    host: config 0 in full (median of 3 solves, and its result, against which
+    the GPU results of config 0 are compared); configs 1-3 by sampling one
+    step at the run-average history length n = M/2 (history term + one step
+    of coefficients and line solves), extrapolated to node*steps/s.  The
+    cpu_baseline object of the line is config 1's."""
+    import numpy as np
+    os.environ.setdefault("OMP_PROC_BIND", "close")
+    from oracle import oracle as O
+    O.lib()
+    cores, model, ncpu = cpu_info()
+    out = {"cores": cores, "host_cpus": ncpu, "cpu_model": model, "omp_proc_bind": os.environ.get("OMP_PROC_BIND")}
+    # config 0: full solves
+    spec0, u0 = F.workload("1d_small", picard_max=picard[0], picard_tol=picard[1])
+    p0 = O.make_params(spec0.nodes, spec0.steps, spec0.tau, spec0.alpha, ksat=spec0.ksat, picard_max=picard[0],
+                       picard_tol=picard[1], nthreads=cores)
+    res = {}
+
+    def full0():
+        res["u"] = O.solve_1d(p0, u0)
+    t0, reps0, _ = _median_time(full0, 3, 0.0)
+    out["config0"] = {"value": spec0.nodes * spec0.steps / t0, "unit": "node*steps/s", "seconds": t0,
+                      "sample": f"full solve N={spec0.nodes} M={spec0.steps}, median of {reps0}"}
+    print(f"[cpu] config 0 full solve {t0:.2f} s ({cores} threads)", file=sys.stderr, flush=True)
+    # config 1: history at n = M/2 over all nodes + one step
+    spec1, u1 = F.workload("1d_large", picard_max=picard[0], picard_tol=picard[1])
+    N1, M1 = spec1.nodes, spec1.steps
+    th, reps, tot = cpu_history_step(O, cores, N1, M1 // 2, spec1.tau, [spec1.alpha], seconds=10.0)
+    p1 = O.make_params(N1, 4, spec1.tau, spec1.alpha, ksat=spec1.ksat, nthreads=cores, picard_max=picard[0],
+                       picard_tol=picard[1])
+    ts, sreps, _ = _median_time(lambda: O.solve_1d(p1, u1), 3, 1.0)
+    ts /= 4
+    out["config1"] = {"value": N1 / (th + ts), "unit": "node*steps/s", "seconds_per_step": th + ts,
+                      "sample": (f"history term at n={M1 // 2} levels x N={N1} nodes (the run average for "
+                                 f"M={M1}): {reps} reps in {tot:.1f} s, median {th * 1e3:.2f} ms; + one step "
+                                 f"{ts * 1e3:.3f} ms (median of {sreps} 4-step solves); extrapolated to the run")}
+    print(f"[cpu] config 1 sampled: {th * 1e3:.1f} ms history + {ts * 1e3:.2f} ms step", file=sys.stderr, flush=True)
+    # config 2: distributed order, 32 Gauss-Legendre orders (the oracle sums per order)
+    spec2, u2 = F.workload("do", picard_max=picard[0], picard_tol=picard[1])
+    al, _ = O.gauss_legendre(spec2.do_q, spec2.do_a0, spec2.do_a1)
+    th2, reps2, tot2 = cpu_history_step(O, cores, spec2.nodes, spec2.steps // 2, spec2.tau, list(al), seconds=5.0)
+    p2 = O.make_params(spec2.nodes, 4, spec2.tau, spec2.alpha, ksat=spec2.ksat, nthreads=cores,
+                       picard_max=picard[0], picard_tol=picard[1], do_q=spec2.do_q, do_rule=spec2.do_rule,
+                       do_a0=spec2.do_a0, do_a1=spec2.do_a1)
+    ts2, sreps2, _ = _median_time(lambda: O.solve_1d(p2, u2), 3, 1.0)
+    ts2 /= 4
+    out["config2"] = {"value": spec2.nodes / (th2 + ts2), "unit": "node*steps/s",
+                      "seconds_per_step": th2 + ts2,
+                      "sample": (f"history term of {spec2.do_q} orders at n={spec2.steps // 2} levels x "
+                                 f"N={spec2.nodes}: {reps2} reps in {tot2:.1f} s, median {th2 * 1e3:.1f} ms; + one "
+                                 f"step {ts2 * 1e3:.2f} ms (median of {sreps2} 4-step solves); extrapolated")}
+    print(f"[cpu] config 2 sampled: {th2 * 1e3:.1f} ms history + {ts2 * 1e3:.2f} ms step", file=sys.stderr,
+          flush=True)
+    # config 3: 128^3, history on 1/8 of the nodes (scaled), one LOD step
+    spec3, u3 = F.workload("3d")
+    nn = spec3.nodes ** 3
+    th3, reps3, tot3 = cpu_history_step(O, cores, nn, spec3.steps // 2, spec3.tau, [spec3.alpha], node_frac=0.125,
+                                        seconds=5.0)
+    p3 = O.make_params(spec3.nodes, 2, spec3.tau, spec3.alpha, ksat=spec3.ksat, nthreads=cores)
+    ts3, sreps3, _ = _median_time(lambda: O.solve_3d(p3, u3), 1, 0.0)
+    ts3 /= 2
+    out["config3"] = {"value": nn / (th3 + ts3), "unit": "node*steps/s", "seconds_per_step": th3 + ts3,
+                      "sample": (f"history term at n={spec3.steps // 2} levels on 1/8 of the {nn} nodes, scaled: "
+                                 f"{reps3} reps in {tot3:.1f} s, median {th3 * 1e3:.0f} ms; + one LOD step "
+                                 f"{ts3 * 1e3:.0f} ms (a 2-step solve); extrapolated")}
+    print(f"[cpu] config 3 sampled: {th3 * 1e3:.0f} ms history + {ts3 * 1e3:.0f} ms step", file=sys.stderr,
+          flush=True)
+    return out, res["u"]
+
+
+def config_runs(F, picard, u_cpu0):
+    """One MI355X solve of each of configs 0, 2 and 3 in the plain FP64 form
+    and the mixed form (after one untimed solve each), with the mixed result's
+    error against the FP64 GPU result, and for config 0 both against the
+    oracle's full CPU solve.  Outside the timed region of the headline."""
+    import numpy as np
+    out = {}
+    for key, name, mixed, block in (("config0", "1d_small", "split_bf16", 64), ("config2", "do", "split_f16", 64),
+                                    ("config3", "3d", "split_bf16", 32)):
+        rs = {}
+        for mode in ("plain", mixed):
+            r = run_variant(F, name, mode, block, 1, 1, picard=picard)
+            rs[mode] = r
+        spec = rs[mixed]["spec"]
+        unknowns = spec.nodes if spec.dims == 1 else spec.nodes ** 3
+        ref = rs["plain"]["u"]
+        e = {}
+        for mode, r in rs.items():
+            t = r["times"][0]
+            e[mode] = {"seconds": t, "node_steps_per_s": unknowns * spec.steps / t}
+        d = np.abs(rs[mixed]["u"] - ref)
+        e[mixed]["rel_err_vs_fp64_gpu"] = float(d.max() / np.abs(ref).max())
+        e[mixed]["max_abs_err_vs_fp64_gpu"] = float(d.max())
+        if key == "config0" and u_cpu0 is not None:
+            for mode, r in rs.items():
+                dc = np.abs(r["u"] - u_cpu0)
+                e[mode]["rel_err_vs_cpu_oracle"] = float(dc.max() / np.abs(u_cpu0).max())
+        out[key] = {"workload": name, "nodes": spec.nodes, "dims": spec.dims, "time_steps": spec.steps,
+                    "mixed": mixed, "block": spec.block, "variants": e,
+                    "speedup_mixed_vs_plain": e["plain"]["seconds"] / e[mixed]["seconds"]}
+        print(f"[configs] {key}: plain {e['plain']['seconds']:.3f} s, {mixed} {e[mixed]['seconds']:.3f} s, "
+              f"rel err {e[mixed]['rel_err_vs_fp64_gpu']:.2e}", file=sys.stderr, flush=True)
+    return out
 
 
 def ensemble_run(F, count, nodes, msteps, hist_mode, picard, reps=3, uneven=False):
@@ -183,35 +295,12 @@ def ensemble_run(F, count, nodes, msteps, hist_mode, picard, reps=3, uneven=Fals
             "batch_max_rel_diff_vs_solo": diff}
 
 
-def serve_mix(F, n_req, rate, seed=1004, quantum=256, budget_gb=200.0, max_active=16, hist_mode="split_f16"):
-    """The serving mix: Poisson arrivals at `rate` req/s; 80% 1D (N~U[500,20000],
-    M~U[1000,40000]), 20% 3D (32^3..96^3, M~U[200,1000]); alpha~U(0.3,0.95);
-    25% distributed order (16 Gauss-Legendre nodes on [0.1,1]); seeded heads."""
+def serve_mix(F, n_req, rate, seed=1004, quantum=256, budget_gb=200.0, max_active=16, hist_mode="split_bf16"):
+    """The serving mix (fpde_hpc_amd.serving_mix): requests submitted at their
+    Poisson arrival times while the server runs; latency = submit to
+    retirement on the server clock."""
     import numpy as np
-    rng = np.random.default_rng(seed)
-    reqs = []
-    t = 0.0
-    for i in range(n_req):
-        t += rng.exponential(1.0 / rate)
-        is3d = rng.random() < 0.2
-        do = rng.random() < 0.25
-        alpha = float(rng.uniform(0.3, 0.95))
-        if is3d:
-            n = int(rng.integers(32, 97))
-            M = int(rng.integers(200, 1001))
-            u0 = F.init_3d(n, 100000 + i)
-            kw = dict(dims=3, block=16)
-        else:
-            n = int(rng.integers(500, 20001))
-            M = int(rng.integers(1000, 40001))
-            u0 = F.init_1d(n, 100000 + i)
-            kw = dict(dims=1, block=64)
-        if do:
-            kw.update(do_q=16, do_rule=0, do_a0=0.1, do_a1=1.0)
-        tau = 1e-3 if is3d else 5e-4
-        spec = F.make_spec(n, M, tau, alpha=alpha, hist_mode=hist_mode, u_top=u0[0], **kw)
-        work = (n ** 3 if is3d else n) * M
-        reqs.append((t, spec, u0, work))
+    reqs = F.serving_mix(n_req, rate, seed, hist_mode)
     srv = F.Server(mem_budget_bytes=int(budget_gb * 2 ** 30), max_active=max_active)
     ids = []
     t0 = time.perf_counter()
@@ -262,6 +351,8 @@ def main():
     ap.add_argument("--nodes", type=int, default=None)
     ap.add_argument("--msteps", type=int, default=None)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-configs", action="store_true", help="skip the configs 0/2/3 solves of the default run")
+    ap.add_argument("--no-serve", action="store_true", help="skip the serving mix of the default run")
     ap.add_argument("--requests", type=int, default=512, help="serving mix size (--workload serve)")
     ap.add_argument("--rate", type=float, default=16.0, help="serving arrival rate, requests/s")
     ap.add_argument("--max-active", type=int, default=16)
@@ -441,20 +532,25 @@ def main():
     else:
         ms = statistics.mean(x["ms_step"] for x in ts)
         nl = statistics.mean(x["step_launches"] for x in ts)
-        # step kernel algorithmic bytes per launch: U^n in, U^{n+1} and D out (24 B/node)
-        # plus the near-field history terms it reads (8 B per term)
         # a launch covers M/launches steps (cooperative block launches: a
-        # whole history block; otherwise one Picard iteration)
+        # whole history block; otherwise one Picard iteration).  Algorithmic
+        # bytes (counted by the solver per step): 100 B per node for the step
+        # itself (coefficients, rows, line solve) + 8 B per near-field history
+        # term the step kernel sums
         steps_per_launch = M / max(nl, 1)
-        bpl = 24.0 * unknowns * steps_per_launch
-        achieved = bpl / (ms * 1e-3 / max(nl, 1)) / 1e9
+        alg = statistics.mean(x["step_alg_bytes"] for x in ts)
+        bpl = alg / max(nl, 1)
+        achieved = alg / (ms * 1e-3) / 1e9
         kname = ("k_step1d_block" if steps_per_launch > 1.5 else "k_step1d") if spec.dims == 1 else "lod3d sweeps"
         roof = {"kernel": kname, "bound": "hbm",
                 "achieved": achieved, "peak": MI355X_HBM_GBS, "unit": "GB/s",
                 "frac": achieved / MI355X_HBM_GBS, "traffic": None,
                 "alg_bytes_per_launch": bpl, "avg_launch_us": ms * 1e3 / max(nl, 1),
                 "steps_per_launch": steps_per_launch,
-                "note": "latency-bound sequential step chain (Picard iterations, grid barriers)"}
+                "alg_bytes_per_node_step": bpl / (unknowns * max(steps_per_launch, 1e-9)),
+                "alg_bytes_rule": "100 B per node and step (coefficients, rows, line solve) + 8 B per near-field "
+                                  "history term summed inside the step kernel",
+                "note": "latency-bound sequential step chain (Picard iterations, grid barriers); see latency"}
 
     # HBM traffic per launch of the dominant kernel, from the committed
     # rocprofv3 --pmc FETCH_SIZE / WRITE_SIZE summary (calibrated on a copy)
@@ -466,12 +562,24 @@ def main():
             roof["traffic_source"] = "profiles/hbm_counters.json (rocprofv3 --pmc, copy-calibrated)"
     except (OSError, ValueError, KeyError):
         pass
-    # the CPU baseline samples the classical scheme (the oracle's spatial sum
-    # is a serial O(N^2) loop per step and is not part of the history path)
-    cpu = None if (args.no_cpu or spec.dims != 1 or spec.do_q or spec.gamma) else cpu_baseline(
-        N, M, alpha=spec.alpha, tau=spec.tau, picard=(spec.picard_max, spec.picard_tol))
+    # time per step of the headline solve and where it goes (profiled solve:
+    # device time of the history kernels and of the step kernels per step)
+    hp = head["prof"]
+    latency = {"us_per_step": t_mean * 1e6 / M,
+               "history_us_per_step": hp["ms_history"] * 1e3 / M, "step_us_per_step": hp["ms_step"] * 1e3 / M,
+               "step_launch_us": roof.get("avg_launch_us") if dom == 1 else None,
+               "picard_iterations_per_step": hp["picard_iters_total"] / max(M, 1)}
+    default_run = (args.workload == "1d_large" and args.nodes is None and args.msteps is None and not args.gamma)
+    # CPU baselines (the oracle on this host's cores, full config 0, sampled
+    # configs 1-3) of the classical scheme
+    cpu, cpu_all, u_cpu0 = None, None, None
+    if default_run and not args.no_cpu:
+        cpu_all, u_cpu0 = cpu_baselines(F, picard)
+        c1 = cpu_all["config1"]
+        cpu = {"value": c1["value"], "unit": c1["unit"], "cores": cpu_all["cores"], "kind": "port",
+               "cpu_model": cpu_all["cpu_model"], "sample": c1["sample"]}
     out = {
-        "metric": "node*steps/s, mixed-MFMA history-sum solver (vs FP64 GPU at <=1e-8 rel err)",
+        "metric": "node*steps/s mixed-MFMA vs FP64 GPU at <=1e-8 rel err; p50/p99 solve latency",
         "value": unknowns * M / t_mean,
         "unit": "node*steps/s",
         "n_gpus": 1,
@@ -496,14 +604,32 @@ def main():
                    "parallelism": "single GPU"},
         "roofline": roof,
         "cpu_baseline": cpu,
+        "latency": latency,
         "variants": variants,
     }
     if results[args.headline].get("tuned"):
         out["config"]["block_autotune"] = results[args.headline]["tuned"]
     if "plain" in variants:
         out["speedup_vs_plain_fp64"] = variants["plain"]["ms_per_solve"] / (t_mean * 1e3)
+        pv = variants["plain"]
+        out["speedup_note"] = (
+            f"plain runs its FP64 history stream between every two steps, so its steps launch per Picard "
+            f"iteration ({pv['step_launches']} step launches, {pv['step_ms']:.0f} ms of step time); the "
+            f"{args.headline} form runs one cooperative launch per history block "
+            f"({variants[args.headline]['step_launches']} launches, {variants[args.headline]['step_ms']:.0f} ms)")
         if ref_u is not None:
             out["rel_err_vs_fp64"] = variants[args.headline].get("rel_err_vs_fp64")
+    if cpu_all is not None:
+        out["cpu_baselines"] = cpu_all
+    if default_run and not args.no_configs:
+        out["configs"] = config_runs(F, picard, u_cpu0)
+    if default_run and not args.no_serve:
+        out["serving"] = serve_mix(F, args.requests, args.rate, quantum=args.quantum, max_active=args.max_active,
+                                   hist_mode=args.headline)
+        sv = out["serving"]
+        print(f"[serve] {sv['completed']}/{sv['requests']} done, p50 {sv['p50_latency_s']:.2f} s, "
+              f"p99 {sv['p99_latency_s']:.2f} s, {sv['node_steps_per_s']:.3e} node*steps/s", file=sys.stderr,
+              flush=True)
     if not args.no_ensemble and args.workload == "1d_large" and args.nodes is None and spec.dims == 1:
         # independent solves of the same column (alpha 0.5..0.9, M=2000), one
         # batch against one after another: not part of the headline figure

@@ -3,8 +3,10 @@
 The library is built in-tree by fpde_hpc_amd.build; loading fails loudly when
 it is missing -- there is no CPU fallback in this package.
 """
+import atexit
 import ctypes
 import os
+import weakref
 
 HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(HERE, "libfpde_hip.so")
@@ -27,8 +29,8 @@ class Spec(ctypes.Structure):
         ("do_q", ctypes.c_int), ("do_rule", ctypes.c_int), ("do_a0", ctypes.c_double),
         ("do_a1", ctypes.c_double), ("hist_mode", ctypes.c_int), ("block", ctypes.c_int),
         ("step_mode", ctypes.c_int), ("sum_alg", ctypes.c_int), ("sum_param", ctypes.c_double),
-        ("taus", ctypes.c_void_p), ("var_tau", ctypes.c_int), ("reserved", ctypes.c_int),
-        ("gamma", ctypes.c_double),
+        ("taus", ctypes.c_void_p), ("var_tau", ctypes.c_int), ("split3d", ctypes.c_int),
+        ("gamma", ctypes.c_double), ("fk", ctypes.c_int), ("fp", ctypes.c_double),
     ]
 
 
@@ -38,7 +40,8 @@ class Stats(ctypes.Structure):
         ("ms_total", ctypes.c_double), ("ms_history", ctypes.c_double), ("ms_step", ctypes.c_double),
         ("history_launches", ctypes.c_int64), ("step_launches", ctypes.c_int64),
         ("history_alg_bytes", ctypes.c_double), ("history_alg_flops", ctypes.c_double),
-        ("device_bytes", ctypes.c_size_t), ("picard_iters_total", ctypes.c_int),
+        ("device_bytes", ctypes.c_size_t), ("picard_iters_total", ctypes.c_int), ("reserved", ctypes.c_int),
+        ("step_alg_bytes", ctypes.c_double), ("steps_done", ctypes.c_int64),
     ]
 
 
@@ -54,6 +57,7 @@ class ResultInfo(ctypes.Structure):
 SIGNATURES = [
     ("fpde_last_error", ctypes.c_char_p, []),
     ("fpde_version", ctypes.c_int, []),
+    ("fpde_shutdown", ctypes.c_int, []),
     ("fpde_device_count", ctypes.c_int, []),
     ("fpde_cl_ctx_create", ctypes.c_int, [ctypes.c_int, ctypes.POINTER(_vp)]),
     ("fpde_cl_ctx_destroy", ctypes.c_int, [_vp]),
@@ -115,6 +119,28 @@ SIGNATURES = [
 ]
 
 _lib = None
+# objects owning device state (Solver, Server): closed by shutdown() before
+# the process exits, so no HIP call is left to interpreter finalisation
+_owners = weakref.WeakSet()
+
+
+def register_owner(obj):
+    _owners.add(obj)
+
+
+def shutdown():
+    """Close every live Solver/Server, then release the library's own device
+    tables (fpde_shutdown).  Registered with atexit on first load: Python's
+    atexit handlers run before the interpreter tears modules down and before
+    the C runtime's exit handlers (where a profiler finalises its tools)."""
+    if _lib is None:
+        return
+    for obj in list(_owners):
+        try:
+            obj.close()
+        except Exception:
+            pass
+    _lib.fpde_shutdown()
 
 
 def lib():
@@ -129,6 +155,7 @@ def lib():
             f.restype = res
             f.argtypes = args
         _lib = L
+        atexit.register(shutdown)
     return _lib
 
 

@@ -33,6 +33,10 @@ struct fpde_cl_prog {
     int sum_alg = 0;
     int64_t os = 0;
     int bs = 16;
+    // fr_do.cpp's program (kernels Fill, DOC, Sum, eq_F, eq_Solve) and its
+    // element type, told apart from fpde_hpc_1d.cpp's by the kernel names
+    bool frdo = false;
+    bool frdo_double = true;
 };
 
 struct fpde_cl_kernel {
@@ -186,6 +190,10 @@ fpde_status fpde_cl_program_create(fpde_cl_ctx* ctx, const char* source, fpde_cl
     p->sum_alg = (int)parse_define(source, "sum_alg", 0);
     p->os = parse_define(source, "OS", 0);
     p->bs = (int)parse_define(source, "BS", 16);
+    const std::string src(source ? source : "");
+    p->frdo = src.find("eq_Solve") != std::string::npos && src.find("DOC") != std::string::npos;
+    // the text is instantiated for T = float or double (cuda_text_macro, fr_do.cpp:301-306)
+    p->frdo_double = src.find("float *df") == std::string::npos && src.find("float* df") == std::string::npos;
     *out = p;
     return FPDE_OK;
 }
@@ -197,14 +205,19 @@ fpde_status fpde_cl_program_destroy(fpde_cl_prog* prog) {
 
 fpde_status fpde_cl_kernel_create(fpde_cl_prog* prog, const char* name, fpde_cl_kernel** out) {
     if (!prog || !name || !out) { set_error("null argument"); return FPDE_ERR_ARG; }
-    if (std::strcmp(name, "Sum") != 0) {
-        set_error(std::string("unknown kernel '") + name + "' (available: Sum)");
+    static const char* frdo_names[] = {"Fill", "DOC", "Sum", "eq_F", "eq_Solve"};
+    bool known = std::strcmp(name, "Sum") == 0;
+    if (prog->frdo)
+        for (const char* nm : frdo_names) known = known || std::strcmp(name, nm) == 0;
+    if (!known) {
+        set_error(std::string("unknown kernel '") + name +
+                  (prog->frdo ? "' (available: Fill, DOC, Sum, eq_F, eq_Solve)" : "' (available: Sum)"));
         return FPDE_ERR_NOTFOUND;
     }
     auto* k = new fpde_cl_kernel;
     k->prog = prog;
     k->name = name;
-    k->args.resize(12);
+    k->args.resize(15);
     *out = k;
     return FPDE_OK;
 }
@@ -236,12 +249,127 @@ fpde_status fpde_cl_kernel_set_buffer_arg(fpde_cl_kernel* k, int idx, fpde_cl_bu
     return FPDE_OK;
 }
 
+}  // extern "C"
+
+namespace {
+
+// fr_do.cpp's kernels, with the argument ABI of its host code (every scalar
+// a double; DO_Cder_CUDA :373-419, eq_one_step_cuda :793-818): decode the
+// arguments, check every index range the kernel will touch against the
+// buffers, then launch.  The global size is the reference's work-item count.
+fpde_status frdo_launch(fpde_cl_ctx* ctx, fpde_cl_kernel* k, int64_t nth) {
+    FrdoLaunch L;
+    L.is_double = k->prog->frdo_double;
+    const int64_t es = L.is_double ? 8 : 4;
+    fpde_status rc;
+    double v[12];
+    fpde_cl_buf* bf[5] = {nullptr, nullptr, nullptr, nullptr, nullptr};
+    auto dbl = [&](std::initializer_list<int> idx) {
+        int q = 0;
+        for (int i : idx)
+            if ((rc = arg_double(k, i, &v[q++])) != FPDE_OK) return false;
+        return true;
+    };
+    auto bufs = [&](std::initializer_list<int> idx) {
+        int q = 0;
+        for (int i : idx) {
+            if ((rc = arg_buf(k, i, &bf[q])) != FPDE_OK) return false;
+            L.b[q] = bf[q]->ptr;
+            q++;
+        }
+        return true;
+    };
+    auto len = [&](int q) { return (int64_t)(bf[q]->bytes / es); };
+    auto bad = [&](const char* what) { set_error(std::string("fr_do ") + k->name + ": " + what); return FPDE_ERR_ARG; };
+    const std::string& nm = k->name;
+    if (nm == "Fill") {                                   // (tau, na, df, bkk) :373-379
+        if (!dbl({0, 1}) || !bufs({2, 3})) return rc;
+        L.kernel = FRDO_FILL;
+        L.tau = v[0];
+        L.na = (int)v[1];
+        L.n = nth;
+        if (L.na < 1 || len(0) < L.n || len(1) < L.n * L.na) return bad("df needs ntau and bkk ntau*na values");
+    } else if (nm == "DOC") {   // (mode, ntau, tau, na, t0, j0, maxt, use_f, F, df, bkk, wa, do1, gs, total_ntau) :383-403
+        if (!dbl({0, 1, 2, 3, 4, 5, 6, 7, 14}) || !bufs({8, 9, 10, 12, 13})) return rc;
+        L.kernel = FRDO_DOC;
+        L.ntau = (int)v[1];
+        L.tau = v[2];
+        L.na = (int)v[3];
+        L.t0 = (int)v[4];
+        L.j0 = (int)v[5];
+        L.maxt = (int)v[6];
+        L.use_f = (int)v[7];
+        L.tot = (int64_t)v[8];
+        if (L.na < 1 || nth < L.na) return bad("the work items must cover na orders");
+        L.nf = (int)(nth / L.na);
+        // bf: 0 F, 1 df, 2 bkk, 3 do1, 4 gs
+        if (L.t0 < 0 || L.ntau < 0) return bad("negative time range");
+        if (len(2) < (int64_t)L.ntau * L.na || len(3) < (int64_t)L.nf * L.na * L.ntau || len(4) < L.na)
+            return bad("bkk, do1 or gs too small");
+        if (L.use_f && (L.ntau > L.tot || len(0) < (int64_t)L.nf * L.tot)) return bad("F too small for the levels");
+        if (!L.use_f && len(1) < L.ntau) return bad("df too small");
+    } else if (nm == "Sum" ) {   // (ntau, ntau2, t0, na, append, do1, do2, wa) :405-415
+        if (!dbl({0, 1, 2, 3, 4}) || !bufs({5, 6, 7})) return rc;
+        L.kernel = FRDO_SUM;
+        L.ntau = (int)v[0];
+        L.ntau2 = (int)v[1];
+        L.t0 = (int)v[2];
+        L.na = (int)v[3];
+        L.append = (int)v[4];
+        const int64_t nt = L.ntau - L.t0;
+        if (nt <= 0) return FPDE_OK;
+        L.nf = (int)(nth / nt);
+        if (L.na < 1 || L.ntau > L.ntau2 || len(0) < (int64_t)L.nf * L.na * L.ntau || len(1) < (int64_t)L.nf * L.ntau2 ||
+            len(2) < L.na)
+            return bad("do1, do2 or wa too small");
+    } else if (nm == "eq_F") {   // (nx, eq_Nt, D, b0, tau, h, j, C, DF, F, ex) :793-808
+        if (!dbl({0, 1, 2, 3, 4, 5, 6, 10}) || !bufs({7, 8, 9})) return rc;
+        L.kernel = FRDO_EQ_F;
+        L.nx = (int)v[0];
+        L.Nt = (int)v[1];
+        L.D = v[2];
+        L.b0 = v[3];
+        L.tau = v[4];
+        L.h = v[5];
+        L.j = (int)v[6];
+        L.ex = v[7];
+        if (nth < L.nx) L.nx = (int)nth;
+        if (L.nx < 1 || L.j < 0 || L.j + 1 >= L.Nt || len(0) < (int64_t)L.nx * L.Nt || len(1) < (int64_t)L.nx * L.Nt ||
+            len(2) < L.nx)
+            return bad("C and DF need nx*Nt values, F nx, and j+1 < Nt");
+    } else if (nm == "eq_Solve") {   // (t, nt, nx, A, C, inv, F) :810-818
+        if (!dbl({0, 1, 2, 3}) || !bufs({4, 5, 6})) return rc;
+        L.kernel = FRDO_EQ_SOLVE;
+        L.j = (int)v[0];
+        L.Nt = (int)v[1];
+        L.nx = (int)v[2];
+        L.A = v[3];
+        if (L.nx < 1 || L.j < 0 || L.j >= L.Nt || len(0) < (int64_t)L.nx * L.Nt || len(1) < (int64_t)L.nx * L.nx ||
+            len(2) < L.nx)
+            return bad("C needs nx*nt values, inv nx*nx, F nx, and t < nt");
+    } else {
+        set_error("unknown fr_do kernel");
+        return FPDE_ERR_NOTFOUND;
+    }
+    rc = launch_frdo(L, ctx->stream);
+    if (rc != FPDE_OK) return rc;
+    // the reference synchronises after every launch (opencl_cuda_base.cpp:710-713)
+    FPDE_HIP_TRY(hipStreamSynchronize(ctx->stream));
+    return FPDE_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
 // "Sum" with the argument ABI of fr_ocl_call_sum (fpde_hpc_1d.cpp:1851-1868):
 // 0 S, 1 old, 2 int old_size, 3 tau, 4 Htmp, 5 row2_precalc, 6 kb_3_cache,
-// 7 double sum_param, 8 gtj, 9 gt, 10 alpha, 11 int tstep.
+// 7 double sum_param, 8 gtj, 9 gt, 10 alpha, 11 int tstep.  The kernels of
+// fr_do.cpp's program go to frdo_launch.
 fpde_status fpde_cl_launch(fpde_cl_ctx* ctx, fpde_cl_kernel* k, int ndims, const size_t* gdims, const size_t* ldims) {
     (void)ldims;
     if (!ctx || !k || ndims < 1 || !gdims) { set_error("bad launch arguments"); return FPDE_ERR_ARG; }
+    if (k->prog->frdo) return frdo_launch(ctx, k, (int64_t)gdims[0]);
     fpde_cl_buf *S, *old, *tau, *kb;
     int64_t old_size;
     double sum_param;
@@ -333,7 +461,7 @@ fpde_status fpde_history_sum_f64(double* S, const double* D, int64_t n, int64_t 
     fpde_status rc = launch_hist_plain(D, ld, (int)nodes, (int)n, (int)n, omega, part, &splits, 0);
     if (rc == FPDE_OK) {
         // combine: S = sum of partial rows (no near levels, no corrections)
-        rc = launch_hist_combine(S, part, splits, ld, 0, D, ld, nodes, 0, 0, omega, nullptr, 0);
+        rc = launch_hist_combine(S, part, splits, ld, 0, D, ld, nodes, 0, 0, omega, nullptr, 1.0, 0);
     }
     hipError_t e = hipDeviceSynchronize();
     hipFree(part);
@@ -425,7 +553,7 @@ fpde_status fpde_history_block_gemm(int mode, const double* D, int64_t ld, int64
         // P[r] = sum over K splits (combine kernel, one row at a time)
         for (int r = 0; r < block && rc == FPDE_OK; r++)
             rc = launch_hist_combine(P + (size_t)r * ld, part, splits, (int64_t)block * ld, (int64_t)r * ld, D, ld,
-                                     nodes, 0, 0, omega, nullptr, 0);
+                                     nodes, 0, 0, omega, nullptr, 1.0, 0);
     }
     e = hipDeviceSynchronize();
     if (rc == FPDE_OK && e != hipSuccess) { rc = FPDE_ERR_HIP; set_error(hipGetErrorString(e)); }

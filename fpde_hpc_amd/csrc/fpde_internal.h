@@ -4,6 +4,7 @@
 #include <cstdint>
 #include <cstddef>
 #include <string>
+#include <map>
 #include <vector>
 
 #include "../../include/fpde_hip.h"
@@ -114,6 +115,7 @@ struct DeviceWeights {
     // prescribed non-uniform grid (spec.taus): cumulative times, step lengths,
     // the local coefficient of every step (host and device copies)
     bool grid = false;
+    std::vector<double> hTime; // host copy of Time when the grid is known at create
     double* Time = nullptr;    // [M+2]
     double* taus = nullptr;    // [M+1]
     double* Lv = nullptr;      // [M]
@@ -135,9 +137,14 @@ struct DeviceWeights {
 // outside t in [t_lo, n).
 fpde_status launch_weights_grid(const fpde_spec& s, const DeviceWeights& w, int n_first, int rows, int len,
                                 int t_lo, int reversed, double* W, int64_t ldw, hipStream_t st);
-// local coefficient of a step of length tn: sum_q cw_q Da_q kkk0(tn) / tn,
-// kkk0 = b(n,n+1;n+1) = tn^(1-a)/(1-a)  (init_time_step, fpde_hpc_1d.cpp:3942-3949)
-double grid_local(const fpde_spec& s, double tn);
+// local coefficient of the step from Tn to Tn1: sum_q cw_q Da_q kkk0 / (Tn1 - Tn),
+// kkk0 = b(n,n+1;n+1) (init_time_step, fpde_hpc_1d.cpp:3942-3949): (Tn1-Tn)^(1-a)/(1-a)
+// for g(t) = t, the integral of the generalized kernel otherwise
+double grid_local(const fpde_spec& s, double Tn, double Tn1);
+// exponent gp of the kernel function g(t) = t^gp of spec.fk (1 for fk 0)
+double kernel_exponent(const fpde_spec& s);
+// int_{p0}^{pa} (pa^gp - s^gp)^-a ds (host; the newest interval's weight)
+double kb_newest(double a, double gp, double p0, double pa);
 fpde_status launch_set_grid_point(const DeviceWeights& w, int n, double t_next, double tau, double L, hipStream_t st);
 
 constexpr int kMaxSeriesTerms = 64;
@@ -220,6 +227,7 @@ struct Step1DArgs {
     unsigned* flagbar;        // Picard barrier flags [2 parities][kMaxChunks][32] (128 B per flag)
     int epar;                 // parity of this cooperative launch
     int* err;                 // set when a grid barrier times out
+    int64_t wait_ticks;       // a grid barrier gives up after this many ticks of the device's wall clock
     // series-compressed history (ser_P > 0): moments H [P][ld], coefficient
     // rows of step n; D is then a ring of two levels (row t & 1)
     int ser_P;
@@ -339,6 +347,10 @@ fpde_status launch_space_apply(const double* F, int N, const double* sp_omega, d
                                hipStream_t st);
 fpde_status launch_space_reduce(double* V, const double* P, const double* F, const double* sp_corr, int N,
                                 int64_t ldp, hipStream_t st);
+// plain FP64 stream over t < t_hi: partial rows part[ks*ld + i], ks < the
+// returned row count of hist_plain_plan (at most kPlainMaxSplits)
+constexpr int kPlainMaxSplits = 64;
+int hist_plain_plan(int nodes, int t_hi, int* chunk_out);
 fpde_status launch_hist_plain(const double* D, int64_t ld, int nodes, int n, int t_hi, const double* omega,
                               double* part, int* splits_out, hipStream_t st);
 fpde_status launch_ref_sum(double* S, const double* old, int64_t old_size, int64_t os, int64_t nodes,
@@ -366,8 +378,50 @@ fpde_status launch_step1d(const Step1DArgs& a, hipStream_t st);
 fpde_status launch_step1d_block(const Step1DArgs& a, hipStream_t st);
 bool step1d_block_supported(int C);
 bool step1d_batch_supported(int64_t wgs);
+// One launch of fr_do.cpp's program kernels (frdo.hip), decoded by the facade.
+enum { FRDO_FILL = 0, FRDO_DOC, FRDO_SUM, FRDO_EQ_F, FRDO_EQ_SOLVE };
+struct FrdoLaunch {
+    int kernel = 0;
+    bool is_double = true;      // the program's element type T
+    int64_t n = 0;              // Fill: levels
+    int nf = 1, ntau = 0, ntau2 = 0, na = 0, t0 = 0, j0 = 0, maxt = -1, use_f = 0, append = 0;
+    int64_t tot = 0;            // DOC: levels per grid point of F (total_ntau)
+    int nx = 0, Nt = 0, j = 0;  // eq_F / eq_Solve (j: the level, t of eq_Solve)
+    double tau = 0, D = 0, b0 = 0, h = 0, A = 0, ex = 0;
+    void* b[5] = {nullptr, nullptr, nullptr, nullptr, nullptr};
+};
+fpde_status launch_frdo(const FrdoLaunch& L, hipStream_t st);
+
+int64_t barrier_wait_ticks();
 bool solver_batchable(const fpde_solver* s);
 int solver_chunks(const fpde_solver* s);
+
+// Device memory arena of the request server: segments of device memory
+// allocated when no free block fits (at least seg_bytes each, the sum capped
+// at limit), handed out first-fit in 256-byte units and returned with
+// coalescing, so that a steady stream of admissions and retirements reuses
+// the same memory without hipMalloc/hipFree.  Not thread-safe (the server's
+// scheduler owns it).
+class DeviceArena {
+public:
+    DeviceArena(size_t seg_bytes, size_t limit) : seg_bytes_(seg_bytes), limit_(limit) {}
+    ~DeviceArena();
+    void* alloc(size_t bytes);          // nullptr when nothing fits within the limit
+    void release(void* p);
+    size_t reserved() const { return reserved_; }   // device memory held in segments
+    size_t in_use() const { return in_use_; }
+    int64_t segment_allocations() const { return (int64_t)segs_.size(); }
+private:
+    struct Block { size_t size; int seg; };
+    size_t seg_bytes_, limit_, reserved_ = 0, in_use_ = 0;
+    std::vector<std::pair<char*, size_t>> segs_;
+    std::map<char*, Block> free_, live_;
+};
+
+// A solver whose device buffers come from `arena` and whose launches go to
+// `stream` (both owned by the caller, the request server): destroying it
+// synchronises the stream and returns the buffers to the arena.
+fpde_status solver_create_in(const fpde_spec* spec, DeviceArena* arena, hipStream_t stream, fpde_solver** out);
 fpde_status launch_step1d_batch(const Step1DArgs* base, const BatchTick& t, hipStream_t st);
 fpde_status launch_coef3d(const double* U, int n, double h, double ksat, int mode, double* K, double* iC,
                           hipStream_t st);
@@ -378,14 +432,14 @@ fpde_status launch_lateral3d(double* U, const double* K, const double* iC, const
 // (else nullptr: the sweep evaluates them itself)
 fpde_status launch_zline3d(double* U, double* Uprev, const double* utop, int n, double h, double h2inv,
                            double ksat, int mode, double L, double* Dlev, double kap, const double* E,
-                           const double* K, const double* iC, hipStream_t st);
+                           const double* K, const double* iC, const double* H, hipStream_t st);
 fpde_status launch_space3d(const double* U, const double* K, int n, int axis, double h, const double* sp_omega,
                            const double* sp_corr, double* E, hipStream_t st);
 fpde_status launch_series3d(double* S, double* H, int P, const double* dnew, const double* ser_c,
-                            const double* ser_v, int64_t nodes, int64_t ld, hipStream_t st);
+                            const double* ser_v, int64_t nodes, int64_t ld, double hscale, hipStream_t st);
 fpde_status launch_hist_combine(double* S, const double* part, int nsplit, int64_t part_stride, int64_t part_off,
                                 const double* D, int64_t ld, int64_t nodes, int n, int near_lo,
-                                const double* omega, const double* corr, hipStream_t st);
+                                const double* omega, const double* corr, double hscale, hipStream_t st);
 int gemm_max_splits(int64_t nodes);
 fpde_status launch_part_reduce(double* P, int splits, int rows, int64_t ld, int nodes, hipStream_t st);
 

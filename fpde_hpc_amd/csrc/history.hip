@@ -555,22 +555,31 @@ fpde_status launch_part_reduce(double* P, int splits, int rows, int64_t ld, int 
 
 // ---------------------------------------------------------------------------
 // Host launchers
-fpde_status launch_hist_plain(const double* D, int64_t ld, int nodes, int n, int t_hi,
-                              const double* omega, double* part, int* splits_out,
-                              hipStream_t st) {
-    if (t_hi <= 0) { *splits_out = 0; return FPDE_OK; }
-    // enough workgroups to fill the chip (~4096 with the node tiles), each
-    // streaming >= 32 levels; at most 64 partial rows
+// t-split plan of the plain stream: enough workgroups to fill the chip
+// (~4096 with the node tiles), each streaming >= 32 levels; at most
+// kPlainMaxSplits partial rows.  Returns the number of partial rows.
+int hist_plain_plan(int nodes, int t_hi, int* chunk_out) {
+    if (t_hi <= 0) { if (chunk_out) *chunk_out = 0; return 0; }
     int tiles = (nodes + 511) / 512;
     int want = (4096 + tiles - 1) / tiles;
-    if (want > 64) want = 64;
+    if (want > kPlainMaxSplits) want = kPlainMaxSplits;
     if (want < 1) want = 1;
     int chunk = (t_hi + want - 1) / want;
     if (chunk < 32) chunk = 32;
-    int splits = (t_hi + chunk - 1) / chunk;
+    if (chunk_out) *chunk_out = chunk;
+    return (t_hi + chunk - 1) / chunk;
+}
+
+fpde_status launch_hist_plain(const double* D, int64_t ld, int nodes, int n, int t_hi,
+                              const double* omega, double* part, int* splits_out,
+                              hipStream_t st) {
+    int chunk = 0;
+    const int splits = hist_plain_plan(nodes, t_hi, &chunk);
+    *splits_out = splits;
+    if (splits == 0) return FPDE_OK;
+    int tiles = (nodes + 511) / 512;
     dim3 grid(tiles, splits);
     hipLaunchKernelGGL(k_hist_plain, grid, dim3(256), 0, st, D, ld, nodes, n, t_hi, chunk, omega, part);
-    *splits_out = splits;
     return FPDE_OK;
 }
 

@@ -2,7 +2,8 @@
 //
 // Field layout idx = z + n*(x + n*y): depth z fastest (the reference's N axis,
 // idx() at :1181).  One step = three sweeps in the reference order:
-//   y (m3d_c=2, carries the history term), x (m3d_c=1), z (m3d_c=0).
+//   y (m3d_c=2, carries the history term), x (m3d_c=1), z (m3d_c=0); with
+//   split3d odd each of the three carries a third of it.
 // Every sweep solves, line by line, L*(U - U_in) = div_axis(K grad U) + H
 // with coefficients evaluated at the field the sweep starts from.
 //  - y and x lines: one thread per line (adjacent threads = adjacent z, so
@@ -126,7 +127,8 @@ __global__ __launch_bounds__(256) void k_zline3d(double* __restrict__ U, double*
                                                  const double* __restrict__ utop, int n, double h,
                                                  double h2inv, double ksat, int mode, double L,
                                                  double* __restrict__ Dlev, double kap,
-                                                 const double* __restrict__ E) {
+                                                 const double* __restrict__ E,
+                                                 const double* __restrict__ H) {
     __shared__ double a[2][256], c[2][256], d[2][256], sK[256];
     const int tid = threadIdx.x;
     const int line = blockIdx.x;                   // x + n*y
@@ -150,7 +152,7 @@ __global__ __launch_bounds__(256) void k_zline3d(double* __restrict__ U, double*
             ar = Km * ic * h2inv * kap;
             cr = Kp * ic * h2inv * kap;
             br = -(ic * ((Kp * h2inv) + (Km * h2inv)) * kap + L);
-            dr = -L * u;
+            dr = -L * u + (H ? H[base + tid] : 0.0);
             if (E) dr -= ic * E[base + tid];
         }
         double ib = 1.0 / br;
@@ -313,7 +315,8 @@ __global__ __launch_bounds__(kZThreads, 2) void k_zline3d_seg(double* __restrict
                                                        double* __restrict__ Dlev, double kap,
                                                        const double* __restrict__ E,
                                                        const double* __restrict__ Kin,
-                                                       const double* __restrict__ iCin) {
+                                                       const double* __restrict__ iCin,
+                                                       const double* __restrict__ H) {
     constexpr int LD = 129;                      // odd row stride: the 16 lines hit distinct banks
     __shared__ double sK[kZLines][LD], sIC[kZLines][LD], sD[kZLines][LD];
     __shared__ double sEnd[kZSeg][kZLines][7];
@@ -334,7 +337,7 @@ __global__ __launch_bounds__(kZThreads, 2) void k_zline3d_seg(double* __restrict
         else vgm_coefs(u, i * h, ksat, mode, k, ic);
         sK[l][i] = k;
         sIC[l][i] = ic;
-        sD[l][i] = -L * u - (E ? ic * E[base0 + q] : 0.0);
+        sD[l][i] = (-L * u + (H ? H[base0 + q] : 0.0)) - (E ? ic * E[base0 + q] : 0.0);
     }
     __syncthreads();
     const int l = tid & (kZLines - 1), sg = tid / kZLines;
@@ -426,12 +429,13 @@ __global__ __launch_bounds__(kZThreads, 2) void k_zline3d_seg(double* __restrict
     }
 }
 
-// S[i] = sum of far partials + near levels [near_lo, n) + two-level corrections.
+// S[i] = sum of far partials + near levels [near_lo, n) + two-level corrections,
+// times hscale (1, or 1/3 when every sweep carries a third of the term).
 __global__ __launch_bounds__(256) void k_hist_combine(double* __restrict__ S, const double* __restrict__ part,
                                                       int nsplit, int64_t part_stride, int64_t part_off,
                                                       const double* __restrict__ D, int64_t ld, int64_t nodes,
                                                       int n, int near_lo, const double* __restrict__ omega,
-                                                      const double* __restrict__ corr) {
+                                                      const double* __restrict__ corr, double hscale) {
     int64_t i = (int64_t)blockIdx.x * 256 + threadIdx.x;
     if (i >= nodes) return;
     constexpr int W = 16;      // independent loads per batch
@@ -462,7 +466,7 @@ __global__ __launch_bounds__(256) void k_hist_combine(double* __restrict__ S, co
     double cr = 0.0;
     if (n >= 1) cr = corr[2 * n] * D[i];
     if (n >= 2) cr = fma(corr[2 * n + 1], D[ld + i], cr);
-    S[i] = far + (nr0 + nr1) + cr;
+    S[i] = (far + (nr0 + nr1) + cr) * hscale;
 }
 
 // Series-compressed history (sum_alg 2): add the newest interval dnew to the
@@ -471,17 +475,18 @@ __global__ __launch_bounds__(256) void k_hist_combine(double* __restrict__ S, co
 __global__ __launch_bounds__(256) void k_series3d(double* __restrict__ S, double* __restrict__ H, int P,
                                                   const double* __restrict__ dnew,
                                                   const double* __restrict__ ser_c,
-                                                  const double* __restrict__ ser_v, int64_t nodes, int64_t ld) {
+                                                  const double* __restrict__ ser_v, int64_t nodes, int64_t ld,
+                                                  double hscale) {
     int64_t i = (int64_t)blockIdx.x * 256 + threadIdx.x;
     if (i >= nodes) return;
     if (!dnew) { S[i] = 0.0; return; }
-    S[i] = series_update(H, ld, i, P, dnew[i], ser_c, ser_v);
+    S[i] = series_update(H, ld, i, P, dnew[i], ser_c, ser_v) * hscale;
 }
 
 fpde_status launch_series3d(double* S, double* H, int P, const double* dnew, const double* ser_c,
-                            const double* ser_v, int64_t nodes, int64_t ld, hipStream_t st) {
+                            const double* ser_v, int64_t nodes, int64_t ld, double hscale, hipStream_t st) {
     hipLaunchKernelGGL(k_series3d, dim3((unsigned)((nodes + 255) / 256)), dim3(256), 0, st, S, H, P, dnew, ser_c,
-                       ser_v, nodes, ld);
+                       ser_v, nodes, ld, hscale);
     return FPDE_OK;
 }
 
@@ -508,24 +513,25 @@ fpde_status launch_lateral3d(double* U, const double* K, const double* iC, const
 
 fpde_status launch_zline3d(double* U, double* Uprev, const double* utop, int n, double h, double h2inv,
                            double ksat, int mode, double L, double* Dlev, double kap, const double* E,
-                           const double* K, const double* iC, hipStream_t st) {
+                           const double* K, const double* iC, const double* H, hipStream_t st) {
     if (n > 256) { set_error("3D grid: at most 256 nodes per axis"); return FPDE_ERR_ARG; }
     if (n <= 8 * kZSeg && L > 0.0) {        // partitioned: 16 lines per workgroup, 16 segments of <= 8 rows
         hipLaunchKernelGGL(k_zline3d_seg, dim3((n * n + kZLines - 1) / kZLines), dim3(kZThreads), 0, st, U, Uprev, utop, n,
-                           h, h2inv, ksat, mode, L, Dlev, kap, E, K, iC);
+                           h, h2inv, ksat, mode, L, Dlev, kap, E, K, iC, H);
         return FPDE_OK;
     }
     int bs = ((n + 63) / 64) * 64;
     hipLaunchKernelGGL(k_zline3d, dim3(n * n), dim3(bs), 0, st, U, Uprev, utop, n, h, h2inv, ksat, mode, L, Dlev,
-                       kap, E);
+                       kap, E, H);
     return FPDE_OK;
 }
 
 fpde_status launch_hist_combine(double* S, const double* part, int nsplit, int64_t part_stride,
                                 int64_t part_off, const double* D, int64_t ld, int64_t nodes, int n,
-                                int near_lo, const double* omega, const double* corr, hipStream_t st) {
+                                int near_lo, const double* omega, const double* corr, double hscale,
+                                hipStream_t st) {
     hipLaunchKernelGGL(k_hist_combine, dim3((unsigned)((nodes + 255) / 256)), dim3(256), 0, st, S, part,
-                       nsplit, part_stride, part_off, D, ld, nodes, n, near_lo, omega, corr);
+                       nsplit, part_stride, part_off, D, ld, nodes, n, near_lo, omega, corr, hscale);
     return FPDE_OK;
 }
 

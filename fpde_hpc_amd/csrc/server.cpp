@@ -13,6 +13,7 @@
 // of a request; only u0 and the final field cross PCIe.
 #include "fpde_internal.h"
 
+#include <algorithm>
 #include <chrono>
 #include <deque>
 #include <map>
@@ -20,6 +21,60 @@
 #include <vector>
 
 using namespace fpde;
+
+// ---------------------------------------------------------------------------
+// DeviceArena (declared in fpde_internal.h)
+DeviceArena::~DeviceArena() {
+    for (auto& sg : segs_) hipFree(sg.first);
+}
+
+void* DeviceArena::alloc(size_t bytes) {
+    const size_t need = (std::max<size_t>(bytes, 1) + 255) / 256 * 256;
+    for (auto it = free_.begin(); it != free_.end(); ++it) {       // first fit
+        if (it->second.size < need) continue;
+        char* p = it->first;
+        Block b = it->second;
+        free_.erase(it);
+        if (b.size > need) free_[p + need] = Block{b.size - need, b.seg};
+        live_[p] = Block{need, b.seg};
+        in_use_ += need;
+        return p;
+    }
+    const size_t seg = std::max(seg_bytes_, need);
+    if (reserved_ + seg > limit_) return nullptr;
+    void* base = nullptr;
+    if (hipMalloc(&base, seg) != hipSuccess) { (void)hipGetLastError(); return nullptr; }
+    const int id = (int)segs_.size();
+    segs_.push_back({(char*)base, seg});
+    reserved_ += seg;
+    if (seg > need) free_[(char*)base + need] = Block{seg - need, id};
+    live_[(char*)base] = Block{need, id};
+    in_use_ += need;
+    return base;
+}
+
+void DeviceArena::release(void* ptr) {
+    auto it = live_.find((char*)ptr);
+    if (it == live_.end()) return;
+    char* p = it->first;
+    Block b = it->second;
+    live_.erase(it);
+    in_use_ -= b.size;
+    // coalesce with the free neighbours of the same segment
+    auto nx = free_.lower_bound(p);
+    if (nx != free_.end() && nx->first == p + b.size && nx->second.seg == b.seg) {
+        b.size += nx->second.size;
+        nx = free_.erase(nx);
+    }
+    if (nx != free_.begin()) {
+        auto pv = std::prev(nx);
+        if (pv->first + pv->second.size == p && pv->second.seg == b.seg) {
+            pv->second.size += b.size;
+            return;
+        }
+    }
+    free_[p] = b;
+}
 
 namespace {
 struct Request {
@@ -30,7 +85,7 @@ struct Request {
     fpde_solver* solver = nullptr;
     size_t bytes = 0;
     double submit_s = 0, start_s = 0, finish_s = 0;
-    int status = -1;       // -1 queued/active, 0 done, 1 rejected, 2 cancelled
+    int status = -1;       // -1 queued/active, 0 done, 1 rejected, 2 cancelled, 3 failed
     bool active = false;
 };
 }  // namespace
@@ -39,10 +94,16 @@ struct fpde_server {
     size_t budget = 0;
     size_t used = 0;
     int max_active = 8;
+    // device memory of every admitted solve (segments reused across requests)
+    // and one stream per active slot, created once
+    std::unique_ptr<DeviceArena> arena;
+    std::vector<hipStream_t> streams;
+    std::vector<hipStream_t> free_streams;
     int64_t next_id = 1;
     int64_t finished = 0;
     std::deque<Request*> queue;
     std::vector<Request*> active;
+    std::map<Request*, hipStream_t> slot_stream;
     std::map<int64_t, std::unique_ptr<Request>> all;
     std::chrono::steady_clock::time_point t0 = std::chrono::steady_clock::now();
     double now() const {
@@ -60,11 +121,16 @@ void retire(fpde_server* srv, Request* r, int status) {
     if (r->solver) {
         if (status == 0) {
             r->result.resize((size_t)n_unknowns(r->spec));
-            if (fpde_solver_get_solution(r->solver, r->result.data()) != FPDE_OK) status = 2;
+            if (fpde_solver_get_solution(r->solver, r->result.data()) != FPDE_OK) status = 3;
         }
-        fpde_solver_destroy(r->solver);
+        fpde_solver_destroy(r->solver);      // synchronises the slot's stream
         r->solver = nullptr;
         srv->used -= r->bytes;
+    }
+    auto st = srv->slot_stream.find(r);
+    if (st != srv->slot_stream.end()) {
+        srv->free_streams.push_back(st->second);
+        srv->slot_stream.erase(st);
     }
     r->status = status;
     r->finish_s = srv->now();
@@ -80,13 +146,17 @@ fpde_status admit(fpde_server* srv) {
         Request* r = srv->queue.front();
         if (r->status != -1) { srv->queue.pop_front(); continue; }
         if (srv->used + r->bytes > srv->budget) break;
-        fpde_status rc = fpde_solver_create(&r->spec, &r->solver);
+        if (srv->free_streams.empty()) break;
+        hipStream_t st = srv->free_streams.back();
+        fpde_status rc = solver_create_in(&r->spec, srv->arena.get(), st, &r->solver);
         if (rc == FPDE_ERR_NOMEM) { r->solver = nullptr; break; }   // retry when memory frees up
         srv->queue.pop_front();
         if (rc != FPDE_OK) { r->status = 1; r->finish_s = srv->now(); srv->finished++; continue; }
+        srv->free_streams.pop_back();                                // the slot's stream
+        srv->slot_stream[r] = st;
         srv->used += r->bytes;
         rc = fpde_solver_set_initial(r->solver, r->u0.data());
-        if (rc != FPDE_OK) { retire(srv, r, 2); continue; }
+        if (rc != FPDE_OK) { retire(srv, r, 3); continue; }
         r->start_s = srv->now();
         r->active = true;
         srv->active.push_back(r);
@@ -99,11 +169,26 @@ fpde_status admit(fpde_server* srv) {
 extern "C" {
 
 fpde_status fpde_server_create(size_t mem_budget_bytes, int max_active, fpde_server** out) {
-    if (!out || max_active < 1) { set_error("bad arguments"); return FPDE_ERR_ARG; }
-    auto* s = new fpde_server;
+    if (!out || max_active < 1 || max_active > 1024) { set_error("bad arguments"); return FPDE_ERR_ARG; }
+    auto s = std::make_unique<fpde_server>();
     s->budget = mem_budget_bytes;
     s->max_active = max_active;
-    *out = s;
+    // segments of at least 8 GiB (or an eighth of the budget), up to the
+    // budget plus one segment of slack for what first-fit leaves between blocks
+    const size_t seg = std::max<size_t>(std::min<size_t>(mem_budget_bytes / 8 + 1, (size_t)8 << 30), (size_t)64 << 20);
+    s->arena = std::make_unique<DeviceArena>(seg, mem_budget_bytes + seg);
+    for (int i = 0; i < max_active; i++) {
+        hipStream_t st = nullptr;
+        hipError_t e = hipStreamCreateWithFlags(&st, hipStreamNonBlocking);
+        if (e != hipSuccess) {
+            for (hipStream_t x : s->streams) hipStreamDestroy(x);
+            set_error(std::string("hipStreamCreate: ") + hipGetErrorString(e));
+            return FPDE_ERR_HIP;
+        }
+        s->streams.push_back(st);
+    }
+    s->free_streams = s->streams;
+    *out = s.release();
     return FPDE_OK;
 }
 
@@ -111,7 +196,8 @@ fpde_status fpde_server_destroy(fpde_server* srv) {
     if (!srv) return FPDE_OK;
     for (auto& kv : srv->all)
         if (kv.second->solver) fpde_solver_destroy(kv.second->solver);
-    delete srv;
+    for (hipStream_t st : srv->streams) hipStreamDestroy(st);
+    delete srv;                          // the arena frees its segments
     return FPDE_OK;
 }
 
@@ -160,7 +246,7 @@ int fpde_server_poll(fpde_server* srv, int quantum) {
     for (Request* r : srv->active) {
         if (fpde_solver_step(r->solver) >= r->spec.steps) continue;
         if (solver_batchable(r->solver)) { coop.push_back(r); continue; }
-        if (fpde_solver_advance(r->solver, quantum) != FPDE_OK) r->status = 2;
+        if (fpde_solver_advance(r->solver, quantum) != FPDE_OK) r->status = 3;
     }
     // the cooperative 1D solves, in groups whose chunk workgroups fit on the
     // GPU together: one batched launch per history block for each group (the
@@ -177,15 +263,22 @@ int fpde_server_poll(fpde_server* srv, int quantum) {
             grp.push_back(coop[g1]->solver);
             g1++;
         }
-        if (fpde_solver_advance_batch(grp.data(), (int)grp.size(), quantum) != FPDE_OK)
-            for (size_t i = g0; i < g1; i++) coop[i]->status = 2;
+        if (fpde_solver_advance_batch(grp.data(), (int)grp.size(), quantum) != FPDE_OK) {
+            // a solve whose barrier timed out or whose launch failed reports
+            // it on sync; the others of the group go on next round
+            bool any = false;
+            for (size_t i = g0; i < g1; i++)
+                if (fpde_solver_sync(coop[i]->solver) != FPDE_OK) { coop[i]->status = 3; any = true; }
+            if (!any)
+                for (size_t i = g0; i < g1; i++) coop[i]->status = 3;
+        }
         g0 = g1;
     }
     // retire drained solves
     std::vector<Request*> keep;
     for (Request* r : srv->active) {
         bool done = fpde_solver_step(r->solver) >= r->spec.steps && fpde_solver_query(r->solver) != 0;
-        if (r->status == 2) { retire(srv, r, 2); continue; }
+        if (r->status == 3) { retire(srv, r, 3); continue; }
         if (done) { retire(srv, r, 0); continue; }
         keep.push_back(r);
     }

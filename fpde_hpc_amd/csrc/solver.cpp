@@ -24,6 +24,12 @@ namespace {
 
 thread_local std::string g_err;
 
+// Per-device argument tables of the batched step launches: allocated on a
+// device's first batch, freed by fpde_shutdown (a hipFree per batch would
+// wait for every stream of the device).  One batch at a time uses a table.
+std::mutex g_rows_mu;
+std::map<int, Step1DArgs*> g_rows_tab;
+
 int64_t round_up(int64_t v, int64_t m) { return (v + m - 1) / m * m; }
 
 bool is_split(int mode) { return mode == HIST_SPLIT_BF16 || mode == HIST_SPLIT_F16 || mode == HIST_SPLIT_BF16X3; }
@@ -66,6 +72,16 @@ fpde_status validate(const fpde_spec& s) {
             set_error("var_tau solvers are 1D, hist_mode 0, sum_alg 0, without a prescribed grid");
             return FPDE_ERR_ARG;
         }
+    }
+    if (s.fk < 0 || s.fk > 3) { set_error("fk must be 0 (g(t)=t), 1 (sqrt t), 2 (t^2) or 3 (t^fp)"); return FPDE_ERR_ARG; }
+    if (s.fk == 3 && !(s.fp >= 0.05 && s.fp <= 20.0)) { set_error("fk 3: fp must be in [0.05, 20]"); return FPDE_ERR_ARG; }
+    if (s.fk != 0) {
+        // generalized kernel: per-step weight rows (the grid path), exact integrals
+        if (s.weight_mode != 0) {
+            set_error("a generalized kernel (fk != 0) takes the exact weight integrals: weight_mode must be 0");
+            return FPDE_ERR_ARG;
+        }
+        if (s.sum_alg != 0) { set_error("a generalized kernel (fk != 0) needs sum_alg 0"); return FPDE_ERR_ARG; }
     }
     if (s.taus) {
         if (s.dims != 1) { set_error("a prescribed time grid (taus) is for 1D solves"); return FPDE_ERR_ARG; }
@@ -122,7 +138,7 @@ size_t footprint_bytes(const fpde_spec& s) {
         size_t ldp = (size_t)round_up(s.nodes, kSpTile);
         if (s.dims == 1) b += sizeof(double) * ldp * (1 + space_chunks((int)(ldp / kSpTile) - 1));
     }
-    if (s.taus || s.var_tau) {                                       // grid, weight rows
+    if (s.taus || s.var_tau || s.fk) {                               // grid, weight rows
         size_t M = (size_t)s.steps, B = (size_t)s.block;
         b += sizeof(double) * (3 * M + 64 + B * (round_up(M + 64, 64) + B + 64));
     }
@@ -186,6 +202,8 @@ struct fpde_solver {
     double* Bt = nullptr;
     double h = 0.0, utop1d = 0.0;
     int C = 0;
+    // 3D: host copy of the initial field (the halo slots of the results3d dump)
+    std::vector<double> u0_h;
     // progress
     int step = 0;
     int64_t launches = 0;
@@ -201,17 +219,26 @@ struct fpde_solver {
     size_t ev_used = 0;
     fpde_stats stats{};
     std::vector<void*> allocs;
+    // server-owned memory and stream (solver_create_in); else its own
+    DeviceArena* arena = nullptr;
+    bool own_stream = true;
+    int64_t wait_ticks = 0;      // grid-barrier bound of the cooperative launches
 };
 
 namespace {
 
 fpde_status dalloc(fpde_solver* s, void** p, size_t bytes) {
     if (bytes == 0) bytes = 8;
-    hipError_t e = hipMalloc(p, bytes);
-    if (e != hipSuccess) {
-        set_error(std::string("hipMalloc failed: ") + hipGetErrorString(e));
-        *p = nullptr;
-        return FPDE_ERR_NOMEM;
+    if (s->arena) {
+        *p = s->arena->alloc(bytes);
+        if (!*p) { set_error("device arena: no free block of that size within the budget"); return FPDE_ERR_NOMEM; }
+    } else {
+        hipError_t e = hipMalloc(p, bytes);
+        if (e != hipSuccess) {
+            set_error(std::string("hipMalloc failed: ") + hipGetErrorString(e));
+            *p = nullptr;
+            return FPDE_ERR_NOMEM;
+        }
     }
     s->allocs.push_back(*p);
     s->stats.device_bytes += bytes;
@@ -254,7 +281,7 @@ void ev_collect(fpde_solver* s) {
 fpde_status solver_alloc(fpde_solver* s) {
     const fpde_spec& sp = s->spec;
     const Plan& p = s->plan;
-    FPDE_HIP_TRY(hipStreamCreateWithFlags(&s->stream, hipStreamNonBlocking));
+    if (!s->stream) FPDE_HIP_TRY(hipStreamCreateWithFlags(&s->stream, hipStreamNonBlocking));
     int M = sp.steps;
     s->w.M = M;
     if (sp.var_tau) {
@@ -270,10 +297,14 @@ fpde_status solver_alloc(fpde_solver* s) {
         FPDE_HIP_TRY(hipMemsetAsync(s->w.Lv, 0, sizeof(double) * M, s->stream));
         DALLOC(s->Wrow, sizeof(double) * (size_t)(M + 64));
     }
-    if (sp.taus) {
+    if (sp.taus || (sp.fk != 0 && !sp.var_tau)) {
+        // a grid known at create: prescribed step lengths, or uniform ones
+        // with a generalized kernel (its weights are not Toeplitz either)
         s->w.grid = true;
-        std::vector<double> T((size_t)M + 2, 0.0), tt((size_t)M + 1);
-        for (int k = 0; k <= M; k++) tt[k] = sp.taus[k < M ? k : M - 1];
+        std::vector<double>& T = s->w.hTime;
+        std::vector<double> tt((size_t)M + 1);
+        T.assign((size_t)M + 2, 0.0);
+        for (int k = 0; k <= M; k++) tt[k] = sp.taus ? sp.taus[k < M ? k : M - 1] : sp.tau;
         for (int k = 1; k <= M + 1; k++) T[k] = T[k - 1] + tt[k - 1];       // init_time_step (:3944)
         DALLOC(s->w.Time, sizeof(double) * (M + 2));
         DALLOC(s->w.taus, sizeof(double) * (M + 1));
@@ -287,7 +318,7 @@ fpde_status solver_alloc(fpde_solver* s) {
             DALLOC(s->Wfar, sizeof(double) * (size_t)sp.block * s->ldwf);
             DALLOC(s->Wnear, sizeof(double) * (size_t)sp.block * s->ldwn);
         }
-        FPDE_HIP_TRY(hipStreamSynchronize(s->stream));    // the host vectors above go out of scope
+        FPDE_HIP_TRY(hipStreamSynchronize(s->stream));    // the host vector tt goes out of scope
     }
     DALLOC(s->w.omega, sizeof(double) * (size_t)(M + 512));
     DALLOC(s->w.corr, sizeof(double) * 2 * (size_t)(M + 1));
@@ -323,6 +354,7 @@ fpde_status solver_alloc(fpde_solver* s) {
         FPDE_HIP_TRY(hipMemsetAsync(s->flagbar, 0, kFlagBarBytes, s->stream));
         DALLOC(s->err, 64);
         FPDE_HIP_TRY(hipMemsetAsync(s->err, 0, 64, s->stream));
+        s->wait_ticks = barrier_wait_ticks();
         // cooperative block launches pay off where a launch covers many steps:
         // the GEMM history forms (one launch per block) and the series form;
         // the plain form needs its history kernel between every two steps
@@ -542,6 +574,7 @@ fpde_status launch_1d(fpde_solver* s, int n, int k, int nsplit, int64_t stride, 
 }
 
 fpde_status finalize_1d(fpde_solver* s);
+void count_steps(fpde_solver* s, int n, int n_end, int near_lo);
 
 // space-fractional sum of step n: U^n complete in memory, then F and the
 // partial sums of E (read by the step launches)
@@ -577,6 +610,7 @@ fpde_status enqueue_1d_step(fpde_solver* s, int n) {
         if (rc != FPDE_OK) return rc;
         s->stats.picard_iters_total++;
     }
+    count_steps(s, n, n + 1, near_lo);
     return FPDE_OK;
 }
 
@@ -614,6 +648,7 @@ Step1DArgs block_args(fpde_solver* s, int n, int n_end, int n0, int nsplit, int6
     a.flagbar = s->flagbar;
     a.epar = (int)(s->coop_launches & 1);
     a.err = s->err;
+    a.wait_ticks = s->wait_ticks;
     a.ser_P = s->w.ser_P;
     if (a.ser_P > 0) {
         a.H = s->H;
@@ -621,6 +656,19 @@ Step1DArgs block_args(fpde_solver* s, int n, int n_end, int n0, int nsplit, int6
         a.ser_v = s->w.ser_v;
     }
     return a;
+}
+
+// Algorithmic bytes of the 1D step kernels for the steps [n, n_end): 100 B
+// per node and step (coefficients, rows, line solve) and 8 B per near-field
+// history term, t in [near_lo, m) for step m (the series form keeps P
+// moments instead: 16 B each, read and written).
+void count_steps(fpde_solver* s, int n, int n_end, int near_lo) {
+    const double N = (double)s->spec.nodes;
+    double terms = 0.0;
+    for (int m = n; m < n_end; m++) terms += (double)std::max(0, m - near_lo);
+    const double per_step = s->w.ser_P > 0 ? 100.0 + 16.0 * s->w.ser_P : 100.0;
+    s->stats.step_alg_bytes += N * (per_step * (n_end - n) + (s->w.ser_P > 0 ? 0.0 : 8.0 * terms));
+    s->stats.steps_done += n_end - n;
 }
 
 // Cooperative block launch for the steps [n, n_end) (no pending solve
@@ -637,6 +685,7 @@ fpde_status launch_1d_block(fpde_solver* s, int n, int n_end, int n0, int nsplit
     s->coop_launches++;
     s->stats.step_launches++;
     s->stats.picard_iters_total += (int)((int64_t)(n_end - n) * sp.picard_max);
+    count_steps(s, n, n_end, near_lo);
     return FPDE_OK;
 }
 
@@ -696,15 +745,21 @@ fpde_status enqueue_3d_step(fpde_solver* s, int n) {
     double h2inv = 1.0 / (s->h * s->h);
     size_t slot = 0;
     const bool series = s->w.ser_P > 0;
+    // split3d odd: every sweep carries a third of the history term (Om1
+    // fpde_hpc_1d.cpp:4172-4173, :4258-4263); even: the y sweep all of it
+    const bool split = (sp.split3d & 1) != 0;
+    const double hscale = split ? 1.0 / 3.0 : 1.0;
     ev_begin(s, 0, &slot);
     if (series) {
         size_t row = (size_t)n * s->w.ser_P;
         const double* dnew = (n >= 1) ? s->D + (size_t)((n - 1) & 1) * p.ld : nullptr;
         rc = launch_series3d(s->S, s->H, s->w.ser_P, dnew, s->w.ser_c + row, s->w.ser_v + row, p.nodes, p.ld,
-                             s->stream);
+                             hscale, s->stream);
     } else {
-        rc = launch_hist_combine(s->S, s->part, nsplit, stride, off, s->D, p.ld, p.nodes, n, near_lo, s->w.omega,
-                                 s->w.corr, s->stream);
+        // on a grid the near levels take this step's weight row (far_part set it)
+        const double* om = (s->w.grid && s->omega_cur) ? s->omega_cur : s->w.omega;
+        rc = launch_hist_combine(s->S, s->part, nsplit, stride, off, s->D, p.ld, p.nodes, n, near_lo, om,
+                                 s->w.corr, hscale, s->stream);
     }
     ev_end(s, slot);
     if (rc != FPDE_OK) return rc;
@@ -718,20 +773,23 @@ fpde_status enqueue_3d_step(fpde_solver* s, int n) {
         return space ? launch_space3d(s->U, s->K, nn, axis, s->h, s->w.sp_omega, s->w.sp_corr, s->spE, s->stream)
                      : FPDE_OK;
     };
+    // local coefficient of this step: per step on a grid (generalized kernel)
+    const double Lc = (s->w.grid && n < sp.steps) ? s->w.hLv[n] : s->w.local;
     if ((rc = launch_coef3d(s->U, nn, s->h, sp.ksat, sp.coef_mode, s->K, s->iC, s->stream)) != FPDE_OK) return rc;
     if ((rc = space_sums(2)) != FPDE_OK) return rc;
-    if ((rc = launch_lateral3d(s->U, s->K, s->iC, s->S, nn, 2, h2inv, s->w.local, s->Al, s->Bt, kap, E, s->stream)) != FPDE_OK) return rc;
+    if ((rc = launch_lateral3d(s->U, s->K, s->iC, s->S, nn, 2, h2inv, Lc, s->Al, s->Bt, kap, E, s->stream)) != FPDE_OK) return rc;
     if ((rc = launch_coef3d(s->U, nn, s->h, sp.ksat, sp.coef_mode, s->K, s->iC, s->stream)) != FPDE_OK) return rc;
     if ((rc = space_sums(1)) != FPDE_OK) return rc;
-    if ((rc = launch_lateral3d(s->U, s->K, s->iC, nullptr, nn, 1, h2inv, s->w.local, s->Al, s->Bt, kap, E, s->stream)) != FPDE_OK) return rc;
+    if ((rc = launch_lateral3d(s->U, s->K, s->iC, split ? s->S : nullptr, nn, 1, h2inv, Lc, s->Al, s->Bt, kap, E,
+                               s->stream)) != FPDE_OK) return rc;
     if (space) {
         if ((rc = launch_coef3d(s->U, nn, s->h, sp.ksat, sp.coef_mode, s->K, s->iC, s->stream)) != FPDE_OK) return rc;
         if ((rc = space_sums(0)) != FPDE_OK) return rc;
     }
     // the depth sweep reuses the coefficient pass of the space-fractional sums
-    if ((rc = launch_zline3d(s->U, s->Uprev, s->utop, nn, s->h, h2inv, sp.ksat, sp.coef_mode, s->w.local,
+    if ((rc = launch_zline3d(s->U, s->Uprev, s->utop, nn, s->h, h2inv, sp.ksat, sp.coef_mode, Lc,
                              s->D + (size_t)(series ? (n & 1) : n) * p.ld, kap, E, space ? s->K : nullptr,
-                             space ? s->iC : nullptr, s->stream)) != FPDE_OK) return rc;
+                             space ? s->iC : nullptr, split ? s->S : nullptr, s->stream)) != FPDE_OK) return rc;
     ev_end(s, slot);
     s->stats.step_launches += space ? 9 : 5;
     s->stats.picard_iters_total++;
@@ -752,6 +810,30 @@ int solver_chunks(const fpde_solver* s) { return s ? s->C : 0; }
 extern "C" {
 
 const char* fpde_last_error(void) { return g_err.c_str(); }
+
+fpde_status fpde_shutdown(void) {
+    std::lock_guard<std::mutex> lock(g_rows_mu);
+    fpde_status rc = FPDE_OK;
+    int cur = 0;
+    const bool have_dev = hipGetDevice(&cur) == hipSuccess;
+    for (auto& kv : g_rows_tab) {
+        if (!kv.second) continue;
+        if (hipSetDevice(kv.first) == hipSuccess) {
+            if (hipDeviceSynchronize() != hipSuccess) rc = FPDE_ERR_HIP;
+            if (hipFree(kv.second) != hipSuccess) rc = FPDE_ERR_HIP;
+        } else {
+            rc = FPDE_ERR_HIP;
+        }
+        kv.second = nullptr;
+    }
+    g_rows_tab.clear();
+    if (have_dev) {
+        (void)hipSetDevice(cur);
+        if (hipDeviceSynchronize() != hipSuccess) rc = FPDE_ERR_HIP;
+    }
+    if (rc != FPDE_OK) set_error("fpde_shutdown: a device call failed while releasing the library's tables");
+    return rc;
+}
 int fpde_version(void) { return 100; }
 int fpde_device_count(void) {
     int n = 0;
@@ -765,11 +847,22 @@ size_t fpde_solver_footprint(const fpde_spec* spec) {
 }
 
 fpde_status fpde_solver_create(const fpde_spec* spec, fpde_solver** out) {
+    return solver_create_in(spec, nullptr, nullptr, out);
+}
+
+}  // extern "C"
+
+fpde_status fpde::solver_create_in(const fpde_spec* spec, DeviceArena* arena, hipStream_t stream, fpde_solver** out) {
     if (!spec || !out) { set_error("null argument"); return FPDE_ERR_ARG; }
     *out = nullptr;
     fpde_status rc = validate(*spec);
     if (rc != FPDE_OK) return rc;
     auto s = std::make_unique<fpde_solver>();
+    s->arena = arena;
+    if (stream) {
+        s->stream = stream;
+        s->own_stream = false;
+    }
     s->spec = *spec;
     if (spec->taus) {
         s->taus_h.assign(spec->taus, spec->taus + spec->steps);
@@ -785,13 +878,18 @@ fpde_status fpde_solver_create(const fpde_spec* spec, fpde_solver** out) {
     return FPDE_OK;
 }
 
+extern "C" {
+
 fpde_status fpde_solver_destroy(fpde_solver* s) {
     if (!s) return FPDE_OK;
     if (s->stream) hipStreamSynchronize(s->stream);
     if (s->gexec) hipGraphExecDestroy(s->gexec);
-    for (void* p : s->allocs) hipFree(p);
+    for (void* p : s->allocs) {
+        if (s->arena) s->arena->release(p);
+        else hipFree(p);
+    }
     for (auto& e : s->evs) { hipEventDestroy(e.a); hipEventDestroy(e.b); }
-    if (s->stream) hipStreamDestroy(s->stream);
+    if (s->stream && s->own_stream) hipStreamDestroy(s->stream);
     delete s;
     return FPDE_OK;
 }
@@ -812,6 +910,7 @@ fpde_status fpde_solver_set_initial(fpde_solver* s, const double* u0) {
             for (int x = 0; x < n; x++) top[(size_t)y * n + x] = u0[(size_t)n * (x + (size_t)n * y)];
         FPDE_HIP_TRY(hipMemcpyAsync(s->Uprev, s->U, sizeof(double) * p.ld, hipMemcpyDeviceToDevice, s->stream));
         FPDE_HIP_TRY(hipMemcpyAsync(s->utop, top.data(), sizeof(double) * top.size(), hipMemcpyHostToDevice, s->stream));
+        s->u0_h.assign(u0, u0 + p.nodes);
     }
     FPDE_HIP_TRY(hipStreamSynchronize(s->stream));
     s->initialized = true;
@@ -905,15 +1004,11 @@ fpde_status fpde_solver_advance_batch(fpde_solver* const* solvers, int count, in
     }
     const int B = (int)blk.size();
     // the argument rows of the batch (pointers and constants of each solve),
-    // in a per-device table that lives until the process ends (a hipFree per
-    // call would wait for every stream of the device); one batch at a time
-    // uses it
-    static std::mutex rows_mu;
-    static std::map<int, Step1DArgs*> rows_tab;
-    std::lock_guard<std::mutex> lock(rows_mu);
+    // in the device's table (g_rows_tab)
+    std::lock_guard<std::mutex> lock(g_rows_mu);
     int dev = 0;
     FPDE_HIP_TRY(hipGetDevice(&dev));
-    Step1DArgs*& drows = rows_tab[dev];
+    Step1DArgs*& drows = g_rows_tab[dev];
     if (!drows) FPDE_HIP_TRY(hipMalloc((void**)&drows, sizeof(Step1DArgs) * kMaxBatch));
     std::vector<Step1DArgs> rows(B);
     for (int j = 0; j < B; j++) rows[j] = block_args(blk[j], blk[j]->step, blk[j]->step, 0, 0, 0, 0);
@@ -969,6 +1064,7 @@ fpde_status fpde_solver_advance_batch(fpde_solver* const* solvers, int count, in
             s->coop_launches++;
             s->stats.step_launches++;
             s->stats.picard_iters_total += (int)((int64_t)(t.n_end[j] - t.n[j]) * s->spec.picard_max);
+            count_steps(s, t.n[j], t.n_end[j], t.near_lo[j]);
             s->step = t.n_end[j];
         }
     }
@@ -1087,7 +1183,7 @@ fpde_status fpde_solver_advance_tau(fpde_solver* s, double tau) {
     if (n >= s->spec.steps) { set_error("var_tau solver: step capacity reached"); return FPDE_ERR_STATE; }
     s->taus_h[n] = tau;
     s->Time_h[n + 1] = s->Time_h[n] + tau;                         // init_time_step (:3944)
-    s->w.hLv[n] = grid_local(s->spec, tau);
+    s->w.hLv[n] = grid_local(s->spec, s->Time_h[n], s->Time_h[n + 1]);
     fpde_status rc = launch_set_grid_point(s->w, n, s->Time_h[n + 1], tau, s->w.hLv[n], s->stream);
     if (rc != FPDE_OK) return rc;
     rc = enqueue_1d_step(s, n);
@@ -1239,23 +1335,40 @@ fpde_status fpde_solver_write_results(fpde_solver* s, const char* path_txt, cons
         return s0 + (s1 - s0) / std::pow(1.0 + std::pow(a * P, n), 1.0 - 1.0 / n);
     };
     const int n = sp.nodes;
+    // the reference's centre line idx(i, _M/2, K/2) with _M = K = n - 1 intervals
+    const int c = (n - 1) / 2;
     for (int i = 0; i < n; i++) {
         double z = i * h;
         if (sp.dims == 1) {
             std::fprintf(f, "%g %g %g %g %g %g \n", t, z, u[i], 0.0, 0.0, theta(z - u[i]));
         } else {
-            size_t id = (size_t)i + (size_t)n * ((size_t)(n / 2) + (size_t)n * (size_t)(n / 2));
-            std::fprintf(f, "%g %g %g %g %g %g %g %g \n", t, z, (n / 2) * h, (n / 2) * h, u[id], 0.0, 0.0,
-                         theta(z - u[id]));
+            size_t id = (size_t)i + (size_t)n * ((size_t)c + (size_t)n * (size_t)c);
+            std::fprintf(f, "%g %g %g %g %g %g %g %g \n", t, z, c * h, c * h, u[id], 0.0, 0.0, theta(z - u[id]));
         }
     }
     std::fclose(f);
     if (sp.dims == 3 && path_3d) {
+        // b_U as the reference stores it: (N+2)^3 values for N = n - 1
+        // intervals per axis, idx(i,j,k) = i + (N+2) j + (N+2)^2 k (depth i,
+        // x j, y k; :1181), nodes 0..N plus one halo slot per axis, which no
+        // sweep writes and keeps its initial value (here: the initial field
+        // at the nearest node)
+        const size_t m = (size_t)n + 1;
+        std::vector<double> b(m * m * m);
+        for (size_t k = 0; k < m; k++)
+            for (size_t j = 0; j < m; j++)
+                for (size_t i = 0; i < m; i++) {
+                    const bool in = i < (size_t)n && j < (size_t)n && k < (size_t)n;
+                    const size_t ii = std::min(i, (size_t)n - 1), jj = std::min(j, (size_t)n - 1),
+                                 kk = std::min(k, (size_t)n - 1);
+                    const size_t id = ii + (size_t)n * (jj + (size_t)n * kk);
+                    b[i + m * (j + m * k)] = in ? u[id] : (s->u0_h.empty() ? 0.0 : s->u0_h[id]);
+                }
         FILE* g = std::fopen(path_3d, "ab");
         if (!g) { set_error(std::string("cannot open ") + path_3d); return FPDE_ERR_ARG; }
-        size_t w = std::fwrite(u.data(), sizeof(double), u.size(), g);
+        size_t w = std::fwrite(b.data(), sizeof(double), b.size(), g);
         std::fclose(g);
-        if (w != u.size()) { set_error("short write to the 3D results file"); return FPDE_ERR_ARG; }
+        if (w != b.size()) { set_error("short write to the 3D results file"); return FPDE_ERR_ARG; }
     }
     return FPDE_OK;
 }

@@ -518,7 +518,13 @@ __global__ __launch_bounds__(256) void k_step1d(Step1DArgs g) {
 // and the precomputed far partials, so steps follow each other without a
 // barrier; U and D of every step are written as in the per-launch form.
 
-constexpr unsigned kSpinLimit = 1u << 18;     // bounded wait of a grid barrier
+// A grid barrier waits at most `wait_ticks` ticks of the device's constant
+// wall clock (wall_clock64; the host sizes it to 2 s from the clock rate).
+// Workgroups of a cooperative launch can become resident late when other
+// streams' kernels hold CUs at dispatch (the 3D and far-part kernels that
+// advance beside a batch): those finish within micro- to milliseconds, far
+// inside the bound, which exists only so that a launch whose workgroups can
+// never all be resident ends with an error instead of hanging.
 
 // Grid barrier of the C chunk workgroups on per-workgroup flags: workgroup
 // c stores the barrier's epoch e to its own flag (128 B apart) after an
@@ -538,14 +544,15 @@ __device__ inline void flag_arrive(unsigned* fl, int c, unsigned e) {
     }
 }
 
-__device__ inline bool flag_wait(unsigned* fl, int C, unsigned e, int* err, int* sh_ok) {
+__device__ inline bool flag_wait(unsigned* fl, int C, unsigned e, int* err, int* sh_ok, int64_t wait_ticks) {
     if (threadIdx.x == 0) *sh_ok = 1;
     __syncthreads();
     for (int j = threadIdx.x; j < C; j += blockDim.x) {
-        unsigned spins = 0;
+        if (__hip_atomic_load(fl + 32 * j, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) >= e) continue;
+        const int64_t t0 = (int64_t)wall_clock64();
         while (__hip_atomic_load(fl + 32 * j, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) < e) {
             __builtin_amdgcn_s_sleep(1);
-            if (++spins > kSpinLimit) {
+            if ((int64_t)wall_clock64() - t0 > wait_ticks) {
                 *sh_ok = 0;
                 __hip_atomic_store(err, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
                 break;
@@ -557,9 +564,9 @@ __device__ inline bool flag_wait(unsigned* fl, int C, unsigned e, int* err, int*
     return *sh_ok != 0;
 }
 
-__device__ inline bool flag_sync(unsigned* fl, int c, int C, unsigned e, int* err, int* sh_ok) {
+__device__ inline bool flag_sync(unsigned* fl, int c, int C, unsigned e, int* err, int* sh_ok, int64_t wait_ticks) {
     flag_arrive(fl, c, e);
-    return flag_wait(fl, C, e, err, sh_ok);
+    return flag_wait(fl, C, e, err, sh_ok, wait_ticks);
 }
 
 // History term of step n at node i (levels t < n-1 in memory, the newest
@@ -689,7 +696,7 @@ __device__ __forceinline__ void step_block(const Step1DArgs& g, const int c) {
             tl_next[2] = g.corr[2 * pre_n + 1];
             hm_next = hist_mem(g, i, pre_n, (int64_t)(pre_n - g.n0) * g.ld, om);
         }
-        if (!flag_wait(fl, C, (unsigned)(bk + 1), g.err, &sh_ok)) return false;
+        if (!flag_wait(fl, C, (unsigned)(bk + 1), g.err, &sh_ok, g.wait_ticks)) return false;
         bk++;
         return true;
     };
@@ -808,7 +815,7 @@ __device__ __forceinline__ void step_block(const Step1DArgs& g, const int c) {
                 // tolerance: decide that first (one barrier on the measure)
                 __syncthreads();
                 if (tid == 0) g.dmax[(size_t)(bk & 1) * C + c] = last_measure();
-                if (!flag_sync(fl, c, C, (unsigned)(bk + 1), g.err, &sh_ok)) return;
+                if (!flag_sync(fl, c, C, (unsigned)(bk + 1), g.err, &sh_ok, g.wait_ticks)) return;
                 bk++;
                 double dv = 0.0;
                 for (int j = lane; j < C; j += 64) dv = fmax(dv, g.dmax[(size_t)((bk - 1) & 1) * C + j]);
@@ -866,6 +873,16 @@ __global__ __launch_bounds__(256) void k_step1d_batch(const Step1DArgs* __restri
     g.epar = t.epar[j];
     g.part_stride = t.part_stride[j];
     step_block(g, wg - t.first[j]);
+}
+
+// Barrier bound in wall-clock ticks: 2 s at the device's wall clock rate
+// (hipDeviceAttributeWallClockRate, kHz).
+int64_t barrier_wait_ticks() {
+    int dev = 0, khz = 0;
+    if (hipGetDevice(&dev) != hipSuccess ||
+        hipDeviceGetAttribute(&khz, hipDeviceAttributeWallClockRate, dev) != hipSuccess || khz <= 0)
+        khz = 100000;                                  // the 100 MHz constant clock of gfx9 parts
+    return (int64_t)khz * 2000;
 }
 
 static bool coop_fits(const void* kern, int64_t wgs) {

@@ -162,14 +162,105 @@ __global__ void k_series_tables(double a, double tau, int M, int P, double eps, 
     }
 }
 
+// ---------------------------------------------------------------------------
+// Generalized Caputo kernel (func_in_kernel 1/2/3, g() and inv_g_der() at
+// fpde_hpc_1d.cpp:1889-1930): g(t) = t^gp with gp = 1/2, 2 or func_power.
+// The L1 weight of an interval is
+//   b(t0,t1;a) = int_{T_t0}^{T_t1} (g(T_a) - g(s))^(-alpha) ds,
+// which the reference evaluates by truncated series (_kb_row, :1933-1996,
+// _kb_row2, :2000-2131, stop at |term| <= global_eps).  Here it is the
+// integral itself, to ~1e-14 relative.  The integrand has its branch points
+// at s = T_a and (gp not an integer) s = 0 only:
+//  - a history interval away from 0 (t0 >= 1): Gauss-Legendre in s, 16, 8 or
+//    4 nodes by the distance of the nearer branch point over the width (at
+//    least 1: convergence factor >= 5.8 per node pair);
+//  - the oldest interval (t0 = 0): the binomial series
+//    (G - s^gp)^-alpha = G^-alpha sum_i (alpha)_i/i! (s/T_a)^(gp i), whose
+//    terms integrate in closed form (ratio (T_1/T_a)^gp <= 2^-gp);
+//  - the newest interval (t1 = a, the local coefficient kkk0, host):
+//    w = (T_a - s)^(1-alpha) moves the singular end into the measure, the
+//    bounded rest is integrated by tanh-sinh; at n = 0 the closed form
+//    T_1^(1-alpha gp) Gamma(1+1/gp) Gamma(1-alpha) / Gamma(1+1/gp-alpha).
+// g(T_a) - g(T_a - d) is formed as -G expm1(gp log1p(-d/T_a)), exact for
+// small d.
+__constant__ double kGL16x[8] = {0.095012509837637454, 0.28160355077925892, 0.45801677765722737,
+                                 0.61787624440264377, 0.755404408355003, 0.86563120238783176,
+                                 0.9445750230732326, 0.98940093499164994};
+__constant__ double kGL16w[8] = {0.18945061045506859, 0.18260341504492361, 0.16915651939500262,
+                                 0.14959598881657676, 0.12462897125553403, 0.095158511682492591,
+                                 0.062253523938647706, 0.027152459411754037};
+__constant__ double kGL8x[4] = {0.18343464249564978, 0.52553240991632899, 0.79666647741362673,
+                                0.96028985649753618};
+__constant__ double kGL8w[4] = {0.36268378337836177, 0.31370664587788705, 0.22238103445337434,
+                                0.10122853629037669};
+__constant__ double kGL4x[2] = {0.33998104358485626, 0.86113631159405257};
+__constant__ double kGL4w[2] = {0.65214515486254621, 0.34785484513745368};
+
+__host__ __device__ inline double g_gap(double pa, double gp, double d, double G) {
+    return -G * expm1(gp * log1p(-d / pa));      // pa^gp - (pa - d)^gp
+}
+
+// b(t0,t1;a) of a history interval, 0 <= p0 < p1 < pa.
+__device__ inline double kb_gen(double a, double gp, double p0, double p1, double pa) {
+    const double G = pow(pa, gp);
+    if (p0 == 0.0) {
+        const double r = pow(p1 / pa, gp);
+        double c = 1.0, sum = 0.0;                // c = (a)_i / i! r^i
+        for (int i = 0; i < 20000; i++) {
+            const double term = c / (gp * i + 1.0);
+            sum += term;
+            if (term <= 1e-17 * sum) break;
+            c *= (a + i) / (i + 1.0) * r;
+        }
+        return pow(G, -a) * p1 * sum;
+    }
+    const double hw = 0.5 * (p1 - p0), gapr = pa - p1;
+    const double dist = fmin(gapr, p0) / (p1 - p0);
+    const double* xs = dist < 4.0 ? kGL16x : (dist < 32.0 ? kGL8x : kGL4x);
+    const double* ws = dist < 4.0 ? kGL16w : (dist < 32.0 ? kGL8w : kGL4w);
+    const int half = dist < 4.0 ? 8 : (dist < 32.0 ? 4 : 2);
+    double sum = 0.0;
+    for (int k = 0; k < half; k++) {
+        // nodes m + hw x and m - hw x; pa - s measured from the right end
+        const double dl = gapr + hw * (1.0 - xs[k]), dr = gapr + hw * (1.0 + xs[k]);
+        sum += ws[k] * (pow(g_gap(pa, gp, dl, G), -a) + pow(g_gap(pa, gp, dr, G), -a));
+    }
+    return hw * sum;
+}
+
+// The local coefficient's interval b(n,n+1;n+1) = int_{p0}^{pa} (G - s^gp)^-a ds (host).
+double kb_newest(double a, double gp, double p0, double pa) {
+    if (a == 1.0) return 1.0;                    // kb() at alpha == 1, t1 == a (:2169-2175)
+    if (gp == 1.0) return std::pow(pa - p0, 1.0 - a) / (1.0 - a);
+    if (p0 == 0.0)
+        return std::pow(pa, 1.0 - a * gp) *
+               std::exp(std::lgamma(1.0 + 1.0 / gp) + std::lgamma(1.0 - a) - std::lgamma(1.0 + 1.0 / gp - a));
+    const double G = std::pow(pa, gp), e1 = 1.0 - a, w0 = std::pow(pa - p0, e1);
+    const double h = 1.0 / 32.0;
+    double sum = 0.0;
+    for (int j = -112; j <= 112; j++) {          // t in [-3.5, 3.5]
+        const double t = j * h, u = 0.5 * M_PI * std::sinh(t);
+        const double w = w0 / (1.0 + std::exp(-2.0 * u));                    // distance from w = 0
+        const double eu = std::exp(u), ch = eu + 1.0 / eu;
+        const double wt = h * w0 * 0.5 * (0.5 * M_PI * std::cosh(t)) * 4.0 / (ch * ch);
+        const double d = std::pow(w, 1.0 / e1);                              // pa - s
+        const double phi = d > 0.0 ? std::pow(g_gap(pa, gp, d, G) / d, -a) : std::pow(gp * std::pow(pa, gp - 1.0), -a);
+        sum += wt * phi;
+    }
+    return sum / e1;
+}
+
 // Prescribed non-uniform grid: the weight of interval t at step n (unknown
 // U^{n+1}) is w_n(t) = sum_q cw_q Da_q kb_q(T_t, T_{t+1}; T_{n+1}) / tau_t, the
 // kb_3_cache of precalc_kb_3 over Time[] (fpde_hpc_1d.cpp:4069-4085) divided by
 // tau[t] in time_summing (:4131).  The closed form ((a-t0)^e - (a-t1)^e)/e,
 // e = 1-alpha, is evaluated as (a-t1)^e expm1(e log1p((t1-t0)/(a-t1)))/e; the
 // two oldest intervals take the _kb_row2 series in weight_mode 1 (:2265).
-__device__ inline double kb_grid(double a, double p0, double p1, double pa, int t, int series, double eps) {
+// With a generalized kernel (gp != 1) the weights are kb_gen's integrals
+// (uniform steps ride on the same per-step rows: they are not Toeplitz).
+__device__ inline double kb_grid(double a, double gp, double p0, double p1, double pa, int t, int series, double eps) {
     if (a == 1.0) return 0.0;                    // history intervals have t1 < a
+    if (gp != 1.0) return kb_gen(a, gp, p0, p1, pa);
     if (series && t <= 1) return l1_series(a, p0, p1, pa, eps);
     double e = 1.0 - a, x = pa - p1;
     return pow(x, e) * expm1(e * log1p((p1 - p0) / x)) / e;
@@ -177,7 +268,7 @@ __device__ inline double kb_grid(double a, double p0, double p1, double pa, int 
 
 __global__ void k_weights_grid(OrderSet os, const double* __restrict__ Time, const double* __restrict__ taus,
                                int M, int n_first, int rows, int len, int t_lo, int reversed, int series,
-                               double eps, double* __restrict__ W, int64_t ldw) {
+                               double eps, double gp, double* __restrict__ W, int64_t ldw) {
     int64_t idx = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (idx >= (int64_t)rows * len) return;
     int r = (int)(idx / len), j = (int)(idx % len);
@@ -187,7 +278,8 @@ __global__ void k_weights_grid(OrderSet os, const double* __restrict__ Time, con
     // rows of a block past the grid (n > M) stay zero: Time holds the M + 2
     // points Time[0..M+1]
     if (n <= M && t >= t_lo && t >= 0 && t < n) {
-        for (int q = 0; q < os.q; q++) w += os.c[q] * kb_grid(os.a[q], Time[t], Time[t + 1], Time[n + 1], t, series, eps);
+        for (int q = 0; q < os.q; q++)
+            w += os.c[q] * kb_grid(os.a[q], gp, Time[t], Time[t + 1], Time[n + 1], t, series, eps);
         w /= taus[t];
     }
     W[(int64_t)r * ldw + j] = w;
@@ -274,13 +366,25 @@ static fpde_status build_orders_grid(const fpde_spec& s, OrderSet& os) {
     return FPDE_OK;
 }
 
-double grid_local(const fpde_spec& s, double tn) {
+double kernel_exponent(const fpde_spec& s) {
+    switch (s.fk) {
+        case 1: return 0.5;
+        case 2: return 2.0;
+        case 3: return s.fp;
+        default: return 1.0;
+    }
+}
+
+// local coefficient of step n on a grid: sum_q c_q Da_q kb_q(n,n+1;n+1)/tau_n
+// (kkk0 of init_time_step, fpde_hpc_1d.cpp:3942-3949), Tn = T_n, Tn1 = T_{n+1}
+double grid_local(const fpde_spec& s, double Tn, double Tn1) {
     OrderSet og;
     if (build_orders_grid(s, og) != FPDE_OK) return 0.0;
+    const double gp = kernel_exponent(s), tn = Tn1 - Tn;
     double L = 0.0;
     for (int q = 0; q < og.q; q++) {
         double a = og.a[q];
-        double kkk0 = (a == 1.0) ? 1.0 : std::pow(tn, 1.0 - a) / (1.0 - a);
+        double kkk0 = (a == 1.0) ? 1.0 : (gp == 1.0 ? std::pow(tn, 1.0 - a) / (1.0 - a) : kb_newest(a, gp, Tn, Tn1));
         L += og.c[q] * kkk0 / tn;
     }
     return L;
@@ -295,7 +399,7 @@ fpde_status launch_weights_grid(const fpde_spec& s, const DeviceWeights& w, int 
     int64_t total = (int64_t)rows * len;
     hipLaunchKernelGGL(k_weights_grid, dim3((unsigned)((total + 255) / 256)), dim3(256), 0, st, os,
                        (const double*)w.Time, (const double*)w.taus, w.M, n_first, rows, len, t_lo, reversed,
-                       s.weight_mode == 1 ? 1 : 0, s.series_eps, W, ldw);
+                       s.weight_mode == 1 ? 1 : 0, s.series_eps, kernel_exponent(s), W, ldw);
     FPDE_HIP_TRY(hipGetLastError());
     return FPDE_OK;
 }
@@ -311,8 +415,8 @@ fpde_status launch_weights(const fpde_spec& s, DeviceWeights& w, hipStream_t st)
         // the weights themselves are generated per step or per block
         int M = w.M;
         w.hLv.assign(M, 0.0);
-        if (s.taus) {
-            for (int n = 0; n < M; n++) w.hLv[n] = grid_local(s, s.taus[n]);
+        if (!w.hTime.empty()) {                  // a grid known in advance (taus, or uniform with a generalized kernel)
+            for (int n = 0; n < M; n++) w.hLv[n] = grid_local(s, w.hTime[n], w.hTime[n + 1]);
             FPDE_HIP_TRY(hipMemcpyAsync(w.Lv, w.hLv.data(), sizeof(double) * M, hipMemcpyHostToDevice, st));
             FPDE_HIP_TRY(hipStreamSynchronize(st));
         }

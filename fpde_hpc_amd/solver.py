@@ -7,10 +7,11 @@ there is no CPU path here.
 """
 import ctypes
 import os
+import sys
 
 import numpy as np
 
-from ._lib import ResultInfo, Spec, Stats, check, lib
+from ._lib import ResultInfo, Spec, Stats, check, lib, register_owner
 
 HIST_MODES = {"plain": 0, "mfma_f64": 1, "split_bf16": 2, "split_f16": 3, "split_bf16x3": 4}
 
@@ -24,18 +25,20 @@ def _p(a):
 def make_spec(nodes, steps, tau, alpha=0.8, dims=1, length=1.0, ksat=None, coef_mode=0,
               picard_max=1, picard_tol=0.0, weight_mode=1, series_eps=1e-5, do_q=0, do_rule=0,
               do_a0=0.1, do_a1=1.0, hist_mode="plain", block=64, step_mode=0, fourier=1.0,
-              u_top=-1.0, sum_alg=0, sum_param=1e-5, taus=None, var_tau=0, gamma=0.0):
+              u_top=-1.0, sum_alg=0, sum_param=1e-5, taus=None, var_tau=0, gamma=0.0, split3d=0, fk=0, fp=0.6):
     """fpde_spec; ksat defaults to the value giving mesh Fourier number `fourier`
     (K*tau/(C*h^2)) at suction -u_top.  `taus` (one length per step) makes the
     time grid non-uniform; `gamma` in (0,1) selects the space-fractional
-    scheme (0 or 1: classical diffusion)."""
+    scheme (0 or 1: classical diffusion); `split3d` odd spreads the 3D history
+    term over the three sweeps; `fk`/`fp` select the generalized kernel
+    g(t) = t, sqrt(t), t^2 or t^fp (fk 0..3; needs weight_mode 0)."""
     if ksat is None:
         h = length / (nodes - 1)
         ksat = lib().fpde_ksat_for_fourier(u_top, h, tau, fourier) if coef_mode == 0 else 1.0
     hm = HIST_MODES[hist_mode] if isinstance(hist_mode, str) else int(hist_mode)
     spec = Spec(dims, nodes, steps, tau, alpha, length, ksat, coef_mode, picard_max, picard_tol,
                 weight_mode, series_eps, do_q, do_rule, do_a0, do_a1, hm, block, step_mode,
-                sum_alg, sum_param, None, var_tau, 0, gamma)
+                sum_alg, sum_param, None, var_tau, split3d, gamma, fk, fp)
     if taus is not None:
         spec._taus = np.ascontiguousarray(taus, dtype=np.float64)   # alive until the solver copies it
         if spec._taus.size != steps:
@@ -63,6 +66,7 @@ class Solver:
         self.spec = spec
         self._h = ctypes.c_void_p()
         check(lib().fpde_solver_create(ctypes.byref(spec), ctypes.byref(self._h)))
+        register_owner(self)
         self.unknowns = spec.nodes if spec.dims == 1 else spec.nodes ** 3
 
     def close(self):
@@ -71,6 +75,10 @@ class Solver:
             self._h = ctypes.c_void_p()
 
     def __del__(self):
+        # at interpreter shutdown the atexit hook has closed it already; a
+        # finaliser must not reach the HIP runtime then
+        if sys.is_finalizing():
+            return
         try:
             self.close()
         except Exception:
@@ -180,6 +188,7 @@ class Server:
     def __init__(self, mem_budget_bytes, max_active=8):
         self._h = ctypes.c_void_p()
         check(lib().fpde_server_create(int(mem_budget_bytes), int(max_active), ctypes.byref(self._h)))
+        register_owner(self)
         self._sizes = {}
 
     def close(self):
@@ -188,10 +197,18 @@ class Server:
             self._h = ctypes.c_void_p()
 
     def __del__(self):
+        if sys.is_finalizing():
+            return
         try:
             self.close()
         except Exception:
             pass
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *a):
+        self.close()
 
     def submit(self, spec, u0):
         u0 = np.ascontiguousarray(u0, dtype=np.float64)
@@ -280,3 +297,37 @@ def workload(name, hist_mode="plain", steps=None, nodes=None, **kw):
         extra.setdefault("do_a1", 1.0)
     spec = make_spec(N, M, tau, alpha=a, dims=dims, hist_mode=hist_mode, u_top=u_top, **extra)
     return spec, u0
+
+
+def serving_mix(n_req=512, rate=16.0, seed=1004, hist_mode="split_bf16"):
+    """Requests of the serving mix: Poisson arrivals at `rate` per second; 80%
+    1D (N ~ U[500, 20000], M ~ U[1000, 40000], tau 5e-4, block 64), 20% 3D
+    (32^3..96^3, M ~ U[200, 1000], tau 1e-3, block 16); alpha ~ U(0.3, 0.95);
+    25% distributed order (16 Gauss-Legendre nodes on [0.1, 1]); each with its
+    own seeded initial head.  Returns (arrival_s, spec, u0, node_steps) tuples
+    in arrival order.  The reference runs such a mix as one process per solve,
+    each configured on its command line (mpi_test.sh:12-23)."""
+    rng = np.random.default_rng(seed)
+    reqs = []
+    t = 0.0
+    for i in range(n_req):
+        t += rng.exponential(1.0 / rate)
+        is3d = rng.random() < 0.2
+        do = rng.random() < 0.25
+        alpha = float(rng.uniform(0.3, 0.95))
+        if is3d:
+            n = int(rng.integers(32, 97))
+            M = int(rng.integers(200, 1001))
+            u0 = init_3d(n, 100000 + i)
+            kw = dict(dims=3, block=16)
+        else:
+            n = int(rng.integers(500, 20001))
+            M = int(rng.integers(1000, 40001))
+            u0 = init_1d(n, 100000 + i)
+            kw = dict(dims=1, block=64)
+        if do:
+            kw.update(do_q=16, do_rule=0, do_a0=0.1, do_a1=1.0)
+        tau = 1e-3 if is3d else 5e-4
+        spec = make_spec(n, M, tau, alpha=alpha, hist_mode=hist_mode, u_top=u0[0], **kw)
+        reqs.append((t, spec, u0, (n ** 3 if is3d else n) * M))
+    return reqs

@@ -34,6 +34,14 @@ typedef enum {
 
 const char *fpde_last_error(void);
 int fpde_version(void);
+/* Release what the library itself holds across calls (the per-device
+ * argument tables of fpde_solver_advance_batch) and wait for the devices it
+ * used.  Call it after destroying every solver, server and facade context,
+ * before the process exits, so that no device resource is left to the HIP
+ * runtime's own teardown.  The library stays usable afterwards (the tables
+ * are allocated again on demand).  No counterpart in the reference, whose
+ * OpenCL_program destructor releases its objects (opencl_cuda_base.cpp:418-434). */
+fpde_status fpde_shutdown(void);
 /* Number of visible devices (0 when no GPU); never fails. */
 int fpde_device_count(void);
 
@@ -172,7 +180,11 @@ typedef struct {
     int var_tau;         /* 1: the step length is given step by step at run time
                             (fpde_solver_advance_tau, fpde_varT_step); 1D,
                             hist_mode 0, sum_alg 0, taus NULL; steps = capacity */
-    int reserved;
+    int split3d;         /* 3D, the reference's split3d flag (rp_split_3d): even =
+                            the history term only in the first (y, m3d_c=2) sweep;
+                            odd = a third of it in each of the three sweeps
+                            (Om1 fpde_hpc_1d.cpp:4172-4173, :4258-4263); 1D
+                            ignores it                                          */
     double gamma;        /* space-fractional order (CLI "bb", H_solver gamma,
                             fpde_hpc_1d.cpp:3510): 0 or 1 = classical diffusion;
                             in (0,1) the reference's explicit one-sided scheme
@@ -180,6 +192,13 @@ typedef struct {
                             Dg kb(gamma,i-1,i,i) (A_/B/R, :3095-3142) and the
                             right-hand side -Dg inv_dw_dh calc_v(F(U^n))
                             (Om1 :4165-4189, calc_v :2303-2335); 1D only       */
+    int fk;              /* kernel function of the generalized Caputo derivative
+                            (CLI "FK", func_in_kernel, g() fpde_hpc_1d.cpp:
+                            1889-1898): 0 g(t)=t, 1 sqrt(t), 2 t^2, 3 t^fp; the
+                            L1 weights are then int (g(T_a)-g(s))^-alpha ds over
+                            each interval, evaluated exactly (weight_mode 0,
+                            sum_alg 0), one weight row per step                */
+    double fp;           /* fk 3: the power (CLI "FP", func_power)             */
 } fpde_spec;
 
 typedef struct fpde_solver fpde_solver;
@@ -194,6 +213,12 @@ typedef struct {
     double history_alg_flops; /* algorithmic flops (2 per term)               */
     size_t device_bytes;      /* device memory held by the solver             */
     int picard_iters_total;   /* Picard iterations actually performed         */
+    int reserved;
+    double step_alg_bytes;    /* algorithmic bytes of the 1D step kernels: per
+                                 node and step 100 B (coefficients, rows and
+                                 the line solve) + 8 B per history term the
+                                 step kernel sums itself (near field)          */
+    int64_t steps_done;       /* time steps completed by the counted launches */
 } fpde_stats;
 
 /* Bytes of device memory a solve of this spec will hold (admission). */
@@ -275,11 +300,13 @@ fpde_status fpde_solver_get_solution(fpde_solver *s, double *u_out);
 /* Output files of the reference driver (fpde_hpc_1d.cpp:9498-9526, opened
  * at :8945-8947), for the current solution at time step*tau: appends to
  * path_txt one row per depth node, "t z u 0 0 theta" (1D) or
- * "t z x y u 0 0 theta" along the depth line through the centre x = y =
- * nodes/2 (3D), printed with %g; the two zeros stand for the C and T fields
- * the moisture-only model does not have, theta is the VGM wetness at
- * P = z - u.  3D also appends the whole field, nodes^3 FP64 values in
- * solver order (depth fastest, no halo layer), to path_3d (NULL: skipped). */
+ * "t z x y u 0 0 theta" along the depth line through the reference's centre
+ * x = y = (nodes-1)/2 (3D), printed with %g; the two zeros stand for the C
+ * and T fields the moisture-only model does not have, theta is the VGM
+ * wetness at P = z - u.  3D also appends the field to path_3d (NULL:
+ * skipped) as the reference's b_U: (nodes+1)^3 FP64 values, depth index
+ * fastest, then x, then y, with one halo slot at the end of each axis that
+ * holds the initial value of the nearest node. */
 fpde_status fpde_solver_write_results(fpde_solver *s, const char *path_txt, const char *path_3d);
 /* Effective weights (local, omega[M], corr[2M]) the solver uses (host copies). */
 fpde_status fpde_solver_get_weights(fpde_solver *s, double *local, double *omega, double *corr);
@@ -301,7 +328,9 @@ typedef struct fpde_server fpde_server;
 typedef struct {
     int64_t id;
     double submit_s, start_s, finish_s;   /* server clock (seconds)            */
-    int status;                           /* 0 done, 1 rejected, 2 cancelled   */
+    int status;                           /* 0 done, 1 rejected, 2 cancelled,
+                                             3 failed (a device error or a grid-
+                                             barrier time-out in its solve)    */
     int steps_done;
 } fpde_result_info;
 

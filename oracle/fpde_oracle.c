@@ -177,6 +177,132 @@ double orc_kb(double alpha, int t0, int a, const double *Time,
     return orc_kb_closed(alpha, p0, p1, pa);
 }
 
+/* ------------------------------------------------------------------------ */
+/* Generalized Caputo kernel (func_in_kernel 1/2/3).                        */
+
+/* g() of fpde_hpc_1d.cpp:1889-1898: t, sqrt(t), t^2, t^fp. */
+double orc_g(int fk, double fp, double t)
+{
+    if (fk == 1) return sqrt(t);
+    if (fk == 2) return t * t;
+    if (fk == 3) return pow(t, fp);
+    return t;
+}
+
+/* inv_g_der of :1901-1930: the n-th derivative of f = g^-1 at t, given the
+ * (n-1)-th as der_nm1 (the recurrence the Taylor series runs on). */
+double orc_inv_g_der(int fk, double fp, double t, int n, double der_nm1)
+{
+    if (fk == 1) {                                   /* f = t^2 */
+        if (n == 0) return t * t;
+        if (n == 1) return 2 * t * (der_nm1 / (t * t));
+        if (n == 2) return 2 * (der_nm1 / (2 * t));
+        return 0.0;
+    }
+    if (fk == 2) {                                   /* f = sqrt(t) */
+        if (n == 0) return sqrt(t);
+        if (n == 1) return (0.5 / sqrt(t)) * (der_nm1 / sqrt(t));
+        return der_nm1 * (0.5 - n + 1) / t;
+    }
+    if (fk == 3) {                                   /* f = t^(1/fp) */
+        if (n == 0) return pow(t, 1.0 / fp);
+        if (n == 1) return ((1.0 / fp) * pow(t, (1.0 / fp) - 1)) * (der_nm1 / pow(t, 1.0 / fp));
+        return der_nm1 * ((1.0 / fp) - n + 1) / t;
+    }
+    return (n == 0) ? t : ((n == 1) ? (der_nm1 / t) : 0.0);
+}
+
+/* _kb_row, :1933-1996, branch g(t1) <= gtj: Taylor series of f' = (g^-1)'
+ * around gtj = g(a), integrated term by term over [g(t0), g(t1)]; stops after
+ * the first |term| <= eps (the reference's global_eps) or max_niter terms. */
+double orc_kb_row_taylor(double alpha, int fk, double fp, double t0, double t1, double gtj,
+                         double eps, int max_niter, int *niter_out)
+{
+    double sum = 0.0, v, v2, v3, v4, i = 0.0;
+    int niter = 0;
+    double g0 = orc_g(fk, fp, t0), g1 = orc_g(fk, fp, t1);
+    if (alpha == 1.0) { if (niter_out) *niter_out = 0; return (g1 == gtj) ? 1.0 : 0.0; }   /* :1939-1942 */
+    v = orc_inv_g_der(fk, fp, gtj, 1, orc_inv_g_der(fk, fp, gtj, 0, 1));                  /* :1945 */
+    v *= pow(gtj, 1 - alpha);
+    v3 = pow(fabs(1.0 - g1 / gtj), 1 - alpha);
+    v4 = pow(fabs(1.0 - g0 / gtj), 1 - alpha);
+    v2 = -(v3 - v4) / (1 - alpha);
+    sum += v2 * v;
+    while (fabs(v * v2) > eps) {                                                          /* :1951 */
+        i += 1.0;
+        v = orc_inv_g_der(fk, fp, gtj, (int)(i + 1.0), v);
+        v *= -gtj / i;
+        v3 *= fabs(1.0 - g1 / gtj);
+        v4 *= fabs(1.0 - g0 / gtj);
+        v2 = -(v3 - v4) / (i - alpha + 1.0);
+        sum += v2 * v;
+        niter++;
+        if (niter > max_niter) break;
+    }
+    if (niter_out) *niter_out = niter;
+    return sum;
+}
+
+/* g(pa) - g(pa - d) for g = t^gp, exact for small d. */
+static double gap_pow(double pa, double gp, double d, double G) { return -G * expm1(gp * log1p(-d / pa)); }
+
+/* b(t0,t1;a) = int_{p0}^{p1} (pa^gp - s^gp)^-alpha ds, the integral the
+ * reference's series approximate, by tanh-sinh quadrature (step 1/64, |t| <=
+ * 4): for p1 < pa directly in s, with pa - s formed from the distance to p1;
+ * for the newest interval (p1 = pa) in w = (pa - s)^(1-alpha), which takes
+ * the (pa - s)^-alpha singularity into the measure and leaves a bounded
+ * integrand. */
+double orc_kb_general(double alpha, double gp, double p0, double p1, double pa)
+{
+    if (alpha == 1.0) return (p1 == pa) ? 1.0 : 0.0;
+    if (gp == 1.0) return orc_kb_closed(alpha, p0, p1, pa);
+    const double G = pow(pa, gp), h = 1.0 / 64.0;
+    double sum = 0.0;
+    if (p1 < pa) {
+        const double L = p1 - p0, gr = pa - p1;
+        for (int j = -256; j <= 256; j++) {
+            double t = j * h, u = 0.5 * M_PI * sinh(t);
+            double dr = L / (1.0 + exp(2.0 * u));                 /* p1 - s */
+            double eu = exp(u), ch = eu + 1.0 / eu;
+            double wt = h * 0.5 * L * (0.5 * M_PI * cosh(t)) * 4.0 / (ch * ch);
+            sum += wt * pow(gap_pow(pa, gp, gr + dr, G), -alpha);
+        }
+        return sum;
+    }
+    const double e1 = 1.0 - alpha, w0 = pow(pa - p0, e1);
+    for (int j = -256; j <= 256; j++) {
+        double t = j * h, u = 0.5 * M_PI * sinh(t);
+        double w = w0 / (1.0 + exp(-2.0 * u));
+        double eu = exp(u), ch = eu + 1.0 / eu;
+        double wt = h * 0.5 * w0 * (0.5 * M_PI * cosh(t)) * 4.0 / (ch * ch);
+        double d = pow(w, 1.0 / e1);
+        double phi = (d > 0.0) ? pow(gap_pow(pa, gp, d, G) / d, -alpha) : pow(gp * pow(pa, gp - 1.0), -alpha);
+        sum += wt * phi;
+    }
+    return sum / e1;
+}
+
+double orc_g_exponent(int fk, double fp)
+{
+    return fk == 1 ? 0.5 : (fk == 2 ? 2.0 : (fk == 3 ? fp : 1.0));
+}
+
+/* kb over a grid with kernel fk: fk 0 = orc_kb; else the integral. */
+double orc_kb_g(double alpha, int fk, double fp, int t0, int a, const double *Time,
+                int weight_mode, double eps)
+{
+    if (fk == 0) return orc_kb(alpha, t0, a, Time, weight_mode, eps);
+    return orc_kb_general(alpha, orc_g_exponent(fk, fp), Time[t0], Time[t0 + 1], Time[a]);
+}
+
+double orc_l1_weights_g(double alpha, int fk, double fp, int n, const double *Time,
+                        int weight_mode, double eps, double *kb3)
+{
+    for (int t = 0; t < n; t++)
+        kb3[t] = orc_kb_g(alpha, fk, fp, t, n + 1, Time, weight_mode, eps);
+    return orc_kb_g(alpha, fk, fp, n, n + 1, Time, weight_mode, eps);
+}
+
 /* init_time_step (:3942-3949) + precalc_kb_3 (:4069-4085) for step n:
  * kkk0 = kb(alpha,n,n+1,n+1); kb3[t] = kb(alpha,t,t+1,n+1), t<n. */
 double orc_l1_weights(double alpha, int n, const double *Time,
@@ -490,7 +616,7 @@ static void history_term(weights_ctx *w, const orc_params *p, int n, int nodes,
     }
     for (int q = 0; q < w->q; q++) {
         double a = w->al[q];
-        double kkk0 = orc_l1_weights(a, n, w->Time, p->weight_mode, p->series_eps, w->kb3);
+        double kkk0 = orc_l1_weights_g(a, p->fk, p->fp, n, w->Time, p->weight_mode, p->series_eps, w->kb3);
         double Da = da_of(a);
         L += w->cw[q] * Da * kkk0 / w->tau[n];
         if (n >= 1 && p->sum_alg == 2) {
@@ -560,8 +686,16 @@ void orc_space_flux(int nodes, double h, const double *K, const double *u, doubl
         F[j] = K[j] * u[j + 1] / hh - (K[j] / hh + K[j - 1] / hh) * u[j] + K[j - 1] * u[j - 1] / hh;
 }
 
+/* the generalized kernel is restated with its exact weights (weight_mode 0)
+ * and the full history sum only */
+static int fk_supported(const orc_params *p)
+{
+    return p->fk == 0 || (p->fk >= 1 && p->fk <= 3 && p->weight_mode == 0 && p->sum_alg == 0);
+}
+
 int orc_solve_1d(const orc_params *p, double *u)
 {
+    if (!fk_supported(p)) return -2;
     int n = p->nodes, M = p->steps;
     double h = p->length / (n - 1);
     weights_ctx w; wctx_init(&w, p);
@@ -630,6 +764,7 @@ int orc_solve_3d(const orc_params *p, double *u) { return orc_solve_3d_trace(p, 
 
 int orc_solve_3d_trace(const orc_params *p, double *u, double *sweeps)
 {
+    if (!fk_supported(p)) return -2;
     int n = p->nodes, M = p->steps;
     size_t nn = (size_t)n * n * n;
     double h = p->length / (n - 1);
@@ -773,4 +908,157 @@ double orc_ksat_for_fourier(double u_top, double h, double tau, double fo)
 {
     double P = 0.0 - u_top;
     return fo * h * h / (tau * orc_kr(P) * orc_inv_dw_dh(P));
+}
+
+/* ------------------------------------------------------------------------ */
+/* fr_do.cpp: the distributed-order diffusion equation (see fpde_oracle.h). */
+
+/* bk of the fr_do kernels (fr_do.cpp:129-134): d1^(1-b) - d2^(1-b), with
+ * either distance 0 giving the other's power alone. */
+static double frdo_bk(double d1, double d2, double beta)
+{
+    if (d1 == 0.0) return pow(d2, 1.0 - beta);
+    if (d2 == 0.0) return pow(d1, 1.0 - beta);
+    return pow(d1, 1.0 - beta) - pow(d2, 1.0 - beta);
+}
+
+/* weight functions: w (:40-43) of the derivative test, eq_w (:467-474) */
+static double frdo_w(int example, double power, double a)
+{
+    if (example < 0) return tgamma(3.0 - a);
+    if (example == 0) return tgamma(5.0 - a);
+    if (example == 1) return pow(a, power) * (1.0 + power);
+    return 0.0;
+}
+
+/* eq_f (:458-465); gpu_pi: eq_ff of the GPU program text (:256-263), where
+ * 2 pi and 800 pi pi are formed in float from a float constant for pi */
+static double frdo_f(int example, int gpu_pi, double t, double x)
+{
+    if (example != 0) return 0.0;
+    if (gpu_pi) {
+        const float pf = (float)M_PI;
+        const double two_pi = (double)(2.0f * pf), k800 = (double)(800.0f * pf * pf);
+        return sin(two_pi * x) * (k800 * t * t * t * t + (2400 * (t * t * t * t - t * t * t) / log(t)));
+    }
+    return sin(2 * M_PI * x) * (800 * M_PI * M_PI * t * t * t * t + (2400 * (t * t * t * t - t * t * t) / log(t)));
+}
+
+/* fill_arrays (:51-72) with F given: bkk[i na + j] = bk((i+1) tau, i tau, a_j)
+ * for ntau intervals and WA_j = w(a_j)/(na-1); gs_j = Gamma(2 - a_j). */
+typedef struct { int ntau, na; double *bkk, *WA, *gs, *CD; } frdo_tables;
+
+static void frdo_fill(frdo_tables *T, int ntau, double tau, int na, int example, double power)
+{
+    T->ntau = ntau;
+    T->na = na;
+    T->bkk = (double *)malloc(sizeof(double) * (size_t)ntau * na);
+    T->WA = (double *)malloc(sizeof(double) * na);
+    T->gs = (double *)malloc(sizeof(double) * na);
+    T->CD = (double *)malloc(sizeof(double) * na);
+    for (int i = 0; i < ntau; i++)
+        for (int j = 0; j < na; j++)
+            T->bkk[(size_t)i * na + j] = frdo_bk((i + 1) * tau, i * tau, (na == 1) ? 0.0 : j / (double)(na - 1));
+    for (int j = 0; j < na; j++) {
+        T->WA[j] = frdo_w(example, power, j / (double)(na - 1)) * (1.0 / (na - 1));
+        T->gs[j] = tgamma(2.0 - (j / (double)(na - 1)));
+    }
+}
+static void frdo_free(frdo_tables *T) { free(T->bkk); free(T->WA); free(T->gs); free(T->CD); }
+
+/* DO_Cder (:89-113) with F given: CD_i = sum_j f_j bkk[(t-j-1) na + i] /
+ * Gamma(2 - a_i), f_j = (F[j+1]-F[j])/tau, then the trapezoid in a. */
+static void frdo_cder(const frdo_tables *T, int ntau, double tau, const double *F, int maxt, int t0, int j0,
+                      double *DF)
+{
+    const int na = T->na;
+    for (int t = t0; t < ntau; t++) {
+        double sum = 0.0;
+        int jhi = (maxt < 0) ? t : ((t > maxt) ? maxt : t);
+        for (int i = 0; i < na; i++) {
+            double sum2 = 0.0;
+            for (int j = j0; j < jhi; j++)
+                if (j >= 0) sum2 += ((F[j + 1] - F[j]) / tau) * T->bkk[(size_t)(t - j - 1) * na + i];
+            sum2 /= T->gs[i];
+            T->CD[i] = sum2;
+        }
+        for (int i = 0; i < na - 1; i++) sum += 0.5 * (T->WA[i] * T->CD[i] + T->WA[i + 1] * T->CD[i + 1]);
+        DF[t] = sum;
+    }
+}
+
+void orc_do_cder(int ntau, double tau, int na, int example, double power, const double *F, int maxt,
+                 int t0, int j0, double *DF)
+{
+    frdo_tables T;
+    frdo_fill(&T, ntau, tau, na, example, power);
+    frdo_cder(&T, ntau, tau, F, maxt, t0, j0, DF);
+    frdo_free(&T);
+}
+
+double orc_doeq_solve(const orc_doeq_params *p, double *C)
+{
+    const int Nx = p->nx, Nt = p->nt, Na = p->na, tb = p->tblock;
+    const double h = p->max_x / (Nx - 1), tau = p->max_t / Nt;
+    const double Aa = 0.5 * p->D / (h * h), Bb = Aa;
+    double *A = (double *)calloc(Nx + 1, sizeof(double)), *B = (double *)calloc(Nx + 1, sizeof(double));
+    double *DFb = (double *)calloc((size_t)Nx * tb, sizeof(double));   /* eq_DF [node][block step] */
+    double *DF = (double *)calloc((size_t)Nt + 1, sizeof(double));
+    frdo_tables T;
+    frdo_fill(&T, Nt, tau, Na, p->example, p->power);                        /* eq_init :685 */
+    for (int i = 0; i < Nx * Nt; i++) C[i] = 0.0;                         /* eq_init :678-681 */
+    for (int i = 0; i < Nx; i++) C[(size_t)i * Nt] = (p->example == 1) ? (i * h) * (1.0 - i * h) : 0.0;
+    /* eq_b0: the trapezoid of WA bkk[0] / Gamma(2-a), bkk[0] = tau^(1-a) (:687-690) */
+    double b0 = 0.0;
+    for (int a = 0; a < Na - 1; a++) {
+        double a0 = a / (double)(Na - 1), a1 = (a + 1) / (double)(Na - 1);
+        double w0 = frdo_w(p->example, p->power, a0) / (Na - 1), w1 = frdo_w(p->example, p->power, a1) / (Na - 1);
+        b0 += 0.5 * (w0 * frdo_bk(tau, 0.0, a0) / tgamma(2.0 - a0) + w1 * frdo_bk(tau, 0.0, a1) / tgamma(2.0 - a1));
+    }
+    const double Ss = (p->D / (h * h)) + (b0 / tau);
+    A[1] = 0.0;                                                              /* u(t,0) = 0 (:693-695) */
+    for (int i = 1; i < Nx - 1; i++) A[i + 1] = Bb / (Ss - Aa * A[i]);
+    for (int i = 0; i < Nt; i += tb) {
+        /* the history before the block, every node (:742-747) */
+        for (int j = 0; j < Nx; j++) {
+            frdo_cder(&T, i + tb, tau, C + (size_t)j * Nt, (i == 0) ? i : i - 1, i, 0, DF);
+            for (int k = 0; k < tb; k++) DFb[(size_t)j * tb + k] = DF[i + k];
+        }
+        for (int j = (i == 0) ? 1 : 0; j < tb; j++) {
+            int lev = i + j;
+            /* eq_one_step (:724-732) with eq_Ff (:656-665) of level lev-1 */
+            B[1] = 0.0;
+            for (int x = 1; x < Nx - 1; x++) {
+                const double *c = C + (size_t)x * Nt;
+                double rp = -0.5 * (p->D / (h * h)) * (C[(size_t)(x - 1) * Nt + lev - 1] - 2.0 * c[lev - 1] + C[(size_t)(x + 1) * Nt + lev - 1]);
+                double lp = -(b0 * c[lev - 1] / tau) - 0.5 * (frdo_f(p->example, p->gpu_pi, (lev - 1) * tau, x * h) +
+                                                              frdo_f(p->example, p->gpu_pi, lev * tau, x * h));
+                double nlp = DFb[(size_t)x * tb + (lev % tb)];
+                if (!isfinite(lp)) lp = -(b0 * c[lev - 1] / tau) - frdo_f(p->example, p->gpu_pi, lev * tau, x * h);
+                double F = rp + lp + nlp;
+                B[x + 1] = (A[x + 1] / Bb) * (Aa * B[x] - F);
+            }
+            C[(size_t)(Nx - 1) * Nt + lev] = 0.0;                          /* u(max_x,t) = 0 */
+            for (int x = Nx - 2; x >= 0; x--) C[(size_t)x * Nt + lev] = A[x + 1] * C[(size_t)(x + 1) * Nt + lev] + B[x + 1];
+            /* the in-block interval [lev-1, lev] joins the next level's history (:751-756) */
+            if (j != tb - 1)
+                for (int x = 0; x < Nx; x++) {
+                    frdo_cder(&T, lev + 2, tau, C + (size_t)x * Nt, lev, lev + 1, i - 1, DF);
+                    DFb[(size_t)x * tb + j + 1] += DF[lev + 1];
+                }
+        }
+    }
+    double err = 0.0;
+    if (p->example == 0) {
+        int i = Nt - 1;
+        for (int x = 0; x < Nx; x++) {
+            double a = 100 * pow(i * tau, 4) * sin(2 * M_PI * x * h);          /* eq_anal_u :635-642 */
+            double d = C[(size_t)x * Nt + i] - a;
+            err += d * d;
+        }
+        err = sqrt(err / Nx);
+    }
+    free(A); free(B); free(DFb); free(DF);
+    frdo_free(&T);
+    return err;
 }

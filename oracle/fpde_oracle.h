@@ -49,6 +49,20 @@ double orc_kb(double alpha, int t0, int a, const double *Time,
 double orc_l1_weights(double alpha, int n, const double *Time,
                       int weight_mode, double eps, double *kb3);
 
+/* ---- generalized kernel g (func_in_kernel fk: 0 t, 1 sqrt t, 2 t^2, 3 t^fp) ---- */
+double orc_g(int fk, double fp, double t);
+double orc_inv_g_der(int fk, double fp, double t, int n, double der_nm1);
+/* the reference's Taylor series _kb_row (branch g(t1) <= gtj) */
+double orc_kb_row_taylor(double alpha, int fk, double fp, double t0, double t1, double gtj,
+                         double eps, int max_niter, int *niter_out);
+/* int_{p0}^{p1} (pa^gp - s^gp)^-alpha ds by tanh-sinh quadrature */
+double orc_kb_general(double alpha, double gp, double p0, double p1, double pa);
+double orc_g_exponent(int fk, double fp);
+double orc_kb_g(double alpha, int fk, double fp, int t0, int a, const double *Time,
+                int weight_mode, double eps);
+double orc_l1_weights_g(double alpha, int fk, double fp, int n, const double *Time,
+                        int weight_mode, double eps, double *kb3);
+
 /* ---- history sum: S[i] = sum_t kb3[t]*(old[t+1][i]-old[t][i])/tau[t];
  * sum_alg 1 stops (newest first) at the first |kb3[t]| < sum_param. */
 void orc_time_summing(int nodes, int levels, const double *old,
@@ -128,6 +142,36 @@ void orc_space_v(int nodes, double h, double gamma, int weight_mode, double eps,
 /* F[j] of Om1 equation 0 (:4178-4179) on a uniform grid, j = 1..nodes-2:
  * (K[j] (u[j+1]-u[j]) - K[j-1] (u[j]-u[j-1])) / h^2; F[0] = F[nodes-1] = 0. */
 void orc_space_flux(int nodes, double h, const double *K, const double *u, double *F);
+
+/* ---- fr_do.cpp: the distributed-order diffusion equation ----
+ *   int_0^1 w(a) D^a u da = D u_xx + f(t,x),  x in [0, max_x], t in [0, max_t],
+ *   u(0,t) = u(max_x,t) = 0, u(x,0) = u0(x)
+ * (eq_solve, fr_do.cpp:733-775): Crank-Nicolson in space (eq_Ff :656-665,
+ * eq_one_step :724-732), the alpha integral as the trapezoid over na nodes
+ * a_i = i/(na-1) with WA_i = w(a_i)/(na-1) (fill_arrays :51-72, DO_Cder
+ * :89-113), L1 weights bk(d1,d2,a) = d1^(1-a) - d2^(1-a) over Gamma(2-a),
+ * local coefficient eq_b0 (:688-690), and the reference's block scheme
+ * (history before the block at its start, in-block intervals added step by
+ * step, :739-761).  example 0: w = Gamma(5-a), u0 = 0, f = sin(2 pi x)
+ * (800 pi^2 t^4 + 2400 (t^4 - t^3)/ln t), exact u = 100 t^4 sin(2 pi x)
+ * (:458-465, :626-633, eq_anal_u); example 1: w = a^p (1+p), u0 = x (1-x),
+ * f = 0.  C: [nx][nt] node-major as the reference's eq_C (level j of node i
+ * at C[i*nt + j]); returns the RMS error against the exact solution of
+ * example 0 at the last level (the figure eq_solve prints), 0 for example 1. */
+typedef struct {
+    int    nx, na, tblock, nt;  /* eq_Nx, eq_Na, eq_tblock, eq_Nt (a multiple of tblock) */
+    int    example;             /* eq_ex */
+    double power;               /* eq_ex1_power */
+    double D, max_x, max_t;     /* eq_D, eq_max_x, eq_max_t */
+    int    gpu_pi;              /* 1: the source as the GPU program text forms it (eq_ff) */
+} orc_doeq_params;
+double orc_doeq_solve(const orc_doeq_params *p, double *C);
+/* The distributed-order derivative of fr_do's DO_Cder on a grid function F
+ * (levels 0..ntau-1, spacing tau): DF[t] for t in [t0, ntau) with history
+ * terms j in [j0, min(t, maxt)) (maxt < 0: no limit), trapezoid over na
+ * orders with weight w (example as above; example < 0: Gamma(3-a)). */
+void orc_do_cder(int ntau, double tau, int na, int example, double power, const double *F, int maxt,
+                 int t0, int j0, double *DF);
 
 /* Effective (folded) weights used by the product for uniform tau:
  * local = coefficient of (U^{n+1}-U^n); omega[j-1] for j=1..M (Toeplitz),

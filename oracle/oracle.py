@@ -17,6 +17,12 @@ _lib = None
 _dp = ctypes.POINTER(ctypes.c_double)
 
 
+class DoEqParams(ctypes.Structure):
+    _fields_ = [("nx", ctypes.c_int), ("na", ctypes.c_int), ("tblock", ctypes.c_int), ("nt", ctypes.c_int),
+                ("example", ctypes.c_int), ("power", ctypes.c_double), ("D", ctypes.c_double),
+                ("max_x", ctypes.c_double), ("max_t", ctypes.c_double), ("gpu_pi", ctypes.c_int)]
+
+
 class Params(ctypes.Structure):
     _fields_ = [
         ("nodes", ctypes.c_int), ("steps", ctypes.c_int), ("tau", ctypes.c_double),
@@ -73,8 +79,24 @@ def lib():
         L.orc_space_v.argtypes = [ctypes.c_int, ctypes.c_double, ctypes.c_double, ctypes.c_int, ctypes.c_double,
                                   _dp, _dp]
         L.orc_space_flux.argtypes = [ctypes.c_int, ctypes.c_double, _dp, _dp, _dp]
+        L.orc_g.argtypes = [ctypes.c_int, ctypes.c_double, ctypes.c_double]
+        L.orc_g.restype = ctypes.c_double
+        L.orc_inv_g_der.argtypes = [ctypes.c_int, ctypes.c_double, ctypes.c_double, ctypes.c_int, ctypes.c_double]
+        L.orc_inv_g_der.restype = ctypes.c_double
+        L.orc_kb_row_taylor.argtypes = [ctypes.c_double, ctypes.c_int] + [ctypes.c_double] * 5 + \
+            [ctypes.c_int, ctypes.POINTER(ctypes.c_int)]
+        L.orc_kb_row_taylor.restype = ctypes.c_double
+        L.orc_kb_general.argtypes = [ctypes.c_double] * 5
+        L.orc_kb_general.restype = ctypes.c_double
+        L.orc_l1_weights_g.argtypes = [ctypes.c_double, ctypes.c_int, ctypes.c_double, ctypes.c_int, _dp,
+                                       ctypes.c_int, ctypes.c_double, _dp]
+        L.orc_l1_weights_g.restype = ctypes.c_double
         L.orc_ksat_for_fourier.argtypes = [ctypes.c_double] * 4
         L.orc_ksat_for_fourier.restype = ctypes.c_double
+        L.orc_doeq_solve.argtypes = [ctypes.POINTER(DoEqParams), _dp]
+        L.orc_doeq_solve.restype = ctypes.c_double
+        L.orc_do_cder.argtypes = [ctypes.c_int, ctypes.c_double, ctypes.c_int, ctypes.c_int, ctypes.c_double, _dp,
+                                  ctypes.c_int, ctypes.c_int, ctypes.c_int, _dp]
         _lib = L
     return _lib
 
@@ -117,6 +139,25 @@ def l1_weights(alpha, n, Time, weight_mode=1, eps=1e-5):
     Time = np.ascontiguousarray(Time, dtype=np.float64)
     kb3 = np.zeros(max(n, 1))
     kkk0 = lib().orc_l1_weights(alpha, n, _p(Time), weight_mode, eps, _p(kb3))
+    return kkk0, kb3[:n]
+
+
+def kb_row_taylor(alpha, fk, fp, t0, t1, gtj, eps=1e-5, max_niter=100000):
+    """The reference's _kb_row Taylor series; returns (value, iterations)."""
+    n = ctypes.c_int(0)
+    v = lib().orc_kb_row_taylor(alpha, fk, fp, t0, t1, gtj, eps, max_niter, ctypes.byref(n))
+    return v, n.value
+
+
+def kb_general(alpha, gp, p0, p1, pa):
+    return lib().orc_kb_general(alpha, gp, p0, p1, pa)
+
+
+def l1_weights_g(alpha, fk, fp, n, Time):
+    """(kkk0, kb_3_cache) of step n with the generalized kernel fk (exact weights)."""
+    Time = np.ascontiguousarray(Time, dtype=np.float64)
+    kb3 = np.zeros(max(n, 1))
+    kkk0 = lib().orc_l1_weights_g(alpha, fk, fp, n, _p(Time), 0, 1e-5, _p(kb3))
     return kkk0, kb3[:n]
 
 
@@ -261,3 +302,28 @@ def space_flux(K, u, h):
 
 def ksat_for_fourier(u_top, h, tau, fo=1.0):
     return lib().orc_ksat_for_fourier(u_top, h, tau, fo)
+
+
+def doeq_params(nx=128, na=64, tblock=32, nt=1280, example=0, power=1.5, D=2.0, max_x=None, max_t=None, gpu_pi=0):
+    """fr_do.cpp's equation settings (set_for_ex: [0, 0.5]^2 for example 0, [0, 1]^2 for 1)."""
+    if max_x is None:
+        max_x = 0.5 if example == 0 else 1.0
+    if max_t is None:
+        max_t = 0.5 if example == 0 else 1.0
+    nt = (nt // tblock) * tblock
+    return DoEqParams(nx, na, tblock, nt, example, power, D, max_x, max_t, gpu_pi)
+
+
+def doeq_solve(p):
+    """(C [nx][nt] node-major, RMS error of the last level against the exact
+    solution of example 0)."""
+    C = np.zeros(p.nx * p.nt)
+    err = lib().orc_doeq_solve(ctypes.byref(p), _p(C))
+    return C.reshape(p.nx, p.nt), err
+
+
+def do_cder(F, tau, na, example=-1, power=1.5, maxt=-1, t0=1, j0=0):
+    F = np.ascontiguousarray(F, dtype=np.float64)
+    DF = np.zeros(F.size)
+    lib().orc_do_cder(F.size, tau, na, example, power, _p(F), maxt, t0, j0, _p(DF))
+    return DF

@@ -19,12 +19,17 @@ def test_more_than_one_gpu_is_refused():
     assert out["n_gpus"] == 2 and "one GPU" in out["error"]
 
 
-def test_cpu_baseline_object():
+def test_cpu_baseline_pieces(monkeypatch):
+    """The host share and CPU model, and the sampled history term of one step
+    (per order; a node share scaled up)."""
     import bench
-    cpu = bench.cpu_baseline(200, 400, seconds_hint=0.2, alpha=0.8, tau=1e-3, seed=1000, picard=(3, 1e-12))
-    assert cpu["value"] > 0 and cpu["unit"] == "node*steps/s"
-    assert cpu["kind"] == "port" and cpu["cores"] >= 1
-    assert "n=200 levels x N=200 nodes" in cpu["sample"] and "reps" in cpu["sample"]
+    from oracle import oracle as O
+    monkeypatch.setenv("OMP_NUM_THREADS", "3")
+    cores, model, ncpu = bench.cpu_info()
+    assert cores == min(3, ncpu) and ncpu >= 1 and model
+    t1, reps, total = bench.cpu_history_step(O, 1, 400, 200, 1e-3, [0.8], seconds=0.1)
+    t2, _, _ = bench.cpu_history_step(O, 1, 400, 200, 1e-3, [0.8, 0.5], node_frac=0.5, seconds=0.1)
+    assert t1 > 0 and t2 > 0 and reps >= 3 and total > 0
 
 
 @pytest.mark.gpu
@@ -50,6 +55,10 @@ def test_bench_json_line():
         assert k in roof, k
     assert roof["bound"] in ("hbm", "mfma") and 0 < roof["frac"] <= 1.0
     assert abs(roof["frac"] - roof["achieved"] / roof["peak"]) < 1e-9
+    if roof["kernel"] == "k_step1d_block":         # 100 B/node*step + 8 B per near-field term
+        assert 100.0 < roof["alg_bytes_per_node_step"] < 100.0 + 8.0 * 129
+    lat = out["latency"]
+    assert lat["us_per_step"] > 0 and lat["step_us_per_step"] > 0 and 1.0 <= lat["picard_iterations_per_step"] <= 3.0
     assert out["rel_err_vs_fp64"] <= 1e-8
     ens = out["ensemble"]                          # six solves as one batch vs one after another
     assert ens["count"] == 6 and ens["batch_max_rel_diff_vs_solo"] == 0.0

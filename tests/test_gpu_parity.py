@@ -115,6 +115,32 @@ def test_reference_sum_kernel_through_facade(fpde, oracle):
     assert np.max(np.abs(S - ref)) <= 1e-13 * np.max(np.abs(ref))
 
 
+@pytest.mark.parametrize("ex,power", [(0, 1.5), (1, 1.5), (1, 0.5)])
+@pytest.mark.parametrize("T", ["d", "f"])
+def test_frdo_equation_through_facade(oracle, tmp_path, ex, power, T):
+    """fr_do.cpp's GPU equation solve (eq_solve_cuda, :824-863) by a compiled
+    C++ caller issuing the program's kernels Fill, DOC, Sum, eq_F and eq_Solve
+    through the OpenCL_* facade, in double and float, against the oracle's
+    restatement of eq_solve with the GPU text's source term."""
+    import os
+    import subprocess
+    exe = os.path.join(os.path.dirname(__file__), "facade", "_build", "frdo_caller")
+    out = tmp_path / "C.bin"
+    nx, na, tb, ent = 64, 32, 32, 320
+    r = subprocess.run([exe, str(nx), str(na), str(tb), str(ent), str(ex), str(power), T, str(out)],
+                       capture_output=True, text=True, timeout=120)
+    assert r.returncode == 0, r.stdout + r.stderr
+    C = np.fromfile(out, dtype=np.float64 if T == "d" else np.float32).astype(np.float64).reshape(nx, -1)
+    ref, err = oracle.doeq_solve(oracle.doeq_params(nx=nx, na=na, tblock=tb, nt=ent, example=ex, power=power,
+                                                    gpu_pi=1))
+    assert C.shape == ref.shape
+    d = rel_err(C, ref)
+    print(f"fr_do equation ex {ex} p {power} T={T}: rel diff {d:.2e}, RMS err vs exact {r.stdout.split()[-1]}")
+    assert d <= (1e-11 if T == "d" else 2e-4)
+    if ex == 0:
+        assert abs(float(r.stdout.split()[-1]) - err) <= (1e-12 if T == "d" else 1e-4)
+
+
 def test_reference_style_cpp_caller_through_facade():
     """The fr_ocl_call_sum call sequence compiled against fpde_opencl_facade.h."""
     import os
@@ -319,6 +345,80 @@ def test_solve_3d_matches_oracle(fpde, oracle, mode):
     assert err <= (TOL_MIXED if mode.startswith("split") else TOL_F64)
 
 
+@pytest.mark.parametrize("mode", ["plain", "mfma_f64", "split_bf16", "split_f16", "split_bf16x3"])
+@pytest.mark.parametrize("fk,fp", [(3, 0.9), (1, 0.6), (2, 0.6)])
+def test_generalized_kernel_1d_matches_oracle(fpde, oracle, mode, fk, fp):
+    """Generalized Caputo kernel g(t) = t^0.9 (FK 3 FP 0.9, the reference
+    scripts' setting, gpu_test.sh:140-141), sqrt(t) (FK 1) and t^2 (FK 2):
+    exact weight integrals per step row, every history form (the GEMM forms in
+    cooperative block launches), Picard <= 3, across GEMM blocks of 32."""
+    nodes, steps, alpha = 300, 150, 0.7
+    u0 = fpde.init_1d(nodes, 1300)
+    spec = fpde.make_spec(nodes, steps, 1e-3, alpha=alpha, picard_max=3, picard_tol=1e-12, hist_mode=mode,
+                          block=32, u_top=u0[0], weight_mode=0, fk=fk, fp=fp)
+    u = fpde.solve(spec, u0)
+
+    def ref(k):
+        return oracle.solve_1d(oracle.make_params(nodes, steps, 1e-3, alpha, ksat=spec.ksat, picard_max=3,
+                                                  picard_tol=1e-12, weight_mode=0, fk=k, fp=fp), u0)
+    r = ref(fk)
+    tol = TOL_F64 if mode in ("plain", "mfma_f64") else TOL_MIXED
+    assert rel_err(ref(0), r) >= 1e3 * tol               # the kernel changes the solution
+    err = rel_err(u, r)
+    print(f"1D FK {fk} FP {fp} {mode}: rel err {err:.3e}")
+    assert err <= tol
+
+
+@pytest.mark.parametrize("mode", ["plain", "mfma_f64", "split_f16"])
+def test_generalized_kernel_3d_and_distributed_order(fpde, oracle, mode):
+    """FK 3 FP 0.9 in the 3D LOD step (12^3, the per-step weight rows feed the
+    history combine and the local coefficient every sweep), and with
+    distributed order in 1D."""
+    n, steps = 12, 40
+    u0 = fpde.init_3d(n, 1003)
+    spec = fpde.make_spec(n, steps, 1e-3, alpha=0.85, dims=3, hist_mode=mode, block=16, u_top=u0[0],
+                          weight_mode=0, fk=3, fp=0.9)
+    u = fpde.solve(spec, u0)
+    ref = oracle.solve_3d(oracle.make_params(n, steps, 1e-3, 0.85, ksat=spec.ksat, weight_mode=0, fk=3, fp=0.9),
+                          u0)
+    tol = TOL_MIXED if mode.startswith("split") else TOL_F64
+    err = rel_err(u, ref)
+    print(f"3D FK 3 {mode}: rel err {err:.3e}")
+    assert err <= tol
+    nodes, M = 200, 120
+    u1 = fpde.init_1d(nodes, 1302)
+    sp1 = fpde.make_spec(nodes, M, 1e-3, picard_max=2, picard_tol=1e-12, hist_mode=mode, block=32, u_top=u1[0],
+                         weight_mode=0, fk=3, fp=0.9, do_q=6, do_rule=0, do_a0=0.1, do_a1=1.0)
+    r1 = oracle.solve_1d(oracle.make_params(nodes, M, 1e-3, ksat=sp1.ksat, picard_max=2, picard_tol=1e-12,
+                                            weight_mode=0, fk=3, fp=0.9, do_q=6, do_rule=0), u1)
+    err = rel_err(fpde.solve(sp1, u1), r1)
+    print(f"1D FK 3 distributed order {mode}: rel err {err:.3e}")
+    assert err <= tol
+
+
+@pytest.mark.parametrize("n,mode", [(12, "plain"), (12, "split_f16"), (12, "series"), (129, "mfma_f64")])
+def test_solve_3d_split3d_matches_oracle(fpde, oracle, n, mode):
+    """split3d odd: each LOD sweep carries a third of the history term (Om1
+    fpde_hpc_1d.cpp:4172-4173, :4258-4263), in the partitioned sweeps (12^3)
+    and the one-thread-per-line / PCR sweeps (129^3), for a GEMM, a split and
+    the series history."""
+    steps = 40 if n == 12 else 12
+    u0 = fpde.init_3d(n, 1003)
+    kw = dict(hist_mode="plain", sum_alg=2, sum_param=16) if mode == "series" else dict(hist_mode=mode)
+    spec = fpde.make_spec(n, steps, 1e-3, alpha=0.85, dims=3, block=16, u_top=u0[0], split3d=1, **kw)
+    u = fpde.solve(spec, u0)
+
+    def ref(split):
+        return oracle.solve_3d(oracle.make_params(n, steps, 1e-3, 0.85, ksat=spec.ksat, split3d=split,
+                                                  sum_alg=spec.sum_alg, sum_param=spec.sum_param), u0)
+    r1 = ref(1)
+    tol = TOL_MIXED if mode.startswith("split") else TOL_F64
+    assert rel_err(ref(0), r1) >= 1e3 * tol         # the flag changes the solution
+    err = rel_err(u, r1)
+    print(f"3D split3d n={n} {mode}: rel err {err:.3e}")
+    assert err <= tol
+
+
 @pytest.mark.parametrize("n", [129, 150])
 def test_solve_3d_long_lines_matches_oracle(fpde, oracle, n):
     """Lines longer than 128 nodes take the one-thread-per-line lateral sweep
@@ -383,6 +483,45 @@ def test_server_results_same_alone_and_in_mix(fpde):
         assert info["status"] == 0
         assert rel_err(srv.result(rid), a) <= 1e-14
     srv.close()
+
+
+def test_serving_mix_512_requests(fpde):
+    """The serving mix at full size (512 requests, seed 1004: 80% 1D with N
+    500..20000 and M 1000..40000, 20% 3D 32^3..96^3, 25% distributed order,
+    split bf16): a quarter submitted up front, the rest 8 per scheduling round
+    while the server runs.  Every request completes, none is rejected, and the
+    smallest 1D, 3D and distributed-order requests equal their solo results."""
+    reqs = fpde.serving_mix(512, 16.0, 1004, "split_bf16")
+    with fpde.Server(200 << 30, max_active=16) as srv:
+        ids, k, rounds = [], 0, 0
+        while k < 128:
+            ids.append(srv.submit(reqs[k][1], reqs[k][2]))
+            k += 1
+        while True:
+            for _ in range(8):
+                if k < len(reqs):
+                    ids.append(srv.submit(reqs[k][1], reqs[k][2]))
+                    k += 1
+            left = srv.poll(256)
+            rounds += 1
+            if rounds % 25 == 0:
+                print(f"serving mix: round {rounds}, {k} submitted, {left} active or queued", flush=True)
+            if k >= len(reqs) and left == 0:
+                break
+        status = [srv.info(i)["status"] for i in ids]
+        assert status.count(0) == 512, {s: status.count(s) for s in set(status)}
+        kinds = {"1d": lambda sp: sp.dims == 1 and not sp.do_q, "3d": lambda sp: sp.dims == 3,
+                 "do": lambda sp: sp.dims == 1 and sp.do_q > 0}
+        sample = []
+        for name, pred in kinds.items():
+            idx = sorted((r[3], j) for j, r in enumerate(reqs) if pred(r[1]))
+            sample += [(name, j) for _, j in idx[:2]]
+        for name, j in sample:
+            _, spec, u0, _ = reqs[j]
+            solo = fpde.solve(spec, u0)
+            err = rel_err(srv.result(ids[j]), solo)
+            print(f"serving mix: request {j} ({name}, N={spec.nodes}, M={spec.steps}) vs solo {err:.1e}", flush=True)
+            assert err <= 1e-13
 
 
 def test_advance_batch_matches_solo(fpde):
@@ -477,6 +616,32 @@ def test_advance_batch_refuses_what_does_not_fit(fpde):
         fpde.advance_batch(solvers[:1], 0)
         assert solvers[0].step == 6
         np.testing.assert_array_equal(solvers[0].solution(), fpde.solve(spec, u0))
+    finally:
+        for s in solvers:
+            s.close()
+
+
+def test_near_full_batch_beside_a_3d_rider(fpde):
+    """Seven N=9000 solves (252 chunk workgroups: one per CU on 256 CUs) in
+    one batch, advanced together with a 64^3 3D solve on its own stream,
+    which is enqueued first, so its sweeps hold CUs while the cooperative
+    launches are dispatched: every grid barrier completes within its bound
+    and every result equals its solo run."""
+    u0 = fpde.init_1d(9000, 77)
+    specs = [fpde.make_spec(9000, 256, 1e-3, alpha=0.5 + 0.05 * j, hist_mode="split_bf16", picard_max=2,
+                            picard_tol=1e-12, u_top=u0[0]) for j in range(7)]
+    u3 = fpde.init_3d(64, 78)
+    sp3 = fpde.make_spec(64, 120, 1e-3, alpha=0.85, dims=3, hist_mode="mfma_f64", block=16, u_top=u3[0])
+    solvers = [fpde.Solver(sp) for sp in specs] + [fpde.Solver(sp3)]
+    try:
+        for s in solvers[:-1]:
+            s.set_initial(u0)
+        solvers[-1].set_initial(u3)
+        fpde.advance_batch(solvers, 0)
+        assert all(s.step == s.spec.steps for s in solvers)
+        for s, sp in zip(solvers[:-1], specs):
+            np.testing.assert_array_equal(s.solution(), fpde.solve(sp, u0))
+        np.testing.assert_array_equal(solvers[-1].solution(), fpde.solve(sp3, u3))
     finally:
         for s in solvers:
             s.close()
@@ -765,10 +930,15 @@ def test_results_files_in_reference_format(fpde, oracle, tmp_path):
         s.write_results(str(t3), str(raw))
     rows = np.loadtxt(t3)
     assert rows.shape == (n, 8)
-    line = u.reshape(n, n, n)[n // 2, n // 2, :]          # y, x, depth
+    c = (n - 1) // 2                                       # idx(i, _M/2, K/2), _M = K = n - 1
+    line = u.reshape(n, n, n)[c, c, :]                     # y, x, depth
     assert np.allclose(rows[:, 4], line, rtol=6e-6)
-    assert np.allclose(rows[:, 2], (n // 2) / (n - 1)) and np.allclose(rows[:, 3], (n // 2) / (n - 1))
-    assert np.array_equal(np.fromfile(raw, dtype=np.float64), u)
+    assert np.allclose(rows[:, 2], c / (n - 1)) and np.allclose(rows[:, 3], c / (n - 1))
+    # b_U with the halo: (N+2)^3 for N = n - 1, idx = i + (N+2) j + (N+2)^2 k
+    b = np.fromfile(raw, dtype=np.float64).reshape(n + 1, n + 1, n + 1)    # k (y), j (x), i (depth)
+    assert np.array_equal(b[:n, :n, :n].ravel(), u)
+    u03 = u3.reshape(n, n, n)
+    assert np.array_equal(b[n, :n, :n], u03[n - 1]) and np.array_equal(b[:n, :n, n], u03[:, :, n - 1])
 
 
 # ------------------------------------------------- prescribed non-uniform time grid

@@ -97,6 +97,118 @@ def test_reference_dispatch_uses_series_for_two_oldest(oracle):
             assert w == oracle.kb_closed(0.7, T[t0], T[t0 + 1], T[n + 1])
 
 
+# ------------------------------------------- generalized kernel (FK 1/2/3)
+_GP = {1: 0.5, 2: 2.0, 3: 0.9}          # g(t) = t^gp (fpde_hpc_1d.cpp:1889-1898), FP 0.9 for FK 3
+
+
+@pytest.mark.parametrize("fk", [1, 2, 3])
+@pytest.mark.parametrize("alpha", [0.3, 0.85])
+def test_generalized_weight_is_the_integral(oracle, fk, alpha):
+    """b(t0,t1;a) = int (g(T_a) - g(s))^-alpha ds of every kind of interval
+    (oldest, middle, next to the newest, the newest itself) against quad."""
+    gp = _GP[fk]
+    tau = 1e-3
+    for n in (1, 2, 9, 200):
+        T = oracle.time_grid(n + 1, tau)
+        Ga = T[n + 1] ** gp
+        for t in sorted({0, 1, n // 2, n - 1, n}):
+            if t > n:
+                continue
+            p0, p1, pa = T[t], T[t + 1], T[n + 1]
+            if t < n:
+                ref, _ = integrate.quad(lambda s: (Ga - s ** gp) ** -alpha, p0, p1, epsabs=0, epsrel=1e-13, limit=200)
+            else:   # (pa - s)^-alpha times a smooth factor: quad's algebraic weight;
+                # g(pa) - g(s) = -G expm1(p log1p(-(pa - s)/pa)) has no cancellation near s = pa
+                def phi(s):
+                    d = pa - s
+                    if d <= 0.0:
+                        return (gp * pa ** (gp - 1)) ** -alpha
+                    return (-Ga * math.expm1(gp * math.log1p(-d / pa)) / d) ** -alpha
+                ref, _ = integrate.quad(phi, p0, pa, weight="alg", wvar=(0, -alpha), epsabs=0, epsrel=1e-13,
+                                        limit=200)
+            v = oracle.kb_general(alpha, gp, p0, p1, pa)
+            assert abs(v - ref) <= 2e-12 * ref, (n, t)
+
+
+@pytest.mark.parametrize("fk", [1, 2, 3])
+@pytest.mark.parametrize("alpha", [0.25, 0.7])
+def test_generalized_l1_exact_for_linear_in_time(oracle, fk, alpha):
+    """For u = c t every interval is linear, so the history plus the local
+    term is the generalized Caputo derivative itself:
+    c/Gamma(1-a) int_0^T (T^p - s^p)^-a ds = c T^(1-a p) Gamma(1+1/p)/Gamma(1+1/p-a),
+    on a uniform and on a random grid."""
+    gp = _GP[fk]
+    rng = np.random.default_rng(31)
+    n, c = 24, 1.7
+    for taus in (np.full(n + 1, 2e-2), 2e-2 * (0.3 + rng.random(n + 1))):
+        T = np.concatenate([[0.0], np.cumsum(taus)])
+        kkk0, kb3 = oracle.l1_weights_g(alpha, fk, gp if fk == 3 else 0.6, n, T)
+        old = (c * T[: n + 1])[:, None] * np.ones((1, 2))
+        hist = oracle.time_summing(old, kb3, taus)[0]
+        approx = (hist + kkk0 * c) / G(1 - alpha)
+        exact = c * T[n + 1] ** (1 - alpha * gp) * G(1 + 1 / gp) / G(1 + 1 / gp - alpha)
+        assert abs(approx - exact) <= 1e-11 * exact
+
+
+@pytest.mark.parametrize("fk", [1, 2, 3])
+def test_generalized_l1_of_t_squared_tends_to_reference_analytic_form(oracle, fk):
+    """The reference's analytic tests use D_g^a t^2 = Gamma(2/p+1)/Gamma(2/p+1-a)
+    t^(2-a p) for g = t^p (g2fp/g2fpa, fpde_hpc_1d.cpp:2716-2720, :2843): the L1
+    scheme converges to it as tau -> 0."""
+    gp, alpha = _GP[fk], 0.6
+    errs = []
+    for n in (40, 80, 160):
+        tau = 1.0 / (n + 1)
+        T = oracle.time_grid(n + 1, tau)
+        kkk0, kb3 = oracle.l1_weights_g(alpha, fk, 0.9, n, T)
+        u = T ** 2
+        hist = oracle.time_summing(u[: n + 1, None], kb3, np.full(n + 1, tau))[0]
+        approx = (hist + kkk0 * (u[n + 1] - u[n]) / tau) / G(1 - alpha)
+        exact = G(2 / gp + 1) / G(2 / gp + 1 - alpha) * T[n + 1] ** (2 - alpha * gp)
+        errs.append(abs(approx - exact) / exact)
+    assert errs[0] < 2e-2 and errs[1] < 0.7 * errs[0] and errs[2] < 0.7 * errs[1]
+
+
+@pytest.mark.parametrize("fk", [1, 2, 3])
+def test_reference_taylor_series_converges_to_the_integral(oracle, fk):
+    """_kb_row (fpde_hpc_1d.cpp:1933-1996) runs the Taylor series of f = g^-1
+    through inv_g_der (:1901-1930); with eps -> 0 it reaches the integral on
+    intervals away from 0.  For g = sqrt(t), f = t^2 and the series has two
+    terms: it is exact at the reference's eps = 1e-5 already."""
+    gp, alpha, tau = _GP[fk], 0.7, 1e-3
+    for n, t in [(5, 2), (40, 20), (40, 38), (300, 150)]:
+        T = oracle.time_grid(n + 1, tau)
+        gtj = T[n + 1] ** gp
+        exact = oracle.kb_general(alpha, gp, T[t], T[t + 1], T[n + 1])
+        v, _ = oracle.kb_row_taylor(alpha, fk, 0.9, T[t], T[t + 1], gtj, eps=1e-18, max_niter=200000)
+        assert abs(v - exact) <= 1e-9 * exact, (n, t)
+        if fk == 1:
+            v5, it = oracle.kb_row_taylor(alpha, fk, 0.9, T[t], T[t + 1], gtj, eps=1e-5)
+            assert abs(v5 - exact) <= 1e-12 * exact and it <= 2       # f''' = 0
+
+
+def test_generalized_linear_solve_converges_to_mittag_leffler(oracle):
+    """K = 1/C = 1 with g = t^0.9: the generalized (psi-) Caputo derivative
+    has the eigenfunction E_a(lambda g(t)^a), so u0 = u_top + A sin(pi z/2)
+    gives u = u_top + A E_a(-(pi/2)^2 g(T)^a) sin(pi z/2)."""
+    alpha, A, ut, Tend, gp = 0.6, 0.1, -1.0, 0.2, 0.9
+    errs = []
+    for nodes, steps in ((81, 100), (161, 200)):
+        z = np.linspace(0, 1, nodes)
+        u0 = ut + A * np.sin(np.pi * z / 2)
+        p = oracle.make_params(nodes, steps, Tend / steps, alpha, coef_mode=1, weight_mode=0, fk=3, fp=gp)
+        u = oracle.solve_1d(p, u0)
+        exact = ut + A * _mittag_leffler(alpha, -(np.pi / 2) ** 2 * (Tend ** gp) ** alpha) * np.sin(np.pi * z / 2)
+        errs.append(np.max(np.abs(u - exact)))
+    assert errs[1] < 0.6 * errs[0] and errs[1] < 2e-3
+
+
+def test_generalized_kernel_needs_exact_weights(oracle):
+    p = oracle.make_params(10, 3, 1e-3, 0.5, coef_mode=1, weight_mode=1, fk=3, fp=0.9)
+    with pytest.raises(RuntimeError):
+        oracle.solve_1d(p, np.zeros(10))
+
+
 # ---------------------------------------------------------------- history sum
 def test_time_summing_brute_force(oracle):
     rng = np.random.default_rng(3)
@@ -657,3 +769,47 @@ def test_3d_space_fractional_lateral_sweeps_are_symmetric(oracle):
     assert np.max(np.abs(rx - ry.transpose(1, 0, 2))) <= 1e-13
     r0 = oracle.solve_3d(oracle.make_params(n, 1, 1e-3, 0.85, coef_mode=1), fx)
     assert np.max(np.abs(rx.ravel() - r0)) >= 1e-4
+
+
+# ------------------------------------------- fr_do.cpp distributed-order equation
+def test_do_equation_converges_to_manufactured_solution(oracle):
+    """eq_solve's Crank-Nicolson scheme for int_0^1 Gamma(5-a) D^a u da =
+    2 u_xx + f on [0, 0.5]^2 (fr_do.cpp:733-775) against its exact solution
+    100 t^4 sin(2 pi x) (eq_anal_u, the source of :458-465): the RMS error of
+    the last level falls with refinement."""
+    errs = []
+    for nx, nt in ((32, 160), (64, 320), (128, 640)):
+        _, err = oracle.doeq_solve(oracle.doeq_params(nx=nx, na=64, tblock=32, nt=nt))
+        errs.append(err)
+    assert errs[0] < 0.012 and errs[1] < 0.7 * errs[0] and errs[2] < 0.7 * errs[1]
+
+
+def test_do_equation_block_scheme_is_exact(oracle):
+    """The block scheme (history before the block at its start, the in-block
+    intervals added level by level, :739-761) is a reordering of the same
+    sums: blocks of 16 and of 64 levels give the same solution."""
+    for ex in (0, 1):
+        a, _ = oracle.doeq_solve(oracle.doeq_params(nx=40, na=33, tblock=16, nt=192, example=ex))
+        b, _ = oracle.doeq_solve(oracle.doeq_params(nx=40, na=33, tblock=64, nt=192, example=ex))
+        assert np.max(np.abs(a - b)) <= 1e-12 * np.max(np.abs(a))
+        assert np.max(np.abs(a[:, -1])) > 0.0
+
+
+def test_frdo_derivative_of_t4_is_the_closed_form(oracle):
+    """DO_Cder with eq_w = Gamma(5-a): int_0^1 Gamma(5-a) D^a t^4 da =
+    24 (t^4 - t^3)/ln t (example 0's source term, fr_do.cpp:458-465)."""
+    n, T = 2000, 0.5
+    tau = T / (n - 1)
+    t = np.arange(n) * tau
+    DF = oracle.do_cder(t ** 4, tau, 128, example=0)
+    exact = 24 * (T ** 4 - T ** 3) / np.log(T)
+    assert abs(DF[-1] - exact) <= 2e-3 * abs(exact)
+
+
+def test_do_equation_gpu_text_source(oracle):
+    """The GPU program text forms the source with pi as a float (eq_ff,
+    :256-263): the solution moves by no more than that rounding."""
+    a, _ = oracle.doeq_solve(oracle.doeq_params(nx=32, na=32, tblock=32, nt=160))
+    b, _ = oracle.doeq_solve(oracle.doeq_params(nx=32, na=32, tblock=32, nt=160, gpu_pi=1))
+    d = np.max(np.abs(a - b)) / np.max(np.abs(a))
+    assert 0.0 < d < 1e-6
