@@ -226,7 +226,7 @@ struct Step1DArgs {
     int n0;                   // first step of the history block (far partial row = n - n0)
     unsigned* flagbar;        // Picard barrier flags [2 parities][kMaxChunks][32] (128 B per flag)
     int epar;                 // parity of this cooperative launch
-    int* err;                 // set when a grid barrier times out
+    int* err;                 // [0] set when a grid barrier times out; [1] Picard solves (cooperative launches)
     int64_t wait_ticks;       // a grid barrier gives up after this many ticks of the device's wall clock
     // series-compressed history (ser_P > 0): moments H [P][ld], coefficient
     // rows of step n; D is then a ring of two levels (row t & 1)
@@ -359,8 +359,12 @@ fpde_status launch_ref_sum(double* S, const double* old, int64_t old_size, int64
 fpde_status launch_ref_series(double* S, const double* old, int64_t old_size, int64_t os, int64_t nodes,
                               const double* tau, int tstep, double* Htmp, const double* coef, int P, double gtj,
                               double gt, double alpha, hipStream_t st);
+// The GEMM launchers write K-split partial rows P[ks][r][i], ks < *splits_out;
+// the readers add them in split order: the 3D history combine itself, the 1D
+// step kernels after launch_part_reduce has folded them into P[0].
 fpde_status launch_gemm_f64(const double* D, int64_t ld, int nodes, int n0, int tc, int block,
                             const double* omega, double* P, int* splits_out, hipStream_t st);
+fpde_status launch_part_reduce(double* P, int splits, int rows, int64_t ld, int nodes, hipStream_t st);
 fpde_status launch_pack_split(int mode, const double* D, int64_t ld, int nodes, int g_lo, int g_hi, void* c0,
                               void* c1, void* c2, float* scale, hipStream_t st);
 fpde_status launch_gemm_f64_dense(const double* D, int64_t ld, int nodes, int tc, int block, const double* W,
@@ -378,6 +382,39 @@ fpde_status launch_step1d(const Step1DArgs& a, hipStream_t st);
 fpde_status launch_step1d_block(const Step1DArgs& a, hipStream_t st);
 bool step1d_block_supported(int C);
 bool step1d_batch_supported(int64_t wgs);
+// Batched 3D LOD step: one step of up to kMax3DBatch 3D solves (n <= 128,
+// partitioned sweeps) in one launch per kernel of the step (history combine,
+// coefficients, y sweep, coefficients, x sweep, z sweep).  Workgroups
+// [first_*[j], first_*[j+1]) of each launch belong to solve j; each solve's
+// row holds what its single-solve launches take.
+constexpr int kMax3DBatch = 8;
+struct Lod3DRow {
+    double *U, *K, *iC, *S, *Uprev, *Dlev;
+    const double *part, *D, *omega, *corr, *utop;
+    int64_t part_stride, part_off, ld, nodes;
+    int n, nsplit, step, near_lo, coef_mode, split;
+    double h, h2inv, ksat, L, hscale;
+    // the space-fractional arguments of the solo sweeps (kap = 1, no sums
+    // here), passed as run-time values so that the batched kernels compile
+    // to the solo kernels' arithmetic: a literal 1.0 or null would let the
+    // compiler fuse products into FMAs differently, changing the rounding
+    double kap;
+    const double *E, *Kz, *iCz;
+};
+struct Lod3DBatch {
+    int count;
+    int first_node[kMax3DBatch + 1];   // 256-node blocks (history combine, coefficients)
+    int first_lat[kMax3DBatch + 1];    // 32-line blocks (lateral sweeps)
+    int first_z[kMax3DBatch + 1];      // 16-line blocks (depth sweep)
+    Lod3DRow row[kMax3DBatch];
+};
+fpde_status launch_lod3d_batch(const Lod3DBatch& b, hipStream_t st);
+constexpr int kZLinesHost = 16;      // depth lines per workgroup of the partitioned z sweep
+bool solver_3d_batchable(const fpde_solver* s);
+fpde_status advance_3d_batch(fpde_solver* const* v, int count, int nsteps);
+// 3D solves in batched LOD steps (groups of kMax3DBatch), the rest on their own streams
+fpde_status advance_riders(fpde_solver* const* v, int count, int nsteps);
+
 // One launch of fr_do.cpp's program kernels (frdo.hip), decoded by the facade.
 enum { FRDO_FILL = 0, FRDO_DOC, FRDO_SUM, FRDO_EQ_F, FRDO_EQ_SOLVE };
 struct FrdoLaunch {
@@ -441,6 +478,5 @@ fpde_status launch_hist_combine(double* S, const double* part, int nsplit, int64
                                 const double* D, int64_t ld, int64_t nodes, int n, int near_lo,
                                 const double* omega, const double* corr, double hscale, hipStream_t st);
 int gemm_max_splits(int64_t nodes);
-fpde_status launch_part_reduce(double* P, int splits, int rows, int64_t ld, int nodes, hipStream_t st);
 
 }  // namespace fpde

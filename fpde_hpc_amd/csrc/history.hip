@@ -212,13 +212,14 @@ __global__ __launch_bounds__(256) void k_gemm_f64(const double* __restrict__ D, 
             }
         }
     }
-    if (!node_ok) return;
+    if (node_ok) {
 #pragma unroll
-    for (int rt = 0; rt < RT; rt++) {
+        for (int rt = 0; rt < RT; rt++) {
 #pragma unroll
-        for (int v = 0; v < 4; v++) {
-            int r = rt * 16 + 4 * v + (lane >> 4);
-            P[((int64_t)ks * BR + r) * ld + node] = acc[rt][v];
+            for (int v = 0; v < 4; v++) {
+                int r = rt * 16 + 4 * v + (lane >> 4);
+                P[((int64_t)ks * BR + r) * ld + node] = acc[rt][v];
+            }
         }
     }
 }
@@ -272,13 +273,14 @@ __global__ __launch_bounds__(256) void k_gemm_f64_dense(const double* __restrict
             }
         }
     }
-    if (!node_ok) return;
+    if (node_ok) {
 #pragma unroll
-    for (int rt = 0; rt < RT; rt++) {
+        for (int rt = 0; rt < RT; rt++) {
 #pragma unroll
-        for (int v = 0; v < 4; v++) {
-            int r = rt * 16 + 4 * v + (lane >> 4);
-            P[((int64_t)ks * BR + r) * ld + node] = acc[rt][v];
+            for (int v = 0; v < 4; v++) {
+                int r = rt * 16 + 4 * v + (lane >> 4);
+                P[((int64_t)ks * BR + r) * ld + node] = acc[rt][v];
+            }
         }
     }
 }
@@ -521,34 +523,46 @@ __global__ __launch_bounds__(256) void k_gemm_split(const T* __restrict__ c0, co
             }
         }
     }
-    if (!node_ok) return;
+    if (node_ok) {
 #pragma unroll
-    for (int rt = 0; rt < RT; rt++) {
+        for (int rt = 0; rt < RT; rt++) {
 #pragma unroll
-        for (int v = 0; v < 4; v++) {
-            int r = rt * 16 + 4 * (lane >> 4) + v;
-            P[((int64_t)ks * BR + r) * ld + node] = acc[rt][v];
+            for (int v = 0; v < 4; v++) {
+                int r = rt * 16 + 4 * (lane >> 4) + v;
+                P[((int64_t)ks * BR + r) * ld + node] = acc[rt][v];
+            }
         }
     }
 }
 
 // ---------------------------------------------------------------------------
-// Fold the K-split partial rows of a GEMM block into split 0, in place:
-// P[0][r][i] += sum_{ks>=1} P[ks][r][i] (fixed order, deterministic), so a
-// step reads one far value per node.
+// Fold the K-split partial rows of a GEMM block into split 0, in place and in
+// split order: P[0][r][i] = ((P[0] + P[1]) + P[2]) + ..., so a 1D step reads
+// one far value per node.  The splits of one (r, i) load as one batch of
+// independent loads.  (Folding inside the GEMM by its last-arriving
+// workgroup per tile needs an agent-scope release per workgroup, an L2
+// write-back on gfx950, and measured slower than this pass.)
 __global__ __launch_bounds__(256) void k_part_reduce(double* __restrict__ P, int splits, int rows,
                                                      int64_t ld, int nodes) {
-    int i = blockIdx.x * 256 + threadIdx.x;
-    int r = blockIdx.y;
+    constexpr int W = 16;
+    const int i = blockIdx.x * 256 + threadIdx.x;
+    const int r = blockIdx.y;
     if (i >= nodes || r >= rows) return;
     double s = P[(int64_t)r * ld + i];
-    for (int ks = 1; ks < splits; ks++) s += P[((int64_t)ks * rows + r) * ld + i];
+    for (int k0 = 1; k0 < splits; k0 += W) {
+        double v[W];
+#pragma unroll
+        for (int j = 0; j < W; j++) v[j] = (k0 + j < splits) ? P[((int64_t)(k0 + j) * rows + r) * ld + i] : 0.0;
+#pragma unroll
+        for (int j = 0; j < W; j++)
+            if (k0 + j < splits) s += v[j];
+    }
     P[(int64_t)r * ld + i] = s;
 }
 
 fpde_status launch_part_reduce(double* P, int splits, int rows, int64_t ld, int nodes, hipStream_t st) {
-    if (splits <= 1) return FPDE_OK;
-    dim3 grid((nodes + 255) / 256, rows);
+    if (splits <= 1 || nodes <= 0) return FPDE_OK;
+    dim3 grid((unsigned)((nodes + 255) / 256), (unsigned)rows);
     hipLaunchKernelGGL(k_part_reduce, grid, dim3(256), 0, st, P, splits, rows, ld, nodes);
     return FPDE_OK;
 }

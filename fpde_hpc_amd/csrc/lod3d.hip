@@ -19,10 +19,9 @@
 namespace fpde {
 
 // K and 1/C of every node at the current field.
-__global__ __launch_bounds__(256) void k_coef3d(const double* __restrict__ U, int n, double h,
-                                                double ksat, int mode, double* __restrict__ K,
-                                                double* __restrict__ iC) {
-    int64_t id = (int64_t)blockIdx.x * 256 + threadIdx.x;
+__device__ __forceinline__ void coef3d_body(const double* __restrict__ U, int n, double h, double ksat, int mode,
+                                            double* __restrict__ K, double* __restrict__ iC, int bid) {
+    int64_t id = (int64_t)bid * 256 + threadIdx.x;
     int64_t nn = (int64_t)n * n * n;
     if (id >= nn) return;
     int z = (int)(id % n);
@@ -30,6 +29,11 @@ __global__ __launch_bounds__(256) void k_coef3d(const double* __restrict__ U, in
     vgm_coefs(U[id], z * h, ksat, mode, k, ic);
     K[id] = k;
     iC[id] = ic;
+}
+__global__ __launch_bounds__(256) void k_coef3d(const double* __restrict__ U, int n, double h,
+                                                double ksat, int mode, double* __restrict__ K,
+                                                double* __restrict__ iC) {
+    coef3d_body(U, n, h, ksat, mode, K, iC, blockIdx.x);
 }
 
 // Lateral sweep, thread per line.  axis 1 = x (stride n), 2 = y (stride n*n).
@@ -202,15 +206,14 @@ __global__ __launch_bounds__(256) void k_zline3d(double* __restrict__ U, double*
 // than that absolute amount.
 constexpr int kLatSeg = 8;
 template <int S>
-__global__ __launch_bounds__(256, 2) void k_lateral3d_seg(double* __restrict__ U, const double* __restrict__ K,
-                                                      const double* __restrict__ iC,
-                                                      const double* __restrict__ H, int n, int axis,
-                                                      double h2inv, double L, double kap,
-                                                      const double* __restrict__ E) {
+__device__ __forceinline__ void lateral3d_seg_body(double* __restrict__ U, const double* __restrict__ K,
+                                                   const double* __restrict__ iC, const double* __restrict__ H,
+                                                   int n, int axis, double h2inv, double L, double kap,
+                                                   const double* __restrict__ E, int bid) {
     __shared__ double sEnd[kLatSeg][32][7];    // x, y, z of the first and the last row
     const int tid = threadIdx.x;
     const int l = tid & 31, sg = tid >> 5;
-    const int line = blockIdx.x * 32 + l;
+    const int line = bid * 32 + l;
     const bool ok_line = line < n * n;
     const int ln = ok_line ? line : 0;
     const int z0 = ln % n, o = ln / n;
@@ -299,6 +302,15 @@ __global__ __launch_bounds__(256, 2) void k_lateral3d_seg(double* __restrict__ U
         if (i <= hi) U[base + stride * i] = xs[j] + ys[j] * vl + cp[j] * vr;
     }
 }
+template <int S>
+__global__ __launch_bounds__(256, 2) void k_lateral3d_seg(double* __restrict__ U, const double* __restrict__ K,
+                                                      const double* __restrict__ iC,
+                                                      const double* __restrict__ H, int n, int axis,
+                                                      double h2inv, double L, double kap,
+                                                      const double* __restrict__ E) {
+    lateral3d_seg_body<S>(U, K, iC, H, n, axis, h2inv, L, kap, E, blockIdx.x);
+}
+
 
 // Depth (z) sweep as a partitioned solve over 16 contiguous lines per
 // workgroup: the lines' U, their VGM coefficients at U and the right-hand
@@ -309,20 +321,19 @@ __global__ __launch_bounds__(256, 2) void k_lateral3d_seg(double* __restrict__ U
 // U_0 = u_top, the rows of A_/B/R/Om, zero-flux bottom U_{n-1} = U_{n-2};
 // the new U, D[n] = U - Uprev and Uprev go out through LDS, coalesced.
 constexpr int kZSeg = 16, kZLines = 16, kZThreads = kZSeg * kZLines;
-__global__ __launch_bounds__(kZThreads, 2) void k_zline3d_seg(double* __restrict__ U, double* __restrict__ Uprev,
-                                                       const double* __restrict__ utop, int n, double h,
-                                                       double h2inv, double ksat, int mode, double L,
-                                                       double* __restrict__ Dlev, double kap,
-                                                       const double* __restrict__ E,
-                                                       const double* __restrict__ Kin,
-                                                       const double* __restrict__ iCin,
-                                                       const double* __restrict__ H) {
+static_assert(kZLines == kZLinesHost, "the host's block count of the depth sweep");
+__device__ __forceinline__ void zline3d_seg_body(double* __restrict__ U, double* __restrict__ Uprev,
+                                                 const double* __restrict__ utop, int n, double h, double h2inv,
+                                                 double ksat, int mode, double L, double* __restrict__ Dlev,
+                                                 double kap, const double* __restrict__ E,
+                                                 const double* __restrict__ Kin, const double* __restrict__ iCin,
+                                                 const double* __restrict__ H, int bid) {
     constexpr int LD = 129;                      // odd row stride: the 16 lines hit distinct banks
     __shared__ double sK[kZLines][LD], sIC[kZLines][LD], sD[kZLines][LD];
     __shared__ double sEnd[kZSeg][kZLines][7];
     const int tid = threadIdx.x;
     const int nlines = n * n;
-    const int line0 = blockIdx.x * kZLines;
+    const int line0 = bid * kZLines;
     const int nl = min(kZLines, nlines - line0);        // lines of this group
     const int64_t base0 = (int64_t)n * line0;
     // stage: K, 1/C and the right-hand side -L U - E/C of every node (four
@@ -428,15 +439,26 @@ __global__ __launch_bounds__(kZThreads, 2) void k_zline3d_seg(double* __restrict
         Uprev[id] = un;
     }
 }
+__global__ __launch_bounds__(kZThreads, 2) void k_zline3d_seg(double* __restrict__ U, double* __restrict__ Uprev,
+                                                       const double* __restrict__ utop, int n, double h,
+                                                       double h2inv, double ksat, int mode, double L,
+                                                       double* __restrict__ Dlev, double kap,
+                                                       const double* __restrict__ E,
+                                                       const double* __restrict__ Kin,
+                                                       const double* __restrict__ iCin,
+                                                       const double* __restrict__ H) {
+    zline3d_seg_body(U, Uprev, utop, n, h, h2inv, ksat, mode, L, Dlev, kap, E, Kin, iCin, H, blockIdx.x);
+}
+
 
 // S[i] = sum of far partials + near levels [near_lo, n) + two-level corrections,
 // times hscale (1, or 1/3 when every sweep carries a third of the term).
-__global__ __launch_bounds__(256) void k_hist_combine(double* __restrict__ S, const double* __restrict__ part,
-                                                      int nsplit, int64_t part_stride, int64_t part_off,
-                                                      const double* __restrict__ D, int64_t ld, int64_t nodes,
-                                                      int n, int near_lo, const double* __restrict__ omega,
-                                                      const double* __restrict__ corr, double hscale) {
-    int64_t i = (int64_t)blockIdx.x * 256 + threadIdx.x;
+__device__ __forceinline__ void hist_combine_body(double* __restrict__ S, const double* __restrict__ part,
+                                                  int nsplit, int64_t part_stride, int64_t part_off,
+                                                  const double* __restrict__ D, int64_t ld, int64_t nodes, int n,
+                                                  int near_lo, const double* __restrict__ omega,
+                                                  const double* __restrict__ corr, double hscale, int bid) {
+    int64_t i = (int64_t)bid * 256 + threadIdx.x;
     if (i >= nodes) return;
     constexpr int W = 16;      // independent loads per batch
     double far = 0.0;
@@ -468,6 +490,14 @@ __global__ __launch_bounds__(256) void k_hist_combine(double* __restrict__ S, co
     if (n >= 2) cr = fma(corr[2 * n + 1], D[ld + i], cr);
     S[i] = (far + (nr0 + nr1) + cr) * hscale;
 }
+__global__ __launch_bounds__(256) void k_hist_combine(double* __restrict__ S, const double* __restrict__ part,
+                                                      int nsplit, int64_t part_stride, int64_t part_off,
+                                                      const double* __restrict__ D, int64_t ld, int64_t nodes,
+                                                      int n, int near_lo, const double* __restrict__ omega,
+                                                      const double* __restrict__ corr, double hscale) {
+    hist_combine_body(S, part, nsplit, part_stride, part_off, D, ld, nodes, n, near_lo, omega, corr, hscale,
+                      blockIdx.x);
+}
 
 // Series-compressed history (sum_alg 2): add the newest interval dnew to the
 // moments and sum the series, S[i] = sum_j v_j (H[j][i] += c_j dnew[i]);
@@ -481,6 +511,62 @@ __global__ __launch_bounds__(256) void k_series3d(double* __restrict__ S, double
     if (i >= nodes) return;
     if (!dnew) { S[i] = 0.0; return; }
     S[i] = series_update(H, ld, i, P, dnew[i], ser_c, ser_v) * hscale;
+}
+
+// ---------------------------------------------------------------------------
+// Batched step of several 3D solves: the same bodies, each workgroup mapped to
+// (solve j, its local block) by the prefix counts of the table.
+__device__ inline int batch_slot(const int* first, int count, int bid) {
+    int j = 0;
+    while (j + 1 < count && bid >= first[j + 1]) j++;
+    return j;
+}
+
+__global__ __launch_bounds__(256) void k_hist_combine_batch(Lod3DBatch b) {
+    const int j = batch_slot(b.first_node, b.count, blockIdx.x);
+    const Lod3DRow& r = b.row[j];
+    hist_combine_body(r.S, r.part, r.nsplit, r.part_stride, r.part_off, r.D, r.ld, r.nodes, r.step, r.near_lo,
+                      r.omega, r.corr, r.hscale, (int)blockIdx.x - b.first_node[j]);
+}
+
+__global__ __launch_bounds__(256) void k_coef3d_batch(Lod3DBatch b) {
+    const int j = batch_slot(b.first_node, b.count, blockIdx.x);
+    const Lod3DRow& r = b.row[j];
+    coef3d_body(r.U, r.n, r.h, r.ksat, r.coef_mode, r.K, r.iC, (int)blockIdx.x - b.first_node[j]);
+}
+
+// y sweep (axis 2) carries the history term; the x sweep (axis 1) a third of
+// it with split3d odd
+__global__ __launch_bounds__(256, 2) void k_lateral3d_batch(Lod3DBatch b, int axis) {
+    const int j = batch_slot(b.first_lat, b.count, blockIdx.x);
+    const Lod3DRow& r = b.row[j];
+    lateral3d_seg_body<16>(r.U, r.K, r.iC, (axis == 2 || r.split) ? r.S : nullptr, r.n, axis, r.h2inv, r.L, r.kap,
+                           r.E, (int)blockIdx.x - b.first_lat[j]);
+}
+
+__global__ __launch_bounds__(kZThreads, 2) void k_zline3d_batch(Lod3DBatch b) {
+    const int j = batch_slot(b.first_z, b.count, blockIdx.x);
+    const Lod3DRow& r = b.row[j];
+    zline3d_seg_body(r.U, r.Uprev, r.utop, r.n, r.h, r.h2inv, r.ksat, r.coef_mode, r.L, r.Dlev, r.kap, r.E,
+                     r.Kz, r.iCz, r.split ? r.S : nullptr, (int)blockIdx.x - b.first_z[j]);
+}
+
+fpde_status launch_lod3d_batch(const Lod3DBatch& b, hipStream_t st) {
+    if (b.count < 1 || b.count > kMax3DBatch) { set_error("batched 3D step: 1 to 8 solves"); return FPDE_ERR_ARG; }
+    for (int j = 0; j < b.count; j++)
+        if (b.row[j].n > 16 * kLatSeg || !(b.row[j].L > 0.0)) {
+            set_error("batched 3D step: partitioned sweeps only (n <= 128)");
+            return FPDE_ERR_ARG;
+        }
+    const int gn = b.first_node[b.count], gl = b.first_lat[b.count], gz = b.first_z[b.count];
+    hipLaunchKernelGGL(k_hist_combine_batch, dim3(gn), dim3(256), 0, st, b);
+    hipLaunchKernelGGL(k_coef3d_batch, dim3(gn), dim3(256), 0, st, b);
+    hipLaunchKernelGGL(k_lateral3d_batch, dim3(gl), dim3(256), 0, st, b, 2);
+    hipLaunchKernelGGL(k_coef3d_batch, dim3(gn), dim3(256), 0, st, b);
+    hipLaunchKernelGGL(k_lateral3d_batch, dim3(gl), dim3(256), 0, st, b, 1);
+    hipLaunchKernelGGL(k_zline3d_batch, dim3(gz), dim3(kZThreads), 0, st, b);
+    FPDE_HIP_TRY(hipGetLastError());
+    return FPDE_OK;
 }
 
 fpde_status launch_series3d(double* S, double* H, int P, const double* dnew, const double* ser_c,

@@ -242,11 +242,19 @@ fpde_status fpde_server_cancel(fpde_server* srv, int64_t id) {
 int fpde_server_poll(fpde_server* srv, int quantum) {
     if (!srv) return -1;
     admit(srv);
-    std::vector<Request*> coop;
+    std::vector<Request*> coop, riders;
     for (Request* r : srv->active) {
         if (fpde_solver_step(r->solver) >= r->spec.steps) continue;
         if (solver_batchable(r->solver)) { coop.push_back(r); continue; }
+        if (solver_3d_batchable(r->solver)) { riders.push_back(r); continue; }
         if (fpde_solver_advance(r->solver, quantum) != FPDE_OK) r->status = 3;
+    }
+    // the 3D requests in batched LOD steps (groups of kMax3DBatch)
+    for (size_t g0 = 0; g0 < riders.size(); g0 += kMax3DBatch) {
+        std::vector<fpde_solver*> grp;
+        for (size_t i = g0; i < riders.size() && i < g0 + kMax3DBatch; i++) grp.push_back(riders[i]->solver);
+        if (advance_3d_batch(grp.data(), (int)grp.size(), quantum) != FPDE_OK)
+            for (size_t i = g0; i < riders.size() && i < g0 + kMax3DBatch; i++) riders[i]->status = 3;
     }
     // the cooperative 1D solves, in groups whose chunk workgroups fit on the
     // GPU together: one batched launch per history block for each group (the

@@ -125,6 +125,7 @@ size_t footprint_bytes(const fpde_spec& s) {
              sizeof(double) * (size_t)(p.levels / kGroup) * (s.block / 16);
     }
     b += sizeof(double) * (size_t)p.part_rows * p.ld;            // partial sums
+    b += sizeof(unsigned) * (size_t)(p.ld / 64);                  // K-split arrival counters
     b += sizeof(double) * (size_t)p.ld * 8;                      // U, S, xyz, ...
     if (s.dims == 3) b += sizeof(double) * (size_t)p.ld * 6;     // Uprev, K, iC, Al, Bt
     b += sizeof(double) * (size_t)(s.steps + 512) * 3;           // weights
@@ -218,6 +219,7 @@ struct fpde_solver {
     std::vector<EvPair> evs;
     size_t ev_used = 0;
     fpde_stats stats{};
+    int picard_dev_base = 0;      // err[1] (Picard solves of cooperative launches) at the last reset
     std::vector<void*> allocs;
     // server-owned memory and stream (solver_create_in); else its own
     DeviceArena* arena = nullptr;
@@ -491,7 +493,9 @@ fpde_status far_part(fpde_solver* s, int n, int* nsplit, int64_t* stride, int64_
             rc = launch_gemm_f64(s->D + (size_t)t_lo * p.ld, p.ld, (int)p.nodes, n - t_lo, tc - t_lo, B, s->w.omega,
                                  s->part, &s->gsplits, s->stream);
         }
-        if (rc == FPDE_OK && s->gsplits > 1) {
+        // 1D: the step kernels read one far value per node (K splits folded
+        // here); 3D: the history combine adds the splits itself
+        if (rc == FPDE_OK && sp.dims == 1 && s->gsplits > 1) {
             rc = launch_part_reduce(s->part, s->gsplits, B, p.ld, (int)p.nodes, s->stream);
             s->gsplits = 1;
         }
@@ -684,8 +688,7 @@ fpde_status launch_1d_block(fpde_solver* s, int n, int n_end, int n0, int nsplit
     s->launches++;
     s->coop_launches++;
     s->stats.step_launches++;
-    s->stats.picard_iters_total += (int)((int64_t)(n_end - n) * sp.picard_max);
-    count_steps(s, n, n_end, near_lo);
+    count_steps(s, n, n_end, near_lo);      // Picard iterations: counted on the device (err[1])
     return FPDE_OK;
 }
 
@@ -803,6 +806,114 @@ namespace fpde {
 // chunk workgroups (the server groups such solvers into batches).
 bool solver_batchable(const fpde_solver* s) { return s && s->spec.dims == 1 && s->block_mode; }
 int solver_chunks(const fpde_solver* s) { return s ? s->C : 0; }
+
+// 3D solves that take the batched LOD step: the partitioned sweeps (n <= 128),
+// the full or restricted history (not the series moments), classical diffusion.
+bool solver_3d_batchable(const fpde_solver* s) {
+    return s && s->spec.dims == 3 && s->spec.nodes <= 128 && s->w.ser_P == 0 && !s->w.sp_on && s->w.local > 0.0;
+}
+
+// Up to kMax3DBatch such solves advanced by nsteps (<= 0: to the end) in
+// lockstep: per step, each solve's far part (its history GEMM when a block
+// starts) and then one launch per kernel of the LOD step for all of them, on
+// the first solve's stream.  Asynchronous: every solve's own stream is made to
+// wait for the batch, so fpde_solver_sync / query cover it.
+fpde_status advance_3d_batch(fpde_solver* const* v, int count, int nsteps) {
+    if (count < 1 || count > kMax3DBatch) { set_error("3D batch: 1 to 8 solvers"); return FPDE_ERR_ARG; }
+    std::vector<int> end(count);
+    for (int j = 0; j < count; j++) {
+        if (!solver_3d_batchable(v[j]) || !v[j]->initialized) { set_error("3D batch: solver not batchable"); return FPDE_ERR_ARG; }
+        end[j] = nsteps <= 0 ? v[j]->spec.steps : std::min(v[j]->spec.steps, v[j]->step + nsteps);
+    }
+    hipStream_t bst = v[0]->stream;
+    std::vector<hipEvent_t> ev(count + 1, nullptr);
+    fpde_status rc = FPDE_OK;
+    auto hip_ok = [&](hipError_t e) {
+        if (e == hipSuccess) return true;
+        set_error(std::string("3D batch: ") + hipGetErrorString(e));
+        rc = FPDE_ERR_HIP;
+        return false;
+    };
+    for (int j = 0; j <= count && rc == FPDE_OK; j++) hip_ok(hipEventCreateWithFlags(&ev[j], hipEventDisableTiming));
+    // the batch follows each solve's earlier work
+    for (int j = 1; j < count && rc == FPDE_OK; j++)
+        if (v[j]->stream != bst && hip_ok(hipEventRecord(ev[j], v[j]->stream))) hip_ok(hipStreamWaitEvent(bst, ev[j], 0));
+    while (rc == FPDE_OK) {
+        Lod3DBatch b{};
+        int k = 0;
+        for (int j = 0; j < count && rc == FPDE_OK; j++) {
+            fpde_solver* s = v[j];
+            const int n = s->step;
+            if (n >= end[j]) continue;
+            const fpde_spec& sp = s->spec;
+            const Plan& p = s->plan;
+            int nsplit = 0, near_lo = 0;
+            int64_t stride = 0, off = 0;
+            hipStream_t own = s->stream;
+            s->stream = bst;                       // the far part goes on the batch's stream
+            rc = far_part(s, n, &nsplit, &stride, &off, &near_lo);
+            s->stream = own;
+            if (rc != FPDE_OK) break;
+            Lod3DRow& r = b.row[k];
+            r.U = s->U; r.K = s->K; r.iC = s->iC; r.S = s->S; r.Uprev = s->Uprev;
+            r.Dlev = s->D + (size_t)n * p.ld;
+            r.part = s->part; r.D = s->D;
+            r.omega = (s->w.grid && s->omega_cur) ? s->omega_cur : s->w.omega;
+            r.corr = s->w.corr; r.utop = s->utop;
+            r.part_stride = stride; r.part_off = off; r.ld = p.ld; r.nodes = p.nodes;
+            r.n = sp.nodes; r.nsplit = nsplit; r.step = n; r.near_lo = near_lo; r.coef_mode = sp.coef_mode;
+            r.split = (sp.split3d & 1) ? 1 : 0;
+            r.h = s->h; r.h2inv = 1.0 / (s->h * s->h); r.ksat = sp.ksat;
+            r.L = (s->w.grid && n < sp.steps) ? s->w.hLv[n] : s->w.local;
+            r.hscale = r.split ? 1.0 / 3.0 : 1.0;
+            r.kap = 1.0; r.E = nullptr; r.Kz = nullptr; r.iCz = nullptr;
+            const int lines = sp.nodes * sp.nodes;
+            b.first_node[k + 1] = b.first_node[k] + (int)((p.nodes + 255) / 256);
+            b.first_lat[k + 1] = b.first_lat[k] + (lines + 31) / 32;
+            b.first_z[k + 1] = b.first_z[k] + (lines + kZLinesHost - 1) / kZLinesHost;
+            k++;
+        }
+        if (rc != FPDE_OK || k == 0) break;
+        b.count = k;
+        rc = launch_lod3d_batch(b, bst);
+        if (rc != FPDE_OK) break;
+        for (int j = 0; j < count; j++) {
+            fpde_solver* s = v[j];
+            if (s->step >= end[j]) continue;
+            s->step++;
+            s->stats.step_launches += 6;
+            s->stats.picard_iters_total++;
+        }
+    }
+    // each solve's own stream waits for the batch
+    if (rc == FPDE_OK && hip_ok(hipEventRecord(ev[count], bst)))
+        for (int j = 1; j < count && rc == FPDE_OK; j++)
+            if (v[j]->stream != bst) hip_ok(hipStreamWaitEvent(v[j]->stream, ev[count], 0));
+    for (hipEvent_t e : ev)
+        if (e) hipEventDestroy(e);
+    return rc;
+}
+
+// Advance the 3D solves among `v` that take the batched step, in groups of
+// kMax3DBatch, and every other solve on its own stream (asynchronous).
+fpde_status advance_riders(fpde_solver* const* v, int count, int nsteps) {
+    std::vector<fpde_solver*> grp;
+    for (int i = 0; i < count; i++) {
+        if (solver_3d_batchable(v[i])) {
+            grp.push_back(v[i]);
+            if ((int)grp.size() == kMax3DBatch) {
+                fpde_status rc = advance_3d_batch(grp.data(), (int)grp.size(), nsteps);
+                if (rc != FPDE_OK) return rc;
+                grp.clear();
+            }
+            continue;
+        }
+        fpde_status rc = fpde_solver_advance(v[i], nsteps);
+        if (rc != FPDE_OK) return rc;
+    }
+    if (!grp.empty()) return advance_3d_batch(grp.data(), (int)grp.size(), nsteps);
+    return FPDE_OK;
+}
 }  // namespace fpde
 
 // ===========================================================================
@@ -931,6 +1042,7 @@ fpde_status fpde_solver_reset(fpde_solver* s, const double* u0) {
     s->packed_groups = 0;
     if (s->flagbar) FPDE_HIP_TRY(hipMemsetAsync(s->flagbar, 0, kFlagBarBytes, s->stream));
     if (s->err) FPDE_HIP_TRY(hipMemsetAsync(s->err, 0, 64, s->stream));
+    s->picard_dev_base = 0;
     s->coop_launches = 0;
     return fpde_solver_set_initial(s, u0);
 }
@@ -992,10 +1104,12 @@ fpde_status fpde_solver_advance_batch(fpde_solver* const* solvers, int count, in
                   " chunk workgroups exceed what the GPU holds at once; split the batch");
         return FPDE_ERR_ARG;
     }
-    for (int i = 0; i < count; i++) {
-        fpde_solver* s = solvers[i];
-        if (s->spec.dims == 1 && s->block_mode) continue;
-        fpde_status rc = fpde_solver_advance(s, nsteps);
+    // the others: batched 3D LOD steps, or their own streams
+    std::vector<fpde_solver*> riders;
+    for (int i = 0; i < count; i++)
+        if (!(solvers[i]->spec.dims == 1 && solvers[i]->block_mode)) riders.push_back(solvers[i]);
+    if (!riders.empty()) {
+        fpde_status rc = advance_riders(riders.data(), (int)riders.size(), nsteps);
         if (rc != FPDE_OK) return rc;
     }
     if (blk.empty()) {
@@ -1063,7 +1177,6 @@ fpde_status fpde_solver_advance_batch(fpde_solver* const* solvers, int count, in
             s->launches++;
             s->coop_launches++;
             s->stats.step_launches++;
-            s->stats.picard_iters_total += (int)((int64_t)(t.n_end[j] - t.n[j]) * s->spec.picard_max);
             count_steps(s, t.n[j], t.n_end[j], t.near_lo[j]);
             s->step = t.n_end[j];
         }
@@ -1389,6 +1502,7 @@ fpde_status fpde_solver_set_profiling(fpde_solver* s, int mask) {
     size_t keep = s->stats.device_bytes;
     s->stats = fpde_stats{};          // counters restart with the new mask
     s->stats.device_bytes = keep;
+    if (s->err) FPDE_HIP_TRY(hipMemcpy(&s->picard_dev_base, s->err + 1, sizeof(int), hipMemcpyDeviceToHost));
     s->prof = mask;
     return FPDE_OK;
 }
@@ -1398,6 +1512,11 @@ fpde_status fpde_solver_get_stats(fpde_solver* s, fpde_stats* out) {
     FPDE_HIP_TRY(hipStreamSynchronize(s->stream));
     ev_collect(s);
     *out = s->stats;
+    if (s->err) {                     // cooperative launches count their Picard solves on the device
+        int dev = 0;
+        FPDE_HIP_TRY(hipMemcpy(&dev, s->err + 1, sizeof(int), hipMemcpyDeviceToHost));
+        out->picard_iters_total += dev - s->picard_dev_base;
+    }
     return FPDE_OK;
 }
 

@@ -780,6 +780,7 @@ __device__ __forceinline__ void step_block(const Step1DArgs& g, const int c) {
     double hm_cur = 0.0;           // that part for the current step
     double tl_cur[3] = {0.0, 0.0, 0.0};
     bool have_hm = false;
+    int nsolve = 0;                // Picard iterations solved (the decisions are uniform)
     int n = n_first;
     while (n < g.n_end) {
         // ---- history term of step n -------------------------------------
@@ -826,6 +827,7 @@ __device__ __forceinline__ void step_block(const Step1DArgs& g, const int c) {
             const int r = solve(pend_check ? 2 : ((k >= 1 && !last_due) ? 1 : 0), uit, uith);
             if (r == 1) break;                                 // U^k is the result
             if (r == 2) { rollback = true; break; }
+            nsolve++;
             pending = false;
         }
         if (rollback) {
@@ -834,6 +836,7 @@ __device__ __forceinline__ void step_block(const Step1DArgs& g, const int c) {
             double ruit = sv_uit, ruith = sv_uith;
             if (!build(ruit, ruith, sv_S, sv_Ln, un, -1)) return;
             solve(0, ruit, ruith);
+            nsolve++;
             finish(sv_n, ruit, ruith);
             pending = false;
             n = sv_n + 1;
@@ -851,6 +854,8 @@ __device__ __forceinline__ void step_block(const Step1DArgs& g, const int c) {
         have_hm = pre_n >= 0;
         n++;
     }
+    // the launch's Picard iterations, counted once per solve (err[1])
+    if (c == 0 && tid == 0) __hip_atomic_fetch_add(g.err + 1, nsolve, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
 }
 
 __global__ __launch_bounds__(256) void k_step1d_block(Step1DArgs g) { step_block(g, (int)blockIdx.x); }

@@ -243,8 +243,10 @@ double kb_newest(double a, double gp, double p0, double pa) {
         const double w = w0 / (1.0 + std::exp(-2.0 * u));                    // distance from w = 0
         const double eu = std::exp(u), ch = eu + 1.0 / eu;
         const double wt = h * w0 * 0.5 * (0.5 * M_PI * std::cosh(t)) * 4.0 / (ch * ch);
-        const double d = std::pow(w, 1.0 / e1);                              // pa - s
-        const double phi = d > 0.0 ? std::pow(g_gap(pa, gp, d, G) / d, -a) : std::pow(gp * std::pow(pa, gp - 1.0), -a);
+        const double d = std::fmin(std::pow(w, 1.0 / e1), pa - p0);          // pa - s, rounded into range
+        // below d = 1e-16 pa, gap/d is its limit g'(pa) in double; from a subnormal d it is not
+        const double phi = d > 1e-16 * pa ? std::pow(g_gap(pa, gp, d, G) / d, -a)
+                                          : std::pow(gp * std::pow(pa, gp - 1.0), -a);
         sum += wt * phi;
     }
     return sum / e1;

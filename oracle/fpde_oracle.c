@@ -275,8 +275,11 @@ double orc_kb_general(double alpha, double gp, double p0, double p1, double pa)
         double w = w0 / (1.0 + exp(-2.0 * u));
         double eu = exp(u), ch = eu + 1.0 / eu;
         double wt = h * 0.5 * w0 * (0.5 * M_PI * cosh(t)) * 4.0 / (ch * ch);
-        double d = pow(w, 1.0 / e1);
-        double phi = (d > 0.0) ? pow(gap_pow(pa, gp, d, G) / d, -alpha) : pow(gp * pow(pa, gp - 1.0), -alpha);
+        /* (pa-p0)^e1^(1/e1) can round above pa-p0; past pa it is outside the domain */
+        double d = fmin(pow(w, 1.0 / e1), pa - p0);
+        /* below d = 1e-16 pa the ratio gap/d equals its limit g'(pa) in double;
+         * formed from subnormal d it would lose its digits or divide 0 by d */
+        double phi = (d > 1e-16 * pa) ? pow(gap_pow(pa, gp, d, G) / d, -alpha) : pow(gp * pow(pa, gp - 1.0), -alpha);
         sum += wt * phi;
     }
     return sum / e1;

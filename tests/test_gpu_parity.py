@@ -304,6 +304,9 @@ def test_cooperative_block_launch_equals_per_iteration_launch(fpde, mode, picard
             for chunk in (7, 50, 1000):
                 s.advance(chunk)
             out.append(s.solution())
+            if step_mode == 0:             # Picard solves counted by the cooperative launches themselves
+                its = s.stats()["picard_iters_total"]
+                assert steps <= its <= picard * steps and (picard > 1 or its == steps), its
     err = rel_err(out[0], out[1])
     print(f"coop vs per-iteration {mode} picard {picard} tol {tol}: {err:.3e}")
     assert err <= (TOL_MIXED if mode.startswith("split") else 1e-12)
@@ -616,6 +619,35 @@ def test_advance_batch_refuses_what_does_not_fit(fpde):
         fpde.advance_batch(solvers[:1], 0)
         assert solvers[0].step == 6
         np.testing.assert_array_equal(solvers[0].solution(), fpde.solve(spec, u0))
+    finally:
+        for s in solvers:
+            s.close()
+
+
+def test_batched_3d_steps_equal_solo(fpde):
+    """3D solves of different sizes, orders, history forms and block sizes
+    (one with split3d odd, one with the generalized kernel FK 3) advance in
+    batched LOD launches (one launch per kernel of the step for all of them,
+    fpde_solver_advance_batch), in uneven pieces: each equals its solo run
+    bit for bit."""
+    cases = [(12, 60, 0.85, "plain", 16, {}), (17, 45, 0.6, "mfma_f64", 16, {}),
+             (32, 70, 0.75, "split_bf16", 32, {"split3d": 1}),
+             (24, 40, 0.8, "mfma_f64", 16, {"weight_mode": 0, "fk": 3, "fp": 0.9})]
+    specs, u0s = [], []
+    for n, M, a, mode, blk, kw in cases:
+        u0 = fpde.init_3d(n, 900 + n)
+        specs.append(fpde.make_spec(n, M, 1e-3, alpha=a, dims=3, hist_mode=mode, block=blk, u_top=u0[0], **kw))
+        u0s.append(u0)
+    solvers = [fpde.Solver(sp) for sp in specs]
+    try:
+        for s, u0 in zip(solvers, u0s):
+            s.set_initial(u0)
+        fpde.advance_batch(solvers, 23)
+        fpde.advance_batch(solvers, 19)
+        fpde.advance_batch(solvers, 0)
+        assert all(s.step == s.spec.steps for s in solvers)
+        for s, sp, u0 in zip(solvers, specs, u0s):
+            np.testing.assert_array_equal(s.solution(), fpde.solve(sp, u0))
     finally:
         for s in solvers:
             s.close()

@@ -102,7 +102,7 @@ _GP = {1: 0.5, 2: 2.0, 3: 0.9}          # g(t) = t^gp (fpde_hpc_1d.cpp:1889-1898
 
 
 @pytest.mark.parametrize("fk", [1, 2, 3])
-@pytest.mark.parametrize("alpha", [0.3, 0.85])
+@pytest.mark.parametrize("alpha", [0.3, 0.85, 0.93, 0.98])
 def test_generalized_weight_is_the_integral(oracle, fk, alpha):
     """b(t0,t1;a) = int (g(T_a) - g(s))^-alpha ds of every kind of interval
     (oldest, middle, next to the newest, the newest itself) against quad."""
@@ -128,6 +128,19 @@ def test_generalized_weight_is_the_integral(oracle, fk, alpha):
                                         limit=200)
             v = oracle.kb_general(alpha, gp, p0, p1, pa)
             assert abs(v - ref) <= 2e-12 * ref, (n, t)
+
+
+@pytest.mark.parametrize("gp", [0.5, 2.0, 0.9])
+def test_generalized_first_step_weight_is_a_beta_function(oracle, gp):
+    """The first step's only interval, [0, pa]: s = pa x^(1/gp) turns the
+    integral into pa^(1 - alpha gp) B(1/gp, 1 - alpha) / gp, across the order
+    range a distributed-order rule samples (end points included)."""
+    for alpha in (0.02, 0.1, 0.3, 0.5, 0.7, 0.9, 0.98):
+        for pa in (1e-3, 0.37, 2.0):
+            v = oracle.kb_general(alpha, gp, 0.0, pa, pa)
+            ref = pa ** (1 - alpha * gp) * math.exp(math.lgamma(1 + 1 / gp) + math.lgamma(1 - alpha)
+                                                    - math.lgamma(1 + 1 / gp - alpha))
+            assert abs(v - ref) <= 1e-12 * ref, (alpha, pa)
 
 
 @pytest.mark.parametrize("fk", [1, 2, 3])
