@@ -80,6 +80,7 @@ SIGNATURES = [
     ("fpde_reference_sum", ctypes.c_int, [_vp, _vp, ctypes.c_int64, ctypes.c_int64, ctypes.c_int64,
                                           _vp, _vp, ctypes.c_int, ctypes.c_double]),
     ("fpde_history_sum_f64", ctypes.c_int, [_vp, _vp, ctypes.c_int64, ctypes.c_int64, ctypes.c_int64, _vp]),
+    ("fpde_history_sum_f64_noalloc", ctypes.c_int, [_vp, _vp, ctypes.c_int64, ctypes.c_int64, ctypes.c_int64, _vp]),
     ("fpde_space_sum", ctypes.c_int, [_vp, _vp, ctypes.c_int64, ctypes.c_double, ctypes.c_double, ctypes.c_int,
                                       ctypes.c_double]),
     ("fpde_history_block_gemm", ctypes.c_int, [ctypes.c_int, _vp, ctypes.c_int64, ctypes.c_int64,

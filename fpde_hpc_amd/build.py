@@ -16,7 +16,7 @@ LIB = os.path.join(HERE, "libfpde_hip.so")
 ARCH = "gfx950"
 
 SOURCES = ["weights.hip", "history.hip", "space.hip", "step1d.hip", "lod3d.hip", "solver.cpp", "facade.cpp",
-           "server.cpp", "frdo.hip"]
+           "server.cpp", "noalloc.hip", "frdo.hip"]
 
 # -packed-fp32-ops off: the compiler must not form v_pk_{add,mul,fma}_f32
 DEVICE_FLAGS = ["-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops"]

@@ -113,6 +113,18 @@ fpde_status fpde_reference_sum(double *S, const double *old, int64_t old_size,
 fpde_status fpde_history_sum_f64(double *S, const double *D, int64_t n, int64_t nodes,
                                  int64_t ld, const double *omega);
 
+/* fpde_history_sum_f64 without any device allocation: the same kernels and
+ * bitwise the same S, with the partial rows in a fixed 32 MiB device array
+ * of the library (one per GPU, held until the process ends) instead of a
+ * buffer allocated per call.  ld must be at most 65536 (else FPDE_ERR_ARG).
+ * Like fpde_history_sum_f64 it returns after the device has finished.
+ * No counterpart in the reference: it is for callers that cannot allocate
+ * device memory (tight serving loops, harnesses that forbid hipMalloc).
+ * All calls on one GPU share the array, so calls that overlap on one GPU
+ * (other threads or streams) race. */
+fpde_status fpde_history_sum_f64_noalloc(double *S, const double *D, int64_t n, int64_t nodes,
+                                         int64_t ld, const double *omega);
+
 /* Space-fractional sum of the explicit scheme, calc_v (fpde_hpc_1d.cpp:2303-2335,
  * one-sided, with_i = 0), on the uniform node grid Z_k = k h:
  * V[i] = Dg * sum_{j=1}^{i-1} kb(gamma, j-1, j, i; Z) * F[j], i = 1..nodes-2,
