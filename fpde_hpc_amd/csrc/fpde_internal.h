@@ -254,6 +254,7 @@ struct BatchTick {
     int first[kMaxBatch + 1];
     int n[kMaxBatch], n_end[kMaxBatch], n0[kMaxBatch], nsplit[kMaxBatch], near_lo[kMaxBatch], epar[kMaxBatch];
     int64_t part_stride[kMaxBatch];
+    const double* part[kMaxBatch];    // the block's partial rows (a solve's buffer alternates by block)
 };
 
 // ---------------------------------------------------------------------------
@@ -359,16 +360,19 @@ fpde_status launch_ref_sum(double* S, const double* old, int64_t old_size, int64
 fpde_status launch_ref_series(double* S, const double* old, int64_t old_size, int64_t os, int64_t nodes,
                               const double* tau, int tstep, double* Htmp, const double* coef, int P, double gtj,
                               double gt, double alpha, hipStream_t st);
-// The GEMM launchers write K-split partial rows P[ks][r][i], ks < *splits_out;
-// the readers add them in split order: the 3D history combine itself, the 1D
-// step kernels after launch_part_reduce has folded them into P[0].
+// The GEMM launchers write K-split partial rows P[ks][r][i], ks < *splits_out
+// (at most max_splits when that is > 0); the readers add them in split order:
+// the 3D history combine itself, the 1D step kernels after launch_part_reduce
+// has folded them into P[0].
 fpde_status launch_gemm_f64(const double* D, int64_t ld, int nodes, int n0, int tc, int block,
-                            const double* omega, double* P, int* splits_out, hipStream_t st);
+                            const double* omega, double* P, int* splits_out, hipStream_t st,
+                            int max_splits = 0);
 fpde_status launch_part_reduce(double* P, int splits, int rows, int64_t ld, int nodes, hipStream_t st);
 fpde_status launch_pack_split(int mode, const double* D, int64_t ld, int nodes, int g_lo, int g_hi, void* c0,
                               void* c1, void* c2, float* scale, hipStream_t st);
 fpde_status launch_gemm_f64_dense(const double* D, int64_t ld, int nodes, int tc, int block, const double* W,
-                                  int64_t ldw, double* P, int* splits_out, hipStream_t st);
+                                  int64_t ldw, double* P, int* splits_out, hipStream_t st,
+                                  int max_splits = 0);
 // W != nullptr: dense weight rows W[r][t] (leading dimension ldw >= tc) of a
 // prescribed grid in place of the Toeplitz vector omega.  afrag / ainv:
 // workspace of split_afrag_bytes(mode, block, tc / 32) bytes and
@@ -377,7 +381,7 @@ size_t split_afrag_bytes(int mode, int block, int groups);
 fpde_status launch_gemm_split(int mode, const void* c0, const void* c1, const void* c2, const float* scale,
                               int64_t ld, int nodes, int n0, int tc, int block, const double* omega,
                               const double* W, int64_t ldw, void* afrag, double* ainv, double* P,
-                              int* splits_out, hipStream_t st);
+                              int* splits_out, hipStream_t st, int max_splits = 0);
 fpde_status launch_step1d(const Step1DArgs& a, hipStream_t st);
 fpde_status launch_step1d_block(const Step1DArgs& a, hipStream_t st);
 bool step1d_block_supported(int C);

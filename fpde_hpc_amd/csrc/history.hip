@@ -616,8 +616,9 @@ int gemm_max_splits(int64_t nodes) {
     return (int)(s < 1 ? 1 : (s > 16 ? 16 : s));
 }
 
-static void gemm_split_plan(int tc, int64_t nodes, int* kchunk, int* splits) {
+static void gemm_split_plan(int tc, int64_t nodes, int* kchunk, int* splits, int max_splits) {
     int ks = gemm_max_splits(nodes);
+    if (max_splits > 0 && ks > max_splits) ks = max_splits;
     int ch = (tc + ks - 1) / ks;
     ch = ((ch + 31) / 32) * 32;
     if (ch < 32) ch = 32;
@@ -626,10 +627,10 @@ static void gemm_split_plan(int tc, int64_t nodes, int* kchunk, int* splits) {
 }
 
 fpde_status launch_gemm_f64_dense(const double* D, int64_t ld, int nodes, int tc, int block, const double* W,
-                                  int64_t ldw, double* P, int* splits_out, hipStream_t st) {
+                                  int64_t ldw, double* P, int* splits_out, hipStream_t st, int max_splits) {
     if (tc <= 0) { *splits_out = 0; return FPDE_OK; }
     int kchunk, splits;
-    gemm_split_plan(tc, nodes, &kchunk, &splits);
+    gemm_split_plan(tc, nodes, &kchunk, &splits, max_splits);
     dim3 grid((unsigned)((nodes + 63) / 64), (unsigned)splits);
     switch (block) {
         case 16: hipLaunchKernelGGL(k_gemm_f64_dense<16>, grid, dim3(256), 0, st, D, ld, nodes, tc, kchunk, W, ldw, P); break;
@@ -644,10 +645,10 @@ fpde_status launch_gemm_f64_dense(const double* D, int64_t ld, int nodes, int tc
 }
 
 fpde_status launch_gemm_f64(const double* D, int64_t ld, int nodes, int n0, int tc, int block,
-                            const double* omega, double* P, int* splits_out, hipStream_t st) {
+                            const double* omega, double* P, int* splits_out, hipStream_t st, int max_splits) {
     if (tc <= 0) { *splits_out = 0; return FPDE_OK; }
     int kchunk, splits;
-    gemm_split_plan(tc, nodes, &kchunk, &splits);
+    gemm_split_plan(tc, nodes, &kchunk, &splits, max_splits);
     dim3 grid((nodes + 63) / 64, splits);
     size_t lds = 0;
     switch (block) {
@@ -719,13 +720,13 @@ size_t split_afrag_bytes(int mode, int block, int groups) {
 fpde_status launch_gemm_split(int mode, const void* c0, const void* c1, const void* c2,
                               const float* scale, int64_t ld, int nodes, int n0, int tc, int block,
                               const double* omega, const double* W, int64_t ldw, void* afrag, double* ainv,
-                              double* P, int* splits_out, hipStream_t st) {
+                              double* P, int* splits_out, hipStream_t st, int max_splits) {
     if (tc <= 0) { *splits_out = 0; return FPDE_OK; }
     if (tc % kGroup != 0) { set_error("split GEMM needs a group-aligned cut"); return FPDE_ERR_ARG; }
     if (W && ldw < tc) { set_error("split GEMM: dense weight rows shorter than the cut"); return FPDE_ERR_ARG; }
     if (!afrag || !ainv) { set_error("split GEMM: no fragment workspace"); return FPDE_ERR_ARG; }
     int kchunk, splits;
-    gemm_split_plan(tc, nodes, &kchunk, &splits);
+    gemm_split_plan(tc, nodes, &kchunk, &splits, max_splits);
     dim3 grid((nodes + 63) / 64, splits);
     fpde_status rc;
     switch (mode) {

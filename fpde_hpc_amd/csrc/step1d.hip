@@ -877,6 +877,7 @@ __global__ __launch_bounds__(256) void k_step1d_batch(const Step1DArgs* __restri
     g.near_lo = t.near_lo[j];
     g.epar = t.epar[j];
     g.part_stride = t.part_stride[j];
+    g.part = t.part[j];
     step_block(g, wg - t.first[j]);
 }
 
