@@ -532,7 +532,8 @@ def test_advance_batch_matches_solo(fpde):
     advanced as one batch (one cooperative launch per history block for the
     block-mode ones; the plain form on its own stream) give each solo result exactly:
     every solve keeps its own barriers and Picard decisions.  One solver
-    starts the batch 37 steps ahead, so the batch's blocks are not aligned."""
+    starts the batch 37 steps ahead, so the batch's blocks are not aligned;
+    another hands a far part already computed ahead over to the batch."""
     cases = [
         dict(n=3000, m=300, alpha=0.6, hist_mode="split_bf16", block=64, picard_max=3, picard_tol=1e-10),
         dict(n=10000, m=200, alpha=0.85, hist_mode="mfma_f64", block=32, picard_max=3, picard_tol=1e-10),
@@ -551,7 +552,11 @@ def test_advance_batch_matches_solo(fpde):
         kw = {k: v for k, v in c.items() if k not in ("n", "m")}
         specs.append(fpde.make_spec(c["n"], c["m"], 1e-3, u_top=u0[0], **kw))
         u0s.append(u0)
-    pre = [37 if i == 2 else 0 for i in range(len(cases))]
+    # case 2 starts 37 steps ahead (its blocks are not aligned with the batch's
+    # calls); case 0 starts two whole 64-step blocks ahead, so the older far part
+    # of its third block is already running on its second stream when the batch
+    # takes over
+    pre = [37 if i == 2 else (128 if i == 0 else 0) for i in range(len(cases))]
 
     def solo(sp, u0, p):                           # the batch's schedule, alone
         with fpde.Solver(sp) as s:
@@ -567,7 +572,9 @@ def test_advance_batch_matches_solo(fpde):
     try:
         for s, u0 in zip(solvers, u0s):
             s.set_initial(u0)
-        solvers[2].advance(pre[2])
+        for s, p in zip(solvers, pre):
+            if p:
+                s.advance(p)
         fpde.advance_batch(solvers, 150)           # part of the way
         assert [s.step for s in solvers] == [min(c["m"], 150 + p) for c, p in zip(cases, pre)]
         fpde.advance_batch(solvers, 0)             # the rest
