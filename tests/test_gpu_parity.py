@@ -243,6 +243,28 @@ def test_solve_1d_matches_oracle(fpde, oracle, case, mode):
     assert err <= tol
 
 
+@pytest.mark.parametrize("mode", ["plain", "mfma_f64", "split_bf16"])
+def test_edge_sizes_match_oracle(fpde, oracle, mode):
+    """The smallest line (4 nodes: one chunk of two interior rows), a single
+    step, lines that end one node past a 256-node chunk, and runs one step
+    past a history block or a 32-level group; 3D at the smallest grid."""
+    tol = TOL_F64 if mode in ("plain", "mfma_f64") else TOL_MIXED
+    for nodes, steps in [(4, 1), (4, 3), (4, 70), (257, 1), (257, 66), (513, 65), (256, 33)]:
+        u0 = fpde.init_1d(nodes, 4000 + nodes + steps)
+        spec = fpde.make_spec(nodes, steps, 1e-3, alpha=0.75, picard_max=3, picard_tol=1e-12, hist_mode=mode,
+                              block=64, u_top=u0[0])
+        err = rel_err(fpde.solve(spec, u0), _oracle_1d(oracle, spec, u0, picard_tol=1e-12))
+        print(f"edge 1D N={nodes} M={steps} {mode}: rel err {err:.3e}", flush=True)
+        assert err <= tol, (nodes, steps)
+    for n, steps in [(4, 1), (4, 20)]:
+        u0 = fpde.init_3d(n, 4100 + steps)
+        spec = fpde.make_spec(n, steps, 1e-3, alpha=0.85, dims=3, hist_mode=mode, block=16, u_top=u0[0])
+        ref = oracle.solve_3d(oracle.make_params(n, steps, 1e-3, 0.85, ksat=spec.ksat), u0)
+        err = rel_err(fpde.solve(spec, u0), ref)
+        print(f"edge 3D n={n} M={steps} {mode}: rel err {err:.3e}", flush=True)
+        assert err <= tol, (n, steps)
+
+
 @pytest.mark.parametrize("tol", [1e-12, 1e-7, 3e-6])
 @pytest.mark.parametrize("mode", ["plain", "split_f16"])
 def test_picard_tolerance_exit_matches_oracle(fpde, oracle, tol, mode):
