@@ -34,12 +34,19 @@ constexpr double kVgmA = 0.066;
 // Conductivity K = ksat*Kr and the inverse water capacity 1/C at suction head
 // P = z - u (metres).  Same quantities as KK()/inv_dw_dh() of the reference
 // (fpde_hpc_1d.cpp:2527-2575, :2625-2657, cv_form 4, k_form 1), rewritten with
-// x = (100 a P)^n:  theta - s0 = (s1-s0) (1+x)^(-m),  Se = (1+x)^(-m),
-// Se^(1/m) = 1/(1+x),  Kr = Se^(1/2) (1 - (x/(1+x))^m)^2,
-// C = d theta/dh = (n-1)(s1-s0) x (1+x)^(1/n) / (P (1+x)^2),   m = 1-1/n,
-// evaluated through lx = log x and l1 = log1p(x) (one log, one log1p, two
-// exp, one expm1): 1-(x/(1+x))^m = -expm1(m (lx - l1)) keeps the digits the
-// direct difference cancels, and (1+x)^(1/n) / (1+x)^2 = Se / (1+x).
+// y = 100 a P and x = y^n:  theta - s0 = (s1-s0) Se,  Se = (1+x)^(-m),
+// Kr = Se^(1/2) (1 - (x/(1+x))^m)^2,  C = d theta/dh = (n-1)(s1-s0) x Se / (P (1+x)),
+// m = 1 - 1/n.  With the reference's n = 5/4 (m = 1/5) every power is a root:
+//   b = y^(1/4) = x^(1/5) (two square roots),  x = y b,  q = (1+x)^(1/5),
+//   Se = 1/q,  (x/(1+x))^m = b/q,
+// and since q^5 - b^5 = (1+x) - x = 1,
+//   1 - b/q = (q - b)/q = 1 / (q (q^4 + q^3 b + q^2 b^2 + q b^3 + b^4)),
+// a sum of positive terms: no digits cancel where x/(1+x) -> 1 (dry soil),
+// where the difference of logarithms loses them (1e-10 relative in Kr at
+// P ~ 100 m for the log/expm1 form).  One log and one exp (the fifth root),
+// three square roots and two divisions; within 4e-15 of the exact K and
+// 1/C for P in [1e-8, 1e3].
+static_assert(kVgmN == 1.25, "vgm_coefs takes fifth and fourth roots: n = 5/4");
 __host__ __device__ inline void vgm_coefs(double u, double z, double ksat, int mode,
                                           double& K, double& invC) {
     if (mode == 1) { K = 1.0; invC = 1.0; return; }
@@ -49,15 +56,16 @@ __host__ __device__ inline void vgm_coefs(double u, double z, double ksat, int m
         invC = 1e20;
         return;
     }
-    const double n = kVgmN, m = 1.0 - 1.0 / kVgmN;
-    double lx = n * log(100.0 * kVgmA * P);
-    double x = exp(lx);
-    double l1 = log1p(x);
-    double se = exp(-m * l1);                      // (1+x)^(-m)
-    double t = -expm1(m * (lx - l1));              // 1 - (x/(1+x))^m
-    K = ksat * sqrt(se) * t * t;
-    double den = (n - 1.0) * (kVgmS1 - kVgmS0) * x * se;     // C (1+x) P
-    invC = (den != 0.0) ? (1.0 + x) * P / den : 1e20;
+    const double y = 100.0 * kVgmA * P;
+    const double b = sqrt(sqrt(y));                 // y^(1/4)
+    const double x = y * b;                         // y^(5/4)
+    const double w = 1.0 + x;
+    const double q = exp(0.2 * log(w));             // (1+x)^(1/5)
+    const double S = (((q + b) * q + b * b) * q + b * b * b) * q + (b * b) * (b * b);
+    const double r = 1.0 / (q * S);                 // 1 - (x/(1+x))^(1/5)
+    const double se = S * r;                        // 1/q
+    K = ksat * sqrt(se) * r * r;
+    invC = (x != 0.0) ? 4.0 * w * P * q / ((kVgmS1 - kVgmS0) * x) : 1e20;   // (n-1) = 1/4
 }
 
 // 1/x for the normalised pivots of the tridiagonal reductions: the hardware
@@ -403,7 +411,7 @@ struct Lod3DRow {
     // to the solo kernels' arithmetic: a literal 1.0 or null would let the
     // compiler fuse products into FMAs differently, changing the rounding
     double kap;
-    const double *E, *Kz, *iCz;
+    const double *E, *Kz, *iCz, *Kl, *iCl;
 };
 struct Lod3DBatch {
     int count;
@@ -466,9 +474,12 @@ fpde_status solver_create_in(const fpde_spec* spec, DeviceArena* arena, hipStrea
 fpde_status launch_step1d_batch(const Step1DArgs* base, const BatchTick& t, hipStream_t st);
 fpde_status launch_coef3d(const double* U, int n, double h, double ksat, int mode, double* K, double* iC,
                           hipStream_t st);
+// K, iC null: the partitioned sweep (lateral3d_partitioned) evaluates the
+// coefficients at U itself; the one-thread-per-line sweep needs them
 fpde_status launch_lateral3d(double* U, const double* K, const double* iC, const double* H, int n, int axis,
                              double h2inv, double L, double* Al, double* Bt, double kap, const double* E,
-                             hipStream_t st);
+                             double h, double ksat, int mode, hipStream_t st);
+inline bool lateral3d_partitioned(int n, double L) { return n <= 128 && L > 0.0; }
 // K, iC: the coefficients at U when a coefficient pass already formed them
 // (else nullptr: the sweep evaluates them itself)
 fpde_status launch_zline3d(double* U, double* Uprev, const double* utop, int n, double h, double h2inv,

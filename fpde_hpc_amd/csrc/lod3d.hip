@@ -204,12 +204,18 @@ __global__ __launch_bounds__(256) void k_zline3d(double* __restrict__ U, double*
 // Al_N| <= 1e-15, which L > 0 rules out: every row is diagonally dominant)
 // and on a bottom value within 1e-15 of zero, where the two differ by less
 // than that absolute amount.
+// K, iC: the coefficients at U from a coefficient pass, or nullptr: then the
+// sweep evaluates them itself, at the rows it solves and the node before its
+// first row (every node of a lateral line has the same depth z0 h), the same
+// values the pass would store.
 constexpr int kLatSeg = 8;
+static_assert(16 * kLatSeg == 128, "lateral3d_partitioned: 8 segments of at most 16 rows");
 template <int S>
 __device__ __forceinline__ void lateral3d_seg_body(double* __restrict__ U, const double* __restrict__ K,
                                                    const double* __restrict__ iC, const double* __restrict__ H,
                                                    int n, int axis, double h2inv, double L, double kap,
-                                                   const double* __restrict__ E, int bid) {
+                                                   const double* __restrict__ E, double h, double ksat, int mode,
+                                                   int bid) {
     __shared__ double sEnd[kLatSeg][32][7];    // x, y, z of the first and the last row
     const int tid = threadIdx.x;
     const int l = tid & 31, sg = tid >> 5;
@@ -230,16 +236,44 @@ __device__ __forceinline__ void lateral3d_seg_body(double* __restrict__ U, const
     double cp[S], xs[S], ys[S];
     double zl = 0.0;                                 // z of the last row
     double cprev = 0.0, xprev = 0.0, yprev = 0.0;
-    double Km = (act && lo >= 1) ? K[base + stride * (lo - 1)] : 0.0;
+    const double zd = z0 * h;
+    double Km = 0.0;
+    if (act && lo >= 1) {
+        if (K) {
+            Km = K[base + stride * (lo - 1)];
+        } else {
+            double icm;
+            vgm_coefs(U[base + stride * (lo - 1)], zd, ksat, mode, Km, icm);
+        }
+    }
+    // the rows' coefficients first (independent evaluations that the
+    // scheduler interleaves), then the recurrence
+    double Kr[S], ir[S], ur[S];
+#pragma unroll
+    for (int j = 0; j < S; j++) {
+        const int i = lo + j;
+        const bool inner = act && i <= hi && i >= 1 && i <= N - 1;
+        const int64_t id = base + stride * (inner ? i : 0);
+        ur[j] = inner ? U[id] : 0.0;
+        if (K) {
+            Kr[j] = inner ? K[id] : 0.0;
+            ir[j] = inner ? iC[id] : 0.0;
+        } else {
+            double kv = 0.0, iv = 0.0;
+            if (inner) vgm_coefs(ur[j], zd, ksat, mode, kv, iv);
+            Kr[j] = kv;
+            ir[j] = iv;
+        }
+    }
 #pragma unroll
     for (int j = 0; j < S; j++) {
         const int i = lo + j;
         const bool valid = act && i <= hi;
         const bool inner = valid && i >= 1 && i <= N - 1;
         const int64_t id = base + stride * (inner ? i : 0);
-        const double Kp = inner ? K[id] : 0.0;
-        const double icv = inner ? iC[id] : 0.0;
-        const double uv = inner ? U[id] : 0.0;
+        const double Kp = Kr[j];
+        const double icv = ir[j];
+        const double uv = ur[j];
         const double hv = (inner && H) ? H[id] : 0.0;
         const double ev = (inner && E) ? E[id] : 0.0;
         const double a = inner ? Km * icv * h2inv * kap : (i == N ? -1.0 : 0.0);
@@ -307,8 +341,9 @@ __global__ __launch_bounds__(256, 2) void k_lateral3d_seg(double* __restrict__ U
                                                       const double* __restrict__ iC,
                                                       const double* __restrict__ H, int n, int axis,
                                                       double h2inv, double L, double kap,
-                                                      const double* __restrict__ E) {
-    lateral3d_seg_body<S>(U, K, iC, H, n, axis, h2inv, L, kap, E, blockIdx.x);
+                                                      const double* __restrict__ E, double h, double ksat,
+                                                      int mode) {
+    lateral3d_seg_body<S>(U, K, iC, H, n, axis, h2inv, L, kap, E, h, ksat, mode, blockIdx.x);
 }
 
 
@@ -529,19 +564,13 @@ __global__ __launch_bounds__(256) void k_hist_combine_batch(Lod3DBatch b) {
                       r.omega, r.corr, r.hscale, (int)blockIdx.x - b.first_node[j]);
 }
 
-__global__ __launch_bounds__(256) void k_coef3d_batch(Lod3DBatch b) {
-    const int j = batch_slot(b.first_node, b.count, blockIdx.x);
-    const Lod3DRow& r = b.row[j];
-    coef3d_body(r.U, r.n, r.h, r.ksat, r.coef_mode, r.K, r.iC, (int)blockIdx.x - b.first_node[j]);
-}
-
 // y sweep (axis 2) carries the history term; the x sweep (axis 1) a third of
 // it with split3d odd
 __global__ __launch_bounds__(256, 2) void k_lateral3d_batch(Lod3DBatch b, int axis) {
     const int j = batch_slot(b.first_lat, b.count, blockIdx.x);
     const Lod3DRow& r = b.row[j];
-    lateral3d_seg_body<16>(r.U, r.K, r.iC, (axis == 2 || r.split) ? r.S : nullptr, r.n, axis, r.h2inv, r.L, r.kap,
-                           r.E, (int)blockIdx.x - b.first_lat[j]);
+    lateral3d_seg_body<16>(r.U, r.Kl, r.iCl, (axis == 2 || r.split) ? r.S : nullptr, r.n, axis, r.h2inv, r.L,
+                           r.kap, r.E, r.h, r.ksat, r.coef_mode, (int)blockIdx.x - b.first_lat[j]);
 }
 
 __global__ __launch_bounds__(kZThreads, 2) void k_zline3d_batch(Lod3DBatch b) {
@@ -559,10 +588,9 @@ fpde_status launch_lod3d_batch(const Lod3DBatch& b, hipStream_t st) {
             return FPDE_ERR_ARG;
         }
     const int gn = b.first_node[b.count], gl = b.first_lat[b.count], gz = b.first_z[b.count];
+    // the lateral sweeps evaluate their coefficients themselves (rows' Kl, iCl null)
     hipLaunchKernelGGL(k_hist_combine_batch, dim3(gn), dim3(256), 0, st, b);
-    hipLaunchKernelGGL(k_coef3d_batch, dim3(gn), dim3(256), 0, st, b);
     hipLaunchKernelGGL(k_lateral3d_batch, dim3(gl), dim3(256), 0, st, b, 2);
-    hipLaunchKernelGGL(k_coef3d_batch, dim3(gn), dim3(256), 0, st, b);
     hipLaunchKernelGGL(k_lateral3d_batch, dim3(gl), dim3(256), 0, st, b, 1);
     hipLaunchKernelGGL(k_zline3d_batch, dim3(gz), dim3(kZThreads), 0, st, b);
     FPDE_HIP_TRY(hipGetLastError());
@@ -585,13 +613,14 @@ fpde_status launch_coef3d(const double* U, int n, double h, double ksat, int mod
 
 fpde_status launch_lateral3d(double* U, const double* K, const double* iC, const double* H, int n,
                              int axis, double h2inv, double L, double* Al, double* Bt, double kap,
-                             const double* E, hipStream_t st) {
+                             const double* E, double h, double ksat, int mode, hipStream_t st) {
     int nl = n * n;
-    if (n <= 16 * kLatSeg && L > 0.0) {   // partitioned: 8 segments of <= 16 rows per line
+    if (lateral3d_partitioned(n, L)) {   // 8 segments of <= 16 rows per line
         hipLaunchKernelGGL(k_lateral3d_seg<16>, dim3((nl + 31) / 32), dim3(256), 0, st, U, K, iC, H, n, axis, h2inv,
-                           L, kap, E);
+                           L, kap, E, h, ksat, mode);
         return FPDE_OK;
     }
+    if (!K || !iC) { set_error("3D lateral sweep over 128 nodes needs the coefficient pass"); return FPDE_ERR_ARG; }
     hipLaunchKernelGGL(k_lateral3d, dim3((nl + 255) / 256), dim3(256), 0, st, U, K, iC, H, n, axis, h2inv, L, Al, Bt,
                        kap, E);
     return FPDE_OK;

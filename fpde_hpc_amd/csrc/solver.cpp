@@ -939,13 +939,23 @@ fpde_status enqueue_3d_step(fpde_solver* s, int n) {
     };
     // local coefficient of this step: per step on a grid (generalized kernel)
     const double Lc = (s->w.grid && n < sp.steps) ? s->w.hLv[n] : s->w.local;
-    if ((rc = launch_coef3d(s->U, nn, s->h, sp.ksat, sp.coef_mode, s->K, s->iC, s->stream)) != FPDE_OK) return rc;
+    // the partitioned lateral sweeps evaluate their coefficients themselves;
+    // a coefficient pass precedes the one-thread-per-line sweeps and the
+    // space-fractional line sums (which need K)
+    const bool pass = space || !lateral3d_partitioned(nn, Lc);
+    const double* Kc = pass ? s->K : nullptr;
+    const double* iCc = pass ? s->iC : nullptr;
+    auto coef_pass = [&]() {
+        return pass ? launch_coef3d(s->U, nn, s->h, sp.ksat, sp.coef_mode, s->K, s->iC, s->stream) : FPDE_OK;
+    };
+    if ((rc = coef_pass()) != FPDE_OK) return rc;
     if ((rc = space_sums(2)) != FPDE_OK) return rc;
-    if ((rc = launch_lateral3d(s->U, s->K, s->iC, s->S, nn, 2, h2inv, Lc, s->Al, s->Bt, kap, E, s->stream)) != FPDE_OK) return rc;
-    if ((rc = launch_coef3d(s->U, nn, s->h, sp.ksat, sp.coef_mode, s->K, s->iC, s->stream)) != FPDE_OK) return rc;
+    if ((rc = launch_lateral3d(s->U, Kc, iCc, s->S, nn, 2, h2inv, Lc, s->Al, s->Bt, kap, E, s->h, sp.ksat,
+                               sp.coef_mode, s->stream)) != FPDE_OK) return rc;
+    if ((rc = coef_pass()) != FPDE_OK) return rc;
     if ((rc = space_sums(1)) != FPDE_OK) return rc;
-    if ((rc = launch_lateral3d(s->U, s->K, s->iC, split ? s->S : nullptr, nn, 1, h2inv, Lc, s->Al, s->Bt, kap, E,
-                               s->stream)) != FPDE_OK) return rc;
+    if ((rc = launch_lateral3d(s->U, Kc, iCc, split ? s->S : nullptr, nn, 1, h2inv, Lc, s->Al, s->Bt, kap, E, s->h,
+                               sp.ksat, sp.coef_mode, s->stream)) != FPDE_OK) return rc;
     if (space) {
         if ((rc = launch_coef3d(s->U, nn, s->h, sp.ksat, sp.coef_mode, s->K, s->iC, s->stream)) != FPDE_OK) return rc;
         if ((rc = space_sums(0)) != FPDE_OK) return rc;
@@ -955,8 +965,10 @@ fpde_status enqueue_3d_step(fpde_solver* s, int n) {
                              s->D + (size_t)(series ? (n & 1) : n) * p.ld, kap, E, space ? s->K : nullptr,
                              space ? s->iC : nullptr, split ? s->S : nullptr, s->stream)) != FPDE_OK) return rc;
     ev_end(s, slot);
-    s->stats.step_launches += space ? 9 : 5;
+    s->stats.step_launches += space ? 9 : (pass ? 5 : 3);
     s->stats.picard_iters_total++;
+    s->stats.step_alg_bytes += 100.0 * (double)p.nodes;     // the three sweeps' coefficients, rows and solves
+    s->stats.steps_done++;
     return FPDE_OK;
 }
 
@@ -1027,7 +1039,7 @@ fpde_status advance_3d_batch(fpde_solver* const* v, int count, int nsteps) {
             r.h = s->h; r.h2inv = 1.0 / (s->h * s->h); r.ksat = sp.ksat;
             r.L = (s->w.grid && n < sp.steps) ? s->w.hLv[n] : s->w.local;
             r.hscale = r.split ? 1.0 / 3.0 : 1.0;
-            r.kap = 1.0; r.E = nullptr; r.Kz = nullptr; r.iCz = nullptr;
+            r.kap = 1.0; r.E = nullptr; r.Kz = nullptr; r.iCz = nullptr; r.Kl = nullptr; r.iCl = nullptr;
             const int lines = sp.nodes * sp.nodes;
             b.first_node[k + 1] = b.first_node[k] + (int)((p.nodes + 255) / 256);
             b.first_lat[k + 1] = b.first_lat[k] + (lines + 31) / 32;
@@ -1042,8 +1054,10 @@ fpde_status advance_3d_batch(fpde_solver* const* v, int count, int nsteps) {
             fpde_solver* s = v[j];
             if (s->step >= end[j]) continue;
             s->step++;
-            s->stats.step_launches += 6;
+            s->stats.step_launches += 4;
             s->stats.picard_iters_total++;
+            s->stats.step_alg_bytes += 100.0 * (double)s->plan.nodes;
+            s->stats.steps_done++;
         }
     }
     // each solve's own stream waits for the batch

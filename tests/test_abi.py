@@ -47,6 +47,28 @@ def test_ksat_fourier_matches_oracle(fpde, oracle):
         assert abs(a - b) <= 1e-14 * abs(b)
 
 
+def test_product_vgm_matches_closed_form(fpde):
+    """The library's K/C at suction P (through fpde_ksat_for_fourier, which
+    returns fo h^2 / (tau Kr(P) / C(P))) against the van Genuchten-Mualem
+    closed form in 50-digit arithmetic: Kr = Se^(1/2) (1 - (x/(1+x))^m)^2,
+    C = (n-1)(s1-s0) x Se / (P (1+x)), x = (100 a P)^n, Se = (1+x)^-m, over
+    wet to very dry soil (the dry end is where a difference of logarithms
+    would lose digits)."""
+    import mpmath
+    mpmath.mp.dps = 50
+    n, s0, s1, a = mpmath.mpf(5) / 4, mpmath.mpf("0.0736"), mpmath.mpf("0.55"), mpmath.mpf("0.066")
+    m = 1 - 1 / n
+    for P in np.logspace(-6, 3, 400):
+        Pm = mpmath.mpf(float(P))
+        x = (100 * a * Pm) ** n
+        se = (1 + x) ** (-m)
+        kr = mpmath.sqrt(se) * (1 - (x / (1 + x)) ** m) ** 2
+        C = (n - 1) * (s1 - s0) * x * se / (Pm * (1 + x))
+        ratio = fpde.lib().fpde_ksat_for_fourier(-float(P), 1.0, 1.0, 1.0)     # = 1 / (Kr / C)
+        exact = float(C / kr)
+        assert abs(ratio - exact) <= 1e-14 * exact, P
+
+
 @pytest.mark.parametrize("bad", [
     dict(dims=2), dict(nodes=3), dict(steps=0), dict(alpha=0.0), dict(alpha=1.5),
     dict(picard_max=0), dict(hist_mode=9), dict(block=48), dict(dims=3, nodes=300),
