@@ -37,15 +37,17 @@ constexpr double kVgmA = 0.066;
 // y = 100 a P and x = y^n:  theta - s0 = (s1-s0) Se,  Se = (1+x)^(-m),
 // Kr = Se^(1/2) (1 - (x/(1+x))^m)^2,  C = d theta/dh = (n-1)(s1-s0) x Se / (P (1+x)),
 // m = 1 - 1/n.  With the reference's n = 5/4 (m = 1/5) every power is a root:
-//   b = y^(1/4) = x^(1/5) (two square roots),  x = y b,  q = (1+x)^(1/5),
-//   Se = 1/q,  (x/(1+x))^m = b/q,
-// and since q^5 - b^5 = (1+x) - x = 1,
-//   1 - b/q = (q - b)/q = 1 / (q (q^4 + q^3 b + q^2 b^2 + q b^3 + b^4)),
-// a sum of positive terms: no digits cancel where x/(1+x) -> 1 (dry soil),
-// where the difference of logarithms loses them (1e-10 relative in Kr at
-// P ~ 100 m for the log/expm1 form).  One log and one exp (the fifth root),
-// three square roots and two divisions; within 4e-15 of the exact K and
-// 1/C for P in [1e-8, 1e3].
+//   b = y^(1/4) = x^(1/5) (two square roots),  x = y b,  w = 1 + x,
+//   Se = r = w^(-1/5),  c = (x/(1+x))^(1/5) = r b,
+// and since (1 - c)(1 + c + c^2 + c^3 + c^4) = 1 - c^5 = 1/w,
+//   1 - c = 1 / (w (1 + c + c^2 + c^3 + c^4)),
+// a sum of positive terms: no digits cancel where c -> 1 (dry soil), where
+// the difference of logarithms loses them (1e-10 relative in Kr at P ~ 100 m
+// for the log/expm1 form).  r comes from a single-precision seed and two
+// Newton steps for the inverse fifth root, r <- r + (r/5)(1 - w r^5) (error
+// e -> 3 e^2: 1e-7 -> 3e-14 -> 1e-27), multiplications only; w beyond the
+// single-precision range takes exp(-log(w)/5).  Three square roots and two
+// divisions in all; within 3e-15 of the exact K and 1/C for P in [1e-8, 1e3].
 static_assert(kVgmN == 1.25, "vgm_coefs takes fifth and fourth roots: n = 5/4");
 __host__ __device__ inline void vgm_coefs(double u, double z, double ksat, int mode,
                                           double& K, double& invC) {
@@ -60,12 +62,21 @@ __host__ __device__ inline void vgm_coefs(double u, double z, double ksat, int m
     const double b = sqrt(sqrt(y));                 // y^(1/4)
     const double x = y * b;                         // y^(5/4)
     const double w = 1.0 + x;
-    const double q = exp(0.2 * log(w));             // (1+x)^(1/5)
-    const double S = (((q + b) * q + b * b) * q + b * b * b) * q + (b * b) * (b * b);
-    const double r = 1.0 / (q * S);                 // 1 - (x/(1+x))^(1/5)
-    const double se = S * r;                        // 1/q
-    K = ksat * sqrt(se) * r * r;
-    invC = (x != 0.0) ? 4.0 * w * P * q / ((kVgmS1 - kVgmS0) * x) : 1e20;   // (n-1) = 1/4
+    double r;                                       // w^(-1/5) = Se
+    if (w < 1e30) {
+        r = (double)exp2f(-0.2f * log2f((float)w));
+        for (int it = 0; it < 2; it++) {
+            const double r2 = r * r;
+            r = fma(r * 0.2, fma(-w, r2 * r2 * r, 1.0), r);
+        }
+    } else {
+        r = exp(-0.2 * log(w));
+    }
+    const double c = r * b;                         // (x/(1+x))^(1/5)
+    const double T = (((c + 1.0) * c + 1.0) * c + 1.0) * c + 1.0;
+    const double t = 1.0 / (w * T);                 // 1 - c
+    K = ksat * sqrt(r) * t * t;
+    invC = (x != 0.0) ? 4.0 * w * P / ((kVgmS1 - kVgmS0) * x * r) : 1e20;   // (n-1) = 1/4
 }
 
 // 1/x for the normalised pivots of the tridiagonal reductions: the hardware
