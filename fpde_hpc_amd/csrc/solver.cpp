@@ -432,14 +432,15 @@ fpde_status solver_alloc(fpde_solver* s) {
 
 // ---- far part one block ahead (lookahead_spec solves) ---------------------
 // The GEMM of block rows n0 .. n0+B-1 over the levels [t_lo, t_hi) on the
-// uniform Toeplitz grid into P (K splits <= max_splits when > 0), on stream st.
+// uniform Toeplitz grid into P (K splits <= max_splits when > 0), on stream st;
+// in the solve's history form, or in FP64 (k_gemm_f64 on D) when fp64.
 fpde_status gemm_range(fpde_solver* s, int n0, int t_lo, int t_hi, double* P, int max_splits, int* splits,
-                       hipStream_t st) {
+                       hipStream_t st, bool fp64 = false) {
     const fpde_spec& sp = s->spec;
     const Plan& p = s->plan;
     *splits = 0;
     if (t_hi <= t_lo) return FPDE_OK;
-    if (is_split(sp.hist_mode)) {
+    if (is_split(sp.hist_mode) && !fp64) {
         size_t cb = 2 * (size_t)t_lo * p.ld;     // 2-byte components, group-major
         auto cp = [&](int c) { return s->comp[c] ? (const void*)((const char*)s->comp[c] + cb) : nullptr; };
         return launch_gemm_split(sp.hist_mode, cp(0), cp(1), cp(2), s->scale + (size_t)(t_lo / kGroup) * p.ld, p.ld,
@@ -451,12 +452,22 @@ fpde_status gemm_range(fpde_solver* s, int n0, int t_lo, int t_hi, double* P, in
 }
 
 // The older part of block n0's far sums, levels [0, tcA), folded into the
-// first B rows of P; returns its split count (0 or 1).  (With a single K
-// split, to draw HBM more slowly beside a step launch, the FP64 GEMM lost
-// its occupancy and the split one gained nothing.)
+// first B rows of P; returns its split count (0 or 1).  The split forms pack
+// the levels it needs first (those not packed yet), on the same stream.
+// (With a single K split, to draw HBM more slowly beside a step launch, the
+// FP64 GEMM lost its occupancy and the split one gained nothing.)
 fpde_status older_part(fpde_solver* s, int n0, int tcA, double* P, hipStream_t st, int* splits) {
+    const fpde_spec& sp = s->spec;
+    fpde_status rc = FPDE_OK;
+    if (is_split(sp.hist_mode) && tcA / kGroup > s->packed_groups) {
+        const int ghi = tcA / kGroup;
+        rc = launch_pack_split(sp.hist_mode, s->D, s->plan.ld, (int)s->plan.nodes, s->packed_groups, ghi,
+                               s->comp[0], s->comp[1], s->comp[2], s->scale, st);
+        s->packed_groups = ghi;
+        if (rc != FPDE_OK) return rc;
+    }
     int sp_ = 0;
-    fpde_status rc = gemm_range(s, n0, 0, tcA, P, 0, &sp_, st);
+    rc = gemm_range(s, n0, 0, tcA, P, 0, &sp_, st);
     if (rc == FPDE_OK && sp_ > 1) rc = launch_part_reduce(P, sp_, s->spec.block, s->plan.ld, (int)s->plan.nodes, st);
     *splits = sp_ > 0 ? 1 : 0;
     if (rc == FPDE_OK && tcA > 0) {
@@ -471,8 +482,11 @@ fpde_status older_part(fpde_solver* s, int n0, int tcA, double* P, hipStream_t s
 // are the older part [0, tcA) -- computed ahead on the side stream during
 // the previous block's step launch when lookahead_issue ran for n0, else
 // here -- plus the newer part [tcA, tc) as one more split, on the main
-// stream.  The two parts' rows sit in this block's parity buffer; the step
-// kernels add the two.
+// stream.  The newer part (about B levels) is an FP64 MFMA GEMM on D in
+// every history form: no packing or fragment kernels on the step chain's
+// path; the split forms pack those levels later, with the next older part.
+// The two parts' rows sit in this block's parity buffer; the step kernels
+// add the two.
 fpde_status far_block_lookahead(fpde_solver* s, int n0, int tc, int tcA) {
     const fpde_spec& sp = s->spec;
     const Plan& p = s->plan;
@@ -492,16 +506,8 @@ fpde_status far_block_lookahead(fpde_solver* s, int n0, int tc, int tcA) {
     ev_begin(s, 0, &slot);
     int spA = (ahead && tcA > 0) ? 1 : 0;
     if (!ahead) rc = older_part(s, n0, tcA, P, s->stream, &spA);
-    if (rc == FPDE_OK && is_split(sp.hist_mode)) {
-        int ghi = tc / kGroup;
-        if (ghi > s->packed_groups) {
-            rc = launch_pack_split(sp.hist_mode, s->D, p.ld, (int)p.nodes, s->packed_groups, ghi, s->comp[0],
-                                   s->comp[1], s->comp[2], s->scale, s->stream);
-            s->packed_groups = ghi;
-        }
-    }
     int spB = 0;
-    if (rc == FPDE_OK) rc = gemm_range(s, n0, tcA, tc, P + (size_t)spA * B * p.ld, 1, &spB, s->stream);
+    if (rc == FPDE_OK) rc = gemm_range(s, n0, tcA, tc, P + (size_t)spA * B * p.ld, 1, &spB, s->stream, true);
     ev_end(s, slot);
     if (rc != FPDE_OK) return rc;
     if (tc > tcA) {
@@ -511,7 +517,7 @@ fpde_status far_block_lookahead(fpde_solver* s, int n0, int tc, int tcA) {
     }
     s->gsplits = spA + spB;
     s->part_cur = P;
-    FPDE_HIP_TRY(hipEventRecord(s->ev_far, s->stream));     // the levels [0, tc) are packed
+    FPDE_HIP_TRY(hipEventRecord(s->ev_far, s->stream));     // the levels [0, tcA) are in memory
     return FPDE_OK;
 }
 
