@@ -412,15 +412,18 @@ bool step1d_batch_supported(int64_t wgs);
 // row holds what its single-solve launches take.
 constexpr int kMax3DBatch = 8;
 struct Lod3DRow {
-    double *U, *K, *iC, *S, *Uprev, *Dlev;
+    double *U, *S, *Uprev, *Dlev;
     const double *part, *D, *omega, *corr, *utop;
     int64_t part_stride, part_off, ld, nodes;
     int n, nsplit, step, near_lo, coef_mode, split;
     double h, h2inv, ksat, L, hscale;
-    // the space-fractional arguments of the solo sweeps (kap = 1, no sums
-    // here), passed as run-time values so that the batched kernels compile
-    // to the solo kernels' arithmetic: a literal 1.0 or null would let the
-    // compiler fuse products into FMAs differently, changing the rounding
+    // arguments the solo sweeps take that a batched solve leaves at their
+    // neutral values: kap = 1 and no line sums E (no space-fractional term),
+    // no coefficient arrays for the depth (Kz, iCz) and lateral (Kl, iCl)
+    // sweeps, which then evaluate K and 1/C themselves.  They are run-time
+    // values so that the batched kernels compile to the solo kernels'
+    // arithmetic: a literal 1.0 or null would let the compiler fold them and
+    // fuse products into FMAs differently, changing the rounding
     double kap;
     const double *E, *Kz, *iCz, *Kl, *iCl;
 };

@@ -1034,7 +1034,7 @@ fpde_status advance_3d_batch(fpde_solver* const* v, int count, int nsteps) {
             s->stream = own;
             if (rc != FPDE_OK) break;
             Lod3DRow& r = b.row[k];
-            r.U = s->U; r.K = s->K; r.iC = s->iC; r.S = s->S; r.Uprev = s->Uprev;
+            r.U = s->U; r.S = s->S; r.Uprev = s->Uprev;
             r.Dlev = s->D + (size_t)n * p.ld;
             r.part = s->part; r.D = s->D;
             r.omega = (s->w.grid && s->omega_cur) ? s->omega_cur : s->w.omega;
