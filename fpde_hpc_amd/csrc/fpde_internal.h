@@ -202,11 +202,11 @@ enum HistMode : int {
 constexpr int kGroup = 32;     // history levels per packed group (K of one 16x16x32 MFMA)
 
 
-// 1D chunks of the partitioned solve (256 nodes each, N <= 65536) and the
-// Picard barrier flags of the cooperative launches: one 128-B flag per chunk
-// and launch parity
+// 1D chunks of the partitioned solve (256 nodes each, N <= 65536)
 constexpr int kMaxChunks = 256;
-constexpr size_t kFlagBarBytes = 2 * (size_t)kMaxChunks * 32 * sizeof(unsigned);
+// grid exchange of the cooperative 1D launches: per barrier parity, 8 words
+// per reduced row (2 per chunk) and 2 per chunk measure, 8 bytes each
+constexpr size_t kXchBytes = 2 * (size_t)(2 * (8 * kMaxChunks + kMaxChunks)) * 8;
 
 // Arguments of one 1D step launch (step1d.hip)
 struct Step1DArgs {
@@ -243,8 +243,8 @@ struct Step1DArgs {
     int n_end;
     int picard_max;
     int n0;                   // first step of the history block (far partial row = n - n0)
-    unsigned* flagbar;        // Picard barrier flags [2 parities][kMaxChunks][32] (128 B per flag)
-    int epar;                 // parity of this cooperative launch
+    unsigned long long* xch;  // grid exchange words [2 parities][kXchBytes / 16] (tagged halves)
+    unsigned xtag;            // this launch's tag base: (launch count mod 65535 + 1) << 16
     int* err;                 // [0] set when a grid barrier times out; [1] Picard solves (cooperative launches)
     int64_t wait_ticks;       // a grid barrier gives up after this many ticks of the device's wall clock
     // series-compressed history (ser_P > 0): moments H [P][ld], coefficient
@@ -271,7 +271,8 @@ constexpr int kMaxBatch = 32;
 struct BatchTick {
     int count;
     int first[kMaxBatch + 1];
-    int n[kMaxBatch], n_end[kMaxBatch], n0[kMaxBatch], nsplit[kMaxBatch], near_lo[kMaxBatch], epar[kMaxBatch];
+    int n[kMaxBatch], n_end[kMaxBatch], n0[kMaxBatch], nsplit[kMaxBatch], near_lo[kMaxBatch];
+    unsigned xtag[kMaxBatch];
     int64_t part_stride[kMaxBatch];
     const double* part[kMaxBatch];    // the block's partial rows (a solve's buffer alternates by block)
 };

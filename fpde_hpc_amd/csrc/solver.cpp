@@ -198,7 +198,7 @@ struct fpde_solver {
     double* spP = nullptr;
     double* spE = nullptr;
     int64_t sp_ldp = 0;
-    unsigned* flagbar = nullptr; // grid-barrier flags of the cooperative 1D block launches
+    unsigned long long* xch = nullptr;  // grid exchange words of the cooperative 1D block launches
     int* err = nullptr;        // barrier time-out flag
     bool block_mode = false;   // 1D steps run as cooperative block launches
     int64_t coop_launches = 0; // selects the barrier counter of the next one
@@ -382,8 +382,8 @@ fpde_status solver_alloc(fpde_solver* s) {
         DALLOC(s->Uit, sizeof(double) * p.ld);
         DALLOC(s->Uit0, sizeof(double) * p.ld);
         DALLOC(s->uend, sizeof(double) * 2 * 2 * s->C);
-        DALLOC(s->flagbar, kFlagBarBytes);
-        FPDE_HIP_TRY(hipMemsetAsync(s->flagbar, 0, kFlagBarBytes, s->stream));
+        DALLOC(s->xch, kXchBytes);
+        FPDE_HIP_TRY(hipMemsetAsync(s->xch, 0, kXchBytes, s->stream));
         DALLOC(s->err, 64);
         FPDE_HIP_TRY(hipMemsetAsync(s->err, 0, 64, s->stream));
         s->wait_ticks = barrier_wait_ticks();
@@ -782,6 +782,12 @@ fpde_status enqueue_1d_step(fpde_solver* s, int n) {
     return FPDE_OK;
 }
 
+// Tag base of the solver's next cooperative launch: the launch count modulo
+// 65535, plus one, in the upper 16 bits (never 0, so no tag equals a zeroed
+// exchange word; consecutive launches differ), the barrier index + 1 modulo
+// 65536 in the lower 16 (set by the kernel).
+unsigned launch_tag(const fpde_solver* s) { return (unsigned)(s->coop_launches % 65535 + 1) << 16; }
+
 // Arguments of the cooperative block launch for the steps [n, n_end).
 Step1DArgs block_args(fpde_solver* s, int n, int n_end, int n0, int nsplit, int64_t stride, int near_lo) {
     const fpde_spec& sp = s->spec;
@@ -813,8 +819,8 @@ Step1DArgs block_args(fpde_solver* s, int n, int n_end, int n0, int nsplit, int6
     a.near_lo = near_lo;
     a.red = s->red;
     a.dmax = s->dmax;
-    a.flagbar = s->flagbar;
-    a.epar = (int)(s->coop_launches & 1);
+    a.xch = s->xch;
+    a.xtag = launch_tag(s);
     a.err = s->err;
     a.wait_ticks = s->wait_ticks;
     a.ser_P = s->w.ser_P;
@@ -1228,7 +1234,7 @@ fpde_status fpde_solver_reset(fpde_solver* s, const double* u0) {
     s->packed_groups = 0;
     s->part_cur = s->part;
     s->la_par = 0;
-    if (s->flagbar) FPDE_HIP_TRY(hipMemsetAsync(s->flagbar, 0, kFlagBarBytes, s->stream));
+    if (s->xch) FPDE_HIP_TRY(hipMemsetAsync(s->xch, 0, kXchBytes, s->stream));
     if (s->err) FPDE_HIP_TRY(hipMemsetAsync(s->err, 0, 64, s->stream));
     s->picard_dev_base = 0;
     s->coop_launches = 0;
@@ -1349,7 +1355,7 @@ fpde_status fpde_solver_advance_batch(fpde_solver* const* solvers, int count, in
             t.near_lo[j] = near_lo;
             t.part_stride[j] = stride;
             t.part[j] = s->part_cur;
-            t.epar[j] = (int)(s->coop_launches & 1);
+            t.xtag[j] = launch_tag(s);
             first += s->C;
             if (s->stream != bst && hip_ok(hipEventRecord(ev[j], s->stream))) hip_ok(hipStreamWaitEvent(bst, ev[j], 0));
         }

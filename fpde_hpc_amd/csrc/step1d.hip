@@ -526,47 +526,44 @@ __global__ __launch_bounds__(256) void k_step1d(Step1DArgs g) {
 // inside the bound, which exists only so that a launch whose workgroups can
 // never all be resident ends with an error instead of hanging.
 
-// Grid barrier of the C chunk workgroups on per-workgroup flags: workgroup
-// c stores the barrier's epoch e to its own flag (128 B apart) after an
-// agent-scope release, and thread j polls flag j (the C flags in parallel,
-// no read-modify-write on a shared counter), then one agent-scope acquire.
-// Each launch uses the flag array of its parity (epochs 1, 2, ... within
-// the launch); its workgroups zero their flags of the other parity for the
-// next launch.  Returns false (and sets *err) after a bounded wait.
-// flag_arrive and flag_wait are its two halves: work placed between them
-// overlaps the wait (the workgroup's own writes made there are not covered
-// by the barrier).
-__device__ inline void flag_arrive(unsigned* fl, int c, unsigned e) {
-    __syncthreads();
-    if (threadIdx.x == 0) {
-        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent");
-        __hip_atomic_store(fl + 32 * c, e, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-    }
-}
+// Grid exchange of the chunk workgroups.  Every value a workgroup publishes
+// for the others (its two reduced rows, its convergence measure) travels as
+// two 64-bit words, each holding one 32-bit half of the double and the 32-bit
+// tag of the barrier it belongs to (xtag | (barrier index + 1) mod 2^16; xtag
+// carries the launch count in its upper 16 bits, never 0).  Words are written
+// and read with relaxed agent-scope atomics: each is single-copy atomic and
+// coherent across the XCDs, and a reader takes a word once it carries the
+// barrier's tag, so it holds that barrier's value.  No fence orders one word
+// against another, so a barrier needs no L2 write-back or invalidate (the
+// release/acquire pair of a flag barrier costs more than the exchange).
+// Readers poll the measure words first (thread j < C polls chunk j's: few
+// pollers, little traffic while the last chunks are still computing), then
+// load the rows, which have landed or are about to; polling every word of
+// every row from the start was measured slower (its traffic delays the
+// writers' stores).
+//  - Words alternate by barrier parity.  A workgroup writes barrier b+1's
+//    words only after it has read every workgroup's barrier-b words, and each
+//    workgroup wrote those after reading its barrier-(b-1) words, so no word
+//    is overwritten while a reader still needs it.
+//  - At launch start each workgroup zeroes its own words (no tag is 0), so no
+//    word an earlier launch left can carry one of this launch's tags.
+//  - A reader waits at most `wait_ticks` ticks of the device's constant wall
+//    clock per wait (the host sizes it to 2 s): workgroups of a cooperative
+//    launch can become resident late when other streams' kernels hold CUs at
+//    dispatch, which ends within micro- to milliseconds; the bound exists so
+//    that a launch whose workgroups can never all be resident ends with an
+//    error instead of hanging.
+// Slots (two words each) per parity: value q of reduced row r at 4 r + q
+// (r < 2C), the measure of chunk j at kXchMeasure + j.
+constexpr int kXchMeasure = 8 * kMaxChunks;
+constexpr int kXchWords = 2 * (kXchMeasure + kMaxChunks);       // per parity
+static_assert(2 * kXchWords * 8 <= (int)kXchBytes, "exchange buffer size");
 
-__device__ inline bool flag_wait(unsigned* fl, int C, unsigned e, int* err, int* sh_ok, int64_t wait_ticks) {
-    if (threadIdx.x == 0) *sh_ok = 1;
-    __syncthreads();
-    for (int j = threadIdx.x; j < C; j += blockDim.x) {
-        if (__hip_atomic_load(fl + 32 * j, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) >= e) continue;
-        const int64_t t0 = (int64_t)wall_clock64();
-        while (__hip_atomic_load(fl + 32 * j, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) < e) {
-            __builtin_amdgcn_s_sleep(1);
-            if ((int64_t)wall_clock64() - t0 > wait_ticks) {
-                *sh_ok = 0;
-                __hip_atomic_store(err, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-                break;
-            }
-        }
-    }
-    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
-    __syncthreads();
-    return *sh_ok != 0;
-}
-
-__device__ inline bool flag_sync(unsigned* fl, int c, int C, unsigned e, int* err, int* sh_ok, int64_t wait_ticks) {
-    flag_arrive(fl, c, e);
-    return flag_wait(fl, C, e, err, sh_ok, wait_ticks);
+__device__ inline void xput(unsigned long long* x, int slot, unsigned tag, double v) {
+    const unsigned long long b = (unsigned long long)__double_as_longlong(v);
+    const unsigned long long t = (unsigned long long)tag << 32;
+    __hip_atomic_store(x + 2 * slot, t | (b & 0xffffffffull), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    __hip_atomic_store(x + 2 * slot + 1, t | (b >> 32), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
 }
 
 // History term of step n at node i (levels t < n-1 in memory, the newest
@@ -585,7 +582,8 @@ __device__ __forceinline__ void step_block(const Step1DArgs& g, const int c) {
     __shared__ double pA[2][256], pC[2][256], pX[2][256], pY[2][256], pZ[2][256];
     __shared__ double sK[258];
     __shared__ double sdp[4];                  // wave maxima of the last solve's |U^{k+1} - U^k|
-    __shared__ int sh_ok;
+    __shared__ double sMeas[kMaxChunks];       // every chunk's published measure of the last barrier
+    __shared__ int sh_fail;                    // a barrier wait ran out of time
     const int tid = threadIdx.x;
     const int N = g.N, C = g.C;
     const int s = (int)(((int64_t)c * N) / C);
@@ -616,13 +614,99 @@ __device__ __forceinline__ void step_block(const Step1DArgs& g, const int c) {
     double d0m = (!series && act && n_first >= 1) ? g.D[i] : 0.0;
     double d1m = (!series && act && n_first >= 2) ? g.D[g.ld + i] : 0.0;
     int bk = 0;                                                    // Picard barriers passed in this launch
-    // this launch synchronises on the flags of its parity and zeroes its own
-    // flag of the other parity for the next launch (the previous launch,
-    // their last user, has finished)
-    unsigned* fl = g.flagbar + (size_t)g.epar * 32 * kMaxChunks;
-    if (tid == 0)
-        __hip_atomic_store(g.flagbar + (size_t)(1 - g.epar) * 32 * kMaxChunks + 32 * c, 0u, __ATOMIC_RELAXED,
-                           __HIP_MEMORY_SCOPE_AGENT);
+    // the exchange words of this workgroup, both parities, zeroed before its
+    // first publication (its rows 2c and 2c+1, its measure)
+    auto xbase = [&](int b) { return g.xch + (size_t)(b & 1) * kXchWords; };
+    auto xtag = [&](int b) { return g.xtag | ((unsigned)(b + 1) & 0xffffu); };
+    if (tid == 0 || tid == nc - 1) {
+        for (int par = 0; par < 2; par++) {
+            unsigned long long* x = g.xch + (size_t)par * kXchWords;
+            const int r = (tid == 0) ? 2 * c : 2 * c + 1;
+            for (int w = 0; w < 8; w++)
+                __hip_atomic_store(x + 8 * r + w, 0ull, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            if (tid == 0)
+                for (int w = 0; w < 2; w++)
+                    __hip_atomic_store(x + 2 * (kXchMeasure + c) + w, 0ull, __ATOMIC_RELAXED,
+                                       __HIP_MEMORY_SCOPE_AGENT);
+        }
+    }
+    if (tid == 0) sh_fail = 0;
+    // the words of barrier b: thread j < C the measure of chunk j (into
+    // sMeas), then (rows = true) thread r reduced rows r and r + 256; then a
+    // workgroup barrier.  False when a wait ran out of time (the launch then
+    // ends; the host reports it).
+    auto read_barrier = [&](int b, bool rows, double (*rrow)[4]) -> bool {
+        const unsigned long long* xr = xbase(b);
+        const unsigned tg = xtag(b);
+        const unsigned long long none = (unsigned long long)tg << 32;
+        const int64_t tmax = g.wait_ticks;
+        // load the words the predicate leaves to this thread until all carry
+        // the tag (missing ones reloaded together); false after tmax ticks
+        auto take = [&](unsigned long long* w, int cnt, const unsigned long long* src) -> bool {
+            int64_t t0 = -1;
+            for (;;) {
+                bool all = true;
+                for (int k = 0; k < cnt; k++) all = all && (unsigned)(w[k] >> 32) == tg;
+                if (all) return true;
+                const int64_t now = (int64_t)wall_clock64();
+                if (t0 < 0) t0 = now;
+                else if (now - t0 > tmax) return false;
+                __builtin_amdgcn_s_sleep(1);
+                for (int k = 0; k < cnt; k++)
+                    if ((unsigned)(w[k] >> 32) != tg)
+                        w[k] = __hip_atomic_load(src + k, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            }
+        };
+        bool ok = true;
+        // first the measures (one pair of words per chunk, the last words a
+        // chunk publishes from its thread 0): few pollers while the others
+        // are still on their way
+        unsigned long long mw[2] = {none, none};
+        if (tid < C) {
+            const unsigned long long* src = xr + 2 * (kXchMeasure + tid);
+            mw[0] = __hip_atomic_load(src, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            mw[1] = __hip_atomic_load(src + 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            ok = take(mw, 2, src);
+        }
+        // then the rows, which have landed or are about to
+        unsigned long long rw[2][8];
+        if (rows) {
+            __syncthreads();
+#pragma unroll
+            for (int j = 0; j < 2; j++) {
+                const int r = tid + 256 * j;
+#pragma unroll
+                for (int w = 0; w < 8; w++)
+                    rw[j][w] = (r < R2) ? __hip_atomic_load(xr + 8 * r + w, __ATOMIC_RELAXED,
+                                                            __HIP_MEMORY_SCOPE_AGENT)
+                                        : none;
+            }
+#pragma unroll
+            for (int j = 0; j < 2; j++)
+                if (tid + 256 * j < R2) ok = ok && take(rw[j], 8, xr + 8 * (tid + 256 * j));
+        }
+        if (!ok) {
+            __hip_atomic_store(g.err, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            sh_fail = 1;
+        }
+        auto dec = [](unsigned long long lo, unsigned long long hi) {
+            return __longlong_as_double((long long)((hi << 32) | (lo & 0xffffffffull)));
+        };
+        if (tid < C) sMeas[tid] = dec(mw[0], mw[1]);
+        if (rows)
+#pragma unroll
+            for (int j = 0; j < 2; j++)
+#pragma unroll
+                for (int qq = 0; qq < 4; qq++) rrow[j][qq] = dec(rw[j][2 * qq], rw[j][2 * qq + 1]);
+        __syncthreads();
+        return sh_fail == 0;
+    };
+    // the maximum of the staged measures, formed by every wave itself
+    auto measures_max = [&]() {
+        double dv = 0.0;
+        for (int j = lane; j < C; j += 64) dv = fmax(dv, sMeas[j]);
+        return wave_max(dv);
+    };
 
     if (tid < 4) sdp[tid] = 0.0;
     // the chunk's max |U^{k+1} - U^k| of the last solve (thread 0, after a
@@ -668,27 +752,27 @@ __device__ __forceinline__ void step_block(const Step1DArgs& g, const int c) {
                                    (tid == nc - 2) ? -cr * ib : 0.0);
         fx = 0.0; fy = 0.0; fz = 0.0;
         if (inner) { fx = pX[pp][q]; fy = pY[pp][q]; fz = pZ[pp][q]; }
-        double* redw = g.red + (size_t)(bk & 1) * R2 * 4;
+        unsigned long long* xw = xbase(bk);
+        const unsigned tg = xtag(bk);
         if (tid == 0) {
             double x, y, z;
             if (nc == 2) { x = 0.0; y = 0.0; z = 1.0; }
             else { x = pX[pp][0]; y = pY[pp][0]; z = pZ[pp][0]; }
-            redw[4 * (2 * c) + 0] = ar;
-            redw[4 * (2 * c) + 1] = br + cr * y;
-            redw[4 * (2 * c) + 2] = cr * z;
-            redw[4 * (2 * c) + 3] = dr - cr * x;
-            g.dmax[(size_t)(bk & 1) * C + c] = last_measure();
+            xput(xw, 4 * (2 * c) + 0, tg, ar);
+            xput(xw, 4 * (2 * c) + 1, tg, br + cr * y);
+            xput(xw, 4 * (2 * c) + 2, tg, cr * z);
+            xput(xw, 4 * (2 * c) + 3, tg, dr - cr * x);
+            xput(xw, kXchMeasure + c, tg, last_measure());
         }
         if (tid == nc - 1) {
             double x, y, z;
             if (nc == 2) { x = 0.0; y = 1.0; z = 0.0; }
             else { x = pX[pp][ni - 1]; y = pY[pp][ni - 1]; z = pZ[pp][ni - 1]; }
-            redw[4 * (2 * c + 1) + 0] = ar * y;
-            redw[4 * (2 * c + 1) + 1] = br + ar * z;
-            redw[4 * (2 * c + 1) + 2] = cr;
-            redw[4 * (2 * c + 1) + 3] = dr - ar * x;
+            xput(xw, 4 * (2 * c + 1) + 0, tg, ar * y);
+            xput(xw, 4 * (2 * c + 1) + 1, tg, br + ar * z);
+            xput(xw, 4 * (2 * c + 1) + 2, tg, cr);
+            xput(xw, 4 * (2 * c + 1) + 3, tg, dr - ar * x);
         }
-        flag_arrive(fl, c, (unsigned)(bk + 1));
         if (pre_n >= 0 && act) {
             const double* om = g.omega + (int64_t)(pre_n - g.n0) * g.omega_stride;
             tl_next[0] = om[0];
@@ -696,8 +780,7 @@ __device__ __forceinline__ void step_block(const Step1DArgs& g, const int c) {
             tl_next[2] = g.corr[2 * pre_n + 1];
             hm_next = hist_mem(g, i, pre_n, (int64_t)(pre_n - g.n0) * g.ld, om);
         }
-        if (!flag_wait(fl, C, (unsigned)(bk + 1), g.err, &sh_ok, g.wait_ticks)) return false;
-        bk++;
+        bk++;                                  // the solve reads this barrier's words
         return true;
     };
     // ---- after a build's barrier: the reduced rows; with check != 0 first
@@ -709,19 +792,10 @@ __device__ __forceinline__ void step_block(const Step1DArgs& g, const int c) {
     // workgroup barrier.  The chunk's max |U^{k+1} - U^k| is left as four
     // wave maxima in sdp for thread 0, which publishes it with the next build.
     auto solve = [&](int check, double& uit, double& uith) -> int {
-        const double* redr = g.red + (size_t)((bk - 1) & 1) * R2 * 4;
         double rrow[2][4];
-#pragma unroll
-        for (int j = 0; j < 2; j++) {
-            int r = tid + 256 * j;
-            bool ok = r < R2;
-#pragma unroll
-            for (int qq = 0; qq < 4; qq++) rrow[j][qq] = ok ? redr[4 * r + qq] : 0.0;
-        }
+        if (!read_barrier(bk - 1, true, rrow)) return -1;
         if (check) {
-            double dv = 0.0;
-            for (int j = lane; j < C; j += 64) dv = fmax(dv, g.dmax[(size_t)((bk - 1) & 1) * C + j]);
-            const bool met = wave_max(dv) <= g.tol;
+            const bool met = measures_max() <= g.tol;
             if (check == 1 && met) return 1;
             if (check == 2 && !met) return 2;
         }
@@ -815,16 +889,15 @@ __device__ __forceinline__ void step_block(const Step1DArgs& g, const int c) {
                 // the last build is skipped when U^k already met the
                 // tolerance: decide that first (one barrier on the measure)
                 __syncthreads();
-                if (tid == 0) g.dmax[(size_t)(bk & 1) * C + c] = last_measure();
-                if (!flag_sync(fl, c, C, (unsigned)(bk + 1), g.err, &sh_ok, g.wait_ticks)) return;
+                if (tid == 0) xput(xbase(bk), kXchMeasure + c, xtag(bk), last_measure());
                 bk++;
-                double dv = 0.0;
-                for (int j = lane; j < C; j += 64) dv = fmax(dv, g.dmax[(size_t)((bk - 1) & 1) * C + j]);
-                if (wave_max(dv) <= g.tol) break;             // U^k is the result (uniform decision)
+                if (!read_barrier(bk - 1, false, nullptr)) return;
+                if (measures_max() <= g.tol) break;           // U^k is the result (uniform decision)
             }
             const bool pend_check = k == 0 && pending;
             if (!build(uit, uith, S, Ln, un, k == 0 ? pre_n : -1)) return;
             const int r = solve(pend_check ? 2 : ((k >= 1 && !last_due) ? 1 : 0), uit, uith);
+            if (r < 0) return;                                 // a barrier wait ran out of time
             if (r == 1) break;                                 // U^k is the result
             if (r == 2) { rollback = true; break; }
             nsolve++;
@@ -835,7 +908,7 @@ __device__ __forceinline__ void step_block(const Step1DArgs& g, const int c) {
             un = sv_un; uh = sv_uh; dh = sv_dh; dlast = sv_dlast; d0m = sv_d0m; d1m = sv_d1m;
             double ruit = sv_uit, ruith = sv_uith;
             if (!build(ruit, ruith, sv_S, sv_Ln, un, -1)) return;
-            solve(0, ruit, ruith);
+            if (solve(0, ruit, ruith) < 0) return;
             nsolve++;
             finish(sv_n, ruit, ruith);
             pending = false;
@@ -875,7 +948,7 @@ __global__ __launch_bounds__(256) void k_step1d_batch(const Step1DArgs* __restri
     g.n0 = t.n0[j];
     g.nsplit = t.nsplit[j];
     g.near_lo = t.near_lo[j];
-    g.epar = t.epar[j];
+    g.xtag = t.xtag[j];
     g.part_stride = t.part_stride[j];
     g.part = t.part[j];
     step_block(g, wg - t.first[j]);
