@@ -246,8 +246,9 @@ def ensemble_run(F, count, nodes, msteps, hist_mode, picard, reps=3, uneven=Fals
     else:
         ns, ms = [nodes] * count, [msteps] * count
     u0s = [F.init_1d(n, 700 + i) for i, n in enumerate(ns)]
+    # 256-node chunks: six N=10^4 solves (240 workgroups) fit one launch
     specs = [F.make_spec(n, m, 1e-3, alpha=float(a), hist_mode=hist_mode, block=64,
-                         picard_max=picard[0], picard_tol=picard[1], u_top=u0[0])
+                         picard_max=picard[0], picard_tol=picard[1], u_top=u0[0], chunk_rows=256)
              for n, m, a, u0 in zip(ns, ms, alphas, u0s)]
     solvers = [F.Solver(sp) for sp in specs]
 
@@ -607,6 +608,8 @@ def main():
         "latency": latency,
         "variants": variants,
     }
+    if spec.dims == 1:
+        out["config"]["chunk_nodes"] = spec.chunk_rows if spec.chunk_rows else "automatic (192)"
     if results[args.headline].get("tuned"):
         out["config"]["block_autotune"] = results[args.headline]["tuned"]
     if "plain" in variants:

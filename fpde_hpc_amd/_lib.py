@@ -31,6 +31,7 @@ class Spec(ctypes.Structure):
         ("step_mode", ctypes.c_int), ("sum_alg", ctypes.c_int), ("sum_param", ctypes.c_double),
         ("taus", ctypes.c_void_p), ("var_tau", ctypes.c_int), ("split3d", ctypes.c_int),
         ("gamma", ctypes.c_double), ("fk", ctypes.c_int), ("fp", ctypes.c_double),
+        ("chunk_rows", ctypes.c_int),
     ]
 
 

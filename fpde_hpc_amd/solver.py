@@ -25,20 +25,23 @@ def _p(a):
 def make_spec(nodes, steps, tau, alpha=0.8, dims=1, length=1.0, ksat=None, coef_mode=0,
               picard_max=1, picard_tol=0.0, weight_mode=1, series_eps=1e-5, do_q=0, do_rule=0,
               do_a0=0.1, do_a1=1.0, hist_mode="plain", block=64, step_mode=0, fourier=1.0,
-              u_top=-1.0, sum_alg=0, sum_param=1e-5, taus=None, var_tau=0, gamma=0.0, split3d=0, fk=0, fp=0.6):
+              u_top=-1.0, sum_alg=0, sum_param=1e-5, taus=None, var_tau=0, gamma=0.0, split3d=0, fk=0, fp=0.6,
+              chunk_rows=0):
     """fpde_spec; ksat defaults to the value giving mesh Fourier number `fourier`
     (K*tau/(C*h^2)) at suction -u_top.  `taus` (one length per step) makes the
     time grid non-uniform; `gamma` in (0,1) selects the space-fractional
     scheme (0 or 1: classical diffusion); `split3d` odd spreads the 3D history
     term over the three sweeps; `fk`/`fp` select the generalized kernel
-    g(t) = t, sqrt(t), t^2 or t^fp (fk 0..3; needs weight_mode 0)."""
+    g(t) = t, sqrt(t), t^2 or t^fp (fk 0..3; needs weight_mode 0); `chunk_rows`
+    sets the 1D nodes per chunk workgroup (0: 192 for the fastest single solve;
+    256 fits more solves into one batched launch)."""
     if ksat is None:
         h = length / (nodes - 1)
         ksat = lib().fpde_ksat_for_fourier(u_top, h, tau, fourier) if coef_mode == 0 else 1.0
     hm = HIST_MODES[hist_mode] if isinstance(hist_mode, str) else int(hist_mode)
     spec = Spec(dims, nodes, steps, tau, alpha, length, ksat, coef_mode, picard_max, picard_tol,
                 weight_mode, series_eps, do_q, do_rule, do_a0, do_a1, hm, block, step_mode,
-                sum_alg, sum_param, None, var_tau, split3d, gamma, fk, fp)
+                sum_alg, sum_param, None, var_tau, split3d, gamma, fk, fp, chunk_rows)
     if taus is not None:
         spec._taus = np.ascontiguousarray(taus, dtype=np.float64)   # alive until the solver copies it
         if spec._taus.size != steps:
@@ -304,7 +307,9 @@ def serving_mix(n_req=512, rate=16.0, seed=1004, hist_mode="split_bf16"):
     1D (N ~ U[500, 20000], M ~ U[1000, 40000], tau 5e-4, block 64), 20% 3D
     (32^3..96^3, M ~ U[200, 1000], tau 1e-3, block 16); alpha ~ U(0.3, 0.95);
     25% distributed order (16 Gauss-Legendre nodes on [0.1, 1]); each with its
-    own seeded initial head.  Returns (arrival_s, spec, u0, node_steps) tuples
+    own seeded initial head; the 1D requests take 256-node chunks, so that
+    more of them share each batched launch.  Returns (arrival_s, spec, u0,
+    node_steps) tuples
     in arrival order.  The reference runs such a mix as one process per solve,
     each configured on its command line (mpi_test.sh:12-23)."""
     rng = np.random.default_rng(seed)
@@ -324,7 +329,7 @@ def serving_mix(n_req=512, rate=16.0, seed=1004, hist_mode="split_bf16"):
             n = int(rng.integers(500, 20001))
             M = int(rng.integers(1000, 40001))
             u0 = init_1d(n, 100000 + i)
-            kw = dict(dims=1, block=64)
+            kw = dict(dims=1, block=64, chunk_rows=256)
         if do:
             kw.update(do_q=16, do_rule=0, do_a0=0.1, do_a1=1.0)
         tau = 1e-3 if is3d else 5e-4

@@ -211,6 +211,17 @@ typedef struct {
                             each interval, evaluated exactly (weight_mode 0,
                             sum_alg 0), one weight row per step                */
     double fp;           /* fk 3: the power (CLI "FP", func_power)             */
+    int chunk_rows;      /* 1D: nodes per chunk of the partitioned line solve
+                            (one workgroup each), 0 or 32..256.  0: 192, or
+                            N/256 rounded up where 192-node chunks would number
+                            more than 256 (the fastest single solve: three
+                            waves of rows per workgroup).  256 packs more solves
+                            into one batched launch (fpde_solver_advance_batch,
+                            the server): ceil(N/chunk_rows) workgroups per
+                            solve, at most 256.  Results depend on the chunking
+                            in the last bits only.  No counterpart in the
+                            reference, whose line solve is one serial Thomas
+                            sweep (fpde_hpc_1d.cpp:4672-4725)                 */
 } fpde_spec;
 
 typedef struct fpde_solver fpde_solver;

@@ -77,6 +77,7 @@ def test_product_vgm_matches_closed_form(fpde):
     dict(sum_alg=2, sum_param=2.5, hist_mode=0), dict(sum_alg=2, sum_param=8.0, hist_mode=1),
     dict(sum_alg=2, sum_param=8.0, hist_mode=0, do_q=4),
     dict(gamma=1.5), dict(gamma=-0.1),
+    dict(chunk_rows=16), dict(chunk_rows=300), dict(nodes=20000, chunk_rows=64),
 ])
 def test_invalid_specs_rejected(fpde, bad):
     kw = dict(nodes=100, steps=10, tau=1e-3, alpha=0.5, hist_mode=1, block=64)

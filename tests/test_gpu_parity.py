@@ -246,10 +246,12 @@ def test_solve_1d_matches_oracle(fpde, oracle, case, mode):
 @pytest.mark.parametrize("mode", ["plain", "mfma_f64", "split_bf16"])
 def test_edge_sizes_match_oracle(fpde, oracle, mode):
     """The smallest line (4 nodes: one chunk of two interior rows), a single
-    step, lines that end one node past a 256-node chunk, and runs one step
-    past a history block or a 32-level group; 3D at the smallest grid."""
+    step, lines that end one node past a 192-node chunk (the automatic size)
+    or a 256-node one, and runs one step past a history block or a 32-level
+    group; 3D at the smallest grid."""
     tol = TOL_F64 if mode in ("plain", "mfma_f64") else TOL_MIXED
-    for nodes, steps in [(4, 1), (4, 3), (4, 70), (257, 1), (257, 66), (513, 65), (256, 33)]:
+    for nodes, steps in [(4, 1), (4, 3), (4, 70), (193, 66), (385, 65), (257, 1), (257, 66), (513, 65),
+                         (256, 33)]:
         u0 = fpde.init_1d(nodes, 4000 + nodes + steps)
         spec = fpde.make_spec(nodes, steps, 1e-3, alpha=0.75, picard_max=3, picard_tol=1e-12, hist_mode=mode,
                               block=64, u_top=u0[0])
@@ -263,6 +265,20 @@ def test_edge_sizes_match_oracle(fpde, oracle, mode):
         err = rel_err(fpde.solve(spec, u0), ref)
         print(f"edge 3D n={n} M={steps} {mode}: rel err {err:.3e}", flush=True)
         assert err <= tol, (n, steps)
+
+
+@pytest.mark.parametrize("rows", [0, 32, 100, 192, 256])
+def test_chunk_sizes_match_oracle(fpde, oracle, rows):
+    """The partitioned line solve in chunks of 32 to 256 nodes (0: the
+    automatic size), on a 2000-node line with Picard to tol 1e-12 in the
+    cooperative block form: every chunking agrees with the oracle."""
+    nodes, steps = 2000, 130
+    u0 = fpde.init_1d(nodes, 4321)
+    spec = fpde.make_spec(nodes, steps, 1e-3, alpha=0.7, picard_max=3, picard_tol=1e-12, hist_mode="mfma_f64",
+                          block=64, u_top=u0[0], chunk_rows=rows)
+    err = rel_err(fpde.solve(spec, u0), _oracle_1d(oracle, spec, u0, picard_tol=1e-12))
+    print(f"chunk rows {rows}: rel err {err:.3e}", flush=True)
+    assert err <= TOL_F64
 
 
 @pytest.mark.parametrize("tol", [1e-12, 1e-7, 3e-6])
@@ -288,7 +304,7 @@ def test_picard_chunk_halo_tight(fpde, oracle, tol, picard):
     sensitive to the coefficient at the node left of each chunk: that node
     must see the same iterate (the extrapolated start, the converged iterate
     when the tolerance was met one launch late) as the chunk's own nodes.
-    Checked to 1e-12 over three chunks."""
+    Checked to 1e-12 over four chunks."""
     nodes, steps = 600, 60
     u0 = fpde.init_1d(nodes, 78)
     spec = fpde.make_spec(nodes, steps, 1e-3, alpha=0.8, picard_max=picard, picard_tol=tol, hist_mode="plain",
@@ -550,16 +566,17 @@ def test_serving_mix_512_requests(fpde):
 
 
 def test_advance_batch_matches_solo(fpde):
-    """Solves of uneven N, M, alpha, history form, block size and time grid
-    advanced as one batch (one cooperative launch per history block for the
+    """Solves of uneven N, M, alpha, history form, block size, chunk size and
+    time grid advanced as one batch (one cooperative launch per history block for the
     block-mode ones; the plain form on its own stream) give each solo result exactly:
     every solve keeps its own barriers and Picard decisions.  One solver
     starts the batch 37 steps ahead, so the batch's blocks are not aligned;
     another hands a far part already computed ahead over to the batch."""
     cases = [
         dict(n=3000, m=300, alpha=0.6, hist_mode="split_bf16", block=64, picard_max=3, picard_tol=1e-10),
-        dict(n=10000, m=200, alpha=0.85, hist_mode="mfma_f64", block=32, picard_max=3, picard_tol=1e-10),
-        dict(n=777, m=513, alpha=0.35, hist_mode="split_f16", block=128),
+        dict(n=10000, m=200, alpha=0.85, hist_mode="mfma_f64", block=32, picard_max=3, picard_tol=1e-10,
+             chunk_rows=256),
+        dict(n=777, m=513, alpha=0.35, hist_mode="split_f16", block=128, chunk_rows=96),
         dict(n=2048, m=150, alpha=0.7, hist_mode="split_bf16", block=16, do_q=4),
         dict(n=1200, m=400, alpha=0.75, hist_mode="plain", sum_alg=2, sum_param=10),
         dict(n=1500, m=120, alpha=0.5, hist_mode="plain"),
@@ -611,7 +628,7 @@ def test_advance_batch_matches_solo(fpde):
 
 
 def test_advance_batch_of_32_small_solves(fpde):
-    """The largest batch: 32 solves of 5..700 nodes (one to three chunk
+    """The largest batch: 32 solves of 5..700 nodes (one to four chunk
     workgroups each, so the workgroup table has 33 uneven bounds), M from 40 to
     300 (solves leave the batch at different blocks), blocks 16..128, all
     equal to their solo results."""
@@ -683,14 +700,14 @@ def test_batched_3d_steps_equal_solo(fpde):
 
 
 def test_near_full_batch_beside_a_3d_rider(fpde):
-    """Seven N=9000 solves (252 chunk workgroups: one per CU on 256 CUs) in
-    one batch, advanced together with a 64^3 3D solve on its own stream,
-    which is enqueued first, so its sweeps hold CUs while the cooperative
+    """Seven N=9000 solves in 256-node chunks (252 chunk workgroups: one per
+    CU on 256 CUs) in one batch, advanced together with a 64^3 3D solve on
+    its own stream, which is enqueued first, so its sweeps hold CUs while the cooperative
     launches are dispatched: every grid barrier completes within its bound
     and every result equals its solo run."""
     u0 = fpde.init_1d(9000, 77)
     specs = [fpde.make_spec(9000, 256, 1e-3, alpha=0.5 + 0.05 * j, hist_mode="split_bf16", picard_max=2,
-                            picard_tol=1e-12, u_top=u0[0]) for j in range(7)]
+                            picard_tol=1e-12, u_top=u0[0], chunk_rows=256) for j in range(7)]
     u3 = fpde.init_3d(64, 78)
     sp3 = fpde.make_spec(64, 120, 1e-3, alpha=0.85, dims=3, hist_mode="mfma_f64", block=16, u_top=u3[0])
     solvers = [fpde.Solver(sp) for sp in specs] + [fpde.Solver(sp3)]
