@@ -935,7 +935,7 @@ __global__ __launch_bounds__(256) void k_step1d_block(Step1DArgs g) { step_block
 
 // Many independent solves in one cooperative launch: workgroups
 // [t.first[j], t.first[j+1]) are the chunks of solve j.  Each solve keeps its
-// own barrier flags, convergence slots and Picard decisions, so its arithmetic
+// own exchange words, convergence slots and Picard decisions, so its arithmetic
 // is that of its single-solve launch; the fields that change from block to
 // block come with the launch, the rest from the solve's row of `base`.
 __global__ __launch_bounds__(256) void k_step1d_batch(const Step1DArgs* __restrict__ base, BatchTick t) {

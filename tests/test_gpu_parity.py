@@ -1409,7 +1409,7 @@ def test_3d_and_distributed_order_full_size_vs_plain(fpde):
 @pytest.mark.parametrize("nodes", [65536, 40001])
 def test_longest_lines_both_launch_forms(fpde, oracle, nodes):
     """The largest 1D lines (256 chunk workgroups, a 512-row reduced system,
-    256 barrier flags) and a ragged one: both launch forms against the oracle,
+    256 chunks in every grid exchange) and a ragged one: both launch forms against the oracle,
     across two history blocks."""
     steps = 130
     u0 = fpde.init_1d(nodes, 33)
