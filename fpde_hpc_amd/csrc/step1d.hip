@@ -938,7 +938,11 @@ __global__ __launch_bounds__(256) void k_step1d_block(Step1DArgs g) { step_block
 // own exchange words, convergence slots and Picard decisions, so its arithmetic
 // is that of its single-solve launch; the fields that change from block to
 // block come with the launch, the rest from the solve's row of `base`.
-__global__ __launch_bounds__(256) void k_step1d_batch(const Step1DArgs* __restrict__ base, BatchTick t) {
+// Registers are held to 256 per lane (a few spills) so that two workgroups
+// share a CU: a batch holds up to 512 chunk workgroups, and the latency-bound
+// waves of two workgroups interleave on each SIMD.  (The single-solve kernel
+// keeps 272 registers and one workgroup per CU, its fastest form.)
+__global__ __launch_bounds__(256, 2) void k_step1d_batch(const Step1DArgs* __restrict__ base, BatchTick t) {
     const int wg = (int)blockIdx.x;
     int j = 0;
     while (j + 1 < t.count && wg >= t.first[j + 1]) j++;

@@ -218,7 +218,8 @@ typedef struct {
                             waves of rows per workgroup).  256 packs more solves
                             into one batched launch (fpde_solver_advance_batch,
                             the server): ceil(N/chunk_rows) workgroups per
-                            solve, at most 256.  Results depend on the chunking
+                            solve, at most 256; a batch holds 512.  Results
+                            depend on the chunking
                             in the last bits only.  No counterpart in the
                             reference, whose line solve is one serial Thomas
                             sweep (fpde_hpc_1d.cpp:4672-4725)                 */

@@ -700,14 +700,15 @@ def test_batched_3d_steps_equal_solo(fpde):
 
 
 def test_near_full_batch_beside_a_3d_rider(fpde):
-    """Seven N=9000 solves in 256-node chunks (252 chunk workgroups: one per
-    CU on 256 CUs) in one batch, advanced together with a 64^3 3D solve on
-    its own stream, which is enqueued first, so its sweeps hold CUs while the cooperative
-    launches are dispatched: every grid barrier completes within its bound
-    and every result equals its solo run."""
+    """Fourteen N=9000 solves in 256-node chunks (504 chunk workgroups: two
+    per CU on 252 of the 256 CUs, the batched kernel's limit) in one batch,
+    advanced together with a 64^3 3D solve on its own stream, which is
+    enqueued first, so its sweeps hold CUs while the cooperative launches are
+    dispatched: every grid exchange completes within its bound and every
+    result equals its solo run."""
     u0 = fpde.init_1d(9000, 77)
-    specs = [fpde.make_spec(9000, 256, 1e-3, alpha=0.5 + 0.05 * j, hist_mode="split_bf16", picard_max=2,
-                            picard_tol=1e-12, u_top=u0[0], chunk_rows=256) for j in range(7)]
+    specs = [fpde.make_spec(9000, 256, 1e-3, alpha=0.5 + 0.025 * j, hist_mode="split_bf16", picard_max=2,
+                            picard_tol=1e-12, u_top=u0[0], chunk_rows=256) for j in range(14)]
     u3 = fpde.init_3d(64, 78)
     sp3 = fpde.make_spec(64, 120, 1e-3, alpha=0.85, dims=3, hist_mode="mfma_f64", block=16, u_top=u3[0])
     solvers = [fpde.Solver(sp) for sp in specs] + [fpde.Solver(sp3)]
