@@ -408,6 +408,12 @@ fpde_status solver_alloc(fpde_solver* s) {
         DALLOC(s->err, 64);
         FPDE_HIP_TRY(hipMemsetAsync(s->err, 0, 64, s->stream));
         s->wait_ticks = barrier_wait_ticks();
+        // tests of the time-out path force a bound of a few ticks: every
+        // workgroup then ends at its first unmet wait and the solve reports it
+        if (const char* e = getenv("FPDE_TEST_WAIT_TICKS")) {
+            const long long v = atoll(e);
+            if (v > 0) s->wait_ticks = v;
+        }
         // cooperative block launches pay off where a launch covers many steps:
         // the GEMM history forms (one launch per block) and the series form;
         // the plain form needs its history kernel between every two steps

@@ -526,6 +526,49 @@ def test_server_results_same_alone_and_in_mix(fpde):
     srv.close()
 
 
+def test_grid_wait_time_out_is_reported(fpde, monkeypatch):
+    """A grid exchange whose wait runs out of time ends every workgroup of
+    the cooperative launch (each leaves at its first unmet wait) and the
+    solve reports it.  The condition is forced from the host: a bound of one
+    tick of the device's constant clock (FPDE_TEST_WAIT_TICKS, read when a
+    solver is created), so every kernel still exits normally.  A solver made
+    without the bound then solves the same spec as usual."""
+    u0 = fpde.init_1d(3000, 91)
+    spec = fpde.make_spec(3000, 130, 1e-3, alpha=0.7, hist_mode="split_bf16", picard_max=3, picard_tol=1e-12,
+                          u_top=u0[0])
+    monkeypatch.setenv("FPDE_TEST_WAIT_TICKS", "1")
+    with pytest.raises(fpde.FpdeError, match="timed out"):
+        fpde.solve(spec, u0)
+    monkeypatch.delenv("FPDE_TEST_WAIT_TICKS")
+    u = fpde.solve(spec, u0)
+    assert np.isfinite(u).all()
+
+
+def test_server_marks_a_timed_out_request_failed(fpde, monkeypatch):
+    """A served 1D request whose grid exchange times out (bound forced from
+    the host as above, while the server admits it) ends with status 3
+    (failed), gives back its memory and slot, and the server goes on: the
+    next request, admitted without the bound, completes and equals its solo
+    result."""
+    u0 = fpde.init_1d(2500, 92)
+    spec = fpde.make_spec(2500, 200, 1e-3, alpha=0.6, hist_mode="split_bf16", picard_max=2, picard_tol=1e-12,
+                          u_top=u0[0])
+    srv = fpde.Server(mem_budget_bytes=2 * fpde.footprint(spec), max_active=1)
+    try:
+        monkeypatch.setenv("FPDE_TEST_WAIT_TICKS", "1")
+        bad = srv.submit(spec, u0)
+        srv.poll()                                   # admitted (solver created) with the bound
+        monkeypatch.delenv("FPDE_TEST_WAIT_TICKS")
+        srv.run()
+        assert srv.info(bad)["status"] == 3
+        good = srv.submit(spec, u0)
+        srv.run()
+        assert srv.info(good)["status"] == 0
+        np.testing.assert_array_equal(srv.result(good), fpde.solve(spec, u0))
+    finally:
+        srv.close()
+
+
 def test_serving_mix_512_requests(fpde):
     """The serving mix at full size (512 requests, seed 1004: 80% 1D with N
     500..20000 and M 1000..40000, 20% 3D 32^3..96^3, 25% distributed order,
