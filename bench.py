@@ -246,7 +246,7 @@ def ensemble_run(F, count, nodes, msteps, hist_mode, picard, reps=3, uneven=Fals
     else:
         ns, ms = [nodes] * count, [msteps] * count
     u0s = [F.init_1d(n, 700 + i) for i, n in enumerate(ns)]
-    # 256-node chunks: six N=10^4 solves (240 workgroups) fit one launch
+    # 256-node chunks: fewer workgroups per solve, the faster form in a batch
     specs = [F.make_spec(n, m, 1e-3, alpha=float(a), hist_mode=hist_mode, block=64,
                          picard_max=picard[0], picard_tol=picard[1], u_top=u0[0], chunk_rows=256)
              for n, m, a, u0 in zip(ns, ms, alphas, u0s)]

@@ -220,9 +220,9 @@ struct fpde_solver {
     double* spE = nullptr;
     int64_t sp_ldp = 0;
     unsigned long long* xch = nullptr;  // grid exchange words of the cooperative 1D block launches
-    int* err = nullptr;        // barrier time-out flag
+    int* err = nullptr;        // grid-exchange time-out flag, Picard solve count
     bool block_mode = false;   // 1D steps run as cooperative block launches
-    int64_t coop_launches = 0; // selects the barrier counter of the next one
+    int64_t coop_launches = 0; // cooperative launches so far (the next one's exchange tag)
     int* flags = nullptr;
     // graph-captured step loop: the executable graph of the last captured chunk
     hipGraphExec_t gexec = nullptr;

@@ -309,8 +309,7 @@ def serving_mix(n_req=512, rate=16.0, seed=1004, hist_mode="split_bf16"):
     25% distributed order (16 Gauss-Legendre nodes on [0.1, 1]); each with its
     own seeded initial head; the 1D requests take 256-node chunks, so that
     more of them share each batched launch.  Returns (arrival_s, spec, u0,
-    node_steps) tuples
-    in arrival order.  The reference runs such a mix as one process per solve,
+    node_steps) tuples in arrival order.  The reference runs such a mix as one process per solve,
     each configured on its command line (mpi_test.sh:12-23)."""
     rng = np.random.default_rng(seed)
     reqs = []
