@@ -304,11 +304,13 @@ def workload(name, hist_mode="plain", steps=None, nodes=None, **kw):
 
 def serving_mix(n_req=512, rate=16.0, seed=1004, hist_mode="split_bf16"):
     """Requests of the serving mix: Poisson arrivals at `rate` per second; 80%
-    1D (N ~ U[500, 20000], M ~ U[1000, 40000], tau 5e-4, block 64), 20% 3D
+    1D (N ~ U[500, 20000], M ~ U[1000, 40000], tau 5e-4, block 128), 20% 3D
     (32^3..96^3, M ~ U[200, 1000], tau 1e-3, block 16); alpha ~ U(0.3, 0.95);
     25% distributed order (16 Gauss-Legendre nodes on [0.1, 1]); each with its
-    own seeded initial head; the 1D requests take 256-node chunks, so that
-    more of them share each batched launch.  Returns (arrival_s, spec, u0,
+    own seeded initial head.  The 1D requests are set for batched launches:
+    256-node chunks, so that more of them share a launch, and history blocks
+    of 128 steps, which halve their far-part GEMMs (on a batch's critical
+    path; a single solve is fastest at 64).  Returns (arrival_s, spec, u0,
     node_steps) tuples in arrival order.  The reference runs such a mix as one process per solve,
     each configured on its command line (mpi_test.sh:12-23)."""
     rng = np.random.default_rng(seed)
@@ -328,7 +330,7 @@ def serving_mix(n_req=512, rate=16.0, seed=1004, hist_mode="split_bf16"):
             n = int(rng.integers(500, 20001))
             M = int(rng.integers(1000, 40001))
             u0 = init_1d(n, 100000 + i)
-            kw = dict(dims=1, block=64, chunk_rows=256)
+            kw = dict(dims=1, block=128, chunk_rows=256)
         if do:
             kw.update(do_q=16, do_rule=0, do_a0=0.1, do_a1=1.0)
         tau = 1e-3 if is3d else 5e-4
