@@ -305,7 +305,7 @@ def workload(name, hist_mode="plain", steps=None, nodes=None, **kw):
 def serving_mix(n_req=512, rate=16.0, seed=1004, hist_mode="split_bf16"):
     """Requests of the serving mix: Poisson arrivals at `rate` per second; 80%
     1D (N ~ U[500, 20000], M ~ U[1000, 40000], tau 5e-4, block 128), 20% 3D
-    (32^3..96^3, M ~ U[200, 1000], tau 1e-3, block 16); alpha ~ U(0.3, 0.95);
+    (32^3..96^3, M ~ U[200, 1000], tau 1e-3, block 32); alpha ~ U(0.3, 0.95);
     25% distributed order (16 Gauss-Legendre nodes on [0.1, 1]); each with its
     own seeded initial head.  The 1D requests are set for batched launches:
     256-node chunks, so that more of them share a launch, and history blocks
@@ -325,7 +325,7 @@ def serving_mix(n_req=512, rate=16.0, seed=1004, hist_mode="split_bf16"):
             n = int(rng.integers(32, 97))
             M = int(rng.integers(200, 1001))
             u0 = init_3d(n, 100000 + i)
-            kw = dict(dims=3, block=16)
+            kw = dict(dims=3, block=32)
         else:
             n = int(rng.integers(500, 20001))
             M = int(rng.integers(1000, 40001))
