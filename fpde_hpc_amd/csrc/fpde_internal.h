@@ -243,6 +243,9 @@ struct Step1DArgs {
     int n_end;
     int picard_max;
     int n0;                   // first step of the history block (far partial row = n - n0)
+    int blk;                  // rows of the history block (far partial and mid rows)
+    double* mid;              // near-field groups folded ahead: mid[p - n0][i] = sum over the
+                              // folded 16-level groups of the step-p weights times D (null: none)
     unsigned long long* xch;  // grid exchange words [2 parities][kXchBytes / 16] (tagged halves)
     unsigned xtag;            // this launch's tag base: (launch count mod 65535 + 1) << 16
     int* err;                 // [0] set when a grid barrier times out; [1] Picard solves (cooperative launches)

@@ -164,6 +164,8 @@ size_t footprint_bytes(const fpde_spec& s) {
     }
     b += sizeof(double) * (size_t)p.part_rows * p.part_bufs * p.ld;   // partial sums
     b += sizeof(double) * (size_t)p.ld * 8;                      // U, S, xyz, ...
+    if (s.dims == 1 && s.hist_mode != HIST_PLAIN_F64 && s.sum_alg != 2)
+        b += sizeof(double) * (size_t)s.block * p.ld;                 // folded near-field groups
     if (s.dims == 3) b += sizeof(double) * (size_t)p.ld * 6;     // Uprev, K, iC, Al, Bt
     b += sizeof(double) * (size_t)(s.steps + 512) * 3;           // weights
     if (s.sum_alg == 2) {
@@ -202,6 +204,7 @@ struct fpde_solver {
     double* part = nullptr;
     double* Uit = nullptr;
     double* dmax = nullptr;
+    double* mid = nullptr;     // near-field groups folded ahead by the cooperative launches [block][ld]
     double* H = nullptr;       // sum_alg 2 moments [P][ld]
     double* Uit0 = nullptr;    // initial Picard iterate
     double* uend = nullptr;    // chunk-end values by step parity
@@ -432,6 +435,7 @@ fpde_status solver_alloc(fpde_solver* s) {
         // with helper workgroups for the sum measured no faster)
         s->block_mode = (sp.step_mode == 0) && blocked_history && !space && step1d_block_supported(s->C);
         s->la = s->block_mode && lookahead_spec(sp);
+        if (s->block_mode && s->w.ser_P == 0) DALLOC(s->mid, sizeof(double) * (size_t)sp.block * p.ld);
         DALLOC(s->dmax, sizeof(double) * 2 * s->C);
         DALLOC(s->flags, sizeof(int) * 16);
     } else {
@@ -830,6 +834,8 @@ Step1DArgs block_args(fpde_solver* s, int n, int n_end, int n0, int nsplit, int6
     a.n = n;
     a.n_end = n_end;
     a.n0 = n0;
+    a.blk = sp.block;
+    a.mid = s->mid;
     a.picard_max = sp.picard_max;
     a.tol = sp.picard_tol;
     a.extrap = sp.picard_max > 1 ? 1 : 0;

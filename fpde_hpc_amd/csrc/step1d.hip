@@ -149,12 +149,17 @@ __device__ inline int chunk_solve(double (*pA)[256], double (*pC)[256], double (
     return pp;
 }
 
+// Levels per near-field group folded ahead by the cooperative form.
+constexpr int kNearGroup = 16;
+
 // Far partial sums plus the near levels t in [near_lo, n-2] already in
 // memory, gathered in 32-wide batches of independent loads (a sum over m
 // terms costs ~m/32 memory round trips).  Every load is guarded: omega is
-// read only at n-t-1 >= 1.
+// read only at n-t-1 >= 1.  With `folded` > 0 the first folded * 16 near
+// levels come as one value of g.mid (near_fold) and the loads start after
+// them.
 __device__ inline double hist_mem(const Step1DArgs& g, int i, int n, int64_t part_off,
-                                  const double* __restrict__ omega) {
+                                  const double* __restrict__ omega, int folded = 0) {
     constexpr int W = 32;
     double far = 0.0;
     for (int sp0 = 0; sp0 < g.nsplit; sp0 += W) {
@@ -165,9 +170,9 @@ __device__ inline double hist_mem(const Step1DArgs& g, int i, int n, int64_t par
 #pragma unroll
         for (int j = 0; j < W; j++) far += v[j];
     }
-    double nr0 = 0.0, nr1 = 0.0;
+    double nr0 = folded > 0 ? g.mid[(int64_t)(n - g.n0) * g.ld + i] : 0.0, nr1 = 0.0;
     const int t_end = n - 1;   // exclusive: D[n-1] is the newest level, added by the caller
-    for (int t0 = g.near_lo; t0 < t_end; t0 += W) {
+    for (int t0 = g.near_lo + kNearGroup * folded; t0 < t_end; t0 += W) {
         double v[W], w[W];
 #pragma unroll
         for (int j = 0; j < W; j++) {
@@ -568,11 +573,50 @@ __device__ inline void xput(unsigned long long* x, int slot, unsigned tag, doubl
 
 // History term of step n at node i (levels t < n-1 in memory, the newest
 // difference dnew in a register, the two-level corrections of the oldest
-// intervals from d0m, d1m; Toeplitz weights or the step's own row).
-__device__ inline double hist_step(const Step1DArgs& g, int i, int n, double dnew, double d0m, double d1m) {
+// intervals from d0m, d1m; Toeplitz weights or the step's own row; the
+// first `folded` near-field groups from g.mid).
+__device__ inline double hist_step(const Step1DArgs& g, int i, int n, double dnew, double d0m, double d1m,
+                                   int folded = 0) {
     const double* om = g.omega + (int64_t)(n - g.n0) * g.omega_stride;
-    double hm = hist_mem(g, i, n, (int64_t)(n - g.n0) * g.ld, om);
+    double hm = hist_mem(g, i, n, (int64_t)(n - g.n0) * g.ld, om, folded);
     return hm + hist_tail(om[0], g.corr[2 * n], g.corr[2 * n + 1], n, g.near_lo, dnew, d0m, d1m);
+}
+
+// Fold the near-field group of levels [lo, lo + 16) of node i into g.mid for
+// the steps p in [p_lo, p_hi) of the history block:
+//   mid[p - n0][i] = (first ? 0 : mid[p - n0][i]) + sum_k w_p[p - lo - k - 1] D[lo + k][i],
+// w_p the weights of step p (one Toeplitz vector or the step's own row).
+// The steps in 8-wide batches: their mid values load together, then the sums.
+__device__ inline void near_fold(const Step1DArgs& g, int i, int lo, int p_lo, int p_hi, bool first) {
+    const double* __restrict__ Dm = g.D;
+    const double* __restrict__ W = g.omega;
+    double* __restrict__ M = g.mid;
+    double dv[kNearGroup];
+#pragma unroll
+    for (int k = 0; k < kNearGroup; k++) dv[k] = Dm[(int64_t)(lo + k) * g.ld + i];
+    constexpr int PB = 8;
+    for (int p0 = p_lo; p0 < p_hi; p0 += PB) {
+        double acc[PB];
+#pragma unroll
+        for (int j = 0; j < PB; j++) {
+            const int p = p0 + j;
+            acc[j] = (!first && p < p_hi) ? M[(int64_t)(p - g.n0) * g.ld + i] : 0.0;
+        }
+#pragma unroll
+        for (int j = 0; j < PB; j++) {
+            const int p = p0 + j;
+            if (p < p_hi) {
+                const double* om = W + (int64_t)(p - g.n0) * g.omega_stride + (p - lo - 1);
+#pragma unroll
+                for (int k = 0; k < kNearGroup; k++) acc[j] = fma(om[-k], dv[k], acc[j]);
+            }
+        }
+#pragma unroll
+        for (int j = 0; j < PB; j++) {
+            const int p = p0 + j;
+            if (p < p_hi) M[(int64_t)(p - g.n0) * g.ld + i] = acc[j];
+        }
+    }
 }
 
 // Steps [g.n, g.n_end) of one solve by its chunk workgroup c (0 <= c < g.C):
@@ -614,6 +658,18 @@ __device__ __forceinline__ void step_block(const Step1DArgs& g, const int c) {
     double d0m = (!series && act && n_first >= 1) ? g.D[i] : 0.0;
     double d1m = (!series && act && n_first >= 2) ? g.D[g.ld + i] : 0.0;
     int bk = 0;                                                    // Picard barriers passed in this launch
+    // Near-field groups of this history block folded into g.mid so far.
+    // The first build of a step s of the block that has a later step in the
+    // block folds every group j (levels near_lo + 16 j ...+15) with
+    // near_lo + 16 j + 17 <= s, i.e. whose levels are final (t <= s - 2)
+    // whatever the previous step's decision; after the steps [n0, s] of the
+    // block that makes (s - near_lo - 1) / 16 groups.  Those of earlier
+    // launches of the block are counted here.
+    int folded = 0;
+    if (g.mid) {
+        const int s_last = min(n_first - 1, g.n0 + g.blk - 2);
+        folded = s_last >= g.n0 ? (s_last - g.near_lo - 1) / kNearGroup : 0;
+    }
     // the exchange words of this workgroup, both parities, zeroed before its
     // first publication (its rows 2c and 2c+1, its measure)
     auto xbase = [&](int b) { return g.xch + (size_t)(b & 1) * kXchWords; };
@@ -722,7 +778,7 @@ __device__ __forceinline__ void step_block(const Step1DArgs& g, const int c) {
     // (far partials and near levels t < n), formed while a barrier is pending
     double hm_next = 0.0;
     double tl_next[3] = {0.0, 0.0, 0.0};       // and its tail weights: omega[0], corr[2n], corr[2n+1]
-    auto build = [&](double uit, double uith, double S, double Ln, double un_, int pre_n) -> bool {
+    auto build = [&](double uit, double uith, double S, double Ln, double un_, int pre_n, int fold_n) -> bool {
         double K = 0.0, invC = 0.0;
         if (halo) {
             double Kl, iCl;
@@ -773,12 +829,21 @@ __device__ __forceinline__ void step_block(const Step1DArgs& g, const int c) {
             xput(xw, 4 * (2 * c + 1) + 2, tg, cr);
             xput(xw, 4 * (2 * c + 1) + 3, tg, dr - ar * x);
         }
+        // the next near-field group whose levels are final (t <= step - 2)
+        // folded for the later steps of the block, while the exchange is
+        // pending (first build of a step only)
+        if (fold_n >= 0 && g.mid) {
+            while (g.near_lo + kNearGroup * folded + kNearGroup + 1 <= fold_n && fold_n + 1 < g.n0 + g.blk) {
+                if (act) near_fold(g, i, g.near_lo + kNearGroup * folded, fold_n + 1, g.n0 + g.blk, folded == 0);
+                folded++;
+            }
+        }
         if (pre_n >= 0 && act) {
             const double* om = g.omega + (int64_t)(pre_n - g.n0) * g.omega_stride;
             tl_next[0] = om[0];
             tl_next[1] = g.corr[2 * pre_n];
             tl_next[2] = g.corr[2 * pre_n + 1];
-            hm_next = hist_mem(g, i, pre_n, (int64_t)(pre_n - g.n0) * g.ld, om);
+            hm_next = hist_mem(g, i, pre_n, (int64_t)(pre_n - g.n0) * g.ld, om, folded);
         }
         bk++;                                  // the solve reads this barrier's words
         return true;
@@ -869,7 +934,7 @@ __device__ __forceinline__ void step_block(const Step1DArgs& g, const int c) {
             } else if (have_hm) {
                 S = hm_cur + hist_tail(tl_cur[0], tl_cur[1], tl_cur[2], n, g.near_lo, dlast, d0m, d1m);
             } else {
-                S = hist_step(g, i, n, dlast, d0m, d1m);
+                S = hist_step(g, i, n, dlast, d0m, d1m, folded);
             }
         }
         // the next step's in-memory history part goes with this step's first
@@ -895,7 +960,7 @@ __device__ __forceinline__ void step_block(const Step1DArgs& g, const int c) {
                 if (measures_max() <= g.tol) break;           // U^k is the result (uniform decision)
             }
             const bool pend_check = k == 0 && pending;
-            if (!build(uit, uith, S, Ln, un, k == 0 ? pre_n : -1)) return;
+            if (!build(uit, uith, S, Ln, un, k == 0 ? pre_n : -1, k == 0 ? n : -1)) return;
             const int r = solve(pend_check ? 2 : ((k >= 1 && !last_due) ? 1 : 0), uit, uith);
             if (r < 0) return;                                 // a barrier wait ran out of time
             if (r == 1) break;                                 // U^k is the result
@@ -907,7 +972,7 @@ __device__ __forceinline__ void step_block(const Step1DArgs& g, const int c) {
             // the tentatively ended step sv_n needs its last iteration
             un = sv_un; uh = sv_uh; dh = sv_dh; dlast = sv_dlast; d0m = sv_d0m; d1m = sv_d1m;
             double ruit = sv_uit, ruith = sv_uith;
-            if (!build(ruit, ruith, sv_S, sv_Ln, un, -1)) return;
+            if (!build(ruit, ruith, sv_S, sv_Ln, un, -1, -1)) return;
             if (solve(0, ruit, ruith) < 0) return;
             nsolve++;
             finish(sv_n, ruit, ruith);
