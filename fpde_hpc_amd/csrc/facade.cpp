@@ -461,7 +461,7 @@ fpde_status fpde_history_sum_f64(double* S, const double* D, int64_t n, int64_t 
     fpde_status rc = launch_hist_plain(D, ld, (int)nodes, (int)n, (int)n, omega, part, &splits, 0);
     if (rc == FPDE_OK) {
         // combine: S = sum of partial rows (no near levels, no corrections)
-        rc = launch_hist_combine(S, part, splits, ld, 0, D, ld, nodes, 0, 0, omega, nullptr, 1.0, 0);
+        rc = launch_hist_combine(S, part, splits, ld, 0, D, ld, nodes, 0, 0, omega, 0, 0, 0, nullptr, nullptr, 1.0, 0);
     }
     hipError_t e = hipDeviceSynchronize();
     hipFree(part);
@@ -553,7 +553,7 @@ fpde_status fpde_history_block_gemm(int mode, const double* D, int64_t ld, int64
         // P[r] = sum over K splits (combine kernel, one row at a time)
         for (int r = 0; r < block && rc == FPDE_OK; r++)
             rc = launch_hist_combine(P + (size_t)r * ld, part, splits, (int64_t)block * ld, (int64_t)r * ld, D, ld,
-                                     nodes, 0, 0, omega, nullptr, 1.0, 0);
+                                     nodes, 0, 0, omega, 0, 0, 0, nullptr, nullptr, 1.0, 0);
     }
     e = hipDeviceSynchronize();
     if (rc == FPDE_OK && e != hipSuccess) { rc = FPDE_ERR_HIP; set_error(hipGetErrorString(e)); }

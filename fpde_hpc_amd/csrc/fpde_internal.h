@@ -90,6 +90,46 @@ __device__ inline double rcp_nr(double x) {
     return fma(r, e, r);
 }
 
+// Near-field history folded ahead in groups of 16 levels (1D cooperative
+// step, 3D history combine): the group [lo, lo + 16) of node i goes into
+// mid for the steps p in [p_lo, p_hi) of the history block that starts at
+// step n0:
+//   mid[p - n0][i] = (first ? 0 : mid[p - n0][i]) + sum_k w_p[p - lo - k - 1] D[lo + k][i],
+// w_p = omega + (p - n0) * omega_stride (one Toeplitz vector, stride 0, or
+// the step's own row).  The steps go in 8-wide batches: their mid values
+// load together, then the sums.
+constexpr int kNearGroup = 16;
+__device__ inline void fold_near_group(const double* __restrict__ D, int64_t ld, const double* __restrict__ omega,
+                                       int64_t omega_stride, int n0, double* __restrict__ mid, int64_t i, int lo,
+                                       int p_lo, int p_hi, bool first) {
+    double dv[kNearGroup];
+#pragma unroll
+    for (int k = 0; k < kNearGroup; k++) dv[k] = D[(int64_t)(lo + k) * ld + i];
+    constexpr int PB = 8;
+    for (int p0 = p_lo; p0 < p_hi; p0 += PB) {
+        double acc[PB];
+#pragma unroll
+        for (int j = 0; j < PB; j++) {
+            const int p = p0 + j;
+            acc[j] = (!first && p < p_hi) ? mid[(int64_t)(p - n0) * ld + i] : 0.0;
+        }
+#pragma unroll
+        for (int j = 0; j < PB; j++) {
+            const int p = p0 + j;
+            if (p < p_hi) {
+                const double* om = omega + (int64_t)(p - n0) * omega_stride + (p - lo - 1);
+#pragma unroll
+                for (int k = 0; k < kNearGroup; k++) acc[j] = fma(om[-k], dv[k], acc[j]);
+            }
+        }
+#pragma unroll
+        for (int j = 0; j < PB; j++) {
+            const int p = p0 + j;
+            if (p < p_hi) mid[(int64_t)(p - n0) * ld + i] = acc[j];
+        }
+    }
+}
+
 // one DPP move of a double (both 32-bit halves) inside the rows of 16 lanes
 template <int CTRL>
 __device__ inline double dpp_move(double v) {
@@ -420,6 +460,10 @@ struct Lod3DRow {
     const double *part, *D, *omega, *corr, *utop;
     int64_t part_stride, part_off, ld, nodes;
     int n, nsplit, step, near_lo, coef_mode, split;
+    // history block: first step, rows, weight-row stride, folded near groups
+    int n0, blk;
+    int64_t omega_stride;
+    double* mid;
     double h, h2inv, ksat, L, hscale;
     // arguments the solo sweeps take that a batched solve leaves at their
     // neutral values: kap = 1 and no line sums E (no space-fractional term),
@@ -507,9 +551,15 @@ fpde_status launch_space3d(const double* U, const double* K, int n, int axis, do
                            const double* sp_corr, double* E, hipStream_t st);
 fpde_status launch_series3d(double* S, double* H, int P, const double* dnew, const double* ser_c,
                             const double* ser_v, int64_t nodes, int64_t ld, double hscale, hipStream_t st);
+// 3D history term of step n (far partials, near levels [near_lo, n),
+// corrections); omega is the weight row of the block's first step n0 (rows
+// omega_stride apart on a prescribed grid, else one Toeplitz vector); with
+// mid (a block of blk rows) the near levels go in 16-level groups folded
+// ahead, each at the step that completes it.
 fpde_status launch_hist_combine(double* S, const double* part, int nsplit, int64_t part_stride, int64_t part_off,
                                 const double* D, int64_t ld, int64_t nodes, int n, int near_lo,
-                                const double* omega, const double* corr, double hscale, hipStream_t st);
+                                const double* omega, int64_t omega_stride, int n0, int blk, double* mid,
+                                const double* corr, double hscale, hipStream_t st);
 int gemm_max_splits(int64_t nodes);
 
 }  // namespace fpde

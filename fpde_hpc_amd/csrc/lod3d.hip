@@ -488,11 +488,19 @@ __global__ __launch_bounds__(kZThreads, 2) void k_zline3d_seg(double* __restrict
 
 // S[i] = sum of far partials + near levels [near_lo, n) + two-level corrections,
 // times hscale (1, or 1/3 when every sweep carries a third of the term).
+// omega is the weight row of the block's first step n0 (rows omega_stride
+// apart).  With mid, the 16-level groups [near_lo + 16 j, +16) of the near
+// field are folded into mid for the steps [s, n0 + blk) at the step s that
+// completes them (the block's first step folds those already complete), and
+// step n takes the complete groups as one mid value plus the levels after
+// them.
 __device__ __forceinline__ void hist_combine_body(double* __restrict__ S, const double* __restrict__ part,
                                                   int nsplit, int64_t part_stride, int64_t part_off,
                                                   const double* __restrict__ D, int64_t ld, int64_t nodes, int n,
-                                                  int near_lo, const double* __restrict__ omega,
+                                                  int near_lo, const double* __restrict__ omega_blk,
+                                                  int64_t omega_stride, int n0, int blk, double* __restrict__ mid,
                                                   const double* __restrict__ corr, double hscale, int bid) {
+    const double* __restrict__ omega = omega_blk + (int64_t)(n - n0) * omega_stride;
     int64_t i = (int64_t)bid * 256 + threadIdx.x;
     if (i >= nodes) return;
     constexpr int W = 16;      // independent loads per batch
@@ -505,7 +513,19 @@ __device__ __forceinline__ void hist_combine_body(double* __restrict__ S, const 
         for (int j = 0; j < W; j++) far += v[j];
     }
     double nr0 = 0.0, nr1 = 0.0;
-    for (int t0 = near_lo; t0 < n; t0 += W) {
+    int lo = near_lo;
+    if (mid) {
+        const int done = n > near_lo ? (n - near_lo) / kNearGroup : 0;       // groups complete by now
+        const int before = (n > n0 && n - 1 > near_lo) ? (n - 1 - near_lo) / kNearGroup : 0;
+        for (int j = before; j < done; j++)
+            fold_near_group(D, ld, omega_blk, omega_stride, n0, mid, i, near_lo + kNearGroup * j, n, n0 + blk,
+                            j == 0);
+        if (done > 0) {
+            nr0 = mid[(int64_t)(n - n0) * ld + i];
+            lo = near_lo + kNearGroup * done;
+        }
+    }
+    for (int t0 = lo; t0 < n; t0 += W) {
         double v[W], w[W];
 #pragma unroll
         for (int j = 0; j < W; j++) {
@@ -529,9 +549,11 @@ __global__ __launch_bounds__(256) void k_hist_combine(double* __restrict__ S, co
                                                       int nsplit, int64_t part_stride, int64_t part_off,
                                                       const double* __restrict__ D, int64_t ld, int64_t nodes,
                                                       int n, int near_lo, const double* __restrict__ omega,
+                                                      int64_t omega_stride, int n0, int blk,
+                                                      double* __restrict__ mid,
                                                       const double* __restrict__ corr, double hscale) {
-    hist_combine_body(S, part, nsplit, part_stride, part_off, D, ld, nodes, n, near_lo, omega, corr, hscale,
-                      blockIdx.x);
+    hist_combine_body(S, part, nsplit, part_stride, part_off, D, ld, nodes, n, near_lo, omega, omega_stride, n0,
+                      blk, mid, corr, hscale, blockIdx.x);
 }
 
 // Series-compressed history (sum_alg 2): add the newest interval dnew to the
@@ -561,7 +583,8 @@ __global__ __launch_bounds__(256) void k_hist_combine_batch(Lod3DBatch b) {
     const int j = batch_slot(b.first_node, b.count, blockIdx.x);
     const Lod3DRow& r = b.row[j];
     hist_combine_body(r.S, r.part, r.nsplit, r.part_stride, r.part_off, r.D, r.ld, r.nodes, r.step, r.near_lo,
-                      r.omega, r.corr, r.hscale, (int)blockIdx.x - b.first_node[j]);
+                      r.omega, r.omega_stride, r.n0, r.blk, r.mid, r.corr, r.hscale,
+                      (int)blockIdx.x - b.first_node[j]);
 }
 
 // y sweep (axis 2) carries the history term; the x sweep (axis 1) a third of
@@ -643,10 +666,11 @@ fpde_status launch_zline3d(double* U, double* Uprev, const double* utop, int n, 
 
 fpde_status launch_hist_combine(double* S, const double* part, int nsplit, int64_t part_stride,
                                 int64_t part_off, const double* D, int64_t ld, int64_t nodes, int n,
-                                int near_lo, const double* omega, const double* corr, double hscale,
-                                hipStream_t st) {
+                                int near_lo, const double* omega, int64_t omega_stride, int n0, int blk,
+                                double* mid, const double* corr, double hscale, hipStream_t st) {
     hipLaunchKernelGGL(k_hist_combine, dim3((unsigned)((nodes + 255) / 256)), dim3(256), 0, st, S, part,
-                       nsplit, part_stride, part_off, D, ld, nodes, n, near_lo, omega, corr, hscale);
+                       nsplit, part_stride, part_off, D, ld, nodes, n, near_lo, omega, omega_stride, n0, blk, mid,
+                       corr, hscale);
     return FPDE_OK;
 }
 

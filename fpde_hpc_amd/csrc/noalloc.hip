@@ -77,7 +77,7 @@ extern "C" fpde_status fpde_history_sum_f64_noalloc(double* S, const double* D, 
     int splits = 0;
     rc = launch_hist_plain(D, ld, (int)nodes, (int)n, (int)n, omega, part, &splits, 0);
     if (rc != FPDE_OK) return rc;
-    rc = launch_hist_combine(S, part, splits, ld, 0, D, ld, nodes, 0, 0, omega, nullptr, 1.0, 0);
+    rc = launch_hist_combine(S, part, splits, ld, 0, D, ld, nodes, 0, 0, omega, 0, 0, 0, nullptr, nullptr, 1.0, 0);
     if (rc != FPDE_OK) return rc;
     FPDE_HIP_TRY(hipGetLastError());
     FPDE_HIP_TRY(hipDeviceSynchronize());

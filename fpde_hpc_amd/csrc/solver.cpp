@@ -164,7 +164,7 @@ size_t footprint_bytes(const fpde_spec& s) {
     }
     b += sizeof(double) * (size_t)p.part_rows * p.part_bufs * p.ld;   // partial sums
     b += sizeof(double) * (size_t)p.ld * 8;                      // U, S, xyz, ...
-    if (s.dims == 1 && s.hist_mode != HIST_PLAIN_F64 && s.sum_alg != 2)
+    if (s.hist_mode != HIST_PLAIN_F64 && s.sum_alg != 2)
         b += sizeof(double) * (size_t)s.block * p.ld;                 // folded near-field groups
     if (s.dims == 3) b += sizeof(double) * (size_t)p.ld * 6;     // Uprev, K, iC, Al, Bt
     b += sizeof(double) * (size_t)(s.steps + 512) * 3;           // weights
@@ -204,7 +204,7 @@ struct fpde_solver {
     double* part = nullptr;
     double* Uit = nullptr;
     double* dmax = nullptr;
-    double* mid = nullptr;     // near-field groups folded ahead by the cooperative launches [block][ld]
+    double* mid = nullptr;     // near-field groups folded ahead [block][ld] (1D cooperative launches, 3D combine)
     double* H = nullptr;       // sum_alg 2 moments [P][ld]
     double* Uit0 = nullptr;    // initial Picard iterate
     double* uend = nullptr;    // chunk-end values by step parity
@@ -451,6 +451,9 @@ fpde_status solver_alloc(fpde_solver* s) {
         DALLOC(s->iC, sizeof(double) * p.ld);
         DALLOC(s->Al, sizeof(double) * (size_t)(n + 1) * n * n);
         DALLOC(s->Bt, sizeof(double) * (size_t)(n + 1) * n * n);
+        // the history combine folds its near field in 16-level groups (GEMM forms)
+        if (sp.hist_mode != HIST_PLAIN_F64 && s->w.ser_P == 0)
+            DALLOC(s->mid, sizeof(double) * (size_t)sp.block * p.ld);
     }
     s->h = sp.length / (sp.nodes - 1);
     fpde_status rc = launch_weights(sp, s->w, s->stream);
@@ -943,6 +946,17 @@ fpde_status finalize_1d(fpde_solver* s) {
     return launch_1d(s, s->step, -1, 0, 0, 0, s->step);
 }
 
+// Weight rows of a 3D step's history term: the GEMM forms give the row of
+// the block's first step (one Toeplitz vector, or on a prescribed grid the
+// block's rows ldwn apart); the plain form the weights of step n itself
+// (far_part set them).
+struct HistRows { const double* omega; int64_t stride; int n0; };
+HistRows hist_rows(const fpde_solver* s, int n) {
+    if (s->spec.hist_mode != HIST_PLAIN_F64)
+        return s->w.grid ? HistRows{s->Wnear, s->ldwn, s->n0} : HistRows{s->w.omega, 0, s->n0};
+    return HistRows{(s->w.grid && s->omega_cur) ? s->omega_cur : s->w.omega, 0, n};
+}
+
 fpde_status enqueue_3d_step(fpde_solver* s, int n) {
     const fpde_spec& sp = s->spec;
     const Plan& p = s->plan;
@@ -965,10 +979,9 @@ fpde_status enqueue_3d_step(fpde_solver* s, int n) {
         rc = launch_series3d(s->S, s->H, s->w.ser_P, dnew, s->w.ser_c + row, s->w.ser_v + row, p.nodes, p.ld,
                              hscale, s->stream);
     } else {
-        // on a grid the near levels take this step's weight row (far_part set it)
-        const double* om = (s->w.grid && s->omega_cur) ? s->omega_cur : s->w.omega;
-        rc = launch_hist_combine(s->S, s->part, nsplit, stride, off, s->D, p.ld, p.nodes, n, near_lo, om,
-                                 s->w.corr, hscale, s->stream);
+        const HistRows hr = hist_rows(s, n);
+        rc = launch_hist_combine(s->S, s->part, nsplit, stride, off, s->D, p.ld, p.nodes, n, near_lo, hr.omega,
+                                 hr.stride, hr.n0, sp.block, s->mid, s->w.corr, hscale, s->stream);
     }
     ev_end(s, slot);
     if (rc != FPDE_OK) return rc;
@@ -1076,7 +1089,8 @@ fpde_status advance_3d_batch(fpde_solver* const* v, int count, int nsteps) {
             r.U = s->U; r.S = s->S; r.Uprev = s->Uprev;
             r.Dlev = s->D + (size_t)n * p.ld;
             r.part = s->part; r.D = s->D;
-            r.omega = (s->w.grid && s->omega_cur) ? s->omega_cur : s->w.omega;
+            const HistRows hr = hist_rows(s, n);
+            r.omega = hr.omega; r.omega_stride = hr.stride; r.n0 = hr.n0; r.blk = sp.block; r.mid = s->mid;
             r.corr = s->w.corr; r.utop = s->utop;
             r.part_stride = stride; r.part_off = off; r.ld = p.ld; r.nodes = p.nodes;
             r.n = sp.nodes; r.nsplit = nsplit; r.step = n; r.near_lo = near_lo; r.coef_mode = sp.coef_mode;

@@ -149,14 +149,11 @@ __device__ inline int chunk_solve(double (*pA)[256], double (*pC)[256], double (
     return pp;
 }
 
-// Levels per near-field group folded ahead by the cooperative form.
-constexpr int kNearGroup = 16;
-
 // Far partial sums plus the near levels t in [near_lo, n-2] already in
 // memory, gathered in 32-wide batches of independent loads (a sum over m
 // terms costs ~m/32 memory round trips).  Every load is guarded: omega is
 // read only at n-t-1 >= 1.  With `folded` > 0 the first folded * 16 near
-// levels come as one value of g.mid (near_fold) and the loads start after
+// levels come as one value of g.mid (fold_near_group) and the loads start after
 // them.
 __device__ inline double hist_mem(const Step1DArgs& g, int i, int n, int64_t part_off,
                                   const double* __restrict__ omega, int folded = 0) {
@@ -582,43 +579,6 @@ __device__ inline double hist_step(const Step1DArgs& g, int i, int n, double dne
     return hm + hist_tail(om[0], g.corr[2 * n], g.corr[2 * n + 1], n, g.near_lo, dnew, d0m, d1m);
 }
 
-// Fold the near-field group of levels [lo, lo + 16) of node i into g.mid for
-// the steps p in [p_lo, p_hi) of the history block:
-//   mid[p - n0][i] = (first ? 0 : mid[p - n0][i]) + sum_k w_p[p - lo - k - 1] D[lo + k][i],
-// w_p the weights of step p (one Toeplitz vector or the step's own row).
-// The steps in 8-wide batches: their mid values load together, then the sums.
-__device__ inline void near_fold(const Step1DArgs& g, int i, int lo, int p_lo, int p_hi, bool first) {
-    const double* __restrict__ Dm = g.D;
-    const double* __restrict__ W = g.omega;
-    double* __restrict__ M = g.mid;
-    double dv[kNearGroup];
-#pragma unroll
-    for (int k = 0; k < kNearGroup; k++) dv[k] = Dm[(int64_t)(lo + k) * g.ld + i];
-    constexpr int PB = 8;
-    for (int p0 = p_lo; p0 < p_hi; p0 += PB) {
-        double acc[PB];
-#pragma unroll
-        for (int j = 0; j < PB; j++) {
-            const int p = p0 + j;
-            acc[j] = (!first && p < p_hi) ? M[(int64_t)(p - g.n0) * g.ld + i] : 0.0;
-        }
-#pragma unroll
-        for (int j = 0; j < PB; j++) {
-            const int p = p0 + j;
-            if (p < p_hi) {
-                const double* om = W + (int64_t)(p - g.n0) * g.omega_stride + (p - lo - 1);
-#pragma unroll
-                for (int k = 0; k < kNearGroup; k++) acc[j] = fma(om[-k], dv[k], acc[j]);
-            }
-        }
-#pragma unroll
-        for (int j = 0; j < PB; j++) {
-            const int p = p0 + j;
-            if (p < p_hi) M[(int64_t)(p - g.n0) * g.ld + i] = acc[j];
-        }
-    }
-}
-
 // Steps [g.n, g.n_end) of one solve by its chunk workgroup c (0 <= c < g.C):
 // the body of the single-solve launch and of the batched one.
 __device__ __forceinline__ void step_block(const Step1DArgs& g, const int c) {
@@ -834,7 +794,9 @@ __device__ __forceinline__ void step_block(const Step1DArgs& g, const int c) {
         // pending (first build of a step only)
         if (fold_n >= 0 && g.mid) {
             while (g.near_lo + kNearGroup * folded + kNearGroup + 1 <= fold_n && fold_n + 1 < g.n0 + g.blk) {
-                if (act) near_fold(g, i, g.near_lo + kNearGroup * folded, fold_n + 1, g.n0 + g.blk, folded == 0);
+                if (act)
+                    fold_near_group(g.D, g.ld, g.omega, g.omega_stride, g.n0, g.mid, i,
+                                    g.near_lo + kNearGroup * folded, fold_n + 1, g.n0 + g.blk, folded == 0);
                 folded++;
             }
         }
