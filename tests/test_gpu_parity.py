@@ -437,6 +437,36 @@ def test_generalized_kernel_3d_and_distributed_order(fpde, oracle, mode):
     assert err <= tol
 
 
+@pytest.mark.parametrize("fk", [0, 3])
+@pytest.mark.parametrize("mode", ["mfma_f64", "split_f16"])
+def test_folded_near_field_long_blocks_match_oracle(fpde, oracle, mode, fk):
+    """History blocks long enough that the near field is folded ahead in
+    several 16-level groups: 3D in 64-step blocks (the history combine folds
+    three groups per block, each into the block's later steps) and 1D in
+    128-step blocks (the cooperative launches fold seven), with the Toeplitz
+    weights and with FK 3's per-step weight rows, against the oracle over two
+    full blocks and part of a third."""
+    kw = dict(weight_mode=0, fk=3, fp=0.9) if fk else {}
+    tol = TOL_MIXED if mode.startswith("split") else TOL_F64
+    n, steps = 12, 150
+    u0 = fpde.init_3d(n, 1777)
+    spec = fpde.make_spec(n, steps, 1e-3, alpha=0.8, dims=3, hist_mode=mode, block=64, u_top=u0[0], **kw)
+    ref = oracle.solve_3d(oracle.make_params(n, steps, 1e-3, 0.8, ksat=spec.ksat, weight_mode=spec.weight_mode,
+                                             fk=fk, fp=0.9), u0)
+    err = rel_err(fpde.solve(spec, u0), ref)
+    print(f"3D block 64 FK {fk} {mode}: rel err {err:.3e}", flush=True)
+    assert err <= tol
+    nodes, M = 500, 300
+    u1 = fpde.init_1d(nodes, 1778)
+    sp1 = fpde.make_spec(nodes, M, 1e-3, alpha=0.7, picard_max=3, picard_tol=1e-12, hist_mode=mode, block=128,
+                         u_top=u1[0], **kw)
+    r1 = oracle.solve_1d(oracle.make_params(nodes, M, 1e-3, 0.7, ksat=sp1.ksat, picard_max=3, picard_tol=1e-12,
+                                            weight_mode=sp1.weight_mode, fk=fk, fp=0.9), u1)
+    err = rel_err(fpde.solve(sp1, u1), r1)
+    print(f"1D block 128 FK {fk} {mode}: rel err {err:.3e}", flush=True)
+    assert err <= tol
+
+
 @pytest.mark.parametrize("n,mode", [(12, "plain"), (12, "split_f16"), (12, "series"), (129, "mfma_f64")])
 def test_solve_3d_split3d_matches_oracle(fpde, oracle, n, mode):
     """split3d odd: each LOD sweep carries a third of the history term (Om1
